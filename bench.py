@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark: stable LSD radix sort of 2^28 uint32 keys on one MI355X.
+
+One step = one full device-resident sort (histogram, plan, four onesweep
+passes) of a distinct copy of the same 2^28-key array (uniform 32-bit keys
+from splitmix64, seed 0x5EED0002), already resident in HBM.  W untimed
+warm-up sorts, then exactly K timed sorts bracketed by device
+synchronisation; the output of the last step is checked against an
+independent sort before anything is printed.
+
+Prints one JSON line: Gkeys/s, the onesweep kernel's roofline (algorithmic
+bytes = 8 B per key per pass, from HIP events on the launch stream) and the
+CPU baseline (the C restatement of ParallelAlgorithm.SortRadixPar, timed on
+this host's cores).  Per-distribution figures go to stderr and to the JSON
+line's "extra" field.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+N_DEFAULT = 1 << 28
+SEED = 0x5EED0002
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_sorts(torch, N, src, copies: int):
+    bufs = [src.clone() for _ in range(copies)]
+    torch.cuda.synchronize()
+    return bufs
+
+
+def run_sort(N, lib, ctx, t, n, stream):
+    N.check(lib.hpcs_dev_sort_radix_u32(ctx, ctypes.c_void_p(t.data_ptr()), n, ctypes.c_void_p(stream)))
+
+
+def measure(torch, N, host_keys: np.ndarray, steps: int, warmup: int, profile: bool):
+    lib, ctx = N.lib(), N.context(0)
+    n = host_keys.size
+    src = torch.from_numpy(host_keys.view(np.int32)).cuda()
+    bufs = gpu_sorts(torch, N, src, warmup + steps)
+    stream = torch.cuda.current_stream().cuda_stream
+    for i in range(warmup):
+        run_sort(N, lib, ctx, bufs[i], n, stream)
+    torch.cuda.synchronize()
+    if profile:
+        N.profile_enable(True)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        run_sort(N, lib, ctx, bufs[warmup + i], n, stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = N.profile_read() if profile else {}
+    if profile:
+        N.profile_enable(False)
+    out = bufs[-1]
+    # correctness before reporting: equal to an independent library sort
+    ref = torch.sort(src.to(torch.int64) & 0xFFFFFFFF)[0]
+    ok = bool(torch.equal(out.to(torch.int64) & 0xFFFFFFFF, ref))
+    del bufs, ref, src, out
+    torch.cuda.empty_cache()
+    return (t1 - t0) / steps, stats, ok
+
+
+def cpu_baseline(host_keys: np.ndarray, reps: int):
+    """ParallelAlgorithm.SortRadixPar restated in C (oracle/), OpenMP threads."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    lib = O.lib()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    a = host_keys.copy()
+    O.sort_radix_par(a)           # warm-up
+    times = []
+    for _ in range(reps):
+        a = host_keys.copy()
+        t0 = time.perf_counter()
+        O.sort_radix_par(a)
+        times.append(time.perf_counter() - t0)
+    assert bool((a[1:] >= a[:-1]).all())
+    del lib
+    t = float(np.median(times))
+    return host_keys.size / t / 1e9, threads, times
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=N_DEFAULT)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dists", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hpcsharp_amd import _native as N
+    from hpcsharp_amd import datagen
+
+    if not torch.cuda.is_available():
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    n = args.n
+    log(f"[bench] generating {n} uniform u32 keys (seed {SEED:#x})")
+    keys = datagen.keys_u32(n, SEED, "uniform")
+    info = N.plan_query(N.HPCS_DTYPE_U32, n)
+    log(f"[bench] plan: {info.passes} passes x {info.digit_bits} bits, tile {info.tile_keys}, "
+        f"{info.tiles} tiles, ideal {info.ideal_bytes / 1e9:.3f} GB")
+    t_step, stats, ok = measure(torch, N, keys, args.steps, args.warmup, profile=True)
+    if not ok:
+        print("bench: sorted output differs from the reference sort", file=sys.stderr)
+        sys.exit(3)
+    gkeys = n / t_step / 1e9
+    log(f"[bench] uniform: {t_step * 1e3:.3f} ms/sort, {gkeys:.2f} Gkeys/s; kernels {stats}")
+
+    ones_launch, ones_ms = stats.get("onesweep", (0, 0.0))
+    ones_avg_ms = ones_ms / max(ones_launch, 1)
+    ones_bytes = n * 4 * 2                      # one read and one write of every key
+    ones_gbs = ones_bytes / (ones_avg_ms * 1e-3) / 1e9 if ones_avg_ms > 0 else 0.0
+    sort_gbs = info.ideal_bytes / t_step / 1e9
+
+    extra = {"sort_ms": round(t_step * 1e3, 4),
+             "sort_ideal_bytes": int(info.ideal_bytes),
+             "sort_roofline_frac": round(sort_gbs / HBM_PEAK_GBS, 4),
+             "kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in stats.items()}}
+    if not args.no_dists:
+        dists = {}
+        for dist in ("low8", "sorted", "constant"):
+            k2 = datagen.keys_u32(n, SEED + 1, dist)
+            t2, _, ok2 = measure(torch, N, k2, max(2, args.steps // 2), 1, profile=False)
+            if not ok2:
+                print(f"bench: {dist} output differs from the reference sort", file=sys.stderr)
+                sys.exit(3)
+            dists[dist] = round(n / t2 / 1e9, 3)
+            log(f"[bench] {dist}: {t2 * 1e3:.3f} ms/sort, {dists[dist]} Gkeys/s")
+        extra["gkeys_by_distribution"] = dists
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline: SortRadixPar restatement on {n} keys")
+        v, threads, times = cpu_baseline(keys, args.cpu_reps)
+        cpu = {"value": round(v, 4), "unit": "Gkeys/s", "cores": threads, "kind": "port",
+               "sample": f"full 2^{n.bit_length() - 1} uniform u32 array, 1 warm-up + median of {args.cpu_reps} "
+                         f"(s: {', '.join(f'{t:.2f}' for t in times)})"}
+        log(f"[bench] CPU: {v:.3f} Gkeys/s on {threads} threads")
+
+    line = {
+        "metric": "u32 LSD radix sort Gkeys/s @2^28 + %HBM roofline; batched vs plain",
+        "value": round(gkeys, 3),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (splitmix64 uniform 32-bit keys, device-resident)",
+        "config": {"workload": f"plain single-array stable LSD radix sort, uint32[2^{n.bit_length() - 1}] uniform",
+                   "n": n, "digit_bits": info.digit_bits, "passes": info.passes, "tile_keys": info.tile_keys},
+        "roofline": {"bound": "hbm", "achieved": round(ones_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ones_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_onesweep (one 8-bit pass: read + write 4 B per key)"},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,206 @@
+"""HPCsharp's radix-sort extension methods, served by the gfx950 library.
+
+``Algorithm`` and ``ParallelAlgorithm`` keep the reference's names, argument
+meaning, results and exceptions (namespace ``HPCsharp``, classes
+``Algorithm`` in RadixSortLsd.cs:90 / Histogram.cs:30 and
+``ParallelAlgorithm`` in RadixSortLsdParallel.cs:25 / HistogramParallel.cs:15).
+Arrays are numpy arrays standing in for the managed arrays: sorts run in
+place on the caller's array and the ``...Func`` forms return that same array,
+as the reference does.  Every call goes through libhpcsort.so on the GPU;
+there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+
+class ArgumentNullException(ValueError):
+    """System.ArgumentNullException (e.g. RadixSortLsd.cs:215-216)."""
+
+
+class ArgumentOutOfRangeException(ValueError):
+    """System.ArgumentOutOfRangeException (e.g. RadixSortLsd.cs:1621-1622)."""
+
+
+class OutOfMemoryException(MemoryError):
+    """System.OutOfMemoryException."""
+
+
+class InvalidOperationException(RuntimeError):
+    """A HIP runtime failure, carrying the library's message."""
+
+
+def _raise(err: N.HpcsError):
+    if err.code == N.HPCS_E_INVALID_ARG:
+        raise ArgumentOutOfRangeException(str(err)) from None
+    if err.code in (N.HPCS_E_OOM, N.HPCS_E_BUDGET):
+        raise OutOfMemoryException(str(err)) from None
+    raise InvalidOperationException(str(err)) from None
+
+
+def _call(fn, *args):
+    try:
+        N.check(fn(N.context(), *args))
+    except N.HpcsError as e:
+        _raise(e)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _require(a, name: str, dtypes):
+    if a is None:
+        raise ArgumentNullException(name)
+    if not isinstance(a, np.ndarray) or a.ndim != 1 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError(f"{name} must be a contiguous one-dimensional numpy array")
+    if a.dtype not in dtypes:
+        raise TypeError(f"{name} has unsupported element type {a.dtype}")
+    if not a.flags["WRITEABLE"]:
+        raise TypeError(f"{name} is read-only")
+
+
+_SORT = {
+    np.dtype(np.uint32): "hpcs_sort_radix_u32",
+    np.dtype(np.uint64): "hpcs_sort_radix_u64",
+    np.dtype(np.float32): "hpcs_sort_radix_f32",
+    np.dtype(np.float64): "hpcs_sort_radix_f64",
+}
+
+
+def _sort_whole(a: np.ndarray) -> None:
+    _call(getattr(N.lib(), _SORT[a.dtype]), _ptr(a), a.size)
+
+
+def _sort_range(a: np.ndarray, startIndex: int, length: int) -> None:
+    if startIndex < 0 or length < 0 or startIndex + length > a.size:
+        raise ArgumentOutOfRangeException("startIndex/length outside the array")
+    if a.dtype == np.uint32:
+        _call(N.lib().hpcs_sort_radix_range_u32, _ptr(a), startIndex, length)
+    else:
+        sub = a[startIndex:startIndex + length]
+        _call(getattr(N.lib(), _SORT[a.dtype]), _ptr(sub), sub.size)
+
+
+class Algorithm:
+    """Serial-surface methods (namespace HPCsharp, class Algorithm)."""
+
+    # -- uint[] / ulong[] LSD --------------------------------------------------
+    @staticmethod
+    def SortRadixLsd(inOutArray: np.ndarray, startIndex: int | None = None, length: int | None = None) -> None:
+        """SortRadixLsd(this uint[]) RadixSortLsd.cs:255-260; (start, length) :242-247
+        with the Experimental overload's sub-range semantics (:1409-1457);
+        ulong[] :424-442."""
+        _require(inOutArray, "inOutArray", (np.uint32, np.uint64))
+        if startIndex is None:
+            _sort_whole(inOutArray)
+        else:
+            _sort_range(inOutArray, startIndex, inOutArray.size - startIndex if length is None else length)
+
+    @staticmethod
+    def SortRadixLsdFunc(inOutArray: np.ndarray, startIndex: int | None = None, length: int | None = None) -> np.ndarray:
+        """SortRadixLsdFunc(this uint[]) RadixSortLsd.cs:213-232 (returns the same array)."""
+        Algorithm.SortRadixLsd(inOutArray, startIndex, length)
+        return inOutArray
+
+    @staticmethod
+    def SortRadix(inputArray: np.ndarray, keys: np.ndarray | None = None):
+        """Two reference shapes share this name:
+        - SortRadix(this uint[]) -- NuGet 3.17.0's name for SortRadixLsdFunc
+          (HPCsharpExamples Program.cs:26): sorts in place, returns the array;
+        - SortRadix<T>(T[] items, UInt32[]/UInt64[] keys) RadixSortLsd.cs:
+          1183-1299: items sorted stably by a parallel key array, returned as
+          (items, keys); here items are uint32 (e.g. indices or handles)."""
+        if keys is None:
+            _require(inputArray, "inputArray", tuple(_SORT))
+            _sort_whole(inputArray)
+            return inputArray
+        _require(inputArray, "inputArray", (np.uint32,))
+        _require(keys, "inKeys", (np.uint32, np.uint64))
+        if keys.size != inputArray.size:
+            raise ArgumentOutOfRangeException("inKeys must have one key per item")
+        fn = N.lib().hpcs_sort_radix_pairs_u64_u32 if keys.dtype == np.uint64 else N.lib().hpcs_sort_radix_pairs_u32_u32
+        _call(fn, _ptr(keys), _ptr(inputArray), keys.size)
+        return inputArray, keys
+
+    # -- float[] / double[] ----------------------------------------------------
+    @staticmethod
+    def SortRadixMsd(arrayToBeSorted: np.ndarray) -> None:
+        """SortRadixMsd(this float[]/double[]) RadixSortMsd.cs:1563-1595: ascending by the
+        radix key (-0.0 before +0.0)."""
+        _require(arrayToBeSorted, "arrayToBeSorted", (np.float32, np.float64))
+        _sort_whole(arrayToBeSorted)
+
+    @staticmethod
+    def SortRadixMsdInPlaceFunc(arrayToBeSorted: np.ndarray) -> np.ndarray:
+        """RadixSortMsd.cs:1578,1602."""
+        Algorithm.SortRadixMsd(arrayToBeSorted)
+        return arrayToBeSorted
+
+    # -- key-extractor records -------------------------------------------------
+    @staticmethod
+    def SortRadixLsd2(inputArray: np.ndarray, getKey: str) -> np.ndarray:
+        """SortRadixLsd2<T>(this T[], Func<T,UInt32/UInt64>) RadixSortLsd.cs:751-775, 821-846.
+
+        ``inputArray`` is a numpy structured array (the unmanaged struct) and
+        ``getKey`` names its key field (uint32/uint64/float32/float64); the
+        records are sorted stably in place and the array is returned."""
+        if inputArray is None:
+            raise ArgumentNullException("inputArray")
+        if getKey is None:
+            raise ArgumentNullException("getKey")
+        if not isinstance(inputArray, np.ndarray) or inputArray.dtype.fields is None:
+            raise TypeError("inputArray must be a numpy structured array")
+        if not inputArray.flags["C_CONTIGUOUS"]:
+            raise TypeError("inputArray must be contiguous")
+        ftype, offset = inputArray.dtype.fields[getKey][:2]
+        kinds = {np.dtype(np.uint32): (32, N.HPCS_KEY_UINT), np.dtype(np.uint64): (64, N.HPCS_KEY_UINT),
+                 np.dtype(np.float32): (32, N.HPCS_KEY_FLOAT), np.dtype(np.float64): (64, N.HPCS_KEY_FLOAT)}
+        if ftype not in kinds:
+            raise TypeError(f"key field {getKey} has unsupported type {ftype}")
+        bits, kind = kinds[ftype]
+        _call(N.lib().hpcs_sort_radix_records, _ptr(inputArray), inputArray.size, inputArray.dtype.itemsize,
+              offset, bits, kind)
+        return inputArray
+
+    SortRadixLsd_T = SortRadixLsd2
+    SortRadixFaster = SortRadixLsd2
+
+    # -- histograms ------------------------------------------------------------
+    @staticmethod
+    def HistogramByteComponents(inArray: np.ndarray, l: int, r: int) -> np.ndarray:
+        """HistogramByteComponents(uint[]/ulong[], int l, int r) Histogram.cs:261-296, 957-990:
+        int32[digits][256] of [l, r] inclusive."""
+        _require(inArray, "inArray", (np.uint32, np.uint64))
+        if l < 0 or (r >= l and r >= inArray.size):
+            raise IndexError("l..r outside the array")
+        P = inArray.dtype.itemsize
+        out = np.zeros((P, 256), dtype=np.int32)
+        fn = N.lib().hpcs_histogram_byte_components_u32 if P == 4 else N.lib().hpcs_histogram_byte_components_u64
+        _call(fn, _ptr(inArray), l, r, _ptr(out))
+        return out
+
+
+class ParallelAlgorithm:
+    """Multicore-surface methods (namespace HPCsharp, class ParallelAlgorithm)."""
+
+    @staticmethod
+    def SortRadixPar(inputArray: np.ndarray, ParallelWorkQuantum: int = 64 * 1024) -> np.ndarray:
+        """SortRadixPar(this uint[], int ParallelWorkQuantum = 64Ki) RadixSortLsdParallel.cs:145-232
+        and (this ulong[]) :690-734.  The work quantum is accepted and unused:
+        the GPU tiles are fixed by the kernel."""
+        if inputArray is None:
+            raise ArgumentNullException("inputArray")
+        _require(inputArray, "inputArray", (np.uint32, np.uint64))
+        _sort_whole(inputArray)
+        return inputArray
+
+    @staticmethod
+    def HistogramByteComponentsPar(inArray: np.ndarray, l: int, r: int, parallelThreshold: int = 16 * 1024) -> np.ndarray:
+        """HistogramByteComponentsPar(uint[]/ulong[], l, r, threshold) HistogramParallel.cs:214-222,
+        545-555: the same counts as uint32[digits][256]."""
+        return Algorithm.HistogramByteComponents(inArray, l, r).astype(np.uint32)

@@ -1,0 +1,623 @@
+// hpcs_api.hip -- host engine and C ABI of libhpcsort.so.
+//
+// The engine owns, per context, a device scratch pool and a stream.  A sort
+// is a fixed sequence of stream-ordered launches (fill, histogram, plan, P
+// onesweep passes, copy-back); no host synchronisation happens inside a
+// device-resident call, so the sequence can be captured in a HIP graph.
+#include "hpcs_kernels.hpp"
+#include "../../include/hpcsort.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace hpcs;
+
+#define HPCS_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Err {
+    int32_t code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int32_t code, const std::string &msg) { throw Err{code, msg}; }
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) fail(HPCS_E_OOM, std::string(what) + ": out of device memory");
+        fail(HPCS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+// Onesweep tile shape: 256 threads (four waves), 16 keys per thread.
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
+constexpr int HIST_BLOCK = 256;
+constexpr size_t ALIGN = 256;
+
+size_t align_up(size_t x, size_t a = ALIGN) { return (x + a - 1) / a * a; }
+
+} // namespace
+
+struct hpcs_ctx {
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    void *scratch = nullptr;
+    size_t scratch_cap = 0;
+    void *io = nullptr;
+    size_t io_cap = 0;
+    // kernel timing
+    struct Pending {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    bool profile = false;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> ev_pool;
+    std::map<std::string, std::pair<int64_t, double>> stats;
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        hip_check(hipGetDevice(&prev), "hipGetDevice");
+        if (prev != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// Grow a cached device buffer.  Work queued on any stream may still use the
+// old buffer, so the device is drained before it is released.
+void *ensure(void *&buf, size_t &cap, size_t need)
+{
+    if (need <= cap && buf) return buf;
+    if (buf) {
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        hip_check(hipFree(buf), "hipFree");
+        buf = nullptr;
+        cap = 0;
+    }
+    size_t want = align_up(std::max(need, size_t(1) << 20), size_t(2) << 20);
+    hip_check(hipMalloc(&buf, want), "hipMalloc");
+    cap = want;
+    return buf;
+}
+
+// Scratch layout of one sort:
+//   [hist P x 256 u32 | DevPlan | look-back region 0]  zero-filled first
+//   [look-back region 1] [keys B] [values B]
+struct Layout {
+    size_t off_hist, off_plan, off_lb0, zero_bytes, off_lb1, off_keysB, off_valsB, total;
+    int64_t tiles;
+};
+
+Layout make_layout(int64_t n, size_t key_size, bool vals)
+{
+    Layout L{};
+    L.tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    const size_t lb_bytes = size_t(L.tiles) * RADIX * sizeof(uint32_t);
+    L.off_hist = 0;
+    L.off_plan = align_up(MAX_PASSES * RADIX * sizeof(uint32_t));
+    L.off_lb0 = align_up(L.off_plan + sizeof(DevPlan));
+    L.zero_bytes = L.off_lb0 + lb_bytes;
+    L.off_lb1 = align_up(L.zero_bytes);
+    L.off_keysB = align_up(L.off_lb1 + lb_bytes);
+    L.off_valsB = align_up(L.off_keysB + size_t(n) * key_size);
+    L.total = align_up(L.off_valsB + (vals ? size_t(n) * sizeof(uint32_t) : 0));
+    return L;
+}
+
+hipEvent_t take_event(hpcs_ctx *c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+// Wait for recorded launch events and fold them into the statistics.
+void harvest(hpcs_ctx *c)
+{
+    for (auto &p : c->pending) {
+        hip_check(hipEventSynchronize(p.b), "hipEventSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+        auto &s = c->stats[p.name];
+        s.first += 1;
+        s.second += ms;
+        c->ev_pool.push_back(p.a);
+        c->ev_pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+// Run `launch` (which enqueues one kernel on st), bracketed by events when
+// profiling is on.
+template <typename F>
+void timed(hpcs_ctx *c, hipStream_t st, const char *name, F &&launch)
+{
+    if (!c->profile) {
+        launch();
+        hip_check(hipGetLastError(), name);
+        return;
+    }
+    if (c->pending.size() >= 4096) harvest(c);
+    hipEvent_t a = take_event(c), b = take_event(c);
+    hip_check(hipEventRecord(a, st), "hipEventRecord");
+    launch();
+    hip_check(hipGetLastError(), name);
+    hip_check(hipEventRecord(b, st), "hipEventRecord");
+    c->pending.push_back({name, a, b});
+}
+
+int grid_for(uint64_t items, int block, int max_blocks)
+{
+    uint64_t g = (items + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > uint64_t(max_blocks)) g = max_blocks;
+    return int(g);
+}
+
+void launch_fill(hpcs_ctx *c, void *p, uint64_t bytes, hipStream_t st)
+{
+    const uint64_t words = bytes / 4;
+    if (!words) return;
+    timed(c, st, "fill", [&] {
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(words, 256, c->num_cus * 8)), dim3(256), 0, st,
+                           static_cast<uint32_t *>(p), words, 0u);
+    });
+}
+
+template <typename K, bool FLOATK>
+void launch_histogram(hpcs_ctx *c, const K *d_keys, uint64_t n, uint32_t *d_hist, hipStream_t st)
+{
+    constexpr int VEC = 16 / int(sizeof(K));
+    const uintptr_t addr = reinterpret_cast<uintptr_t>(d_keys);
+    if (addr % sizeof(K)) fail(HPCS_E_INVALID_ARG, "keys pointer is not aligned to the key size");
+    const uint32_t head = uint32_t(((16 - (addr & 15)) & 15) / sizeof(K));
+    const uint64_t chunks = (n > head ? n - head : 0) / (WAVE * VEC);
+    const int waves_per_block = HIST_BLOCK / WAVE;
+    const int grid = grid_for(chunks, waves_per_block, c->num_cus * 4);
+    timed(c, st, "histogram", [&] {
+        hipLaunchKernelGGL((k_histogram<K, FLOATK, HIST_BLOCK>), dim3(grid), dim3(HIST_BLOCK), 0, st, d_keys, n,
+                           head, d_hist);
+    });
+}
+
+// The device-resident sort: keys (and optional uint32 values) in place.
+template <typename K, bool HAS_VALS, bool FLOATK>
+void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
+{
+    if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
+    if (n > 0 && !d_keys) fail(HPCS_E_INVALID_ARG, "keys is null");
+    if (HAS_VALS && n > 0 && !d_vals) fail(HPCS_E_INVALID_ARG, "values is null");
+    if (n <= 1) return;
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    if (reinterpret_cast<uintptr_t>(d_keys) % sizeof(K)) fail(HPCS_E_INVALID_ARG, "keys pointer is misaligned");
+    constexpr int P = int(sizeof(K));
+    const Layout L = make_layout(n, sizeof(K), HAS_VALS);
+    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
+    uint32_t *hist = reinterpret_cast<uint32_t *>(base + L.off_hist);
+    DevPlan *plan = reinterpret_cast<DevPlan *>(base + L.off_plan);
+    uint32_t *lb[2] = {reinterpret_cast<uint32_t *>(base + L.off_lb0), reinterpret_cast<uint32_t *>(base + L.off_lb1)};
+    K *keysB = reinterpret_cast<K *>(base + L.off_keysB);
+    uint32_t *valsB = HAS_VALS ? reinterpret_cast<uint32_t *>(base + L.off_valsB) : nullptr;
+
+    launch_fill(c, base, L.zero_bytes, st);
+    launch_histogram<K, FLOATK>(c, d_keys, uint64_t(n), hist, st);
+    timed(c, st, "plan", [&] {
+        hipLaunchKernelGGL((k_plan<P>), dim3(1), dim3(RADIX), 0, st, hist, plan, uint64_t(n), FLOATK ? 1 : 0);
+    });
+    for (int p = 0; p < P; ++p) {
+        uint32_t *next = (p + 1 < P) ? lb[(p + 1) & 1] : nullptr;
+        timed(c, st, "onesweep", [&] {
+            hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, SORT_BLOCK, SORT_ITEMS>), dim3(unsigned(L.tiles)),
+                               dim3(SORT_BLOCK), 0, st, d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan,
+                               lb[p & 1], next);
+        });
+    }
+    timed(c, st, "copyback", [&] {
+        hipLaunchKernelGGL((k_copyback<K>), dim3(grid_for(uint64_t(n), 256, c->num_cus * 8)), dim3(256), 0, st,
+                           d_keys, keysB, HAS_VALS ? d_vals : nullptr, valsB, uint64_t(n), plan);
+    });
+}
+
+template <typename K>
+void run_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset, int32_t key_kind,
+                 hipStream_t st)
+{
+    // Work buffer [keys n][idx n][gathered records n*rec_size], stream-ordered
+    // allocation; the sort's own scratch is the context's.
+    const size_t off_idx = align_up(size_t(n) * sizeof(K));
+    const size_t off_out = align_up(off_idx + size_t(n) * sizeof(uint32_t));
+    const size_t need = align_up(off_out + size_t(n) * size_t(rec_size));
+    void *work = nullptr;
+    hip_check(hipMallocAsync(&work, need, st), "hipMallocAsync");
+    char *w = static_cast<char *>(work);
+    K *keys = reinterpret_cast<K *>(w);
+    uint32_t *idx = reinterpret_cast<uint32_t *>(w + off_idx);
+    unsigned char *out = reinterpret_cast<unsigned char *>(w + off_out);
+    const int grid = grid_for(uint64_t(n), 256, c->num_cus * 8);
+    hipLaunchKernelGGL((k_extract_keys<K>), dim3(grid), dim3(256), 0, st,
+                       static_cast<const unsigned char *>(d_recs), uint64_t(n), rec_size, key_offset,
+                       key_kind == HPCS_KEY_FLOAT ? 1 : 0, keys, idx);
+    hip_check(hipGetLastError(), "k_extract_keys launch");
+    run_sort<K, true, false>(c, keys, idx, n, st);
+    const int words = ((rec_size & 3) == 0 && (reinterpret_cast<uintptr_t>(d_recs) & 3) == 0) ? 1 : 0;
+    hipLaunchKernelGGL(k_gather_records, dim3(grid_for(uint64_t(n) * uint64_t(rec_size) / 4 + 1, 256, c->num_cus * 8)),
+                       dim3(256), 0, st, static_cast<const unsigned char *>(d_recs), out, idx, uint64_t(n), rec_size,
+                       words);
+    hip_check(hipGetLastError(), "k_gather_records launch");
+    hip_check(hipMemcpyAsync(d_recs, out, size_t(n) * size_t(rec_size), hipMemcpyDeviceToDevice, st),
+              "hipMemcpyAsync records");
+    hip_check(hipFreeAsync(work, st), "hipFreeAsync");
+}
+
+void check_records_args(int64_t n, int32_t rec_size, int32_t key_offset, int32_t key_bits, int32_t key_kind)
+{
+    if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
+    if (key_bits != 32 && key_bits != 64) fail(HPCS_E_INVALID_ARG, "key_bits must be 32 or 64");
+    if (key_kind != HPCS_KEY_UINT && key_kind != HPCS_KEY_FLOAT) fail(HPCS_E_INVALID_ARG, "unknown key_kind");
+    if (rec_size <= 0 || key_offset < 0 || key_offset + key_bits / 8 > rec_size)
+        fail(HPCS_E_INVALID_ARG, "key does not fit inside the record");
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+}
+
+void dev_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset, int32_t key_bits,
+                 int32_t key_kind, hipStream_t st)
+{
+    check_records_args(n, rec_size, key_offset, key_bits, key_kind);
+    if (n <= 1) return;
+    if (!d_recs) fail(HPCS_E_INVALID_ARG, "records is null");
+    if (key_bits == 32)
+        run_records<uint32_t>(c, d_recs, n, rec_size, key_offset, key_kind, st);
+    else
+        run_records<uint64_t>(c, d_recs, n, rec_size, key_offset, key_kind, st);
+}
+
+template <typename K, bool FLOATK>
+void dev_histogram(hpcs_ctx *c, const K *d_keys, int64_t n, uint32_t *d_counts, hipStream_t st)
+{
+    if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
+    if (!d_counts) fail(HPCS_E_INVALID_ARG, "counts is null");
+    if (n > 0 && !d_keys) fail(HPCS_E_INVALID_ARG, "keys is null");
+    if (n >= (int64_t(1) << 32)) fail(HPCS_E_UNSUPPORTED, "histogram counts are 32-bit");
+    launch_fill(c, d_counts, sizeof(K) * RADIX * sizeof(uint32_t), st);
+    if (n > 0) launch_histogram<K, FLOATK>(c, d_keys, uint64_t(n), d_counts, st);
+}
+
+hipStream_t pick_stream(hpcs_ctx *c, void *stream) { return stream ? static_cast<hipStream_t>(stream) : c->stream; }
+
+// Host-array call: copy in, sort on the device, copy out, wait.
+template <typename K, bool HAS_VALS, bool FLOATK>
+void host_sort(hpcs_ctx *c, K *keys, uint32_t *vals, int64_t n)
+{
+    if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
+    if (n > 0 && !keys) fail(HPCS_E_INVALID_ARG, "keys is null");
+    if (HAS_VALS && n > 0 && !vals) fail(HPCS_E_INVALID_ARG, "values is null");
+    if (n <= 1) return;
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    const size_t kbytes = size_t(n) * sizeof(K);
+    const size_t voff = align_up(kbytes);
+    const size_t need = voff + (HAS_VALS ? size_t(n) * sizeof(uint32_t) : 0);
+    char *io = static_cast<char *>(ensure(c->io, c->io_cap, need));
+    K *d_keys = reinterpret_cast<K *>(io);
+    uint32_t *d_vals = HAS_VALS ? reinterpret_cast<uint32_t *>(io + voff) : nullptr;
+    hipStream_t st = c->stream;
+    hip_check(hipMemcpyAsync(d_keys, keys, kbytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
+    if (HAS_VALS) hip_check(hipMemcpyAsync(d_vals, vals, size_t(n) * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
+    run_sort<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
+    hip_check(hipMemcpyAsync(keys, d_keys, kbytes, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+    if (HAS_VALS) hip_check(hipMemcpyAsync(vals, d_vals, size_t(n) * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+template <typename K>
+void host_histogram(hpcs_ctx *c, const K *a, int64_t l, int64_t r, int32_t *counts)
+{
+    if (!a) fail(HPCS_E_INVALID_ARG, "inArray is null");
+    if (!counts) fail(HPCS_E_INVALID_ARG, "counts is null");
+    if (l < 0) fail(HPCS_E_INVALID_ARG, "l must be non-negative");
+    const int64_t n = r >= l ? r - l + 1 : 0;
+    if (n >= (int64_t(1) << 31)) fail(HPCS_E_UNSUPPORTED, "int32 counts limit the range to 2^31 - 1 keys");
+    constexpr int P = int(sizeof(K));
+    const size_t hoff = align_up(size_t(n) * sizeof(K));
+    char *io = static_cast<char *>(ensure(c->io, c->io_cap, hoff + P * RADIX * sizeof(uint32_t)));
+    K *d_keys = reinterpret_cast<K *>(io);
+    uint32_t *d_counts = reinterpret_cast<uint32_t *>(io + hoff);
+    hipStream_t st = c->stream;
+    if (n > 0) hip_check(hipMemcpyAsync(d_keys, a + l, size_t(n) * sizeof(K), hipMemcpyHostToDevice, st), "H2D");
+    dev_histogram<K, false>(c, d_keys, n, d_counts, st);
+    uint32_t tmp[8 * RADIX];
+    hip_check(hipMemcpyAsync(tmp, d_counts, P * RADIX * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (int i = 0; i < P * RADIX; ++i) counts[i] = int32_t(tmp[i]);
+}
+
+template <typename F>
+int32_t guarded(hpcs_ctx *c, F &&f)
+{
+    if (!c) {
+        g_last_error = "context is null";
+        return HPCS_E_INVALID_ARG;
+    }
+    try {
+        std::lock_guard<std::mutex> lock(c->mu);
+        DeviceGuard dg(c->device);
+        f();
+        return HPCS_OK;
+    } catch (const Err &e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host out of memory";
+        return HPCS_E_OOM;
+    } catch (...) {
+        g_last_error = "unexpected C++ exception";
+        return HPCS_E_HIP;
+    }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+HPCS_API int32_t hpcs_version(void) { return 100; }
+
+HPCS_API const char *hpcs_last_error(void) { return g_last_error.c_str(); }
+
+HPCS_API int32_t hpcs_ctx_create(int32_t device, hpcs_ctx **out)
+{
+    if (!out) {
+        g_last_error = "out is null";
+        return HPCS_E_INVALID_ARG;
+    }
+    *out = nullptr;
+    hpcs_ctx *c = nullptr;
+    try {
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count) fail(HPCS_E_INVALID_ARG, "no such HIP device");
+        c = new hpcs_ctx();
+        c->device = device;
+        DeviceGuard dg(device);
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        *out = c;
+        return HPCS_OK;
+    } catch (const Err &e) {
+        delete c;
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        delete c;
+        g_last_error = "host out of memory";
+        return HPCS_E_OOM;
+    }
+}
+
+HPCS_API int32_t hpcs_ctx_trim(hpcs_ctx *ctx)
+{
+    return guarded(ctx, [&] {
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        if (ctx->scratch) hip_check(hipFree(ctx->scratch), "hipFree");
+        if (ctx->io) hip_check(hipFree(ctx->io), "hipFree");
+        ctx->scratch = ctx->io = nullptr;
+        ctx->scratch_cap = ctx->io_cap = 0;
+    });
+}
+
+HPCS_API int32_t hpcs_ctx_destroy(hpcs_ctx *ctx)
+{
+    if (!ctx) return HPCS_OK;
+    int32_t rc = hpcs_ctx_trim(ctx);
+    try {
+        DeviceGuard dg(ctx->device);
+        for (auto &p : ctx->pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    } catch (const Err &) {
+    }
+    delete ctx;
+    return rc;
+}
+
+HPCS_API int32_t hpcs_profile_enable(hpcs_ctx *ctx, int32_t on)
+{
+    return guarded(ctx, [&] {
+        harvest(ctx);
+        ctx->stats.clear();
+        ctx->profile = on != 0;
+    });
+}
+
+HPCS_API int32_t hpcs_profile_read(hpcs_ctx *ctx, hpcs_kernel_stat *out, int32_t max, int32_t *count)
+{
+    return guarded(ctx, [&] {
+        if (!count || (max > 0 && !out)) fail(HPCS_E_INVALID_ARG, "out/count is null");
+        harvest(ctx);
+        int32_t i = 0;
+        for (const auto &kv : ctx->stats) {
+            if (i >= max) break;
+            std::memset(out[i].name, 0, sizeof(out[i].name));
+            std::strncpy(out[i].name, kv.first.c_str(), sizeof(out[i].name) - 1);
+            out[i].launches = kv.second.first;
+            out[i].total_ms = kv.second.second;
+            ++i;
+        }
+        *count = i;
+    });
+}
+
+HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_t with_values, hpcs_plan_info *info)
+{
+    if (!info) {
+        g_last_error = "info is null";
+        return HPCS_E_INVALID_ARG;
+    }
+    (void)ctx;
+    size_t ks;
+    switch (dtype) {
+    case HPCS_DTYPE_U32:
+    case HPCS_DTYPE_F32: ks = 4; break;
+    case HPCS_DTYPE_U64:
+    case HPCS_DTYPE_F64: ks = 8; break;
+    default: g_last_error = "unknown dtype"; return HPCS_E_INVALID_ARG;
+    }
+    if (n < 0) {
+        g_last_error = "n must be non-negative";
+        return HPCS_E_INVALID_ARG;
+    }
+    const int passes = int(ks * 8 / RADIX_BITS);
+    const Layout L = make_layout(n, ks, with_values != 0);
+    const int64_t per_key = int64_t(ks) + passes * 2 * (int64_t(ks) + (with_values ? 4 : 0));
+    info->digit_bits = RADIX_BITS;
+    info->passes = passes;
+    info->tile_keys = SORT_TILE;
+    info->block_threads = SORT_BLOCK;
+    info->tiles = L.tiles;
+    info->ideal_bytes = n * per_key;
+    info->scratch_bytes = int64_t(L.total);
+    return HPCS_OK;
+}
+
+// host arrays
+HPCS_API int32_t hpcs_sort_radix_u32(hpcs_ctx *ctx, uint32_t *keys, int64_t n)
+{
+    return guarded(ctx, [&] { host_sort<uint32_t, false, false>(ctx, keys, nullptr, n); });
+}
+HPCS_API int32_t hpcs_sort_radix_range_u32(hpcs_ctx *ctx, uint32_t *keys, int64_t start, int64_t length)
+{
+    return guarded(ctx, [&] {
+        if (start < 0 || length < 0) fail(HPCS_E_INVALID_ARG, "start and length must be non-negative");
+        if (length > 0 && !keys) fail(HPCS_E_INVALID_ARG, "keys is null");
+        host_sort<uint32_t, false, false>(ctx, keys ? keys + start : keys, nullptr, length);
+    });
+}
+HPCS_API int32_t hpcs_sort_radix_u64(hpcs_ctx *ctx, uint64_t *keys, int64_t n)
+{
+    return guarded(ctx, [&] { host_sort<uint64_t, false, false>(ctx, keys, nullptr, n); });
+}
+HPCS_API int32_t hpcs_sort_radix_f32(hpcs_ctx *ctx, float *keys, int64_t n)
+{
+    return guarded(ctx, [&] { host_sort<uint32_t, false, true>(ctx, reinterpret_cast<uint32_t *>(keys), nullptr, n); });
+}
+HPCS_API int32_t hpcs_sort_radix_f64(hpcs_ctx *ctx, double *keys, int64_t n)
+{
+    return guarded(ctx, [&] { host_sort<uint64_t, false, true>(ctx, reinterpret_cast<uint64_t *>(keys), nullptr, n); });
+}
+HPCS_API int32_t hpcs_sort_radix_pairs_u64_u32(hpcs_ctx *ctx, uint64_t *keys, uint32_t *vals, int64_t n)
+{
+    return guarded(ctx, [&] { host_sort<uint64_t, true, false>(ctx, keys, vals, n); });
+}
+HPCS_API int32_t hpcs_sort_radix_pairs_u32_u32(hpcs_ctx *ctx, uint32_t *keys, uint32_t *vals, int64_t n)
+{
+    return guarded(ctx, [&] { host_sort<uint32_t, true, false>(ctx, keys, vals, n); });
+}
+HPCS_API int32_t hpcs_sort_radix_records(hpcs_ctx *ctx, void *recs, int64_t n, int32_t rec_size, int32_t key_offset,
+                                         int32_t key_bits, int32_t key_kind)
+{
+    return guarded(ctx, [&] {
+        check_records_args(n, rec_size, key_offset, key_bits, key_kind);
+        if (n > 0 && !recs) fail(HPCS_E_INVALID_ARG, "records is null");
+        if (n <= 1) return;
+        const size_t bytes = size_t(n) * size_t(rec_size);
+        void *d = ensure(ctx->io, ctx->io_cap, bytes);
+        hip_check(hipMemcpyAsync(d, recs, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
+        dev_records(ctx, d, n, rec_size, key_offset, key_bits, key_kind, ctx->stream);
+        hip_check(hipMemcpyAsync(recs, d, bytes, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+HPCS_API int32_t hpcs_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *a, int64_t l, int64_t r,
+                                                    int32_t *counts)
+{
+    return guarded(ctx, [&] { host_histogram<uint32_t>(ctx, a, l, r, counts); });
+}
+HPCS_API int32_t hpcs_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *a, int64_t l, int64_t r,
+                                                    int32_t *counts)
+{
+    return guarded(ctx, [&] { host_histogram<uint64_t>(ctx, a, l, r, counts); });
+}
+
+// device-resident
+HPCS_API int32_t hpcs_dev_sort_radix_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream)
+{
+    return guarded(ctx, [&] { run_sort<uint32_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });
+}
+HPCS_API int32_t hpcs_dev_sort_radix_u64(hpcs_ctx *ctx, uint64_t *d_keys, int64_t n, void *stream)
+{
+    return guarded(ctx, [&] { run_sort<uint64_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });
+}
+HPCS_API int32_t hpcs_dev_sort_radix_f32(hpcs_ctx *ctx, float *d_keys, int64_t n, void *stream)
+{
+    return guarded(ctx, [&] {
+        run_sort<uint32_t, false, true>(ctx, reinterpret_cast<uint32_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
+    });
+}
+HPCS_API int32_t hpcs_dev_sort_radix_f64(hpcs_ctx *ctx, double *d_keys, int64_t n, void *stream)
+{
+    return guarded(ctx, [&] {
+        run_sort<uint64_t, false, true>(ctx, reinterpret_cast<uint64_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
+    });
+}
+HPCS_API int32_t hpcs_dev_sort_radix_pairs_u64_u32(hpcs_ctx *ctx, uint64_t *d_keys, uint32_t *d_vals, int64_t n,
+                                                   void *stream)
+{
+    return guarded(ctx, [&] { run_sort<uint64_t, true, false>(ctx, d_keys, d_vals, n, pick_stream(ctx, stream)); });
+}
+HPCS_API int32_t hpcs_dev_sort_radix_pairs_u32_u32(hpcs_ctx *ctx, uint32_t *d_keys, uint32_t *d_vals, int64_t n,
+                                                   void *stream)
+{
+    return guarded(ctx, [&] { run_sort<uint32_t, true, false>(ctx, d_keys, d_vals, n, pick_stream(ctx, stream)); });
+}
+HPCS_API int32_t hpcs_dev_sort_radix_records(hpcs_ctx *ctx, void *d_recs, int64_t n, int32_t rec_size,
+                                             int32_t key_offset, int32_t key_bits, int32_t key_kind, void *stream)
+{
+    return guarded(ctx, [&] {
+        dev_records(ctx, d_recs, n, rec_size, key_offset, key_bits, key_kind, pick_stream(ctx, stream));
+    });
+}
+HPCS_API int32_t hpcs_dev_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n,
+                                                        uint32_t *d_counts, void *stream)
+{
+    return guarded(ctx, [&] { dev_histogram<uint32_t, false>(ctx, d_keys, n, d_counts, pick_stream(ctx, stream)); });
+}
+HPCS_API int32_t hpcs_dev_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n,
+                                                        uint32_t *d_counts, void *stream)
+{
+    return guarded(ctx, [&] { dev_histogram<uint64_t, false>(ctx, d_keys, n, d_counts, pick_stream(ctx, stream)); });
+}

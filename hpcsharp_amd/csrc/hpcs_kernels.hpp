@@ -1,0 +1,472 @@
+// hpcs_kernels.hpp -- gfx950 kernels of the stable LSD radix sort.
+//
+// Pipeline of one sort of n keys (8-bit digits, P = sizeof(key) passes):
+//   k_fill        zero the histogram, the plan and look-back region 0
+//   k_histogram   one read of the keys, every digit's 256-bin count at once
+//                 (the GPU form of HistogramByteComponents, Histogram.cs:261-296)
+//   k_plan        per pass: exclusive digit starts (ComputeStartOfBins,
+//                 RadixSortLsd.cs:112-125), whether the pass is trivial (all
+//                 keys share the digit: a stable identity), ping-pong buffer
+//                 selection, where the float key transform goes in and out
+//   k_onesweep    one scatter pass (SortRadixPar's per-quantum pass,
+//                 RadixSortLsdParallel.cs:184-226): tiles take ids in launch
+//                 order, rank their keys stably in LDS, find their global
+//                 offsets by decoupled look-back over earlier tiles, and write
+//                 each digit's run contiguously from LDS
+//   k_copyback    if an odd number of passes ran, move the result back
+//
+// Stability: the tile order equals the input order and the in-tile rank
+// follows (wave, item, lane) = input order, so equal digits keep their order
+// in every pass, which is what makes LSD radix sort correct.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hpcs {
+
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int MAX_PASSES = 8;
+constexpr int WAVE = 64;
+
+// Look-back word: [31:30] status, [29:0] count.  Status 0 = not yet
+// published, AGG = this tile's own count, INC = count of all tiles up to and
+// including this one.
+constexpr uint32_t LB_AGG = 1u << 30;
+constexpr uint32_t LB_INC = 2u << 30;
+constexpr uint32_t LB_VALUE = (1u << 30) - 1;
+constexpr int64_t MAX_KEYS = int64_t(1) << 30;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+template <typename K> struct Vec16;
+template <> struct Vec16<uint32_t> { using type = u32x4; };
+template <> struct Vec16<uint64_t> { using type = u64x2; };
+
+// Device-side plan, written by k_plan and read by the pass kernels.  It sits in
+// the zero-filled part of the scratch so that the tile counters start at 0.
+struct DevPlan {
+    uint32_t gstart[MAX_PASSES][RADIX]; // exclusive global start of each digit
+    uint32_t trivial[MAX_PASSES];       // 1 if every key has the same digit
+    uint32_t src[MAX_PASSES];           // 0: data in buffer A, 1: in buffer B
+    uint32_t xin[MAX_PASSES];           // apply the float key transform on load
+    uint32_t xout[MAX_PASSES];          // apply the inverse transform on store
+    uint32_t tile_counter[MAX_PASSES];  // dynamic tile ids per pass
+    uint32_t copyback;                  // final data sits in buffer B
+    uint32_t pad[63];
+};
+
+// ---------------------------------------------------------------------------
+// Key traits: the float/double radix key of RadixSortMsd.cs:1379-1383 and
+// :1484-1488 (non-negative: flip the sign bit; negative: flip every bit).
+// ---------------------------------------------------------------------------
+template <typename K> struct KeyOps;
+template <> struct KeyOps<uint32_t> {
+    static constexpr uint32_t SIGN = 0x80000000u;
+    __device__ static inline uint32_t to_key(uint32_t b) { return (b & SIGN) ? ~b : (b ^ SIGN); }
+    __device__ static inline uint32_t from_key(uint32_t k) { return (k & SIGN) ? (k ^ SIGN) : ~k; }
+    __device__ static inline uint32_t digit(uint32_t k, int shift) { return (k >> shift) & (RADIX - 1); }
+};
+template <> struct KeyOps<uint64_t> {
+    static constexpr uint64_t SIGN = 0x8000000000000000ull;
+    __device__ static inline uint64_t to_key(uint64_t b) { return (b & SIGN) ? ~b : (b ^ SIGN); }
+    __device__ static inline uint64_t from_key(uint64_t k) { return (k & SIGN) ? (k ^ SIGN) : ~k; }
+    __device__ static inline uint32_t digit(uint64_t k, int shift) { return uint32_t(k >> shift) & (RADIX - 1); }
+};
+
+__device__ inline uint32_t lb_load(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void lb_store(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Lanes of the wave whose 8-bit digit equals this lane's: eight ballots, one
+// per digit bit; a lane keeps the ballot where its bit is set and the
+// complement where it is clear.
+__device__ inline uint64_t match_digit8(uint32_t d)
+{
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int b = 0; b < RADIX_BITS; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ inline uint32_t count_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// ---------------------------------------------------------------------------
+// k_fill: zero (or fill) a word range.  Used instead of hipMemsetAsync so the
+// sequence stays capturable in a HIP graph.
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t words, uint32_t value)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// k_histogram: every digit's 256-bin count in one read.
+// Waves walk 1 KiB chunks (64 lanes x one 16-byte vector) in a grid-stride
+// loop; each digit update is an LDS atomic on the block's [P][256] table,
+// except that a wave whose 64 lanes share a digit adds 64 once (sorted and
+// constant inputs).  The tail below a whole chunk is counted lane by lane.
+// Block tables are added into the global [P][256] result with one atomic per
+// non-zero bin.
+// ---------------------------------------------------------------------------
+// `head` (< VEC) keys precede the first 16-byte aligned key; they are counted
+// lane by lane together with the tail.
+template <typename K, bool FLOATK, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys, uint64_t n, uint32_t head,
+                                                     uint32_t *__restrict__ hist)
+{
+    constexpr int P = int(sizeof(K));
+    constexpr int VEC = 16 / int(sizeof(K)); // keys per 16-byte vector
+    constexpr int CHUNK = WAVE * VEC;
+    constexpr int WAVES = BLOCK / WAVE;
+    using Vec = typename Vec16<K>::type;
+    __shared__ uint32_t sh[P * RADIX];
+
+    for (int i = threadIdx.x; i < P * RADIX; i += BLOCK) sh[i] = 0;
+    __syncthreads();
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t body = n > head ? n - head : 0;
+    const uint64_t nchunks = body / CHUNK;
+    const uint64_t wave_id = uint64_t(blockIdx.x) * WAVES + (threadIdx.x >> 6);
+    const uint64_t nwaves = uint64_t(gridDim.x) * WAVES;
+    const Vec *vkeys = reinterpret_cast<const Vec *>(keys + head);
+
+    for (uint64_t c = wave_id; c < nchunks; c += nwaves) {
+        Vec v = __builtin_nontemporal_load(&vkeys[c * WAVE + lane]);
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+            const K raw = v[j];
+            K key = FLOATK ? KeyOps<K>::to_key(raw) : raw;
+#pragma unroll
+            for (int d = 0; d < P; ++d) {
+                const uint32_t dg = KeyOps<K>::digit(key, 8 * d);
+                const uint32_t d0 = __builtin_amdgcn_readfirstlane(dg);
+                if (__all(dg == d0)) {
+                    if (lane == 0) atomicAdd(&sh[d * RADIX + d0], uint32_t(WAVE));
+                } else {
+                    atomicAdd(&sh[d * RADIX + dg], 1u);
+                }
+            }
+        }
+    }
+    // head keys [0, head) and tail keys [head + nchunks * CHUNK, n), block 0
+    if (blockIdx.x == 0) {
+        const uint64_t tail0 = head + nchunks * CHUNK;
+        const uint64_t extra = (n < head ? n : uint64_t(head)) + (n > tail0 ? n - tail0 : 0);
+        for (uint64_t t = threadIdx.x; t < extra; t += BLOCK) {
+            const uint64_t i = t < head ? t : tail0 + (t - head);
+            K key = FLOATK ? KeyOps<K>::to_key(keys[i]) : keys[i];
+#pragma unroll
+            for (int d = 0; d < P; ++d) atomicAdd(&sh[d * RADIX + KeyOps<K>::digit(key, 8 * d)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < P * RADIX; i += BLOCK) {
+        const uint32_t c = sh[i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_plan: one block of RADIX threads; thread b owns bin b of every pass.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(RADIX) void k_plan(const uint32_t *__restrict__ hist, DevPlan *__restrict__ plan,
+                                                uint64_t n, int floatk)
+{
+    __shared__ uint32_t s[RADIX];
+    __shared__ uint32_t s_trivial[MAX_PASSES];
+    const int b = threadIdx.x;
+    if (b < MAX_PASSES) s_trivial[b] = 0;
+    __syncthreads();
+    for (int p = 0; p < P; ++p) {
+        const uint32_t c = hist[p * RADIX + b];
+        if (uint64_t(c) == n) s_trivial[p] = 1; // one bin holds every key
+        s[b] = c;
+        __syncthreads();
+        // Hillis-Steele inclusive scan over 256 bins
+        for (int off = 1; off < RADIX; off <<= 1) {
+            const uint32_t add = b >= off ? s[b - off] : 0u;
+            __syncthreads();
+            s[b] += add;
+            __syncthreads();
+        }
+        plan->gstart[p][b] = s[b] - c;
+        __syncthreads();
+    }
+    if (b == 0) {
+        uint32_t cur = 0;
+        int first = -1, last = -1;
+        for (int p = 0; p < P; ++p) {
+            const uint32_t t = (n <= 1) ? 1u : s_trivial[p];
+            plan->trivial[p] = t;
+            plan->src[p] = cur;
+            if (!t) {
+                cur ^= 1u;
+                if (first < 0) first = p;
+                last = p;
+            }
+        }
+        for (int p = 0; p < P; ++p) {
+            plan->xin[p] = (floatk && p == first) ? 1u : 0u;
+            plan->xout[p] = (floatk && p == last) ? 1u : 0u;
+        }
+        plan->copyback = cur;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_onesweep: one stable scatter pass over 8-bit digit `pass`.
+//
+// Tile = BLOCK threads x ITEMS keys.  Wave w of the tile owns the contiguous
+// span [w*64*ITEMS, (w+1)*64*ITEMS) and loads it lane-striped (item i, lane l
+// -> key i*64 + l), so every load instruction reads 64 consecutive keys.
+//
+// 1. rank: items in order; eight ballots give the lanes holding the same
+//    digit; the wave's LDS counter for that digit gives how many earlier
+//    items of the wave had it; the group's first lane advances the counter.
+// 2. thread b (one per digit) turns the per-wave counts into per-wave bases,
+//    publishes the tile's count for the look-back and scans the tile counts
+//    over digits.
+// 3. keys (and values) go to LDS at their in-tile sorted position.
+// 4. decoupled look-back: thread b walks back over earlier tiles' words for
+//    digit b, summing own-counts until it meets an inclusive count, then
+//    publishes its own inclusive count.
+// 5. the tile is written from LDS in order; each digit's run lands at
+//    consecutive addresses.
+// ---------------------------------------------------------------------------
+template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__restrict__ keysB,
+                                                    uint32_t *__restrict__ valsA, uint32_t *__restrict__ valsB,
+                                                    uint32_t n, int pass, DevPlan *__restrict__ plan,
+                                                    uint32_t *__restrict__ lb_cur, uint32_t *__restrict__ lb_next)
+{
+    static_assert(BLOCK >= RADIX && BLOCK % WAVE == 0, "one thread per digit is required");
+    constexpr int WAVES = BLOCK / WAVE;
+    constexpr int TILE = BLOCK * ITEMS;
+    constexpr int WAVE_TILE = WAVE * ITEMS;
+    constexpr int SCAN_WAVES = RADIX / WAVE;
+
+    __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint32_t s_base[RADIX];
+    __shared__ uint32_t s_scan[SCAN_WAVES];
+    __shared__ uint32_t s_tile;
+    __shared__ K s_keys[TILE];
+    __shared__ uint32_t s_vals[HAS_VALS ? TILE : 1];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wave = tid >> 6;
+
+    // Clear the next pass's look-back words for tile blockIdx.x (the grid has
+    // exactly one block per tile); that region's previous pass has finished.
+    if (lb_next != nullptr && tid < RADIX) lb_next[size_t(blockIdx.x) * RADIX + tid] = 0u;
+
+    if (plan->trivial[pass]) return;
+
+    const uint32_t src = plan->src[pass];
+    const K *__restrict__ kin = src ? keysB : keysA;
+    K *__restrict__ kout = src ? keysA : keysB;
+    const uint32_t *__restrict__ vin = src ? valsB : valsA;
+    uint32_t *__restrict__ vout = src ? valsA : valsB;
+    const bool xin = FLOATK && plan->xin[pass];
+    const bool xout = FLOATK && plan->xout[pass];
+    const int shift = pass * RADIX_BITS;
+
+    if (tid == 0) s_tile = atomicAdd(&plan->tile_counter[pass], 1u);
+    for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t tile_base = tile * uint32_t(TILE);
+    const uint32_t wave_base = tile_base + uint32_t(wave * WAVE_TILE);
+    // keys of this wave that exist: items with i*64 + lane < wave_valid
+    const int wave_valid = n > wave_base ? int(min(uint32_t(WAVE_TILE), n - wave_base)) : 0;
+
+    // ---- load ----
+    K keys[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int off = i * WAVE + lane;
+        if (off < wave_valid) {
+            K k = kin[wave_base + off];
+            keys[i] = xin ? KeyOps<K>::to_key(k) : k;
+        } else {
+            keys[i] = ~K(0);
+        }
+    }
+
+    // ---- 1. rank within the wave ----
+    uint32_t rank[ITEMS];
+    uint32_t *cnt = s_cnt[wave];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t d = KeyOps<K>::digit(keys[i], shift);
+        const bool valid = i * WAVE + lane < wave_valid;
+        const uint64_t peers = match_digit8(d) & __ballot(valid);
+        const uint32_t below = count_below(peers);
+        const uint32_t old = cnt[d];
+        if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
+        rank[i] = old + below;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // ---- 2. per-digit wave bases, tile count, publish, scan ----
+    uint32_t tile_cnt = 0, tile_start = 0;
+    if (tid < RADIX) {
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = s_cnt[w][tid];
+            s_cnt[w][tid] = tile_cnt;
+            tile_cnt += c;
+        }
+        lb_store(&lb_cur[size_t(tile) * RADIX + tid], (tile == 0 ? LB_INC : LB_AGG) | tile_cnt);
+        // inclusive scan over the 64 digits of this wave
+        uint32_t x = tile_cnt;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == WAVE - 1) s_scan[wave] = x;
+        tile_start = x - tile_cnt;
+    }
+    __syncthreads();
+    if (tid < RADIX) {
+        for (int w = 0; w < wave; ++w) tile_start += s_scan[w];
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) s_cnt[w][tid] += tile_start;
+    }
+    __syncthreads();
+
+    // ---- 3. stage the tile in LDS in sorted order ----
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        if (i * WAVE + lane < wave_valid) {
+            const uint32_t d = KeyOps<K>::digit(keys[i], shift);
+            rank[i] += cnt[d];
+            s_keys[rank[i]] = keys[i];
+        }
+    }
+    if constexpr (HAS_VALS) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int off = i * WAVE + lane;
+            if (off < wave_valid) s_vals[rank[i]] = vin[wave_base + off];
+        }
+    }
+
+    // ---- 4. decoupled look-back ----
+    if (tid < RADIX) {
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int64_t p = int64_t(tile) - 1;
+            while (true) {
+                const uint32_t v = lb_load(&lb_cur[size_t(p) * RADIX + tid]);
+                const uint32_t st = v & ~LB_VALUE;
+                if (st == 0u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += v & LB_VALUE;
+                if (st == LB_INC) break;
+                --p;
+            }
+            lb_store(&lb_cur[size_t(tile) * RADIX + tid], LB_INC | (excl + tile_cnt));
+        }
+        s_base[tid] = plan->gstart[pass][tid] + excl - tile_start;
+    }
+    __syncthreads();
+
+    // ---- 5. write the tile out, each digit's run contiguous ----
+    const int tile_valid = n > tile_base ? int(min(uint32_t(TILE), n - tile_base)) : 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int j = i * BLOCK + tid;
+        if (j < tile_valid) {
+            const K k = s_keys[j];
+            const uint32_t dst = s_base[KeyOps<K>::digit(k, shift)] + uint32_t(j);
+            kout[dst] = xout ? KeyOps<K>::from_key(k) : k;
+            if constexpr (HAS_VALS) vout[dst] = s_vals[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_copyback: if the result sits in buffer B, copy it to A.
+// ---------------------------------------------------------------------------
+template <typename K>
+__global__ void k_copyback(K *__restrict__ keysA, const K *__restrict__ keysB, uint32_t *__restrict__ valsA,
+                           const uint32_t *__restrict__ valsB, uint64_t n, const DevPlan *__restrict__ plan)
+{
+    if (!plan->copyback) return;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        keysA[i] = keysB[i];
+        if (valsA) valsA[i] = valsB[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Records: extract (radix key, index) pairs, and gather records by index.
+// ---------------------------------------------------------------------------
+template <typename K>
+__global__ void k_extract_keys(const unsigned char *__restrict__ recs, uint64_t n, int rec_size, int key_offset,
+                               int floatk, K *__restrict__ keys, uint32_t *__restrict__ idx)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const unsigned char *p = recs + i * uint64_t(rec_size) + key_offset;
+        K k;
+        if ((reinterpret_cast<uintptr_t>(p) & (sizeof(K) - 1)) == 0) {
+            k = *reinterpret_cast<const K *>(p);
+        } else {
+            k = 0;
+            for (int b = 0; b < int(sizeof(K)); ++b) k |= K(p[b]) << (8 * b);
+        }
+        keys[i] = floatk ? KeyOps<K>::to_key(k) : k;
+        idx[i] = uint32_t(i);
+    }
+}
+
+// One thread per 4-byte word of output when rec_size and the buffer allow it
+// (words4), one per byte otherwise.
+__global__ void k_gather_records(const unsigned char *__restrict__ in, unsigned char *__restrict__ out,
+                                 const uint32_t *__restrict__ idx, uint64_t n, int rec_size, int words4)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    if (words4) {
+        const int words = rec_size >> 2;
+        const uint64_t total = n * uint64_t(words);
+        for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
+            const uint64_t r = t / words;
+            const int w = int(t - r * words);
+            reinterpret_cast<uint32_t *>(out)[t] =
+                reinterpret_cast<const uint32_t *>(in + uint64_t(idx[r]) * rec_size)[w];
+        }
+    } else {
+        const uint64_t total = n * uint64_t(rec_size);
+        for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
+            const uint64_t r = t / rec_size;
+            const int b = int(t - r * rec_size);
+            out[t] = in[uint64_t(idx[r]) * rec_size + b];
+        }
+    }
+}
+
+} // namespace hpcs

@@ -1,0 +1,151 @@
+/*
+ * hpcsort.h -- C ABI of libhpcsort.so, the MI355X (gfx950) implementation of
+ * HPCsharp's stable LSD radix sort and byte-component histogram.
+ *
+ * Plain C: pointers, sizes and status codes only.  No HIP, C++ or torch
+ * types cross this boundary; a stream is passed as `void*` (a hipStream_t,
+ * NULL = the context's own stream).
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the HPCsharp source tree).  The managed binding a .NET caller would use
+ * ([LibraryImport] over pinned spans) is given in INTEGRATION.md.
+ *
+ * Conventions shared by every entry point:
+ *  - Return HPCS_OK (0) or a negative HPCS_E_* status; hpcs_last_error()
+ *    returns a thread-local message for the last failure on this thread.
+ *  - "Host-array" calls are synchronous and sort the caller's array in place
+ *    (the reference's in-place interface: the result is in the caller's
+ *    array, RadixSortLsd.cs:196-201, RadixSortLsdParallel.cs:223-231).
+ *  - "hpcs_dev_*" calls take device pointers, are ordered on `stream` and
+ *    return without synchronising.  Scratch memory belongs to the context;
+ *    calls that share a context are serialised on the context's lock, and two
+ *    streams must not use one context's scratch concurrently (use one context
+ *    per stream for concurrent work).
+ *  - Sorting is stable: equal keys keep their input order, so key-value and
+ *    record sorts carry the exact stable permutation of the reference.
+ *  - Floating-point keys use the reference's radix key (non-negative values
+ *    flip the sign bit, negative values flip every bit, RadixSortMsd.cs:
+ *    1379-1383, 1484-1488): ascending order with -0.0 before +0.0.
+ *  - Element counts are int64_t; n must be below 2^30 elements per array in
+ *    this version (HPCS_E_UNSUPPORTED otherwise).
+ */
+#ifndef HPCSORT_H
+#define HPCSORT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPCS_OK 0
+#define HPCS_E_INVALID_ARG (-1) /* ArgumentNullException / ArgumentOutOfRangeException */
+#define HPCS_E_OOM (-2)         /* OutOfMemoryException */
+#define HPCS_E_HIP (-3)         /* a HIP runtime error (InvalidOperationException) */
+#define HPCS_E_BUDGET (-4)      /* request exceeds the context's memory budget */
+#define HPCS_E_UNSUPPORTED (-5) /* size or option outside this version */
+#define HPCS_E_BUSY (-6)        /* resource in use */
+
+#define HPCS_KEY_UINT 0  /* record key is an unsigned integer */
+#define HPCS_KEY_FLOAT 1 /* record key is an IEEE float/double (radix key transform) */
+
+#define HPCS_DTYPE_U32 0
+#define HPCS_DTYPE_U64 1
+#define HPCS_DTYPE_F32 2
+#define HPCS_DTYPE_F64 3
+
+typedef struct hpcs_ctx hpcs_ctx;
+
+/* Sort plan for a dtype and size: what the sort moves, for roofline accounting. */
+typedef struct hpcs_plan_info {
+    int32_t digit_bits;      /* bits per radix digit */
+    int32_t passes;          /* scatter passes in the full plan */
+    int32_t tile_keys;       /* keys per onesweep tile (one workgroup) */
+    int32_t block_threads;   /* threads per workgroup */
+    int64_t tiles;           /* tiles per pass */
+    int64_t ideal_bytes;     /* histogram read + (read + write) per pass, keys (+ values) */
+    int64_t scratch_bytes;   /* device scratch the plan needs */
+} hpcs_plan_info;
+
+/* Per-kernel timing collected with HIP events recorded on the launch stream. */
+typedef struct hpcs_kernel_stat {
+    char name[32];     /* kernel name, e.g. "onesweep" */
+    int64_t launches;  /* timed launches since profiling was enabled */
+    double total_ms;   /* sum of event-measured durations */
+} hpcs_kernel_stat;
+
+/* ---- library and context ------------------------------------------------ */
+
+int32_t hpcs_version(void);
+const char *hpcs_last_error(void);
+/* Create a context on HIP device `device`; owns scratch memory and a stream. */
+int32_t hpcs_ctx_create(int32_t device, hpcs_ctx **out);
+int32_t hpcs_ctx_destroy(hpcs_ctx *ctx);
+/* Release the context's cached scratch memory. */
+int32_t hpcs_ctx_trim(hpcs_ctx *ctx);
+int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_t with_values, hpcs_plan_info *info);
+/* Kernel timing (no counterpart in the reference, whose Stopwatch timing is
+ * commented out, RadixSortLsd.cs:146-153): on != 0 clears the statistics and
+ * brackets every later launch of this context with HIP events on its stream;
+ * hpcs_profile_read waits for those events and returns up to max entries. */
+int32_t hpcs_profile_enable(hpcs_ctx *ctx, int32_t on);
+int32_t hpcs_profile_read(hpcs_ctx *ctx, hpcs_kernel_stat *out, int32_t max, int32_t *count);
+
+/* ---- host-array drop-ins (synchronous, in place) ------------------------ */
+
+/* Algorithm.SortRadixLsd(this uint[]) RadixSortLsd.cs:255-260 (and its
+ * SortRadix compatibility name, HPCsharpExamples Program.cs:26);
+ * ParallelAlgorithm.SortRadixPar(this uint[], int) RadixSortLsdParallel.cs:145-232. */
+int32_t hpcs_sort_radix_u32(hpcs_ctx *ctx, uint32_t *keys, int64_t n);
+/* Algorithm.SortRadixLsd(this uint[], int startIndex, int length) with the
+ * Experimental overload's correct sub-range semantics, RadixSortLsd.cs:1409-1457. */
+int32_t hpcs_sort_radix_range_u32(hpcs_ctx *ctx, uint32_t *keys, int64_t start, int64_t length);
+/* Algorithm.SortRadixLsd(this ulong[]) RadixSortLsd.cs:359-442;
+ * ParallelAlgorithm.SortRadixPar(this ulong[]) RadixSortLsdParallel.cs:690-734. */
+int32_t hpcs_sort_radix_u64(hpcs_ctx *ctx, uint64_t *keys, int64_t n);
+/* Algorithm.SortRadixMsd(this float[]) RadixSortMsd.cs:1563-1571 (float order). */
+int32_t hpcs_sort_radix_f32(hpcs_ctx *ctx, float *keys, int64_t n);
+/* Algorithm.SortRadixMsd(this double[]) RadixSortMsd.cs:1587-1595 (double order). */
+int32_t hpcs_sort_radix_f64(hpcs_ctx *ctx, double *keys, int64_t n);
+/* Algorithm.SortRadix<T>(T[] items, UInt64[] keys) RadixSortLsd.cs:1245-1299,
+ * full 64-bit key, uint32 items; both arrays are permuted in place. */
+int32_t hpcs_sort_radix_pairs_u64_u32(hpcs_ctx *ctx, uint64_t *keys, uint32_t *vals, int64_t n);
+/* Algorithm.SortRadix<T>(T[] items, UInt32[] keys) RadixSortLsd.cs:1183-1237. */
+int32_t hpcs_sort_radix_pairs_u32_u32(hpcs_ctx *ctx, uint32_t *keys, uint32_t *vals, int64_t n);
+/* Key-extractor record sorts: SortRadixLsd2<T>(Func<T,UInt64>) RadixSortLsd.cs:
+ * 821-846 (key_bits 64), SortRadixLsd2<T>(Func<T,UInt32>) :751-775 (key_bits 32),
+ * for unmanaged records of rec_size bytes whose key sits at key_offset. */
+int32_t hpcs_sort_radix_records(hpcs_ctx *ctx, void *recs, int64_t n, int32_t rec_size, int32_t key_offset,
+                                int32_t key_bits, int32_t key_kind);
+/* Algorithm.HistogramByteComponents(uint[], int l, int r) Histogram.cs:261-296;
+ * ParallelAlgorithm.HistogramByteComponentsPar HistogramParallel.cs:214-222.
+ * l..r inclusive; counts is int32[4][256], digit-major. */
+int32_t hpcs_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *a, int64_t l, int64_t r, int32_t *counts);
+/* Algorithm.HistogramByteComponents(ulong[], int l, int r) Histogram.cs:957-990;
+ * HistogramByteComponentsPar(ulong[]) HistogramParallel.cs:545-555.  counts is int32[8][256]. */
+int32_t hpcs_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *a, int64_t l, int64_t r, int32_t *counts);
+
+/* ---- device-resident, stream-ordered ------------------------------------ */
+
+/* The same sorts on device buffers (d_* are device pointers). */
+int32_t hpcs_dev_sort_radix_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream);
+int32_t hpcs_dev_sort_radix_u64(hpcs_ctx *ctx, uint64_t *d_keys, int64_t n, void *stream);
+int32_t hpcs_dev_sort_radix_f32(hpcs_ctx *ctx, float *d_keys, int64_t n, void *stream);
+int32_t hpcs_dev_sort_radix_f64(hpcs_ctx *ctx, double *d_keys, int64_t n, void *stream);
+int32_t hpcs_dev_sort_radix_pairs_u64_u32(hpcs_ctx *ctx, uint64_t *d_keys, uint32_t *d_vals, int64_t n, void *stream);
+int32_t hpcs_dev_sort_radix_pairs_u32_u32(hpcs_ctx *ctx, uint32_t *d_keys, uint32_t *d_vals, int64_t n, void *stream);
+int32_t hpcs_dev_sort_radix_records(hpcs_ctx *ctx, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset,
+                                    int32_t key_bits, int32_t key_kind, void *stream);
+/* Byte-component histogram of d_keys[0, n): d_counts is uint32[4][256]
+ * (u32) or uint32[8][256] (u64) on the device, overwritten. */
+int32_t hpcs_dev_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n, uint32_t *d_counts,
+                                               void *stream);
+int32_t hpcs_dev_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, uint32_t *d_counts,
+                                               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPCSORT_H */

@@ -1,0 +1,60 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every entry
+point include/hpcsort.h declares, and rejects bad arguments without a GPU."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from hpcsharp_amd import _native as N
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "hpcsort.h")
+
+
+def declared_functions() -> list[str]:
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hpcs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_binding_table():
+    assert declared_functions() == sorted(N.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(N.LIB_PATH):
+        pytest.skip("libhpcsort.so not built")
+    out = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [f for f in declared_functions() if f not in exported]
+    assert not missing, missing
+    lib = N.lib()
+    for f in declared_functions():
+        assert getattr(lib, f) is not None
+
+
+def test_header_compiles_as_c():
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-x", "c", HEADER],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_null_context_is_rejected_without_a_gpu():
+    if not os.path.exists(N.LIB_PATH):
+        pytest.skip("libhpcsort.so not built")
+    lib = N.lib()
+    a = np.arange(4, dtype=np.uint32)
+    rc = lib.hpcs_sort_radix_u32(None, a.ctypes.data_as(ctypes.c_void_p), 4)
+    assert rc == N.HPCS_E_INVALID_ARG
+    assert b"null" in lib.hpcs_last_error()
+    assert lib.hpcs_version() >= 100
+    info = N.PlanInfo()
+    assert lib.hpcs_plan_query(None, N.HPCS_DTYPE_U32, 1 << 28, 0, ctypes.byref(info)) == 0
+    assert info.passes == 4 and info.digit_bits == 8
+    assert info.ideal_bytes == (1 << 28) * 36     # one histogram read + read and write per pass
+    assert lib.hpcs_plan_query(None, 99, 10, 0, ctypes.byref(info)) == N.HPCS_E_INVALID_ARG

@@ -1,0 +1,252 @@
+"""GPU parity: libhpcsort.so against the oracle and the golden fixtures.
+
+Bit-exact throughout (integer work).  Host-array calls go through the
+reference-shaped surface (hpcsharp_amd.algorithm); device-resident calls go
+through the C ABI on torch buffers.  The full-size case (2^28 keys) is
+checked by size-independent properties: equality with an independent
+library sort of the same keys and byte histograms of input and output.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from hpcsharp_amd import datagen
+from hpcsharp_amd import _native as N
+from hpcsharp_amd.algorithm import (Algorithm, ArgumentNullException, ArgumentOutOfRangeException,
+                                    ParallelAlgorithm)
+from tests._fixtures import FIXTURES, float_order, hist_bytes, record_dtype, sha
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    N.context(0)
+    yield
+
+
+def test_reference_examples():
+    ex = FIXTURES["reference_examples"]
+    a = np.array(ex["program_cs_uint"]["input"], dtype=np.uint32)
+    assert Algorithm.SortRadix(a) is a
+    assert a.tolist() == ex["program_cs_uint"]["expected"]
+    b = np.array(ex["sorting_usage_long"]["input"], dtype=np.uint64)
+    assert ParallelAlgorithm.SortRadixPar(b) is b
+    assert b.tolist() == ex["sorting_usage_long"]["expected"]
+    u = ex["user_defined_class"]
+    rec = np.zeros(6, dtype=record_dtype(np.uint32))
+    rec["key"] = u["keys"]
+    rec["payload"] = u["index"]
+    Algorithm.SortRadixLsd2(rec, "key")
+    assert rec["key"].tolist() == u["expected_keys"]
+    assert rec["payload"].tolist() == u["expected_index"]
+
+
+@pytest.mark.parametrize("case", FIXTURES["u32"], ids=lambda c: f"{c['dist']}-{c['n']}")
+def test_u32_golden_and_oracle(oracle, case):
+    a = datagen.keys_u32(case["n"], case["seed"], case["dist"])
+    ref = a.copy()
+    oracle.sort_radix_par(ref)
+    Algorithm.SortRadixLsd(a)
+    assert sha(a) == case["sorted_sha"]
+    assert np.array_equal(a, ref)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "low8", "sorted", "constant"])
+def test_u32_config1_2pow20(oracle, dist):
+    n = 1 << 20
+    a = datagen.keys_u32(n, 0x5EED0001, dist)
+    ref = a.copy()
+    oracle.sort_radix_lsd(ref)
+    h_gpu = Algorithm.HistogramByteComponents(a, 0, n - 1)
+    h_ref = oracle.histogram_byte_components_u32(a, 0, n - 1)
+    assert np.array_equal(h_gpu, h_ref)
+    assert np.array_equal(ParallelAlgorithm.HistogramByteComponentsPar(a, 0, n - 1),
+                          oracle.histogram_byte_components_par(a, 0, n - 1))
+    ParallelAlgorithm.SortRadixPar(a)
+    assert np.array_equal(a, ref)
+
+
+@pytest.mark.parametrize("case", FIXTURES["u64"], ids=lambda c: str(c["n"]))
+def test_u64_golden(oracle, case):
+    a = datagen.keys_u64(case["n"], case["seed"])
+    ref = a.copy()
+    oracle.sort_radix_lsd(ref)
+    Algorithm.SortRadixLsd(a)
+    assert sha(a) == case["sorted_sha"]
+    assert np.array_equal(a, ref)
+
+
+@pytest.mark.parametrize("case", FIXTURES["f64"], ids=lambda c: str(c["n"]))
+def test_f64_golden(oracle, case):
+    x = datagen.doubles_special(case["n"], case["seed"])
+    ref = x.copy()
+    oracle.sort_radix_float(ref)
+    Algorithm.SortRadixMsd(x)
+    assert sha(x) == case["sorted_sha"]
+    assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("case", FIXTURES["f32"], ids=lambda c: str(c["n"]))
+def test_f32_golden(oracle, case):
+    x = datagen.floats_finite(case["n"], case["seed"])
+    ref = x.copy()
+    oracle.sort_radix_float(ref)
+    Algorithm.SortRadixMsd(x)
+    assert sha(x) == case["sorted_sha"]
+    assert np.array_equal(x.view(np.uint32), ref.view(np.uint32))
+
+
+def test_f64_specials():
+    sp = FIXTURES["f64_specials"]
+    x = np.array([int(h, 16) for h in sp["input_bits"]], dtype=np.uint64).view(np.float64)
+    Algorithm.SortRadixMsd(x)
+    assert [hex(v) for v in x.view(np.uint64)] == sp["expected_bits"]
+
+
+def test_f64_all_trivial_but_one_pass(oracle):
+    # only the sign differs: every pass but the top one is trivial, so the
+    # key transform must enter and leave in that one pass
+    x = np.array([1.0, -1.0, 1.0, -1.0, -1.0] * 1000, dtype=np.float64)
+    ref = x.copy()
+    oracle.sort_radix_float(ref)
+    Algorithm.SortRadixMsd(x)
+    assert np.array_equal(x.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("case", FIXTURES["pairs"], ids=lambda c: f"{c['n']}-{c['mask']}")
+def test_pairs_stable_golden(case):
+    k = datagen.keys_u64(case["n"], case["seed"]) & np.uint64(int(case["mask"], 16))
+    items = np.arange(case["n"], dtype=np.uint32)
+    out_items, out_keys = Algorithm.SortRadix(items, k)
+    assert out_items is items and out_keys is k
+    assert sha(k) == case["sorted_keys_sha"]
+    assert sha(items) == case["perm_sha"]
+
+
+def test_pairs_u32_stable(oracle):
+    n = 300001
+    k = datagen.keys_u32(n, 42) & np.uint32(0xFFF)
+    v = np.arange(n, dtype=np.uint32)
+    rk, rv = k.copy(), v.copy()
+    oracle.sort_radix_pairs(rk, rv)
+    Algorithm.SortRadix(v, k)
+    assert np.array_equal(k, rk) and np.array_equal(v, rv)
+
+
+@pytest.mark.parametrize("key_type,mask", [(np.uint64, 0xFFFF), (np.uint64, 0xFFFFFFFFFFFFFFFF), (np.uint32, 0xFF)])
+def test_records_stable(oracle, key_type, mask):
+    n = 200003
+    rec = np.zeros(n, dtype=record_dtype(key_type))
+    rec["key"] = (datagen.keys_u64(n, 7) & np.uint64(mask)).astype(key_type)
+    rec["payload"] = np.arange(n, dtype=np.uint32)
+    ref = rec.copy()
+    oracle.sort_radix_records(ref, 0, np.dtype(key_type).itemsize * 8)
+    assert Algorithm.SortRadixLsd2(rec, "key") is rec
+    assert rec.tobytes() == ref.tobytes()
+
+
+def test_records_float_key_unaligned(oracle):
+    n = 70001
+    dt = np.dtype({"names": ["a", "key", "payload"], "formats": [np.uint8, np.float64, np.uint32],
+                   "offsets": [0, 1, 9], "itemsize": 13})           # packed: key at an odd offset
+    rec = np.zeros(n, dtype=dt)
+    rec["key"] = datagen.doubles_special(n, 99)
+    rec["payload"] = np.arange(n, dtype=np.uint32)
+    ref = rec.copy()
+    oracle.sort_radix_records(ref, 1, 64, 1)
+    Algorithm.SortRadixLsd2(rec, "key")
+    assert rec.tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("case", FIXTURES["hist"], ids=lambda c: str(c["n"]))
+def test_histograms(case):
+    n = case["n"]
+    a = datagen.keys_u32(n, case["seed_u32"], "uniform")
+    assert sha(Algorithm.HistogramByteComponents(a, 0, n - 1)) == case["u32_sha"]
+    b = datagen.keys_u64(n, case["seed_u64"])
+    assert sha(Algorithm.HistogramByteComponents(b, 0, n - 1)) == case["u64_sha"]
+
+
+def test_histogram_subrange_and_types(oracle):
+    a = datagen.keys_u32(100000, 3)
+    h = Algorithm.HistogramByteComponents(a, 17, 99000)
+    assert h.dtype == np.int32
+    assert np.array_equal(h, oracle.histogram_byte_components_u32(a, 17, 99000))
+    hp = ParallelAlgorithm.HistogramByteComponentsPar(a, 17, 99000)
+    assert hp.dtype == np.uint32 and np.array_equal(hp, h.astype(np.uint32))
+
+
+def test_sub_range_sort(oracle):
+    a = datagen.keys_u32(200000, 11)
+    ref = a.copy()
+    oracle.sort_radix_lsd(ref, 12345, 150000)
+    Algorithm.SortRadixLsd(a, 12345, 150000)
+    assert np.array_equal(a, ref)
+
+
+def test_argument_errors():
+    with pytest.raises(ArgumentNullException):
+        Algorithm.SortRadixLsd(None)
+    with pytest.raises(ArgumentOutOfRangeException):
+        Algorithm.SortRadixLsd(np.zeros(10, dtype=np.uint32), 5, 10)
+    rc = N.lib().hpcs_sort_radix_records(N.context(), None, 10, 16, 12, 64, 0)
+    assert rc == N.HPCS_E_INVALID_ARG
+
+
+def _torch_u32(a: np.ndarray):
+    import torch
+    return torch.from_numpy(a.view(np.int32)).cuda()
+
+
+def test_device_api_u32_and_stream(oracle):
+    import torch
+    n = 3 * (1 << 20) + 12345
+    a = datagen.keys_u32(n, 1234)
+    ref = a.copy()
+    oracle.sort_radix_lsd(ref)
+    t = _torch_u32(a)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        N.check(N.lib().hpcs_dev_sort_radix_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n,
+                                                ctypes.c_void_p(s.cuda_stream)))
+    s.synchronize()
+    assert np.array_equal(t.cpu().numpy().view(np.uint32), ref)
+    counts = torch.empty(4 * 256, dtype=torch.int32, device="cuda")
+    N.check(N.lib().hpcs_dev_histogram_byte_components_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n,
+                                                           ctypes.c_void_p(counts.data_ptr()), None))
+    torch.cuda.synchronize()
+    N.check(N.lib().hpcs_ctx_trim(N.context()))
+    assert np.array_equal(counts.cpu().numpy().reshape(4, 256), hist_bytes(a))
+
+
+def test_full_size_2pow28_properties():
+    """2^28 uniform u32 (the headline size): the output equals an independent
+    library sort of the same keys (torch.sort on an order-preserving int64
+    copy), and input and output have the same byte histograms."""
+    import torch
+    n = 1 << 28
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x5EED0002)
+    t = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device="cuda", generator=g)
+    h_in = torch.empty(1024, dtype=torch.int32, device="cuda")
+    N.check(N.lib().hpcs_dev_histogram_byte_components_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n,
+                                                           ctypes.c_void_p(h_in.data_ptr()), None))
+    ref = torch.sort((t.to(torch.int64) & 0xFFFFFFFF))[0]
+    N.check(N.lib().hpcs_dev_sort_radix_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n, None))
+    torch.cuda.synchronize()
+    got = t.to(torch.int64) & 0xFFFFFFFF
+    assert torch.equal(got, ref)
+    del ref, got
+    h_out = torch.empty(1024, dtype=torch.int32, device="cuda")
+    N.check(N.lib().hpcs_dev_histogram_byte_components_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n,
+                                                           ctypes.c_void_p(h_out.data_ptr()), None))
+    torch.cuda.synchronize()
+    assert torch.equal(h_in, h_out)
+    assert int(h_in.view(4, 256).sum(dim=1).min()) == n

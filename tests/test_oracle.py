@@ -1,0 +1,223 @@
+"""Pin the CPU oracle (oracle/hpcs_oracle.c) before trusting it.
+
+Each oracle function is checked against something that does not restate it:
+the reference's own example vectors, golden fixtures computed with numpy
+closed forms (tests/golden/make_golden.py), brute-force counts, and
+invariants (permutation, sortedness, stability by index payload).  A planted
+mistake in a copy of the oracle's output must fail the same checks.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from hpcsharp_amd import datagen
+from tests._fixtures import FIXTURES, float_order, hist_bytes, record_dtype, sha
+
+
+def test_splitmix64_known_answers():
+    got = [hex(int(v)) for v in datagen.splitmix64(0, 3)]
+    assert got == [hex(int(x, 16)) for x in FIXTURES["splitmix64_seed0_first3"]]
+
+
+def test_reference_example_vectors(oracle):
+    ex = FIXTURES["reference_examples"]
+    a = np.array(ex["program_cs_uint"]["input"], dtype=np.uint32)
+    oracle.sort_radix_lsd(a)
+    assert a.tolist() == ex["program_cs_uint"]["expected"]
+    a = np.array(ex["program_cs_uint"]["input"], dtype=np.uint32)
+    oracle.sort_radix_par(a)
+    assert a.tolist() == ex["program_cs_uint"]["expected"]
+    b = np.array(ex["sorting_usage_long"]["input"], dtype=np.uint64)
+    oracle.sort_radix_lsd(b)
+    assert b.tolist() == ex["sorting_usage_long"]["expected"]
+    u = ex["user_defined_class"]
+    rec = np.zeros(6, dtype=record_dtype(np.uint32))
+    rec["key"] = u["keys"]
+    rec["payload"] = u["index"]
+    oracle.sort_radix_records(rec, 0, 32)
+    assert rec["key"].tolist() == u["expected_keys"]
+    assert rec["payload"].tolist() == u["expected_index"]
+
+
+@pytest.mark.parametrize("case", FIXTURES["u32"], ids=lambda c: f"{c['dist']}-{c['n']}")
+def test_lsd_u32_golden(oracle, case):
+    a = datagen.keys_u32(case["n"], case["seed"], case["dist"])
+    assert sha(a) == case["input_sha"]
+    oracle.sort_radix_lsd(a)
+    assert sha(a) == case["sorted_sha"]
+
+
+@pytest.mark.parametrize("case", [c for c in FIXTURES["u32"] if c["n"] >= 4096], ids=lambda c: f"{c['dist']}-{c['n']}")
+def test_par_u32_golden(oracle, case):
+    a = datagen.keys_u32(case["n"], case["seed"], case["dist"])
+    oracle.sort_radix_par(a, quantum=4096)   # several quanta even at small n
+    assert sha(a) == case["sorted_sha"]
+    b = datagen.keys_u32(case["n"], case["seed"], case["dist"])
+    oracle.sort_radix_par(b)
+    assert sha(b) == case["sorted_sha"]
+
+
+@pytest.mark.parametrize("case", FIXTURES["u64"], ids=lambda c: str(c["n"]))
+def test_lsd_and_par_u64_golden(oracle, case):
+    for f in (oracle.sort_radix_lsd, oracle.sort_radix_par):
+        a = datagen.keys_u64(case["n"], case["seed"])
+        assert sha(a) == case["input_sha"]
+        f(a)
+        assert sha(a) == case["sorted_sha"]
+
+
+@pytest.mark.parametrize("bits", [1, 3, 8, 11, 16])
+def test_nbit_sorts(oracle, bits):
+    a = datagen.keys_u32(30011, 17 + bits)
+    ref = np.sort(a)
+    oracle.sort_radix_lsd_nbit(a, bits)
+    assert np.array_equal(a, ref)
+    b = datagen.keys_u64(20011, 19 + bits)
+    refb = np.sort(b)
+    oracle.sort_radix_lsd_nbit(b, bits)
+    assert np.array_equal(b, refb)
+
+
+def test_sub_range_sort_touches_only_the_range(oracle):
+    a = datagen.keys_u32(10000, 5)
+    orig = a.copy()
+    oracle.sort_radix_lsd(a, 1234, 5000)
+    assert np.array_equal(a[:1234], orig[:1234])
+    assert np.array_equal(a[6234:], orig[6234:])
+    assert np.array_equal(a[1234:6234], np.sort(orig[1234:6234]))
+
+
+@pytest.mark.parametrize("case", FIXTURES["f64"], ids=lambda c: str(c["n"]))
+def test_f64_golden(oracle, case):
+    x = datagen.doubles_special(case["n"], case["seed"])
+    assert sha(x) == case["input_sha"]
+    oracle.sort_radix_float(x)
+    assert sha(x) == case["sorted_sha"]
+
+
+@pytest.mark.parametrize("case", FIXTURES["f32"], ids=lambda c: str(c["n"]))
+def test_f32_golden(oracle, case):
+    x = datagen.floats_finite(case["n"], case["seed"])
+    assert sha(x) == case["input_sha"]
+    oracle.sort_radix_float(x)
+    assert sha(x) == case["sorted_sha"]
+
+
+def test_f64_specials_order(oracle):
+    sp = FIXTURES["f64_specials"]
+    x = np.array([int(h, 16) for h in sp["input_bits"]], dtype=np.uint64).view(np.float64)
+    oracle.sort_radix_float(x)
+    assert [hex(v) for v in x.view(np.uint64)] == sp["expected_bits"]
+    # spelled out: -0.0 sits right before +0.0, subnormals between 0 and the normals
+    s = x.view(np.uint64)
+    i = list(s).index(np.uint64(0x8000000000000000))
+    assert s[i + 1] == 0 or s[i + 1] == np.uint64(0x8000000000000000)
+
+
+def test_float_key_transform_closed_form(oracle):
+    # the key of +x is x's bits with the sign flipped; of -x, all bits flipped
+    assert oracle.f32_key(0x3F800000) == 0xBF800000      # +1.0
+    assert oracle.f32_key(0xBF800000) == 0x407FFFFF      # -1.0
+    assert oracle.f32_key(0x80000000) == 0x7FFFFFFF      # -0.0
+    assert oracle.f32_key(0x00000000) == 0x80000000      # +0.0
+    assert oracle.f64_key(0x8000000000000000) == 0x7FFFFFFFFFFFFFFF
+
+
+@pytest.mark.parametrize("case", FIXTURES["pairs"], ids=lambda c: f"{c['n']}-{c['mask']}")
+def test_pairs_stability_golden(oracle, case):
+    k = datagen.keys_u64(case["n"], case["seed"]) & np.uint64(int(case["mask"], 16))
+    assert sha(k) == case["input_sha"]
+    v = np.arange(case["n"], dtype=np.uint32)
+    oracle.sort_radix_pairs(k, v)
+    assert sha(k) == case["sorted_keys_sha"]
+    assert sha(v) == case["perm_sha"]
+
+
+def test_pairs_u32_and_records_stable(oracle):
+    n = 50000
+    k = datagen.keys_u32(n, 77) & np.uint32(0x3FF)
+    perm = np.argsort(k, kind="stable")
+    v = np.arange(n, dtype=np.uint32)
+    kk = k.copy()
+    oracle.sort_radix_pairs(kk, v)
+    assert np.array_equal(v, perm.astype(np.uint32))
+    rec = np.zeros(n, dtype=record_dtype(np.uint64))
+    rec["key"] = datagen.keys_u64(n, 78) & np.uint64(0xFFFF)
+    rec["payload"] = np.arange(n, dtype=np.uint32)
+    want = np.argsort(rec["key"], kind="stable")
+    oracle.sort_radix_records(rec, 0, 64)
+    assert np.array_equal(rec["payload"], want.astype(np.uint32))
+
+
+def test_records_float_key(oracle):
+    n = 3000
+    dt = np.dtype({"names": ["pad", "key", "payload"], "formats": [np.uint32, np.float64, np.uint32],
+                   "offsets": [0, 8, 16], "itemsize": 24})
+    rec = np.zeros(n, dtype=dt)
+    rec["key"] = datagen.doubles_special(n, 3)
+    rec["payload"] = np.arange(n, dtype=np.uint32)
+    want = np.lexsort((rec["payload"], ~np.signbit(rec["key"]), rec["key"]))
+    oracle.sort_radix_records(rec, 8, 64, 1)
+    assert np.array_equal(rec["payload"], want.astype(np.uint32))
+
+
+@pytest.mark.parametrize("case", FIXTURES["hist"], ids=lambda c: str(c["n"]))
+def test_histograms_golden(oracle, case):
+    n = case["n"]
+    a = datagen.keys_u32(n, case["seed_u32"], "uniform")
+    h = oracle.histogram_byte_components_u32(a, 0, n - 1)
+    assert sha(h) == case["u32_sha"]
+    assert np.array_equal(oracle.histogram_byte_components_par(a, 0, n - 1).astype(np.int32), h)
+    b = datagen.keys_u64(n, case["seed_u64"])
+    h8 = oracle.histogram_byte_components_u64(b, 0, n - 1)
+    assert sha(h8) == case["u64_sha"]
+    assert np.array_equal(oracle.histogram_byte_components_par(b, 0, n - 1).astype(np.int32), h8)
+
+
+def test_histogram_closed_forms(oracle):
+    n = 1 << 16
+    c = np.full(n, 0xA1B2C3D4, dtype=np.uint32)
+    h = oracle.histogram_byte_components_u32(c, 0, n - 1)
+    for d, byte in enumerate([0xD4, 0xC3, 0xB2, 0xA1]):
+        assert h[d, byte] == n and h[d].sum() == n
+    s = np.arange(n, dtype=np.uint32)           # key = i: closed-form counts
+    h = oracle.histogram_byte_components_u32(s, 0, n - 1)
+    assert (h[0] == n // 256).all() and (h[1] == 256).all()
+    assert h[2, 0] == n and h[3, 0] == n
+    # inclusive l..r sub-range
+    a = datagen.keys_u32(5000, 9)
+    assert np.array_equal(oracle.histogram_byte_components_u32(a, 100, 4000), hist_bytes(a[100:4001]))
+    # nbit: 11-bit digits, top digit 10 bits
+    h11 = oracle.histogram_nbit_components(a, 0, a.size - 1, 11)
+    assert h11.shape == (3, 2048)
+    assert np.array_equal(h11[0], np.bincount(a & 0x7FF, minlength=2048))
+    assert np.array_equal(h11[2], np.bincount(a >> 22, minlength=2048))
+
+
+def test_segments(oracle):
+    lens = datagen.segment_lengths(300, 4, 16, 5000)
+    begin = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    a = datagen.keys_u32(int(begin[-1]) + 7, 5)
+    orig = a.copy()
+    oracle.sort_segments_u32(a, begin)
+    for s in range(len(lens)):
+        assert np.array_equal(a[begin[s]:begin[s + 1]], np.sort(orig[begin[s]:begin[s + 1]]))
+    assert np.array_equal(a[begin[-1]:], orig[begin[-1]:])
+
+
+def test_planted_mistakes_fail_the_checks(oracle):
+    """A mistake planted in the oracle's output must be caught by the checks above."""
+    case = next(c for c in FIXTURES["u32"] if c["n"] == 4097 and c["dist"] == "uniform")
+    a = datagen.keys_u32(case["n"], case["seed"], case["dist"])
+    oracle.sort_radix_lsd(a)
+    bad = a.copy()
+    bad[[10, 11]] = bad[[11, 10]]                    # transposed pair
+    assert sha(bad) != case["sorted_sha"]
+    pc = FIXTURES["pairs"][0]
+    k = datagen.keys_u64(pc["n"], pc["seed"]) & np.uint64(int(pc["mask"], 16))
+    v = np.arange(pc["n"], dtype=np.uint32)
+    oracle.sort_radix_pairs(k, v)
+    eq = np.flatnonzero(k[1:] == k[:-1])[0]          # swap two equal keys' payloads: unstable
+    v[[eq, eq + 1]] = v[[eq + 1, eq]]
+    assert sha(v) != pc["perm_sha"]
