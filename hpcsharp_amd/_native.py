@@ -24,6 +24,7 @@ HPCS_E_BUSY = -6
 HPCS_KEY_UINT = 0
 HPCS_KEY_FLOAT = 1
 HPCS_DTYPE_U32, HPCS_DTYPE_U64, HPCS_DTYPE_F32, HPCS_DTYPE_F64 = 0, 1, 2, 3
+HPCS_OPT_RANK, HPCS_OPT_TILE_SHAPE, HPCS_OPT_FALLBACK_ROUNDS, HPCS_OPT_LOOKBACK_WINDOW, HPCS_OPT_PASS_ALGO = 1, 2, 3, 4, 5
 
 # Every symbol include/hpcsort.h declares, with its argument types.
 _p, _i64, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
@@ -52,6 +53,7 @@ SIGNATURES = {
     "hpcs_ctx_destroy": ([_p], _i32),
     "hpcs_ctx_trim": ([_p], _i32),
     "hpcs_plan_query": ([_p, _i32, _i64, _i32, ctypes.POINTER(PlanInfo)], _i32),
+    "hpcs_ctx_set_option": ([_p, _i32, _i64], _i32),
     "hpcs_profile_enable": ([_p, _i32], _i32),
     "hpcs_profile_read": ([_p, ctypes.POINTER(KernelStat), _i32, ctypes.POINTER(_i32)], _i32),
     "hpcs_sort_radix_u32": ([_p, _p, _i64], _i32),
@@ -145,3 +147,7 @@ def profile_read(ctx=None) -> dict[str, tuple[int, float]]:
     cnt = ctypes.c_int32()
     check(lib().hpcs_profile_read(ctx if ctx is not None else context(), buf, 64, ctypes.byref(cnt)))
     return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(cnt.value)}
+
+
+def set_option(option: int, value: int, ctx=None) -> None:
+    check(lib().hpcs_ctx_set_option(ctx if ctx is not None else context(), option, value))

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -41,7 +42,16 @@ void hip_check(hipError_t e, const char *what)
     }
 }
 
-// Onesweep tile shape: 256 threads (four waves), 16 keys per thread.
+constexpr int PASS_ONESWEEP = 0;
+constexpr int PASS_RTS = 1;
+
+// Onesweep tile shapes (threads x keys per thread).  Shape 0 serves every key
+// type; the larger u32 shapes halve the tiles (and look-back work) per pass.
+struct TileShape {
+    int block, items;
+};
+constexpr TileShape TILE_SHAPES[] = {{256, 16}, {512, 16}, {256, 32}, {512, 32}};
+constexpr int NUM_TILE_SHAPES = 4;
 constexpr int SORT_BLOCK = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
@@ -61,6 +71,11 @@ struct hpcs_ctx {
     size_t scratch_cap = 0;
     void *io = nullptr;
     size_t io_cap = 0;
+    int rank_mode = RANK_LDSOR;  // tuning knob, env HPCS_RANK=ballot|ldsor
+    int tile_shape = 1;          // tuning knob for u32 keys, env HPCS_TILE_SHAPE=0..3
+    int fallback_rounds = FALLBACK_ROUNDS;
+    int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
+    int pass_algo = PASS_RTS;     // PASS_ONESWEEP or PASS_RTS (reduce-then-scan)
     // kernel timing
     struct Pending {
         const char *name;
@@ -113,10 +128,10 @@ struct Layout {
     int64_t tiles;
 };
 
-Layout make_layout(int64_t n, size_t key_size, bool vals)
+Layout make_layout(int64_t n, size_t key_size, bool vals, int tile_keys = SORT_TILE)
 {
     Layout L{};
-    L.tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    L.tiles = (n + tile_keys - 1) / tile_keys;
     const size_t lb_bytes = size_t(L.tiles) * RADIX * sizeof(uint32_t);
     L.off_hist = 0;
     L.off_plan = align_up(MAX_PASSES * RADIX * sizeof(uint32_t));
@@ -210,6 +225,85 @@ void launch_histogram(hpcs_ctx *c, const K *d_keys, uint64_t n, uint32_t *d_hist
     });
 }
 
+template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int W>
+void launch_onesweep_w(int rank_mode, int fb, dim3 grid, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb,
+                       uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
+{
+    if (rank_mode == RANK_BALLOT && BLOCK * ITEMS == SORT_TILE)
+        hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_BALLOT, W>), grid, dim3(BLOCK), 0, st,
+                           a, b, va, vb, n, pass, plan, lb, next, fb);
+    else
+        hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_LDSOR, W>), grid, dim3(BLOCK), 0, st,
+                           a, b, va, vb, n, pass, plan, lb, next, fb);
+}
+
+template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS>
+void launch_onesweep_shape(int rank_mode, int window, int fb, dim3 grid, hipStream_t st, K *a, K *b, uint32_t *va,
+                           uint32_t *vb, uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
+{
+    if (window == 0)
+        return launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 0>(rank_mode, fb, grid, st, a, b, va, vb, n,
+                                                                       pass, plan, lb, next);
+    if constexpr (sizeof(K) == 4 && !HAS_VALS) {
+        if (window <= 16)
+            return launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 16>(rank_mode, fb, grid, st, a, b, va, vb, n,
+                                                                            pass, plan, lb, next);
+        if (window >= 64)
+            return launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 64>(rank_mode, fb, grid, st, a, b, va, vb, n,
+                                                                            pass, plan, lb, next);
+    }
+    launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 32>(rank_mode, fb, grid, st, a, b, va, vb, n, pass, plan,
+                                                             lb, next);
+}
+
+template <typename K, bool HAS_VALS, bool FLOATK>
+void launch_onesweep(int rank_mode, int shape, int window, int fb, dim3 grid, hipStream_t st, K *a, K *b,
+                     uint32_t *va, uint32_t *vb, uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
+{
+    if constexpr (sizeof(K) == 4 && !HAS_VALS) {
+        if (shape == 1)
+            return launch_onesweep_shape<K, HAS_VALS, FLOATK, 512, 16>(rank_mode, window, fb, grid, st, a, b, va, vb,
+                                                                       n, pass, plan, lb, next);
+        if (shape == 2)
+            return launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 32>(rank_mode, window, fb, grid, st, a, b, va, vb,
+                                                                       n, pass, plan, lb, next);
+        if (shape == 3)
+            return launch_onesweep_shape<K, HAS_VALS, FLOATK, 512, 32>(rank_mode, window, fb, grid, st, a, b, va, vb,
+                                                                       n, pass, plan, lb, next);
+    }
+    launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 16>(rank_mode, window, fb, grid, st, a, b, va, vb, n, pass, plan,
+                                                        lb, next);
+}
+
+template <typename K, bool FLOATK>
+void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const K *b, uint32_t n, int pass,
+                        const DevPlan *plan, uint32_t *counts)
+{
+    const int block = TILE_SHAPES[shape].block, items = TILE_SHAPES[shape].items;
+    if (block == 256 && items == 16)
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts);
+    else if (block == 512 && items == 16)
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts);
+    else if (block == 256 && items == 32)
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts);
+    else
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts);
+}
+
+template <typename K, bool FLOATK>
+void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n, uint32_t *counts, K *orslots)
+{
+    const int block = TILE_SHAPES[shape].block, items = TILE_SHAPES[shape].items;
+    if (block == 256 && items == 16)
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, n, counts, orslots);
+    else if (block == 512 && items == 16)
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, n, counts, orslots);
+    else if (block == 256 && items == 32)
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, n, counts, orslots);
+    else
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, n, counts, orslots);
+}
+
 // The device-resident sort: keys (and optional uint32 values) in place.
 template <typename K, bool HAS_VALS, bool FLOATK>
 void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
@@ -221,7 +315,9 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
     if (reinterpret_cast<uintptr_t>(d_keys) % sizeof(K)) fail(HPCS_E_INVALID_ARG, "keys pointer is misaligned");
     constexpr int P = int(sizeof(K));
-    const Layout L = make_layout(n, sizeof(K), HAS_VALS);
+    const int shape = (sizeof(K) == 4 && !HAS_VALS) ? c->tile_shape : 0;
+    const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
+    const Layout L = make_layout(n, sizeof(K), HAS_VALS, tile_keys);
     char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
     uint32_t *hist = reinterpret_cast<uint32_t *>(base + L.off_hist);
     DevPlan *plan = reinterpret_cast<DevPlan *>(base + L.off_plan);
@@ -229,17 +325,53 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     K *keysB = reinterpret_cast<K *>(base + L.off_keysB);
     uint32_t *valsB = HAS_VALS ? reinterpret_cast<uint32_t *>(base + L.off_valsB) : nullptr;
 
-    launch_fill(c, base, L.zero_bytes, st);
-    launch_histogram<K, FLOATK>(c, d_keys, uint64_t(n), hist, st);
-    timed(c, st, "plan", [&] {
-        hipLaunchKernelGGL((k_plan<P>), dim3(1), dim3(RADIX), 0, st, hist, plan, uint64_t(n), FLOATK ? 1 : 0);
-    });
+    const uint32_t tiles = uint32_t(L.tiles);
+    const uint32_t nranges = (tiles + SCAN_RANGE - 1) / SCAN_RANGE;
+    if (c->pass_algo == PASS_RTS) {
+        // the histogram area holds the OR slots; counts in look-back region 0
+        launch_fill(c, base, L.off_lb0, st);
+        K *orslots = reinterpret_cast<K *>(hist);
+        timed(c, st, "upsweep", [&] {
+            launch_upsweep<K, FLOATK>(shape, dim3(tiles), st, d_keys, uint32_t(n), lb[0], orslots);
+        });
+        timed(c, st, "plan", [&] {
+            hipLaunchKernelGGL((k_plan_rts<K>), dim3(1), dim3(64), 0, st, orslots, plan, uint64_t(n),
+                               FLOATK ? 1 : 0);
+        });
+    } else {
+        launch_fill(c, base, L.zero_bytes, st);
+        launch_histogram<K, FLOATK>(c, d_keys, uint64_t(n), hist, st);
+        timed(c, st, "plan", [&] {
+            hipLaunchKernelGGL((k_plan<P>), dim3(1), dim3(RADIX), 0, st, hist, plan, uint64_t(n), FLOATK ? 1 : 0);
+        });
+    }
     for (int p = 0; p < P; ++p) {
+        if (c->pass_algo == PASS_RTS) {
+            // counts in look-back region 0, range sums in region 1
+            uint32_t *counts = lb[0], *totals = lb[1];
+            if (p > 0)
+                timed(c, st, "tile_counts", [&] {
+                    launch_tile_counts<K, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan,
+                                                  counts);
+                });
+            timed(c, st, "scan", [&] {
+                hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals, p, plan);
+                hipLaunchKernelGGL(k_scan_ranges, dim3(1), dim3(4 * RADIX), 0, st, totals, nranges, p, plan);
+                hipLaunchKernelGGL(k_range_offsets, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals, p,
+                                   plan);
+            });
+            timed(c, st, "scatter", [&] {
+                launch_onesweep<K, HAS_VALS, FLOATK>(c->rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
+                                                     d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
+                                                     nullptr);
+            });
+            continue;
+        }
         uint32_t *next = (p + 1 < P) ? lb[(p + 1) & 1] : nullptr;
         timed(c, st, "onesweep", [&] {
-            hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, SORT_BLOCK, SORT_ITEMS>), dim3(unsigned(L.tiles)),
-                               dim3(SORT_BLOCK), 0, st, d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan,
-                               lb[p & 1], next);
+            launch_onesweep<K, HAS_VALS, FLOATK>(c->rank_mode, shape, c->lookback_window, c->fallback_rounds,
+                                                 dim3(unsigned(L.tiles)), st, d_keys, keysB, d_vals, valsB,
+                                                 uint32_t(n), p, plan, lb[p & 1], next);
         });
     }
     timed(c, st, "copyback", [&] {
@@ -312,7 +444,8 @@ void dev_histogram(hpcs_ctx *c, const K *d_keys, int64_t n, uint32_t *d_counts, 
     if (n > 0) launch_histogram<K, FLOATK>(c, d_keys, uint64_t(n), d_counts, st);
 }
 
-hipStream_t pick_stream(hpcs_ctx *c, void *stream) { return stream ? static_cast<hipStream_t>(stream) : c->stream; }
+// Device-resident calls run on the caller's stream; NULL is HIP's null stream.
+hipStream_t pick_stream(hpcs_ctx *, void *stream) { return static_cast<hipStream_t>(stream); }
 
 // Host-array call: copy in, sort on the device, copy out, wait.
 template <typename K, bool HAS_VALS, bool FLOATK>
@@ -412,6 +545,11 @@ HPCS_API int32_t hpcs_ctx_create(int32_t device, hpcs_ctx **out)
         hipDeviceProp_t prop;
         hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        if (const char *r = std::getenv("HPCS_RANK")) c->rank_mode = std::strcmp(r, "ballot") == 0 ? RANK_BALLOT : RANK_LDSOR;
+        if (const char *t = std::getenv("HPCS_TILE_SHAPE")) {
+            const int v = std::atoi(t);
+            c->tile_shape = (v >= 0 && v < NUM_TILE_SHAPES) ? v : 0;
+        }
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         *out = c;
         return HPCS_OK;
@@ -453,6 +591,35 @@ HPCS_API int32_t hpcs_ctx_destroy(hpcs_ctx *ctx)
     }
     delete ctx;
     return rc;
+}
+
+HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t value)
+{
+    return guarded(ctx, [&] {
+        switch (option) {
+        case HPCS_OPT_RANK:
+            if (value != RANK_BALLOT && value != RANK_LDSOR) fail(HPCS_E_INVALID_ARG, "rank mode must be 0 or 1");
+            ctx->rank_mode = int(value);
+            break;
+        case HPCS_OPT_TILE_SHAPE:
+            if (value < 0 || value >= NUM_TILE_SHAPES) fail(HPCS_E_INVALID_ARG, "unknown tile shape");
+            ctx->tile_shape = int(value);
+            break;
+        case HPCS_OPT_LOOKBACK_WINDOW:
+            if (value != 16 && value != 32 && value != 64) fail(HPCS_E_INVALID_ARG, "window must be 16, 32 or 64");
+            ctx->lookback_window = int(value);
+            break;
+        case HPCS_OPT_PASS_ALGO:
+            if (value != PASS_ONESWEEP && value != PASS_RTS) fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0 or 1");
+            ctx->pass_algo = int(value);
+            break;
+        case HPCS_OPT_FALLBACK_ROUNDS:
+            if (value < 0 || value > (1 << 20)) fail(HPCS_E_INVALID_ARG, "fallback rounds out of range");
+            ctx->fallback_rounds = int(value);
+            break;
+        default: fail(HPCS_E_INVALID_ARG, "unknown option");
+        }
+    });
 }
 
 HPCS_API int32_t hpcs_profile_enable(hpcs_ctx *ctx, int32_t on)
@@ -502,12 +669,14 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
         return HPCS_E_INVALID_ARG;
     }
     const int passes = int(ks * 8 / RADIX_BITS);
-    const Layout L = make_layout(n, ks, with_values != 0);
+    const int shape = (ctx && ks == 4 && !with_values) ? ctx->tile_shape : 0;
+    const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
+    const Layout L = make_layout(n, ks, with_values != 0, tile_keys);
     const int64_t per_key = int64_t(ks) + passes * 2 * (int64_t(ks) + (with_values ? 4 : 0));
     info->digit_bits = RADIX_BITS;
     info->passes = passes;
-    info->tile_keys = SORT_TILE;
-    info->block_threads = SORT_BLOCK;
+    info->tile_keys = tile_keys;
+    info->block_threads = TILE_SHAPES[shape].block;
     info->tiles = L.tiles;
     info->ideal_bytes = n * per_key;
     info->scratch_bytes = int64_t(L.total);

@@ -51,9 +51,8 @@ struct DevPlan {
     uint32_t src[MAX_PASSES];           // 0: data in buffer A, 1: in buffer B
     uint32_t xin[MAX_PASSES];           // apply the float key transform on load
     uint32_t xout[MAX_PASSES];          // apply the inverse transform on store
-    uint32_t tile_counter[MAX_PASSES];  // dynamic tile ids per pass
     uint32_t copyback;                  // final data sits in buffer B
-    uint32_t pad[63];
+    uint32_t pad[55];
 };
 
 // ---------------------------------------------------------------------------
@@ -96,6 +95,17 @@ __device__ inline uint64_t match_digit8(uint32_t d)
         peers &= bit ? m : ~m;
     }
     return peers;
+}
+
+// Workgroup b runs on XCD b % 8; map it to tile start(b % 8) + b / 8 so that
+// XCD x processes the contiguous tiles [start(x), start(x + 1)).  A bijection
+// on [0, g) for any g.
+constexpr uint32_t NUM_XCD = 8;
+__device__ inline uint32_t xcd_contiguous(uint32_t b, uint32_t g)
+{
+    const uint32_t x = b % NUM_XCD, i = b / NUM_XCD;
+    const uint32_t per = g / NUM_XCD, rem = g % NUM_XCD;
+    return x * per + min(x, rem) + i;
 }
 
 // Number of set bits of `mask` below this lane.
@@ -235,37 +245,64 @@ __global__ __launch_bounds__(RADIX) void k_plan(const uint32_t *__restrict__ his
 //
 // Tile = BLOCK threads x ITEMS keys.  Wave w of the tile owns the contiguous
 // span [w*64*ITEMS, (w+1)*64*ITEMS) and loads it lane-striped (item i, lane l
-// -> key i*64 + l), so every load instruction reads 64 consecutive keys.
+// -> key i*64 + l), so every load instruction reads 64 consecutive keys and
+// (wave, item, lane) order is input order.
 //
-// 1. rank: items in order; eight ballots give the lanes holding the same
-//    digit; the wave's LDS counter for that digit gives how many earlier
-//    items of the wave had it; the group's first lane advances the counter.
+// 1. count: each wave counts its digits with LDS atomics (order-free).
 // 2. thread b (one per digit) turns the per-wave counts into per-wave bases,
-//    publishes the tile's count for the look-back and scans the tile counts
-//    over digits.
-// 3. keys (and values) go to LDS at their in-tile sorted position.
-// 4. decoupled look-back: thread b walks back over earlier tiles' words for
-//    digit b, summing own-counts until it meets an inclusive count, then
-//    publishes its own inclusive count.
+//    publishes the tile's own count for the look-back at once (so later
+//    tiles rarely wait on it), and scans the tile counts over digits.
+// 3. rank + stage: items in order; the lanes holding the same digit are found
+//    (RANK), the group's first lane advances the wave's slot counter for that
+//    digit, and each key goes to LDS at its in-tile sorted position.
+// 4. decoupled look-back: thread b reads LOOKBACK_WINDOW earlier tiles' words
+//    for digit b in one round trip, adds own-counts from the newest back until
+//    it meets an inclusive count, and repeats only if the window ran out or
+//    met an unpublished tile; then it publishes its own inclusive count.
+//    The tile id is blockIdx.x (no global counter: a single-address atomic
+//    per tile serialises at the memory side).  Nothing assumes that earlier
+//    tiles are already running: when the block has waited FALLBACK_ROUNDS
+//    rounds on an unpublished tile, it reads that tile's keys and counts its
+//    digits itself (decoupled fallback), so every wait is bounded.
 // 5. the tile is written from LDS in order; each digit's run lands at
 //    consecutive addresses.
+//
+// RANK selects how a wave finds the lanes that share a digit:
+//   RANK_BALLOT  eight ballots (VALU work, no LDS traffic);
+//   RANK_LDSOR   each lane ORs its lane bit into a per-wave LDS mask of its
+//                digit and reads the mask back (LDS work, few VALU ops); OR is
+//                commutative, so the mask does not depend on the order in
+//                which the LDS unit applies the lanes' updates.
 // ---------------------------------------------------------------------------
-template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS>
+constexpr int RANK_BALLOT = 0;
+constexpr int RANK_LDSOR = 1;
+// Default look-back rounds without progress after which a block counts the
+// digits of the unpublished earlier tile itself (decoupled fallback); the
+// kernel takes the value as an argument so tests can force the fallback.
+constexpr int FALLBACK_ROUNDS = 24;
+
+template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int RANK, int LOOKBACK_WINDOW>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__restrict__ keysB,
                                                     uint32_t *__restrict__ valsA, uint32_t *__restrict__ valsB,
                                                     uint32_t n, int pass, DevPlan *__restrict__ plan,
-                                                    uint32_t *__restrict__ lb_cur, uint32_t *__restrict__ lb_next)
+                                                    uint32_t *__restrict__ lb_cur, uint32_t *__restrict__ lb_next,
+                                                    int fb_rounds)
 {
     static_assert(BLOCK >= RADIX && BLOCK % WAVE == 0, "one thread per digit is required");
     constexpr int WAVES = BLOCK / WAVE;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int WAVE_TILE = WAVE * ITEMS;
     constexpr int SCAN_WAVES = RADIX / WAVE;
+    // LOOKBACK_WINDOW == 0: the tile's global digit offsets were computed
+    // beforehand (k_tile_counts + scans) and lb_cur holds them, [tile][digit].
+    constexpr bool PRESCANNED = LOOKBACK_WINDOW == 0;
 
     __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint64_t s_match[RANK == RANK_LDSOR ? WAVES : 1][RANK == RANK_LDSOR ? RADIX : 1];
     __shared__ uint32_t s_base[RADIX];
     __shared__ uint32_t s_scan[SCAN_WAVES];
-    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_fb[RADIX];
+    __shared__ uint32_t s_fb_tile;
     __shared__ K s_keys[TILE];
     __shared__ uint32_t s_vals[HAS_VALS ? TILE : 1];
 
@@ -288,10 +325,16 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     const bool xout = FLOATK && plan->xout[pass];
     const int shift = pass * RADIX_BITS;
 
-    if (tid == 0) s_tile = atomicAdd(&plan->tile_counter[pass], 1u);
     for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
+    if constexpr (RANK == RANK_LDSOR)
+        for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_match[0][0])[i] = 0ull;
     __syncthreads();
-    const uint32_t tile = s_tile;
+    // With precomputed offsets the tile order is free: give each XCD (work-
+    // groups are dealt to the 8 XCDs round-robin) a contiguous range of tiles,
+    // so the partial cache lines at the ends of neighbouring tiles' digit runs
+    // meet in one L2 and leave it as full lines.  The look-back form keeps
+    // tile = blockIdx.x: its chain wants neighbouring tiles to start together.
+    const uint32_t tile = PRESCANNED ? xcd_contiguous(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t tile_base = tile * uint32_t(TILE);
     const uint32_t wave_base = tile_base + uint32_t(wave * WAVE_TILE);
     // keys of this wave that exist: items with i*64 + lane < wave_valid
@@ -310,20 +353,11 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
         }
     }
 
-    // ---- 1. rank within the wave ----
-    uint32_t rank[ITEMS];
+    // ---- 1. per-wave digit counts ----
     uint32_t *cnt = s_cnt[wave];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const uint32_t d = KeyOps<K>::digit(keys[i], shift);
-        const bool valid = i * WAVE + lane < wave_valid;
-        const uint64_t peers = match_digit8(d) & __ballot(valid);
-        const uint32_t below = count_below(peers);
-        const uint32_t old = cnt[d];
-        if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
-        rank[i] = old + below;
-        __builtin_amdgcn_wave_barrier();
-    }
+    for (int i = 0; i < ITEMS; ++i)
+        if (i * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[i], shift)], 1u);
     __syncthreads();
 
     // ---- 2. per-digit wave bases, tile count, publish, scan ----
@@ -335,7 +369,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
             s_cnt[w][tid] = tile_cnt;
             tile_cnt += c;
         }
-        lb_store(&lb_cur[size_t(tile) * RADIX + tid], (tile == 0 ? LB_INC : LB_AGG) | tile_cnt);
+        if constexpr (!PRESCANNED)
+            lb_store(&lb_cur[size_t(tile) * RADIX + tid], (tile == 0 ? LB_INC : LB_AGG) | tile_cnt);
         // inclusive scan over the 64 digits of this wave
         uint32_t x = tile_cnt;
 #pragma unroll
@@ -354,42 +389,132 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     }
     __syncthreads();
 
-    // ---- 3. stage the tile in LDS in sorted order ----
+    // ---- 3. rank and stage the tile in LDS in sorted order ----
+    uint32_t pos[HAS_VALS ? ITEMS : 1];
+    if constexpr (RANK == RANK_BALLOT) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        if (i * WAVE + lane < wave_valid) {
+        for (int i = 0; i < ITEMS; ++i) {
             const uint32_t d = KeyOps<K>::digit(keys[i], shift);
-            rank[i] += cnt[d];
-            s_keys[rank[i]] = keys[i];
+            const bool valid = i * WAVE + lane < wave_valid;
+            const uint64_t peers = match_digit8(d) & __ballot(valid);
+            const uint32_t below = count_below(peers);
+            const uint32_t old = cnt[d];
+            if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
+            if (valid) s_keys[old + below] = keys[i];
+            if constexpr (HAS_VALS) pos[i] = old + below;
+            __builtin_amdgcn_wave_barrier();
+        }
+    } else {
+        uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
+        const uint32_t lanebit = 1u << (lane & 31);
+        const int half = lane >> 5;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint32_t d = KeyOps<K>::digit(keys[i], shift);
+            const bool valid = i * WAVE + lane < wave_valid;
+            if (valid) atomicOr(&match32[2 * d + half], lanebit);
+            __builtin_amdgcn_wave_barrier();
+            const uint64_t peers = s_match[wave][d];
+            const uint32_t below = count_below(peers);
+            const uint32_t old = cnt[d];
+            if (valid && below == 0) {
+                cnt[d] = old + uint32_t(__popcll(peers));
+                s_match[wave][d] = 0ull;
+            }
+            if (valid) s_keys[old + below] = keys[i];
+            if constexpr (HAS_VALS) pos[i] = old + below;
+            __builtin_amdgcn_wave_barrier();
         }
     }
     if constexpr (HAS_VALS) {
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
             const int off = i * WAVE + lane;
-            if (off < wave_valid) s_vals[rank[i]] = vin[wave_base + off];
+            if (off < wave_valid) s_vals[pos[i]] = vin[wave_base + off];
         }
     }
 
-    // ---- 4. decoupled look-back ----
-    if (tid < RADIX) {
+    // ---- 4. decoupled look-back over a window of earlier tiles ----
+    // A round reads LOOKBACK_WINDOW words at once; after meeting an
+    // unpublished word the thread polls only that word (with a pause) until it
+    // is published, so waiting costs one load per round, not a window.
+    if constexpr (PRESCANNED) {
+        if (tid < RADIX) s_base[tid] = lb_cur[size_t(tile) * RADIX + tid] - tile_start;
+    } else {
         uint32_t excl = 0;
-        if (tile > 0) {
-            int64_t p = int64_t(tile) - 1;
-            while (true) {
-                const uint32_t v = lb_load(&lb_cur[size_t(p) * RADIX + tid]);
-                const uint32_t st = v & ~LB_VALUE;
-                if (st == 0u) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+        int64_t p = int64_t(tile) - 1; // newest earlier tile not yet added
+        bool done = tid >= RADIX || tile == 0;
+        bool polling = false;
+        int idle = 0;                  // rounds in a row without progress
+        while (true) {
+            if (!done) {
+                if (polling) {
+                    const uint32_t w = lb_load(&lb_cur[size_t(p) * RADIX + tid]);
+                    const uint32_t st = w & ~LB_VALUE;
+                    if (st != 0u) {
+                        excl += w & LB_VALUE;
+                        done = st == LB_INC;
+                        p -= 1;
+                        polling = false;
+                        idle = 0;
+                    } else {
+                        ++idle;
+                    }
+                } else {
+                    const int avail = p + 1 < LOOKBACK_WINDOW ? int(p + 1) : LOOKBACK_WINDOW;
+                    uint32_t v[LOOKBACK_WINDOW > 0 ? LOOKBACK_WINDOW : 1];
+#pragma unroll
+                    for (int k = 0; k < LOOKBACK_WINDOW; ++k)
+                        v[k] = k < avail ? lb_load(&lb_cur[size_t(p - k) * RADIX + tid]) : 0u;
+                    int used = 0;
+#pragma unroll
+                    for (int k = 0; k < LOOKBACK_WINDOW; ++k) {
+                        if (!done && used == k && k < avail) {
+                            const uint32_t st = v[k] & ~LB_VALUE;
+                            if (st != 0u) {
+                                excl += v[k] & LB_VALUE;
+                                used = k + 1;
+                                done = st == LB_INC;
+                            }
+                        }
+                    }
+                    p -= used;
+                    polling = !done && used < avail; // stopped at an unpublished word
+                    idle = used ? 0 : idle + 1;
                 }
-                excl += v & LB_VALUE;
-                if (st == LB_INC) break;
-                --p;
             }
-            lb_store(&lb_cur[size_t(tile) * RADIX + tid], LB_INC | (excl + tile_cnt));
+            if (!__syncthreads_or(!done)) break;
+            if (__syncthreads_or(!done && idle >= fb_rounds)) {
+                // count the newest unpublished tile's digits here
+                if (tid == 0) s_fb_tile = 0u;
+                for (int i = tid; i < RADIX; i += BLOCK) s_fb[i] = 0u;
+                __syncthreads();
+                if (!done && idle >= fb_rounds) atomicMax(&s_fb_tile, uint32_t(p));
+                __syncthreads();
+                const uint32_t q = s_fb_tile;
+                const uint32_t qbase = q * uint32_t(TILE);
+                const uint32_t qn = min(uint32_t(TILE), n - qbase);
+                for (uint32_t j = tid; j < qn; j += BLOCK) {
+                    const K k = kin[qbase + j];
+                    atomicAdd(&s_fb[KeyOps<K>::digit(xin ? KeyOps<K>::to_key(k) : k, shift)], 1u);
+                }
+                __syncthreads();
+                if (!done && uint32_t(p) == q) {
+                    excl += s_fb[tid];
+                    p -= 1;
+                    idle = 0;
+                    polling = false;
+                    done = p < 0; // counted every earlier tile
+                }
+                __syncthreads();
+            } else if (!done && polling) {
+                __builtin_amdgcn_s_sleep(8);
+            }
         }
-        s_base[tid] = plan->gstart[pass][tid] + excl - tile_start;
+        if (tid < RADIX) {
+            if (tile > 0) lb_store(&lb_cur[size_t(tile) * RADIX + tid], LB_INC | (excl + tile_cnt));
+            s_base[tid] = plan->gstart[pass][tid] + excl - tile_start;
+        }
     }
     __syncthreads();
 
@@ -405,6 +530,236 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
             if constexpr (HAS_VALS) vout[dst] = s_vals[j];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Reduce-then-scan form of a pass (no waits between workgroups):
+//   k_tile_counts    per-tile digit counts, [tile][digit]
+//   k_range_totals   per range of SCAN_RANGE tiles, the sum of its counts
+//   k_scan_ranges    exclusive scan of the range sums over ranges, seeded with
+//                    the digit's global start
+//   k_range_offsets  per tile, the global offset of each digit's run
+// followed by k_onesweep<..., LOOKBACK_WINDOW = 0> reading those offsets.
+// It reads the keys twice per pass (12 B per u32 key instead of 8) and has
+// no chain of dependent loads between tiles.
+// ---------------------------------------------------------------------------
+constexpr int SCAN_RANGE = 64;
+
+template <typename K, bool FLOATK, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ keysA, const K *__restrict__ keysB,
+                                                       uint32_t n, int pass, const DevPlan *__restrict__ plan,
+                                                       uint32_t *__restrict__ counts)
+{
+    constexpr int TILE = BLOCK * ITEMS;
+    __shared__ uint32_t sh[RADIX];
+    if (plan->trivial[pass]) return;
+    const K *__restrict__ kin = plan->src[pass] ? keysB : keysA;
+    const bool xin = FLOATK && plan->xin[pass];
+    const int shift = pass * RADIX_BITS;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * uint32_t(TILE);
+    const uint32_t cnt = min(uint32_t(TILE), n - base);
+    K k[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t j = uint32_t(i * BLOCK + tid);
+        k[i] = j < cnt ? kin[base + j] : K(0);
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t j = uint32_t(i * BLOCK + tid);
+        const bool valid = j < cnt;
+        const uint32_t d = KeyOps<K>::digit(xin ? KeyOps<K>::to_key(k[i]) : k[i], shift);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        if (__all(valid && d == d0)) {
+            if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
+        } else if (valid) {
+            atomicAdd(&sh[d], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
+}
+
+__global__ __launch_bounds__(RADIX) void k_range_totals(const uint32_t *__restrict__ counts, uint32_t tiles,
+                                                        uint32_t *__restrict__ totals, int pass,
+                                                        const DevPlan *__restrict__ plan)
+{
+    if (plan->trivial[pass]) return;
+    const uint32_t r = blockIdx.x, d = threadIdx.x;
+    const uint32_t t0 = r * SCAN_RANGE, t1 = min(tiles, t0 + SCAN_RANGE);
+    uint32_t sum = 0;
+#pragma unroll 16
+    for (uint32_t t = t0; t < t1; ++t) sum += counts[size_t(t) * RADIX + d];
+    totals[size_t(r) * RADIX + d] = sum;
+}
+
+// One block of 4 x RADIX threads: four threads per digit split the ranges.
+// The range sums add up to the pass's digit histogram; its exclusive scan
+// over digits is the digit's global start (ComputeStartOfBins,
+// RadixSortLsd.cs:112-125), recorded in the plan and used as the seed.
+__global__ __launch_bounds__(4 * RADIX) void k_scan_ranges(uint32_t *__restrict__ totals, uint32_t nranges, int pass,
+                                                           DevPlan *__restrict__ plan)
+{
+    __shared__ uint32_t s_part[4][RADIX];
+    __shared__ uint32_t s_start[RADIX];
+    __shared__ uint32_t s_wsum[RADIX / WAVE];
+    if (plan->trivial[pass]) return;
+    const uint32_t d = threadIdx.x & (RADIX - 1), q = threadIdx.x / RADIX;
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    const uint32_t per = (nranges + 3) / 4;
+    const uint32_t r0 = min(nranges, q * per), r1 = min(nranges, r0 + per);
+    uint32_t sum = 0;
+#pragma unroll 8
+    for (uint32_t r = r0; r < r1; ++r) sum += totals[size_t(r) * RADIX + d];
+    s_part[q][d] = sum;
+    __syncthreads();
+    if (q == 0) {
+        const uint32_t tot = s_part[0][d] + s_part[1][d] + s_part[2][d] + s_part[3][d];
+        uint32_t x = tot;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= uint32_t(off)) x += y;
+        }
+        if (lane == WAVE - 1) s_wsum[wave] = x;
+        s_start[d] = x - tot;
+    }
+    __syncthreads();
+    if (q == 0) {
+        uint32_t add = 0;
+        for (uint32_t w = 0; w < wave; ++w) add += s_wsum[w];
+        s_start[d] += add;
+        plan->gstart[pass][d] = s_start[d];
+    }
+    __syncthreads();
+    uint32_t run = s_start[d];
+    for (uint32_t k = 0; k < q; ++k) run += s_part[k][d];
+    for (uint32_t r = r0; r < r1; ++r) {
+        const uint32_t c = totals[size_t(r) * RADIX + d];
+        totals[size_t(r) * RADIX + d] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(RADIX) void k_range_offsets(uint32_t *__restrict__ counts, uint32_t tiles,
+                                                         const uint32_t *__restrict__ totals, int pass,
+                                                         const DevPlan *__restrict__ plan)
+{
+    if (plan->trivial[pass]) return;
+    const uint32_t r = blockIdx.x, d = threadIdx.x;
+    const uint32_t t0 = r * SCAN_RANGE, t1 = min(tiles, t0 + SCAN_RANGE);
+    uint32_t run = totals[size_t(r) * RADIX + d];
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t c = counts[size_t(t) * RADIX + d];
+        counts[size_t(t) * RADIX + d] = run;
+        run += c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_upsweep: the first read of a reduce-then-scan sort.  Per tile it writes
+// the digit-0 counts (pass 0's reduce step, [tile][digit]) and folds the keys
+// into OR(keys) and OR(~keys): a bit varies across the keys iff it is set in
+// both, so digit position p is trivial (one value for every key: a stable
+// identity pass) iff no bit of its byte varies.  The two ORs go to one of
+// ORSLOTS word pairs (by block) to keep atomics on one address few.
+// ---------------------------------------------------------------------------
+constexpr int ORSLOTS = 64;
+
+template <typename K, bool FLOATK, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, uint32_t n,
+                                                   uint32_t *__restrict__ counts, K *__restrict__ orslots)
+{
+    constexpr int TILE = BLOCK * ITEMS;
+    constexpr int WAVES = BLOCK / WAVE;
+    __shared__ uint32_t sh[RADIX];
+    __shared__ K s_or[WAVES], s_nor[WAVES];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * uint32_t(TILE);
+    const uint32_t cnt = min(uint32_t(TILE), n - base);
+    K k[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t j = uint32_t(i * BLOCK + tid);
+        k[i] = j < cnt ? keys[base + j] : K(0);
+    }
+    K vor = 0, vnor = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t j = uint32_t(i * BLOCK + tid);
+        const bool valid = j < cnt;
+        const K key = FLOATK ? KeyOps<K>::to_key(k[i]) : k[i];
+        if (valid) {
+            vor |= key;
+            vnor |= ~key;
+        }
+        const uint32_t d = KeyOps<K>::digit(key, 0);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        if (__all(valid && d == d0)) {
+            if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
+        } else if (valid) {
+            atomicAdd(&sh[d], 1u);
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        vor |= __shfl_xor(vor, off);
+        vnor |= __shfl_xor(vnor, off);
+    }
+    if (lane == 0) {
+        s_or[wave] = vor;
+        s_nor[wave] = vnor;
+    }
+    __syncthreads();
+    for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
+    if (tid == 0) {
+        K a = 0, b = 0;
+        for (int w = 0; w < WAVES; ++w) {
+            a |= s_or[w];
+            b |= s_nor[w];
+        }
+        const int slot = blockIdx.x % ORSLOTS;
+        atomicOr(&orslots[2 * slot], a);
+        atomicOr(&orslots[2 * slot + 1], b);
+    }
+}
+
+// k_plan_rts: trivial passes from the varying bits; buffer selection and the
+// float transform's first/last pass.  Digit starts come later, per pass, from
+// that pass's counts (k_scan_ranges).
+template <typename K>
+__global__ void k_plan_rts(const K *__restrict__ orslots, DevPlan *__restrict__ plan, uint64_t n, int floatk)
+{
+    if (threadIdx.x != 0) return;
+    constexpr int P = int(sizeof(K));
+    K a = 0, b = 0;
+    for (int s = 0; s < ORSLOTS; ++s) {
+        a |= orslots[2 * s];
+        b |= orslots[2 * s + 1];
+    }
+    const K varying = a & b;
+    uint32_t cur = 0;
+    int first = -1, last = -1;
+    for (int p = 0; p < P; ++p) {
+        const uint32_t t = (n <= 1 || ((varying >> (8 * p)) & K(0xFF)) == 0) ? 1u : 0u;
+        plan->trivial[p] = t;
+        plan->src[p] = cur;
+        if (!t) {
+            cur ^= 1u;
+            if (first < 0) first = p;
+            last = p;
+        }
+    }
+    for (int p = 0; p < P; ++p) {
+        plan->xin[p] = (floatk && p == first) ? 1u : 0u;
+        plan->xout[p] = (floatk && p == last) ? 1u : 0u;
+    }
+    plan->copyback = cur;
 }
 
 // ---------------------------------------------------------------------------

@@ -4,7 +4,7 @@
  *
  * Plain C: pointers, sizes and status codes only.  No HIP, C++ or torch
  * types cross this boundary; a stream is passed as `void*` (a hipStream_t,
- * NULL = the context's own stream).
+ * NULL = HIP's null stream).
  *
  * Each entry point names the reference interface it replaces (paths relative
  * to the HPCsharp source tree).  The managed binding a .NET caller would use
@@ -50,6 +50,15 @@ extern "C" {
 #define HPCS_KEY_UINT 0  /* record key is an unsigned integer */
 #define HPCS_KEY_FLOAT 1 /* record key is an IEEE float/double (radix key transform) */
 
+/* Tuning options for hpcs_ctx_set_option. */
+#define HPCS_OPT_RANK 1       /* 0: ballot match, 1: LDS-OR match (default) */
+#define HPCS_OPT_TILE_SHAPE 2 /* u32 keys: 0: 256x16, 1: 512x16 (default), 2: 256x32, 3: 512x32
+                                 (threads x keys per thread); other key types use 256x16 */
+#define HPCS_OPT_FALLBACK_ROUNDS 3 /* look-back rounds on an unpublished tile before counting it locally
+                                      (default 24; 0 makes every wait use the local count, for tests) */
+#define HPCS_OPT_LOOKBACK_WINDOW 4 /* u32 keys: earlier tiles read per look-back round: 16, 32 (default), 64 */
+#define HPCS_OPT_PASS_ALGO 5       /* 0: single-pass onesweep with look-back, 1: reduce-then-scan (default) */
+
 #define HPCS_DTYPE_U32 0
 #define HPCS_DTYPE_U64 1
 #define HPCS_DTYPE_F32 2
@@ -85,6 +94,10 @@ int32_t hpcs_ctx_destroy(hpcs_ctx *ctx);
 /* Release the context's cached scratch memory. */
 int32_t hpcs_ctx_trim(hpcs_ctx *ctx);
 int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_t with_values, hpcs_plan_info *info);
+/* Set a kernel tuning option (HPCS_OPT_*) for later calls on this context;
+ * no counterpart in the reference (whose knobs are static properties such as
+ * SortRadixParallelWorkQuanta, RadixSortLsdParallel.cs:763). */
+int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t value);
 /* Kernel timing (no counterpart in the reference, whose Stopwatch timing is
  * commented out, RadixSortLsd.cs:146-153): on != 0 clears the statistics and
  * brackets every later launch of this context with HIP events on its stream;

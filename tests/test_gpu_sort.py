@@ -146,10 +146,31 @@ def test_records_stable(oracle, key_type, mask):
     rec = np.zeros(n, dtype=record_dtype(key_type))
     rec["key"] = (datagen.keys_u64(n, 7) & np.uint64(mask)).astype(key_type)
     rec["payload"] = np.arange(n, dtype=np.uint32)
-    ref = rec.copy()
+    ref = _byte_copy(rec)
     oracle.sort_radix_records(ref, 0, np.dtype(key_type).itemsize * 8)
     assert Algorithm.SortRadixLsd2(rec, "key") is rec
-    assert rec.tobytes() == ref.tobytes()
+    _assert_records_equal(rec, ref)
+
+
+def _byte_copy(rec):
+    # ndarray.copy() of a structured dtype with gaps leaves the padding bytes
+    # uninitialised; the record sorts move padding too, so copy every byte
+    return np.frombuffer(bytearray(rec.tobytes()), dtype=rec.dtype)
+
+
+def _assert_records_equal(rec, ref):
+    if rec.tobytes() == ref.tobytes():
+        return
+    for f in rec.dtype.names:
+        bad = np.flatnonzero(rec[f] != ref[f])
+        if bad.size:
+            i = int(bad[0])
+            raise AssertionError(f"field {f}: {bad.size} records differ, first at {i}: got {rec[i]} want {ref[i]}; "
+                                 f"got[{i}-1..{i}+1]={rec[max(0, i - 1):i + 2]} want={ref[max(0, i - 1):i + 2]}")
+    rb = np.frombuffer(rec.tobytes(), np.uint8).reshape(rec.size, -1)
+    fb = np.frombuffer(ref.tobytes(), np.uint8).reshape(ref.size, -1)
+    i = int(np.flatnonzero((rb != fb).any(axis=1))[0])
+    raise AssertionError(f"padding bytes differ first at record {i}: got {rb[i].tolist()} want {fb[i].tolist()}")
 
 
 def test_records_float_key_unaligned(oracle):
@@ -159,10 +180,10 @@ def test_records_float_key_unaligned(oracle):
     rec = np.zeros(n, dtype=dt)
     rec["key"] = datagen.doubles_special(n, 99)
     rec["payload"] = np.arange(n, dtype=np.uint32)
-    ref = rec.copy()
+    ref = _byte_copy(rec)
     oracle.sort_radix_records(ref, 1, 64, 1)
     Algorithm.SortRadixLsd2(rec, "key")
-    assert rec.tobytes() == ref.tobytes()
+    _assert_records_equal(rec, ref)
 
 
 @pytest.mark.parametrize("case", FIXTURES["hist"], ids=lambda c: str(c["n"]))
@@ -232,21 +253,102 @@ def test_full_size_2pow28_properties():
     copy), and input and output have the same byte histograms."""
     import torch
     n = 1 << 28
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)   # order with torch's own work
     g = torch.Generator(device="cuda")
     g.manual_seed(0x5EED0002)
     t = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device="cuda", generator=g)
     h_in = torch.empty(1024, dtype=torch.int32, device="cuda")
     N.check(N.lib().hpcs_dev_histogram_byte_components_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n,
-                                                           ctypes.c_void_p(h_in.data_ptr()), None))
+                                                           ctypes.c_void_p(h_in.data_ptr()), st))
     ref = torch.sort((t.to(torch.int64) & 0xFFFFFFFF))[0]
-    N.check(N.lib().hpcs_dev_sort_radix_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n, None))
-    torch.cuda.synchronize()
+    N.check(N.lib().hpcs_dev_sort_radix_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n, st))
     got = t.to(torch.int64) & 0xFFFFFFFF
     assert torch.equal(got, ref)
     del ref, got
     h_out = torch.empty(1024, dtype=torch.int32, device="cuda")
     N.check(N.lib().hpcs_dev_histogram_byte_components_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n,
-                                                           ctypes.c_void_p(h_out.data_ptr()), None))
+                                                           ctypes.c_void_p(h_out.data_ptr()), st))
     torch.cuda.synchronize()
     assert torch.equal(h_in, h_out)
     assert int(h_in.view(4, 256).sum(dim=1).min()) == n
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_forced_lookback_fallback(oracle, shape):
+    """Every look-back wait resolved by counting the earlier tile locally
+    (fallback threshold 0) still gives the oracle's output."""
+    ctx = N.new_context(0)
+    try:
+        N.set_option(N.HPCS_OPT_PASS_ALGO, 0, ctx)          # single-pass look-back
+        N.set_option(N.HPCS_OPT_FALLBACK_ROUNDS, 0, ctx)
+        N.set_option(N.HPCS_OPT_TILE_SHAPE, shape, ctx)
+        a = datagen.keys_u32((1 << 20) + 333, 4242 + shape)
+        ref = a.copy()
+        oracle.sort_radix_lsd(ref)
+        N.check(N.lib().hpcs_sort_radix_u32(ctx, a.ctypes.data_as(ctypes.c_void_p), a.size))
+        assert np.array_equal(a, ref)
+        k = datagen.keys_u64(300001, 77) & np.uint64(0xFFFFF)
+        v = np.arange(k.size, dtype=np.uint32)
+        rk, rv = k.copy(), v.copy()
+        oracle.sort_radix_pairs(rk, rv)
+        N.check(N.lib().hpcs_sort_radix_pairs_u64_u32(ctx, k.ctypes.data_as(ctypes.c_void_p),
+                                                      v.ctypes.data_as(ctypes.c_void_p), k.size))
+        assert np.array_equal(k, rk) and np.array_equal(v, rv)
+        x = datagen.doubles_special(200003, 5)
+        xr = x.copy()
+        oracle.sort_radix_float(xr)
+        N.check(N.lib().hpcs_sort_radix_f64(ctx, x.ctypes.data_as(ctypes.c_void_p), x.size))
+        assert np.array_equal(x.view(np.uint64), xr.view(np.uint64))
+    finally:
+        N.destroy_context(ctx)
+
+
+@pytest.mark.parametrize("algo", [0, 1])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+def test_tile_shapes_u32(oracle, shape, algo):
+    ctx = N.new_context(0)
+    try:
+        N.set_option(N.HPCS_OPT_TILE_SHAPE, shape, ctx)
+        N.set_option(N.HPCS_OPT_PASS_ALGO, algo, ctx)
+        for n, dist in [(8191, "uniform"), (8192, "sorted"), (8193, "low8"), (3 * (1 << 20) + 7, "uniform")]:
+            a = datagen.keys_u32(n, 99 + n, dist)
+            ref = a.copy()
+            oracle.sort_radix_lsd(ref)
+            N.check(N.lib().hpcs_sort_radix_u32(ctx, a.ctypes.data_as(ctypes.c_void_p), a.size))
+            assert np.array_equal(a, ref), (n, dist)
+    finally:
+        N.destroy_context(ctx)
+
+
+def test_both_pass_algorithms_all_types(oracle):
+    """The single-pass look-back form and the reduce-then-scan form give the
+    oracle's output for every key type and for key-value pairs."""
+    for algo in (0, 1):
+        ctx = N.new_context(0)
+        try:
+            N.set_option(N.HPCS_OPT_PASS_ALGO, algo, ctx)
+            P = ctypes.c_void_p
+            a = datagen.keys_u64(400009, 31 + algo)
+            ra = a.copy()
+            oracle.sort_radix_lsd(ra)
+            N.check(N.lib().hpcs_sort_radix_u64(ctx, a.ctypes.data_as(P), a.size))
+            assert np.array_equal(a, ra)
+            x = datagen.floats_finite(300007, 41 + algo)
+            rx = x.copy()
+            oracle.sort_radix_float(rx)
+            N.check(N.lib().hpcs_sort_radix_f32(ctx, x.ctypes.data_as(P), x.size))
+            assert np.array_equal(x.view(np.uint32), rx.view(np.uint32))
+            k = datagen.keys_u32(250001, 51 + algo) & np.uint32(0x3FFF)
+            v = np.arange(k.size, dtype=np.uint32)
+            rk, rv = k.copy(), v.copy()
+            oracle.sort_radix_pairs(rk, rv)
+            N.check(N.lib().hpcs_sort_radix_pairs_u32_u32(ctx, k.ctypes.data_as(P), v.ctypes.data_as(P), k.size))
+            assert np.array_equal(k, rk) and np.array_equal(v, rv)
+            for dist in ("sorted", "low8", "constant"):
+                b = datagen.keys_u32(1 << 20, 61, dist)
+                rb = b.copy()
+                oracle.sort_radix_lsd(rb)
+                N.check(N.lib().hpcs_sort_radix_u32(ctx, b.ctypes.data_as(P), b.size))
+                assert np.array_equal(b, rb), dist
+        finally:
+            N.destroy_context(ctx)
