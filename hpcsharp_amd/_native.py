@@ -25,6 +25,7 @@ HPCS_KEY_UINT = 0
 HPCS_KEY_FLOAT = 1
 HPCS_DTYPE_U32, HPCS_DTYPE_U64, HPCS_DTYPE_F32, HPCS_DTYPE_F64 = 0, 1, 2, 3
 HPCS_OPT_RANK, HPCS_OPT_TILE_SHAPE, HPCS_OPT_FALLBACK_ROUNDS, HPCS_OPT_LOOKBACK_WINDOW, HPCS_OPT_PASS_ALGO = 1, 2, 3, 4, 5
+HPCS_OPT_PERSISTENT = 6
 
 # Every symbol include/hpcsort.h declares, with its argument types.
 _p, _i64, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32

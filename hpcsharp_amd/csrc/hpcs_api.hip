@@ -76,6 +76,7 @@ struct hpcs_ctx {
     int fallback_rounds = FALLBACK_ROUNDS;
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
     int pass_algo = PASS_RTS;     // PASS_ONESWEEP or PASS_RTS (reduce-then-scan)
+    int persistent = 0;           // reduce-then-scan scatter: persistent workgroups with prefetch
     // kernel timing
     struct Pending {
         const char *name;
@@ -229,7 +230,7 @@ template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int W>
 void launch_onesweep_w(int rank_mode, int fb, dim3 grid, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb,
                        uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
 {
-    if (rank_mode == RANK_BALLOT && BLOCK * ITEMS == SORT_TILE)
+    if (rank_mode == RANK_BALLOT)
         hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_BALLOT, W>), grid, dim3(BLOCK), 0, st,
                            a, b, va, vb, n, pass, plan, lb, next, fb);
     else
@@ -288,6 +289,53 @@ void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const 
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts);
     else
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts);
+}
+
+template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int RANK>
+void launch_scatter_r(int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb, uint32_t n, int pass,
+                      DevPlan *plan, const uint32_t *offsets, uint32_t tiles)
+{
+    static int occ = -1; // resident workgroups per CU for this instantiation
+    if (occ < 0) {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK>, BLOCK,
+                                                         0) != hipSuccess || o < 1)
+            o = 1;
+        occ = o;
+    }
+    uint32_t grid = uint32_t(num_cus) * uint32_t(occ);
+    grid = grid / NUM_XCD * NUM_XCD;
+    if (grid < NUM_XCD) grid = NUM_XCD;
+    if (grid > tiles) grid = tiles;
+    hipLaunchKernelGGL((k_scatter<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK>), dim3(grid), dim3(BLOCK), 0, st, a, b, va,
+                       vb, n, pass, plan, offsets, tiles);
+}
+
+template <typename K, bool HAS_VALS, bool FLOATK>
+void launch_scatter(int rank_mode, int shape, int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb,
+                    uint32_t n, int pass, DevPlan *plan, const uint32_t *offsets, uint32_t tiles)
+{
+    if constexpr (sizeof(K) == 4 && !HAS_VALS) {
+        if (shape == 1) {
+            if (rank_mode == RANK_BALLOT)
+                return launch_scatter_r<K, HAS_VALS, FLOATK, 512, 16, RANK_BALLOT>(num_cus, st, a, b, va, vb, n, pass,
+                                                                                    plan, offsets, tiles);
+            return launch_scatter_r<K, HAS_VALS, FLOATK, 512, 16, RANK_LDSOR>(num_cus, st, a, b, va, vb, n, pass,
+                                                                               plan, offsets, tiles);
+        }
+        if (shape == 2) {
+            if (rank_mode == RANK_BALLOT)
+                return launch_scatter_r<K, HAS_VALS, FLOATK, 256, 32, RANK_BALLOT>(num_cus, st, a, b, va, vb, n, pass,
+                                                                                    plan, offsets, tiles);
+            return launch_scatter_r<K, HAS_VALS, FLOATK, 256, 32, RANK_LDSOR>(num_cus, st, a, b, va, vb, n, pass,
+                                                                               plan, offsets, tiles);
+        }
+    }
+    if (rank_mode == RANK_BALLOT)
+        return launch_scatter_r<K, HAS_VALS, FLOATK, 256, 16, RANK_BALLOT>(num_cus, st, a, b, va, vb, n, pass, plan,
+                                                                            offsets, tiles);
+    launch_scatter_r<K, HAS_VALS, FLOATK, 256, 16, RANK_LDSOR>(num_cus, st, a, b, va, vb, n, pass, plan, offsets,
+                                                                tiles);
 }
 
 template <typename K, bool FLOATK>
@@ -361,9 +409,14 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
                                    plan);
             });
             timed(c, st, "scatter", [&] {
-                launch_onesweep<K, HAS_VALS, FLOATK>(c->rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
-                                                     d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
-                                                     nullptr);
+                // the persistent kernel keeps two tiles in registers: not for 512x32 tiles
+                if (c->persistent && shape != 3)
+                    launch_scatter<K, HAS_VALS, FLOATK>(c->rank_mode, shape, c->num_cus, st, d_keys, keysB, d_vals,
+                                                        valsB, uint32_t(n), p, plan, counts, tiles);
+                else
+                    launch_onesweep<K, HAS_VALS, FLOATK>(c->rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
+                                                         d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
+                                                         nullptr);
             });
             continue;
         }
@@ -612,6 +665,10 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
         case HPCS_OPT_PASS_ALGO:
             if (value != PASS_ONESWEEP && value != PASS_RTS) fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0 or 1");
             ctx->pass_algo = int(value);
+            break;
+        case HPCS_OPT_PERSISTENT:
+            if (value != 0 && value != 1) fail(HPCS_E_INVALID_ARG, "persistent must be 0 or 1");
+            ctx->persistent = int(value);
             break;
         case HPCS_OPT_FALLBACK_ROUNDS:
             if (value < 0 || value > (1 << 20)) fail(HPCS_E_INVALID_ARG, "fallback rounds out of range");

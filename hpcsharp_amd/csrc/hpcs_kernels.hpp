@@ -83,18 +83,21 @@ __device__ inline void lb_store(uint32_t *p, uint32_t v)
 }
 
 // Lanes of the wave whose 8-bit digit equals this lane's: eight ballots, one
-// per digit bit; a lane keeps the ballot where its bit is set and the
-// complement where it is clear.
+// per digit bit.  Lane j differs from this lane in bit b iff ballot_b[j]
+// differs from this lane's bit, i.e. (ballot_b ^ splat_b)[j] = 1 with
+// splat_b = all-ones when this lane's bit is set; the peers are the lanes
+// that differ in no bit.
 __device__ inline uint64_t match_digit8(uint32_t d)
 {
-    uint64_t peers = ~0ull;
+    uint32_t diff_lo = 0u, diff_hi = 0u;
 #pragma unroll
     for (int b = 0; b < RADIX_BITS; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
+        const uint32_t sp = uint32_t(__builtin_amdgcn_sbfe(int(d), b, 1)); // ~0 if bit b set
+        const uint64_t m = __ballot(sp != 0u);
+        diff_lo |= uint32_t(m) ^ sp;         // one three-input bit operation per half
+        diff_hi |= uint32_t(m >> 32) ^ sp;
     }
-    return peers;
+    return ~((uint64_t(diff_hi) << 32) | diff_lo);
 }
 
 // Workgroup b runs on XCD b % 8; map it to tile start(b % 8) + b / 8 so that
@@ -354,10 +357,29 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     }
 
     // ---- 1. per-wave digit counts ----
+    // LDS-OR ranking stages keys while it ranks, so it needs the wave bases
+    // first and counts with LDS atomics here.  Ballot ranking ranks first
+    // with wave-local counters (their final values are the counts) and stages
+    // afterwards, so it issues no counting atomics.
     uint32_t *cnt = s_cnt[wave];
+    uint32_t rank[RANK == RANK_BALLOT ? ITEMS : 1];
+    if constexpr (RANK == RANK_BALLOT) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i)
-        if (i * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[i], shift)], 1u);
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint32_t d = KeyOps<K>::digit(keys[i], shift);
+            const bool valid = i * WAVE + lane < wave_valid;
+            const uint64_t peers = match_digit8(d) & __ballot(valid);
+            const uint32_t below = count_below(peers);
+            const uint32_t old = cnt[d];
+            if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
+            rank[i] = old + below;
+            __builtin_amdgcn_wave_barrier();
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+            if (i * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[i], shift)], 1u);
+    }
     __syncthreads();
 
     // ---- 2. per-digit wave bases, tile count, publish, scan ----
@@ -392,17 +414,14 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     // ---- 3. rank and stage the tile in LDS in sorted order ----
     uint32_t pos[HAS_VALS ? ITEMS : 1];
     if constexpr (RANK == RANK_BALLOT) {
+        // cnt[d] now holds the wave's base for digit d
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
-            const uint32_t d = KeyOps<K>::digit(keys[i], shift);
-            const bool valid = i * WAVE + lane < wave_valid;
-            const uint64_t peers = match_digit8(d) & __ballot(valid);
-            const uint32_t below = count_below(peers);
-            const uint32_t old = cnt[d];
-            if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
-            if (valid) s_keys[old + below] = keys[i];
-            if constexpr (HAS_VALS) pos[i] = old + below;
-            __builtin_amdgcn_wave_barrier();
+            if (i * WAVE + lane < wave_valid) {
+                const uint32_t at = cnt[KeyOps<K>::digit(keys[i], shift)] + rank[i];
+                s_keys[at] = keys[i];
+                if constexpr (HAS_VALS) pos[i] = at;
+            }
         }
     } else {
         uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
@@ -529,6 +548,210 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
             kout[dst] = xout ? KeyOps<K>::from_key(k) : k;
             if constexpr (HAS_VALS) vout[dst] = s_vals[j];
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_scatter: the scatter step of the reduce-then-scan pass, persistent.
+//
+// Offsets were computed beforehand (offsets[tile][digit] = global position of
+// the tile's first key of each digit), so tiles are independent and the tile
+// order is free.  The grid is sized to the resident workgroups; workgroup b
+// serves the tiles of XCD b % 8's contiguous range (xcd_contiguous), taking
+// every (grid / 8)-th one, so the tiles an XCD has in flight are neighbours
+// and the partial cache lines at the ends of their digit runs meet in its L2.
+// While a tile is ranked and written, the next tile's keys and offsets are
+// already loading into registers, so each CU keeps reads in flight through
+// the compute phases (one tile per workgroup at a time left HBM half idle).
+//
+// Per tile: per-wave ranks (RANK, as in k_onesweep), per-wave bases from the
+// per-wave counts, tile digit starts by a block scan, keys staged in LDS in
+// sorted order, then written in order so each digit's run is contiguous.
+// ---------------------------------------------------------------------------
+template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int RANK>
+__global__ __launch_bounds__(BLOCK) void k_scatter(K *__restrict__ keysA, K *__restrict__ keysB,
+                                                   uint32_t *__restrict__ valsA, uint32_t *__restrict__ valsB,
+                                                   uint32_t n, int pass, const DevPlan *__restrict__ plan,
+                                                   const uint32_t *__restrict__ offsets, uint32_t tiles)
+{
+    static_assert(BLOCK >= RADIX && BLOCK % WAVE == 0, "one thread per digit is required");
+    constexpr int WAVES = BLOCK / WAVE;
+    constexpr int TILE = BLOCK * ITEMS;
+    constexpr int WAVE_TILE = WAVE * ITEMS;
+    constexpr int SCAN_WAVES = RADIX / WAVE;
+
+    __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint64_t s_match[RANK == RANK_LDSOR ? WAVES : 1][RANK == RANK_LDSOR ? RADIX : 1];
+    __shared__ uint32_t s_base[RADIX];
+    __shared__ uint32_t s_scan[SCAN_WAVES];
+    __shared__ K s_keys[TILE];
+    __shared__ uint32_t s_vals[HAS_VALS ? TILE : 1];
+
+    if (plan->trivial[pass]) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wave = tid >> 6;
+    const uint32_t src = plan->src[pass];
+    const K *__restrict__ kin = src ? keysB : keysA;
+    K *__restrict__ kout = src ? keysA : keysB;
+    const uint32_t *__restrict__ vin = src ? valsB : valsA;
+    uint32_t *__restrict__ vout = src ? valsA : valsB;
+    const bool xin = FLOATK && plan->xin[pass];
+    const bool xout = FLOATK && plan->xout[pass];
+    const int shift = pass * RADIX_BITS;
+
+    // this workgroup's tiles: XCD x's range, every `stride`-th from i
+    const uint32_t x = blockIdx.x % NUM_XCD;
+    const uint32_t per = tiles / NUM_XCD, rem = tiles % NUM_XCD;
+    const uint32_t x_start = x * per + min(x, rem), x_count = per + (x < rem ? 1u : 0u);
+    const uint32_t stride = (gridDim.x - x + NUM_XCD - 1) / NUM_XCD; // workgroups on XCD x
+    uint32_t i = blockIdx.x / NUM_XCD;
+    if (i >= x_count) return;
+
+    auto load_tile = [&](uint32_t tile, K (&k)[ITEMS], uint32_t &off) {
+        const uint32_t wbase = tile * uint32_t(TILE) + uint32_t(wave * WAVE_TILE);
+        const int wv = n > wbase ? int(min(uint32_t(WAVE_TILE), n - wbase)) : 0;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int o = j * WAVE + lane;
+            k[j] = o < wv ? kin[wbase + o] : ~K(0);
+        }
+        off = tid < RADIX ? offsets[size_t(tile) * RADIX + tid] : 0u;
+    };
+
+    K cur[ITEMS];
+    uint32_t cur_off;
+    load_tile(x_start + i, cur, cur_off);
+    while (true) {
+        const uint32_t tile = x_start + i;
+        const uint32_t inext = i + stride;
+        const bool has_next = inext < x_count;
+        K nxt[ITEMS];
+        uint32_t nxt_off = 0;
+        if (has_next) load_tile(x_start + inext, nxt, nxt_off);
+
+        const uint32_t tile_base = tile * uint32_t(TILE);
+        const uint32_t wave_base = tile_base + uint32_t(wave * WAVE_TILE);
+        const int wave_valid = n > wave_base ? int(min(uint32_t(WAVE_TILE), n - wave_base)) : 0;
+        K keys[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+            keys[j] = (xin && j * WAVE + lane < wave_valid) ? KeyOps<K>::to_key(cur[j]) : cur[j];
+
+        for (int q = tid; q < WAVES * RADIX; q += BLOCK) (&s_cnt[0][0])[q] = 0u;
+        if constexpr (RANK == RANK_LDSOR)
+            for (int q = tid; q < WAVES * RADIX; q += BLOCK) (&s_match[0][0])[q] = 0ull;
+        __syncthreads();
+
+        // ---- per-wave counts (ballot ranks first; LDS-OR counts first) ----
+        uint32_t *cnt = s_cnt[wave];
+        uint32_t rank[RANK == RANK_BALLOT ? ITEMS : 1];
+        if constexpr (RANK == RANK_BALLOT) {
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                const uint32_t d = KeyOps<K>::digit(keys[j], shift);
+                const bool valid = j * WAVE + lane < wave_valid;
+                const uint64_t peers = match_digit8(d) & __ballot(valid);
+                const uint32_t below = count_below(peers);
+                const uint32_t old = cnt[d];
+                if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
+                rank[j] = old + below;
+                __builtin_amdgcn_wave_barrier();
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j)
+                if (j * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[j], shift)], 1u);
+        }
+        __syncthreads();
+
+        // ---- wave bases, tile digit starts, global bases ----
+        uint32_t tile_cnt = 0, tile_start = 0;
+        if (tid < RADIX) {
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) {
+                const uint32_t c = s_cnt[w][tid];
+                s_cnt[w][tid] = tile_cnt;
+                tile_cnt += c;
+            }
+            uint32_t xs = tile_cnt;
+#pragma unroll
+            for (int off = 1; off < WAVE; off <<= 1) {
+                const uint32_t y = __shfl_up(xs, off);
+                if (lane >= off) xs += y;
+            }
+            if (lane == WAVE - 1) s_scan[wave] = xs;
+            tile_start = xs - tile_cnt;
+        }
+        __syncthreads();
+        if (tid < RADIX) {
+            for (int w = 0; w < wave; ++w) tile_start += s_scan[w];
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) s_cnt[w][tid] += tile_start;
+            s_base[tid] = cur_off - tile_start;
+        }
+        __syncthreads();
+
+        // ---- stage the tile in LDS in sorted order ----
+        uint32_t pos[HAS_VALS ? ITEMS : 1];
+        if constexpr (RANK == RANK_BALLOT) {
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                if (j * WAVE + lane < wave_valid) {
+                    const uint32_t at = cnt[KeyOps<K>::digit(keys[j], shift)] + rank[j];
+                    s_keys[at] = keys[j];
+                    if constexpr (HAS_VALS) pos[j] = at;
+                }
+            }
+        } else {
+            uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
+            const uint32_t lanebit = 1u << (lane & 31);
+            const int half = lane >> 5;
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                const uint32_t d = KeyOps<K>::digit(keys[j], shift);
+                const bool valid = j * WAVE + lane < wave_valid;
+                if (valid) atomicOr(&match32[2 * d + half], lanebit);
+                __builtin_amdgcn_wave_barrier();
+                const uint64_t peers = s_match[wave][d];
+                const uint32_t below = count_below(peers);
+                const uint32_t old = cnt[d];
+                if (valid && below == 0) {
+                    cnt[d] = old + uint32_t(__popcll(peers));
+                    s_match[wave][d] = 0ull;
+                }
+                if (valid) s_keys[old + below] = keys[j];
+                if constexpr (HAS_VALS) pos[j] = old + below;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if constexpr (HAS_VALS) {
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                const int o = j * WAVE + lane;
+                if (o < wave_valid) s_vals[pos[j]] = vin[wave_base + o];
+            }
+        }
+        __syncthreads();
+
+        // ---- write the tile out, each digit's run contiguous ----
+        const int tile_valid = n > tile_base ? int(min(uint32_t(TILE), n - tile_base)) : 0;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int jj = j * BLOCK + tid;
+            if (jj < tile_valid) {
+                const K k = s_keys[jj];
+                const uint32_t dst = s_base[KeyOps<K>::digit(k, shift)] + uint32_t(jj);
+                kout[dst] = xout ? KeyOps<K>::from_key(k) : k;
+                if constexpr (HAS_VALS) vout[dst] = s_vals[jj];
+            }
+        }
+        if (!has_next) break;
+        __syncthreads(); // LDS is reused by the next tile
+        i = inext;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) cur[j] = nxt[j];
+        cur_off = nxt_off;
     }
 }
 

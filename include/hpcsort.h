@@ -58,6 +58,8 @@ extern "C" {
                                       (default 24; 0 makes every wait use the local count, for tests) */
 #define HPCS_OPT_LOOKBACK_WINDOW 4 /* u32 keys: earlier tiles read per look-back round: 16, 32 (default), 64 */
 #define HPCS_OPT_PASS_ALGO 5       /* 0: single-pass onesweep with look-back, 1: reduce-then-scan (default) */
+#define HPCS_OPT_PERSISTENT 6      /* reduce-then-scan scatter: 0 one workgroup per tile (default), 1 persistent
+                                      workgroups that prefetch the next tile */
 
 #define HPCS_DTYPE_U32 0
 #define HPCS_DTYPE_U64 1
