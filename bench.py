@@ -72,6 +72,66 @@ def measure(torch, N, host_keys: np.ndarray, steps: int, warmup: int, profile: b
     return (t1 - t0) / steps, stats, ok
 
 
+def batched_vs_plain(torch, N, datagen, nseg: int, seed: int, reps: int):
+    """4096 uint32 segments with lengths log-uniform in [16, 2^20] (uniform
+    keys): one batched call (captured once in a HIP graph, replayed) against
+    a loop of plain per-array calls on the same buffers.  Outputs must be
+    identical.  Returns a dict for the JSON line."""
+    lens = datagen.segment_lengths(nseg, seed, 16, 1 << 20)
+    begin = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(begin[-1])
+    keys = datagen.keys_u32(total, seed + 1, "uniform")
+    lib, ctx = N.lib(), N.context(0)
+    src = torch.from_numpy(keys.view(np.int32)).cuda()
+    d_begin = torch.from_numpy(begin).cuda()
+    s = torch.cuda.Stream()
+    work = src.clone()
+    P = ctypes.c_void_p
+
+    def batched():
+        N.check(lib.hpcs_dev_sort_segments_u32(ctx, P(work.data_ptr()), total, P(d_begin.data_ptr()), nseg,
+                                               P(s.cuda_stream)))
+
+    with torch.cuda.stream(s):
+        batched()                                     # warm-up: sizes the scratch
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        batched()
+    t_b = []
+    for _ in range(reps):
+        work.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.replay()
+        torch.cuda.synchronize()
+        t_b.append(time.perf_counter() - t0)
+    out_b = work.clone()
+    # plain loop: one device-resident sort call per segment
+    plain = src.clone()
+    base = plain.data_ptr()
+    t_p = []
+    for _ in range(max(1, reps // 3)):
+        plain.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(nseg):
+            n_i = int(lens[i])
+            if n_i > 1:
+                N.check(lib.hpcs_dev_sort_radix_u32(ctx, P(base + 4 * int(begin[i])), n_i, P(s.cuda_stream)))
+        torch.cuda.synchronize()
+        t_p.append(time.perf_counter() - t0)
+    identical = bool(torch.equal(out_b, plain))
+    tb, tp = float(np.median(t_b)), float(np.median(t_p))
+    del work, out_b, plain, src, g
+    torch.cuda.empty_cache()
+    return {"workload": f"{nseg} uint32 segments, lengths log-uniform in [16, 2^20], uniform keys, seed {seed:#x}",
+            "keys": total, "batched_ms": round(tb * 1e3, 3), "plain_loop_ms": round(tp * 1e3, 3),
+            "batched_gkeys": round(total / tb / 1e9, 3), "plain_loop_gkeys": round(total / tp / 1e9, 3),
+            "speedup": round(tp / tb, 2), "identical_outputs": identical,
+            "batched": "one hipGraph replay of the batched call (fixed 14-launch sequence)"}
+
+
 def cpu_baseline(host_keys: np.ndarray, reps: int):
     """ParallelAlgorithm.SortRadixPar restated in C (oracle/), OpenMP threads."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -101,6 +161,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dists", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)
@@ -154,6 +215,14 @@ def main():
             dists[dist] = round(n / t2 / 1e9, 3)
             log(f"[bench] {dist}: {t2 * 1e3:.3f} ms/sort, {dists[dist]} Gkeys/s")
         extra["gkeys_by_distribution"] = dists
+
+    if not args.no_batched:
+        log("[bench] batched vs plain: 4096 segments")
+        extra["batched_vs_plain"] = batched_vs_plain(torch, N, datagen, 4096, 0x5EED0008, 6)
+        log(f"[bench] batched: {extra['batched_vs_plain']}")
+        if not extra["batched_vs_plain"]["identical_outputs"]:
+            print("bench: batched and plain outputs differ", file=sys.stderr)
+            sys.exit(3)
 
     cpu = None
     if not args.no_cpu:

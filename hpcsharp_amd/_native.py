@@ -67,6 +67,8 @@ SIGNATURES = {
     "hpcs_sort_radix_records": ([_p, _p, _i64, _i32, _i32, _i32, _i32], _i32),
     "hpcs_histogram_byte_components_u32": ([_p, _p, _i64, _i64, _p], _i32),
     "hpcs_histogram_byte_components_u64": ([_p, _p, _i64, _i64, _p], _i32),
+    "hpcs_sort_segments_u32": ([_p, _p, _p, _i32], _i32),
+    "hpcs_dev_sort_segments_u32": ([_p, _p, _i64, _p, _i32, _p], _i32),
     "hpcs_dev_sort_radix_u32": ([_p, _p, _i64, _p], _i32),
     "hpcs_dev_sort_radix_u64": ([_p, _p, _i64, _p], _i32),
     "hpcs_dev_sort_radix_f32": ([_p, _p, _i64, _p], _i32),

@@ -5,6 +5,7 @@
 // onesweep passes, copy-back); no host synchronisation happens inside a
 // device-resident call, so the sequence can be captured in a HIP graph.
 #include "hpcs_kernels.hpp"
+#include "hpcs_segments.hpp"
 #include "../../include/hpcsort.h"
 
 #include <hip/hip_runtime.h>
@@ -433,6 +434,74 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     });
 }
 
+// Batched segments: scratch [SegPlan | seg_tile0 | small | large | tile_seg |
+// counts | keys B].  The plan's word `bad` counts rejected segments.
+struct SegLayout {
+    size_t off_plan, off_tile0, off_small, off_large, off_tileseg, off_counts, off_keysB, total;
+    uint64_t max_tiles;
+};
+
+SegLayout seg_layout(int64_t n, int nseg)
+{
+    SegLayout L{};
+    L.max_tiles = uint64_t(n) / SEG_TILE + uint64_t(nseg) + 1;
+    L.off_plan = 0;
+    L.off_tile0 = align_up(sizeof(SegPlan));
+    L.off_small = align_up(L.off_tile0 + size_t(nseg) * 4);
+    L.off_large = align_up(L.off_small + size_t(nseg) * 4);
+    L.off_tileseg = align_up(L.off_large + size_t(nseg) * 4);
+    L.off_counts = align_up(L.off_tileseg + L.max_tiles * 4);
+    L.off_keysB = align_up(L.off_counts + L.max_tiles * RADIX * 4);
+    L.total = align_up(L.off_keysB + size_t(n) * 4);
+    return L;
+}
+
+void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_begin, int nseg, hipStream_t st)
+{
+    if (n < 0 || nseg < 0) fail(HPCS_E_INVALID_ARG, "n and nseg must be non-negative");
+    if (nseg == 0 || n <= 1) return;
+    if (!d_keys || !d_begin) fail(HPCS_E_INVALID_ARG, "keys or segment offsets are null");
+    if (n >= (int64_t(1) << 32)) fail(HPCS_E_UNSUPPORTED, "segments must lie below 2^32 keys");
+    const SegLayout L = seg_layout(n, nseg);
+    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
+    SegPlan *sp = reinterpret_cast<SegPlan *>(base + L.off_plan);
+    uint32_t *tile0 = reinterpret_cast<uint32_t *>(base + L.off_tile0);
+    uint32_t *small = reinterpret_cast<uint32_t *>(base + L.off_small);
+    uint32_t *large = reinterpret_cast<uint32_t *>(base + L.off_large);
+    uint32_t *tileseg = reinterpret_cast<uint32_t *>(base + L.off_tileseg);
+    uint32_t *counts = reinterpret_cast<uint32_t *>(base + L.off_counts);
+    uint32_t *keysB = reinterpret_cast<uint32_t *>(base + L.off_keysB);
+    timed(c, st, "seg_plan", [&] {
+        hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, d_begin, nseg, n, sp, tile0, tileseg, small, large);
+    });
+    const int small_grid = std::min(nseg, c->num_cus * 8);
+    timed(c, st, "seg_small", [&] {
+        hipLaunchKernelGGL(k_seg_small, dim3(small_grid), dim3(SEG_BLOCK), 0, st, d_keys, d_begin, sp, small);
+    });
+    const int count_grid = c->num_cus * 8;
+    const int scan_grid = std::min(nseg, c->num_cus * 8);
+    static int occ = -1;
+    if (occ < 0) {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_seg_scatter, SEG_BLOCK, 0) != hipSuccess || o < 1) o = 1;
+        occ = o;
+    }
+    const int scatter_grid = std::max(int(NUM_XCD), c->num_cus * occ / int(NUM_XCD) * int(NUM_XCD));
+    for (int p = 0; p < 4; ++p) {
+        timed(c, st, "seg_counts", [&] {
+            hipLaunchKernelGGL(k_seg_counts, dim3(count_grid), dim3(SEG_BLOCK), 0, st, d_keys, keysB, p, d_begin, sp,
+                               tile0, tileseg, counts);
+        });
+        timed(c, st, "seg_scan", [&] {
+            hipLaunchKernelGGL(k_seg_scan, dim3(scan_grid), dim3(RADIX), 0, st, d_begin, sp, tile0, large, counts);
+        });
+        timed(c, st, "seg_scatter", [&] {
+            hipLaunchKernelGGL(k_seg_scatter, dim3(scatter_grid), dim3(SEG_BLOCK), 0, st, d_keys, keysB, p, d_begin,
+                               sp, tile0, tileseg, counts);
+        });
+    }
+}
+
 template <typename K>
 void run_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset, int32_t key_kind,
                  hipStream_t st)
@@ -799,7 +868,37 @@ HPCS_API int32_t hpcs_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_
     return guarded(ctx, [&] { host_histogram<uint64_t>(ctx, a, l, r, counts); });
 }
 
+HPCS_API int32_t hpcs_sort_segments_u32(hpcs_ctx *ctx, uint32_t *keys, const int64_t *seg_begin, int32_t nseg)
+{
+    return guarded(ctx, [&] {
+        if (nseg < 0) fail(HPCS_E_INVALID_ARG, "nseg must be non-negative");
+        if (nseg == 0) return;
+        if (!seg_begin) fail(HPCS_E_INVALID_ARG, "seg_begin is null");
+        if (seg_begin[0] < 0) fail(HPCS_E_INVALID_ARG, "segments must start at or after index 0");
+        for (int32_t s = 0; s < nseg; ++s)
+            if (seg_begin[s + 1] < seg_begin[s]) fail(HPCS_E_INVALID_ARG, "segment offsets must be non-decreasing");
+        const int64_t n = seg_begin[nseg];
+        if (n > 0 && !keys) fail(HPCS_E_INVALID_ARG, "keys is null");
+        if (n <= 1) return;
+        const size_t kbytes = size_t(n) * 4, boff = align_up(kbytes);
+        char *io = static_cast<char *>(ensure(ctx->io, ctx->io_cap, boff + size_t(nseg + 1) * 8));
+        uint32_t *d_keys = reinterpret_cast<uint32_t *>(io);
+        int64_t *d_begin = reinterpret_cast<int64_t *>(io + boff);
+        hip_check(hipMemcpyAsync(d_keys, keys, kbytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
+        hip_check(hipMemcpyAsync(d_begin, seg_begin, size_t(nseg + 1) * 8, hipMemcpyHostToDevice, ctx->stream), "H2D");
+        run_segments(ctx, d_keys, n, d_begin, nseg, ctx->stream);
+        hip_check(hipMemcpyAsync(keys, d_keys, kbytes, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
 // device-resident
+HPCS_API int32_t hpcs_dev_sort_segments_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, const int64_t *d_seg_begin,
+                                            int32_t nseg, void *stream)
+{
+    return guarded(ctx, [&] { run_segments(ctx, d_keys, n, d_seg_begin, nseg, pick_stream(ctx, stream)); });
+}
+
 HPCS_API int32_t hpcs_dev_sort_radix_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] { run_sort<uint32_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });

@@ -1,0 +1,364 @@
+// hpcs_segments.hpp -- batched sort of many uint32 segments in one fixed
+// sequence of launches (the GPU form of calling the Experimental
+// SortRadixLsd(uint[], startIndex, length), RadixSortLsd.cs:1409-1457, once
+// per segment).
+//
+// Segment s is keys[begin[s], begin[s+1]).  Everything is planned on the
+// device, so the launch sequence does not depend on the segment lengths and
+// can be captured once in a HIP graph and replayed:
+//   k_seg_plan     classify: segments of at most SEG_SMALL keys go to a list
+//                  sorted by k_seg_small; larger ones get a range of tiles of
+//                  TILE keys (a tile never crosses a segment)
+//   k_seg_small    one workgroup per small segment: four LSD passes entirely
+//                  in LDS (one HBM read and one write per key)
+//   per pass p of the large segments:
+//     k_seg_counts   per tile, the digit counts
+//     k_seg_scan     per segment, digit starts (segment base + exclusive scan
+//                    of the segment's digit totals) and per-tile offsets
+//     k_seg_scatter  rank, stage in LDS, write each digit run contiguously
+// Four passes leave every large segment back in the caller's buffer.
+#pragma once
+#include "hpcs_kernels.hpp"
+
+namespace hpcs {
+
+constexpr int SEG_BLOCK = 256;
+constexpr int SEG_ITEMS = 16;
+constexpr int SEG_TILE = SEG_BLOCK * SEG_ITEMS; // 4096 keys
+constexpr int SEG_SMALL = SEG_TILE;             // largest segment sorted in LDS
+
+struct SegPlan {
+    uint32_t num_tiles; // tiles of the large segments
+    uint32_t num_small; // small segments
+    uint32_t num_large; // large segments
+    uint32_t bad;       // segments with end < begin or outside [0, n): skipped
+    uint32_t pad[60];
+};
+
+// Ballot ranking of one tile held lane-striped in registers (wave w, item i,
+// lane l = key w*64*ITEMS + i*64 + l): on return every valid key has been
+// stored to s_keys at its stable in-tile position sorted by the digit at
+// `shift`, and s_start[d] holds the tile's first position of digit d.
+template <typename K, int BLOCK, int ITEMS>
+__device__ inline void tile_rank_stage(const K (&keys)[ITEMS], int wave_valid, int shift,
+                                       uint32_t (*s_cnt)[RADIX], uint32_t *s_scan, uint32_t *s_start, K *s_keys)
+{
+    constexpr int WAVES = BLOCK / WAVE;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    for (int q = tid; q < WAVES * RADIX; q += BLOCK) (&s_cnt[0][0])[q] = 0u;
+    __syncthreads();
+    uint32_t *cnt = s_cnt[wave];
+    uint32_t rank[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t d = KeyOps<K>::digit(keys[j], shift);
+        const bool valid = j * WAVE + lane < wave_valid;
+        const uint64_t peers = match_digit8(d) & __ballot(valid);
+        const uint32_t below = count_below(peers);
+        const uint32_t old = cnt[d];
+        if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
+        rank[j] = old + below;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    uint32_t tile_cnt = 0, tile_start = 0;
+    if (tid < RADIX) {
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = s_cnt[w][tid];
+            s_cnt[w][tid] = tile_cnt;
+            tile_cnt += c;
+        }
+        uint32_t xs = tile_cnt;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(xs, off);
+            if (lane >= off) xs += y;
+        }
+        if (lane == WAVE - 1) s_scan[wave] = xs;
+        tile_start = xs - tile_cnt;
+    }
+    __syncthreads();
+    if (tid < RADIX) {
+        for (int w = 0; w < wave; ++w) tile_start += s_scan[w];
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) s_cnt[w][tid] += tile_start;
+        s_start[tid] = tile_start;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+        if (j * WAVE + lane < wave_valid) s_keys[cnt[KeyOps<K>::digit(keys[j], shift)] + rank[j]] = keys[j];
+    __syncthreads();
+}
+
+// One block of 1024 threads.  begin has nseg + 1 entries.
+__global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ begin, int nseg, int64_t n,
+                                                   SegPlan *__restrict__ sp, uint32_t *__restrict__ seg_tile0,
+                                                   uint32_t *__restrict__ tile_seg, uint32_t *__restrict__ small_list,
+                                                   uint32_t *__restrict__ large_list)
+{
+    __shared__ uint32_t s_wsum[16];
+    __shared__ uint32_t s_carry[3];
+    __shared__ uint32_t s_bad;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    if (tid < 3) s_carry[tid] = 0u;
+    if (tid == 0) s_bad = 0u;
+    __syncthreads();
+    for (int base = 0; base < nseg; base += 1024) {
+        const int s = base + tid;
+        uint32_t tiles = 0, small = 0;
+        if (s < nseg) {
+            const int64_t b = begin[s], e = begin[s + 1];
+            if (b < 0 || e < b || e > n) {
+                atomicAdd(&s_bad, 1u);
+            } else {
+                const int64_t len = e - b;
+                if (len > SEG_SMALL) tiles = uint32_t((len + SEG_TILE - 1) / SEG_TILE);
+                else if (len > 1) small = 1;
+            }
+        }
+        const uint32_t large = tiles ? 1u : 0u;
+        // block-wide exclusive scans of (tiles, small, large), one at a time
+        uint32_t vals[3] = {tiles, small, large}, excl[3];
+        for (int k = 0; k < 3; ++k) {
+            uint32_t x = vals[k];
+#pragma unroll
+            for (int off = 1; off < WAVE; off <<= 1) {
+                const uint32_t y = __shfl_up(x, off);
+                if (lane >= off) x += y;
+            }
+            if (lane == WAVE - 1) s_wsum[wave] = x;
+            __syncthreads();
+            uint32_t add = s_carry[k];
+            for (int w = 0; w < wave; ++w) add += s_wsum[w];
+            excl[k] = add + x - vals[k];
+            __syncthreads();
+            if (tid == 1023) s_carry[k] = add + x;
+            __syncthreads();
+        }
+        if (s < nseg) {
+            seg_tile0[s] = excl[0];
+            for (uint32_t t = 0; t < tiles; ++t) tile_seg[excl[0] + t] = uint32_t(s);
+            if (small) small_list[excl[1]] = uint32_t(s);
+            if (large) large_list[excl[2]] = uint32_t(s);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        sp->num_tiles = s_carry[0];
+        sp->num_small = s_carry[1];
+        sp->num_large = s_carry[2];
+        sp->bad = s_bad;
+    }
+}
+
+// One workgroup per small segment (grid-stride): four LSD passes in LDS.
+// Digit positions where every key of the segment agrees are skipped.
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_small(uint32_t *__restrict__ keys,
+                                                         const int64_t *__restrict__ begin,
+                                                         const SegPlan *__restrict__ sp,
+                                                         const uint32_t *__restrict__ small_list)
+{
+    constexpr int WAVES = SEG_BLOCK / WAVE;
+    constexpr int WAVE_TILE = WAVE * SEG_ITEMS;
+    __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint32_t s_scan[RADIX / WAVE];
+    __shared__ uint32_t s_start[RADIX];
+    __shared__ uint32_t s_keys[SEG_TILE];
+    __shared__ uint32_t s_or[WAVES], s_nor[WAVES];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    const uint32_t nsmall = sp->num_small;
+    for (uint32_t k = blockIdx.x; k < nsmall; k += gridDim.x) {
+        const uint32_t s = small_list[k];
+        const int64_t b0 = begin[s];
+        const int len = int(begin[s + 1] - b0);
+        uint32_t *seg = keys + b0;
+        const int wave_valid = len > wave * WAVE_TILE ? min(WAVE_TILE, len - wave * WAVE_TILE) : 0;
+        uint32_t v[SEG_ITEMS];
+        uint32_t vor = 0, vnor = 0;
+#pragma unroll
+        for (int j = 0; j < SEG_ITEMS; ++j) {
+            const int o = j * WAVE + lane;
+            v[j] = o < wave_valid ? seg[wave * WAVE_TILE + o] : ~0u;
+            if (o < wave_valid) {
+                vor |= v[j];
+                vnor |= ~v[j];
+            }
+        }
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            vor |= __shfl_xor(vor, off);
+            vnor |= __shfl_xor(vnor, off);
+        }
+        if (lane == 0) {
+            s_or[wave] = vor;
+            s_nor[wave] = vnor;
+        }
+        __syncthreads();
+        uint32_t varying = 0, a = 0, c = 0;
+        for (int w = 0; w < WAVES; ++w) {
+            a |= s_or[w];
+            c |= s_nor[w];
+        }
+        varying = a & c;
+        bool staged = false; // keys currently in s_keys in sorted order
+        for (int pass = 0; pass < 4; ++pass) {
+            if (((varying >> (8 * pass)) & 0xFFu) == 0u) continue;
+            if (staged) {
+#pragma unroll
+                for (int j = 0; j < SEG_ITEMS; ++j) {
+                    const int o = j * WAVE + lane;
+                    v[j] = o < wave_valid ? s_keys[wave * WAVE_TILE + o] : ~0u;
+                }
+                __syncthreads();
+            }
+            tile_rank_stage<uint32_t, SEG_BLOCK, SEG_ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_scan, s_start, s_keys);
+            staged = true;
+        }
+        if (staged) {
+            for (int j = tid; j < len; j += SEG_BLOCK) seg[j] = s_keys[j];
+        }
+        __syncthreads();
+    }
+}
+
+// Grid-stride over the large segments' tiles: digit counts of pass `pass`.
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
+                                                          const uint32_t *__restrict__ keysB, int pass,
+                                                          const int64_t *__restrict__ begin,
+                                                          const SegPlan *__restrict__ sp,
+                                                          const uint32_t *__restrict__ seg_tile0,
+                                                          const uint32_t *__restrict__ tile_seg,
+                                                          uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t sh[RADIX];
+    const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const int shift = 8 * pass;
+    const uint32_t ntiles = sp->num_tiles;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t s = tile_seg[t];
+        const int64_t tb = begin[s] + int64_t(t - seg_tile0[s]) * SEG_TILE;
+        const int cnt = int(min(int64_t(SEG_TILE), begin[s + 1] - tb));
+        sh[tid] = 0u;
+        __syncthreads();
+        uint32_t k[SEG_ITEMS];
+#pragma unroll
+        for (int j = 0; j < SEG_ITEMS; ++j) {
+            const int o = j * SEG_BLOCK + tid;
+            k[j] = o < cnt ? kin[tb + o] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < SEG_ITEMS; ++j) {
+            const bool valid = j * SEG_BLOCK + tid < cnt;
+            const uint32_t d = (k[j] >> shift) & 0xFFu;
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            if (__all(valid && d == d0)) {
+                if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
+            } else if (valid) {
+                atomicAdd(&sh[d], 1u);
+            }
+        }
+        __syncthreads();
+        counts[size_t(t) * RADIX + tid] = sh[tid];
+        __syncthreads();
+    }
+}
+
+// Grid-stride over the large segments, one workgroup (thread = digit) each:
+// digit starts of the segment and the per-tile offsets, in place of counts.
+__global__ __launch_bounds__(RADIX) void k_seg_scan(const int64_t *__restrict__ begin,
+                                                    const SegPlan *__restrict__ sp,
+                                                    const uint32_t *__restrict__ seg_tile0,
+                                                    const uint32_t *__restrict__ large_list,
+                                                    uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t s_wsum[RADIX / WAVE];
+    const int d = threadIdx.x, lane = d & (WAVE - 1), wave = d >> 6;
+    const uint32_t nlarge = sp->num_large;
+    for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
+        const uint32_t s = large_list[k];
+        const uint32_t t0 = seg_tile0[s];
+        const uint32_t nt = uint32_t((begin[s + 1] - begin[s] + SEG_TILE - 1) / SEG_TILE);
+        uint32_t tot = 0;
+#pragma unroll 8
+        for (uint32_t t = 0; t < nt; ++t) tot += counts[size_t(t0 + t) * RADIX + d];
+        uint32_t x = tot;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == WAVE - 1) s_wsum[wave] = x;
+        __syncthreads();
+        uint32_t run = x - tot;
+        for (int w = 0; w < wave; ++w) run += s_wsum[w];
+        // offsets are positions within the segment (added to begin[s] at use)
+        for (uint32_t t = 0; t < nt; ++t) {
+            const size_t at = size_t(t0 + t) * RADIX + d;
+            const uint32_t c = counts[at];
+            counts[at] = run;
+            run += c;
+        }
+        __syncthreads();
+    }
+}
+
+// Scatter of pass `pass` for the large segments' tiles.  Workgroup b walks
+// the tiles of XCD (b % 8)'s contiguous share of [0, num_tiles) so that
+// neighbouring tiles (adjacent digit runs) meet in one L2.
+__global__ __launch_bounds__(SEG_BLOCK) void k_seg_scatter(uint32_t *__restrict__ keysA, uint32_t *__restrict__ keysB,
+                                                           int pass, const int64_t *__restrict__ begin,
+                                                           const SegPlan *__restrict__ sp,
+                                                           const uint32_t *__restrict__ seg_tile0,
+                                                           const uint32_t *__restrict__ tile_seg,
+                                                           const uint32_t *__restrict__ offsets)
+{
+    constexpr int WAVES = SEG_BLOCK / WAVE;
+    constexpr int WAVE_TILE = WAVE * SEG_ITEMS;
+    __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint32_t s_scan[RADIX / WAVE];
+    __shared__ uint32_t s_start[RADIX];
+    __shared__ uint32_t s_base[RADIX];
+    __shared__ uint32_t s_keys[SEG_TILE];
+    const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
+    uint32_t *__restrict__ kout = (pass & 1) ? keysA : keysB;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    const int shift = 8 * pass;
+    const uint32_t ntiles = sp->num_tiles;
+    const uint32_t x = blockIdx.x % NUM_XCD;
+    const uint32_t per = ntiles / NUM_XCD, rem = ntiles % NUM_XCD;
+    const uint32_t x_start = x * per + min(x, rem), x_count = per + (x < rem ? 1u : 0u);
+    const uint32_t stride = (gridDim.x - x + NUM_XCD - 1) / NUM_XCD;
+    for (uint32_t i = blockIdx.x / NUM_XCD; i < x_count; i += stride) {
+        const uint32_t t = x_start + i;
+        const uint32_t s = tile_seg[t];
+        const int64_t sb = begin[s];
+        const int64_t tb = sb + int64_t(t - seg_tile0[s]) * SEG_TILE;
+        const int cnt = int(min(int64_t(SEG_TILE), begin[s + 1] - tb));
+        const int wave_valid = cnt > wave * WAVE_TILE ? min(WAVE_TILE, cnt - wave * WAVE_TILE) : 0;
+        uint32_t v[SEG_ITEMS];
+#pragma unroll
+        for (int j = 0; j < SEG_ITEMS; ++j) {
+            const int o = j * WAVE + lane;
+            v[j] = o < wave_valid ? kin[tb + wave * WAVE_TILE + o] : ~0u;
+        }
+        const uint32_t off = offsets[size_t(t) * RADIX + (tid & (RADIX - 1))];
+        tile_rank_stage<uint32_t, SEG_BLOCK, SEG_ITEMS>(v, wave_valid, shift, s_cnt, s_scan, s_start, s_keys);
+        if (tid < RADIX) s_base[tid] = off - s_start[tid];
+        __syncthreads();
+        uint32_t *out = kout + sb;
+#pragma unroll
+        for (int j = 0; j < SEG_ITEMS; ++j) {
+            const int jj = j * SEG_BLOCK + tid;
+            if (jj < cnt) {
+                const uint32_t k = s_keys[jj];
+                out[s_base[(k >> shift) & 0xFFu] + uint32_t(jj)] = k;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+} // namespace hpcs

@@ -141,6 +141,13 @@ int32_t hpcs_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *a, int
  * HistogramByteComponentsPar(ulong[]) HistogramParallel.cs:545-555.  counts is int32[8][256]. */
 int32_t hpcs_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *a, int64_t l, int64_t r, int32_t *counts);
 
+/* A batch of uint32 segments, each sorted in place: segment s is
+ * keys[seg_begin[s], seg_begin[s+1]) (nseg + 1 non-decreasing offsets);
+ * keys outside every segment are untouched.  Replaces one call per segment of
+ * HPCsharpExperimental.Algorithm.SortRadixLsd(uint[], startIndex, length),
+ * RadixSortLsd.cs:1409-1457; segments must lie below 2^32 keys. */
+int32_t hpcs_sort_segments_u32(hpcs_ctx *ctx, uint32_t *keys, const int64_t *seg_begin, int32_t nseg);
+
 /* ---- device-resident, stream-ordered ------------------------------------ */
 
 /* The same sorts on device buffers (d_* are device pointers). */
@@ -152,6 +159,12 @@ int32_t hpcs_dev_sort_radix_pairs_u64_u32(hpcs_ctx *ctx, uint64_t *d_keys, uint3
 int32_t hpcs_dev_sort_radix_pairs_u32_u32(hpcs_ctx *ctx, uint32_t *d_keys, uint32_t *d_vals, int64_t n, void *stream);
 int32_t hpcs_dev_sort_radix_records(hpcs_ctx *ctx, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset,
                                     int32_t key_bits, int32_t key_kind, void *stream);
+/* Batched segments on the device: d_seg_begin is a device array of nseg + 1
+ * offsets into d_keys[0, n).  The launch sequence does not depend on the
+ * segment lengths (planning runs on the device), so after one warm-up call
+ * (which sizes the scratch) the call can be captured in a HIP graph. */
+int32_t hpcs_dev_sort_segments_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, const int64_t *d_seg_begin,
+                                   int32_t nseg, void *stream);
 /* Byte-component histogram of d_keys[0, n): d_counts is uint32[4][256]
  * (u32) or uint32[8][256] (u64) on the device, overwritten. */
 int32_t hpcs_dev_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n, uint32_t *d_counts,

@@ -352,3 +352,83 @@ def test_both_pass_algorithms_all_types(oracle):
                 assert np.array_equal(b, rb), dist
         finally:
             N.destroy_context(ctx)
+
+
+def _segments(nseg, seed, lo, hi, gaps=False):
+    """Log-uniform lengths with some empty and single-key segments; offsets
+    are the running sum (segments tile [0, n))."""
+    lens = datagen.segment_lengths(nseg, seed, lo, hi)
+    rng = np.random.default_rng(seed)
+    lens[rng.integers(0, nseg, size=max(1, nseg // 20))] = 0          # empty segments
+    lens[rng.integers(0, nseg, size=max(1, nseg // 20))] = 1          # single keys
+    begin = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    return begin, lens
+
+
+@pytest.mark.parametrize("nseg,lo,hi", [(300, 16, 1 << 13), (40, 4000, 1 << 20), (7, 1, 9000)])
+def test_segments_host(oracle, nseg, lo, hi):
+    begin, lens = _segments(nseg, 1000 + nseg, lo, hi, gaps=False)
+    n = int(begin[-1])
+    a = datagen.keys_u32(n + 5, 77 + nseg)
+    ref = a.copy()
+    oracle.sort_segments_u32(ref, begin)
+    N.check(N.lib().hpcs_sort_segments_u32(N.context(), a.ctypes.data_as(ctypes.c_void_p),
+                                           begin.ctypes.data_as(ctypes.c_void_p), nseg))
+    assert np.array_equal(a, ref)
+
+
+def test_segments_device_with_gaps_and_distributions(oracle):
+    import torch
+    _, lens = _segments(500, 4242, 2, 1 << 15, gaps=False)
+    # three untouched keys on either side of the batch
+    begin = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64) + 3
+    n = int(begin[-1]) + 3
+    a = datagen.keys_u32(n, 99)
+    a[: n // 3] &= np.uint32(0xFF)               # low-8 keys in the first third
+    a[n // 3: n // 2] = np.arange(n // 2 - n // 3, dtype=np.uint32)   # sorted run
+    ref = a.copy()
+    oracle.sort_segments_u32(ref, begin)
+    t = torch.from_numpy(a.view(np.int32)).cuda()
+    b = torch.from_numpy(begin).cuda()
+    st = torch.cuda.current_stream().cuda_stream
+    N.check(N.lib().hpcs_dev_sort_segments_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n,
+                                               ctypes.c_void_p(b.data_ptr()), len(lens), ctypes.c_void_p(st)))
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy().view(np.uint32), ref)
+
+
+def test_segments_graph_replays_match_eager(oracle):
+    """The batched call captured once in a HIP graph: every replay (with eager
+    work between replays) equals the eager result."""
+    import torch
+    lens = datagen.segment_lengths(256, 5, 16, 1 << 17)
+    begin = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n = int(begin[-1])
+    a = datagen.keys_u32(n, 8)
+    ref = a.copy()
+    oracle.sort_segments_u32(ref, begin)
+    src = torch.from_numpy(a.view(np.int32)).cuda()
+    work = src.clone()
+    b = torch.from_numpy(begin).cuda()
+    ctx = N.context()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        N.check(N.lib().hpcs_dev_sort_segments_u32(ctx, ctypes.c_void_p(work.data_ptr()), n,
+                                                   ctypes.c_void_p(b.data_ptr()), len(lens),
+                                                   ctypes.c_void_p(s.cuda_stream)))   # warm-up sizes scratch
+    s.synchronize()
+    assert np.array_equal(work.cpu().numpy().view(np.uint32), ref)
+    g = torch.cuda.CUDAGraph()
+    work.copy_(src)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        N.check(N.lib().hpcs_dev_sort_segments_u32(ctx, ctypes.c_void_p(work.data_ptr()), n,
+                                                   ctypes.c_void_p(b.data_ptr()), len(lens),
+                                                   ctypes.c_void_p(s.cuda_stream)))
+    for rep in range(3):
+        work.copy_(src)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(work.cpu().numpy().view(np.uint32), ref), rep
+        _ = (src[:1000].to(torch.int64) * 3).sum().item()    # eager work between replays
