@@ -73,7 +73,7 @@ struct hpcs_ctx {
     void *io = nullptr;
     size_t io_cap = 0;
     int rank_mode = RANK_LDSOR;  // tuning knob, env HPCS_RANK=ballot|ldsor
-    int tile_shape = 1;          // tuning knob for u32 keys, env HPCS_TILE_SHAPE=0..3
+    int tile_shape = 0;          // tuning knob for u32 keys, env HPCS_TILE_SHAPE=0..3
     int fallback_rounds = FALLBACK_ROUNDS;
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
     int pass_algo = PASS_RTS;     // PASS_ONESWEEP or PASS_RTS (reduce-then-scan)
@@ -126,9 +126,11 @@ void *ensure(void *&buf, size_t &cap, size_t need)
 //   [hist P x 256 u32 | DevPlan | look-back region 0]  zero-filled first
 //   [look-back region 1] [keys B] [values B]
 struct Layout {
-    size_t off_hist, off_plan, off_lb0, zero_bytes, off_lb1, off_keysB, off_valsB, total;
+    size_t off_hist, off_plan, off_dtot, off_lb0, zero_bytes, off_lb1, off_keysB, off_valsB, total;
     int64_t tiles;
 };
+
+static_assert(ORSLOTS * 128 <= MAX_PASSES * RADIX * sizeof(uint32_t), "OR slots live in the histogram area");
 
 Layout make_layout(int64_t n, size_t key_size, bool vals, int tile_keys = SORT_TILE)
 {
@@ -137,7 +139,8 @@ Layout make_layout(int64_t n, size_t key_size, bool vals, int tile_keys = SORT_T
     const size_t lb_bytes = size_t(L.tiles) * RADIX * sizeof(uint32_t);
     L.off_hist = 0;
     L.off_plan = align_up(MAX_PASSES * RADIX * sizeof(uint32_t));
-    L.off_lb0 = align_up(L.off_plan + sizeof(DevPlan));
+    L.off_dtot = align_up(L.off_plan + sizeof(DevPlan));
+    L.off_lb0 = align_up(L.off_dtot + size_t(MAX_PASSES) * RADIX * LINE_WORDS * sizeof(uint32_t));
     L.zero_bytes = L.off_lb0 + lb_bytes;
     L.off_lb1 = align_up(L.zero_bytes);
     L.off_keysB = align_up(L.off_lb1 + lb_bytes);
@@ -277,19 +280,26 @@ void launch_onesweep(int rank_mode, int shape, int window, int fb, dim3 grid, hi
                                                         lb, next);
 }
 
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
 template <typename K, bool FLOATK>
 void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const K *b, uint32_t n, int pass,
                         const DevPlan *plan, uint32_t *counts)
 {
     const int block = TILE_SHAPES[shape].block, items = TILE_SHAPES[shape].items;
+    const int vec = aligned16(a) && aligned16(b);
     if (block == 256 && items == 16)
-        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts);
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
+                           vec);
     else if (block == 512 && items == 16)
-        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts);
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts,
+                           vec);
     else if (block == 256 && items == 32)
-        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts);
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
+                           vec);
     else
-        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts);
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts,
+                           vec);
 }
 
 template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int RANK>
@@ -343,14 +353,15 @@ template <typename K, bool FLOATK>
 void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n, uint32_t *counts, K *orslots)
 {
     const int block = TILE_SHAPES[shape].block, items = TILE_SHAPES[shape].items;
+    const int vec = aligned16(a);
     if (block == 256 && items == 16)
-        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, n, counts, orslots);
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, n, counts, orslots, vec);
     else if (block == 512 && items == 16)
-        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, n, counts, orslots);
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, n, counts, orslots, vec);
     else if (block == 256 && items == 32)
-        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, n, counts, orslots);
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, n, counts, orslots, vec);
     else
-        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, n, counts, orslots);
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, n, counts, orslots, vec);
 }
 
 // The device-resident sort: keys (and optional uint32 values) in place.
@@ -371,6 +382,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     uint32_t *hist = reinterpret_cast<uint32_t *>(base + L.off_hist);
     DevPlan *plan = reinterpret_cast<DevPlan *>(base + L.off_plan);
     uint32_t *lb[2] = {reinterpret_cast<uint32_t *>(base + L.off_lb0), reinterpret_cast<uint32_t *>(base + L.off_lb1)};
+    uint32_t *dtot = reinterpret_cast<uint32_t *>(base + L.off_dtot); // per-pass digit totals, zero-filled
     K *keysB = reinterpret_cast<K *>(base + L.off_keysB);
     uint32_t *valsB = HAS_VALS ? reinterpret_cast<uint32_t *>(base + L.off_valsB) : nullptr;
 
@@ -404,8 +416,10 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
                                                   counts);
                 });
             timed(c, st, "scan", [&] {
-                hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals, p, plan);
-                hipLaunchKernelGGL(k_scan_ranges, dim3(1), dim3(4 * RADIX), 0, st, totals, nranges, p, plan);
+                uint32_t *dtot_p = dtot + size_t(p) * RADIX * LINE_WORDS;
+                hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals, dtot_p,
+                                   p, plan);
+                hipLaunchKernelGGL(k_scan_columns, dim3(RADIX), dim3(1024), 0, st, totals, nranges, dtot_p, p, plan);
                 hipLaunchKernelGGL(k_range_offsets, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals, p,
                                    plan);
             });

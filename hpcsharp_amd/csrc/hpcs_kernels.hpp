@@ -52,8 +52,18 @@ struct DevPlan {
     uint32_t xin[MAX_PASSES];           // apply the float key transform on load
     uint32_t xout[MAX_PASSES];          // apply the inverse transform on store
     uint32_t copyback;                  // final data sits in buffer B
-    uint32_t pad[55];
+    uint32_t flags[MAX_PASSES];         // the four fields above packed, PF_*
+    uint32_t pad[47];
 };
+constexpr uint32_t PF_TRIVIAL = 1u, PF_SRC_B = 2u, PF_XIN = 4u, PF_XOUT = 8u;
+
+// Pack the per-pass fields into flags[] (one thread, after they are set).
+__device__ inline void pack_plan_flags(DevPlan *plan, int P)
+{
+    for (int p = 0; p < P; ++p)
+        plan->flags[p] = (plan->trivial[p] ? PF_TRIVIAL : 0u) | (plan->src[p] ? PF_SRC_B : 0u) |
+                         (plan->xin[p] ? PF_XIN : 0u) | (plan->xout[p] ? PF_XOUT : 0u);
+}
 
 // ---------------------------------------------------------------------------
 // Key traits: the float/double radix key of RadixSortMsd.cs:1379-1383 and
@@ -240,6 +250,7 @@ __global__ __launch_bounds__(RADIX) void k_plan(const uint32_t *__restrict__ his
             plan->xout[p] = (floatk && p == last) ? 1u : 0u;
         }
         plan->copyback = cur;
+        pack_plan_flags(plan, P);
     }
 }
 
@@ -317,21 +328,19 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     // exactly one block per tile); that region's previous pass has finished.
     if (lb_next != nullptr && tid < RADIX) lb_next[size_t(blockIdx.x) * RADIX + tid] = 0u;
 
-    if (plan->trivial[pass]) return;
+    // one packed word of per-pass flags (one dependent load before the keys)
+    const uint32_t flags = plan->flags[pass];
+    if (flags & PF_TRIVIAL) return;
 
-    const uint32_t src = plan->src[pass];
+    const uint32_t src = flags & PF_SRC_B;
     const K *__restrict__ kin = src ? keysB : keysA;
     K *__restrict__ kout = src ? keysA : keysB;
     const uint32_t *__restrict__ vin = src ? valsB : valsA;
     uint32_t *__restrict__ vout = src ? valsA : valsB;
-    const bool xin = FLOATK && plan->xin[pass];
-    const bool xout = FLOATK && plan->xout[pass];
+    const bool xin = FLOATK && (flags & PF_XIN);
+    const bool xout = FLOATK && (flags & PF_XOUT);
     const int shift = pass * RADIX_BITS;
 
-    for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
-    if constexpr (RANK == RANK_LDSOR)
-        for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_match[0][0])[i] = 0ull;
-    __syncthreads();
     // With precomputed offsets the tile order is free: give each XCD (work-
     // groups are dealt to the 8 XCDs round-robin) a contiguous range of tiles,
     // so the partial cache lines at the ends of neighbouring tiles' digit runs
@@ -343,17 +352,25 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     // keys of this wave that exist: items with i*64 + lane < wave_valid
     const int wave_valid = n > wave_base ? int(min(uint32_t(WAVE_TILE), n - wave_base)) : 0;
 
-    // ---- load ----
+    // ---- load (issued before the LDS set-up so their latency overlaps it) ----
     K keys[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const int off = i * WAVE + lane;
-        if (off < wave_valid) {
-            K k = kin[wave_base + off];
-            keys[i] = xin ? KeyOps<K>::to_key(k) : k;
-        } else {
-            keys[i] = ~K(0);
-        }
+        keys[i] = off < wave_valid ? kin[wave_base + off] : ~K(0);
+    }
+    uint32_t pre_off = 0;
+    if constexpr (PRESCANNED)
+        if (tid < RADIX) pre_off = lb_cur[size_t(tile) * RADIX + tid];
+
+    for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
+    if constexpr (RANK == RANK_LDSOR)
+        for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_match[0][0])[i] = 0ull;
+    __syncthreads();
+    if (xin) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+            if (i * WAVE + lane < wave_valid) keys[i] = KeyOps<K>::to_key(keys[i]);
     }
 
     // ---- 1. per-wave digit counts ----
@@ -458,7 +475,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     // unpublished word the thread polls only that word (with a pause) until it
     // is published, so waiting costs one load per round, not a window.
     if constexpr (PRESCANNED) {
-        if (tid < RADIX) s_base[tid] = lb_cur[size_t(tile) * RADIX + tid] - tile_start;
+        if (tid < RADIX) s_base[tid] = pre_off - tile_start;
     } else {
         uint32_t excl = 0;
         int64_t p = int64_t(tile) - 1; // newest earlier tile not yet added
@@ -768,16 +785,47 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(K *__restrict__ keysA, K *__r
 // ---------------------------------------------------------------------------
 constexpr int SCAN_RANGE = 64;
 
+// Keys of one tile for counting (order does not matter): with `vec` (the
+// buffer is 16-byte aligned) a full tile is read in 16-byte vectors, thread t
+// taking vectors t, t + BLOCK, ...; key slot i*BLOCK + t of the scalar form
+// is replaced by a permutation of the same keys.  Missing keys read as 0 and
+// are masked by the caller (slot index >= cnt).
+template <typename K, int BLOCK, int ITEMS>
+__device__ inline void load_tile_keys(const K *__restrict__ p, uint32_t cnt, bool vec, K (&k)[ITEMS])
+{
+    constexpr int VK = 16 / int(sizeof(K));
+    constexpr int NV = ITEMS / VK;
+    using V = typename Vec16<K>::type;
+    const int tid = threadIdx.x;
+    if (vec && cnt == uint32_t(BLOCK * ITEMS)) {
+        const V *pv = reinterpret_cast<const V *>(p);
+        V v[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = pv[i * BLOCK + tid];
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int e = 0; e < VK; ++e) k[i * VK + e] = v[i][e];
+    } else {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint32_t j = uint32_t(i * BLOCK + tid);
+            k[i] = j < cnt ? p[j] : K(0);
+        }
+    }
+}
+
 template <typename K, bool FLOATK, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ keysA, const K *__restrict__ keysB,
                                                        uint32_t n, int pass, const DevPlan *__restrict__ plan,
-                                                       uint32_t *__restrict__ counts)
+                                                       uint32_t *__restrict__ counts, int vec)
 {
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ uint32_t sh[RADIX];
-    if (plan->trivial[pass]) return;
-    const K *__restrict__ kin = plan->src[pass] ? keysB : keysA;
-    const bool xin = FLOATK && plan->xin[pass];
+    const uint32_t flags = plan->flags[pass];
+    if (flags & PF_TRIVIAL) return;
+    const K *__restrict__ kin = (flags & PF_SRC_B) ? keysB : keysA;
+    const bool xin = FLOATK && (flags & PF_XIN);
     const int shift = pass * RADIX_BITS;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
     for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
@@ -785,11 +833,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
     const uint32_t base = blockIdx.x * uint32_t(TILE);
     const uint32_t cnt = min(uint32_t(TILE), n - base);
     K k[ITEMS];
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const uint32_t j = uint32_t(i * BLOCK + tid);
-        k[i] = j < cnt ? kin[base + j] : K(0);
-    }
+    load_tile_keys<K, BLOCK, ITEMS>(kin + base, cnt, vec, k);
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t j = uint32_t(i * BLOCK + tid);
@@ -806,79 +850,89 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
     for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
 }
 
+// Spacing of per-digit accumulators that take atomics from many workgroups:
+// one 128-byte line each, so the atomics do not queue behind each other.
+constexpr int LINE_WORDS = 32;
+
+// totals[r][d] = sum of counts[t][d] over the tiles of range r, and the
+// pass's digit totals dtot[d * LINE_WORDS] += totals[r][d].
 __global__ __launch_bounds__(RADIX) void k_range_totals(const uint32_t *__restrict__ counts, uint32_t tiles,
-                                                        uint32_t *__restrict__ totals, int pass,
-                                                        const DevPlan *__restrict__ plan)
+                                                        uint32_t *__restrict__ totals, uint32_t *__restrict__ dtot,
+                                                        int pass, const DevPlan *__restrict__ plan)
 {
-    if (plan->trivial[pass]) return;
+    if (plan->flags[pass] & PF_TRIVIAL) return;
     const uint32_t r = blockIdx.x, d = threadIdx.x;
     const uint32_t t0 = r * SCAN_RANGE, t1 = min(tiles, t0 + SCAN_RANGE);
+    uint32_t c[SCAN_RANGE];
+#pragma unroll
+    for (int k = 0; k < SCAN_RANGE; ++k) c[k] = t0 + k < t1 ? counts[size_t(t0 + k) * RADIX + d] : 0u;
     uint32_t sum = 0;
-#pragma unroll 16
-    for (uint32_t t = t0; t < t1; ++t) sum += counts[size_t(t) * RADIX + d];
+#pragma unroll
+    for (int k = 0; k < SCAN_RANGE; ++k) sum += c[k];
     totals[size_t(r) * RADIX + d] = sum;
+    if (sum) atomicAdd(&dtot[d * LINE_WORDS], sum);
 }
 
-// One block of 4 x RADIX threads: four threads per digit split the ranges.
-// The range sums add up to the pass's digit histogram; its exclusive scan
-// over digits is the digit's global start (ComputeStartOfBins,
-// RadixSortLsd.cs:112-125), recorded in the plan and used as the seed.
-__global__ __launch_bounds__(4 * RADIX) void k_scan_ranges(uint32_t *__restrict__ totals, uint32_t nranges, int pass,
-                                                           DevPlan *__restrict__ plan)
+// One workgroup per digit d: the digit's global start (ComputeStartOfBins,
+// RadixSortLsd.cs:112-125: the sum of the smaller digits' totals), then the
+// exclusive scan of column d of the range totals, seeded with that start.
+__global__ __launch_bounds__(1024) void k_scan_columns(uint32_t *__restrict__ totals, uint32_t nranges,
+                                                       const uint32_t *__restrict__ dtot, int pass,
+                                                       DevPlan *__restrict__ plan)
 {
-    __shared__ uint32_t s_part[4][RADIX];
-    __shared__ uint32_t s_start[RADIX];
-    __shared__ uint32_t s_wsum[RADIX / WAVE];
-    if (plan->trivial[pass]) return;
-    const uint32_t d = threadIdx.x & (RADIX - 1), q = threadIdx.x / RADIX;
-    const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-    const uint32_t per = (nranges + 3) / 4;
-    const uint32_t r0 = min(nranges, q * per), r1 = min(nranges, r0 + per);
-    uint32_t sum = 0;
-#pragma unroll 8
-    for (uint32_t r = r0; r < r1; ++r) sum += totals[size_t(r) * RADIX + d];
-    s_part[q][d] = sum;
-    __syncthreads();
-    if (q == 0) {
-        const uint32_t tot = s_part[0][d] + s_part[1][d] + s_part[2][d] + s_part[3][d];
-        uint32_t x = tot;
+    __shared__ uint32_t s_w[1024 / WAVE];
+    if (plan->flags[pass] & PF_TRIVIAL) return;
+    const uint32_t d = blockIdx.x, tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    // global start of digit d
+    uint32_t v = (tid < d) ? dtot[tid * LINE_WORDS] : 0u;
 #pragma unroll
-        for (int off = 1; off < WAVE; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off);
-            if (lane >= uint32_t(off)) x += y;
-        }
-        if (lane == WAVE - 1) s_wsum[wave] = x;
-        s_start[d] = x - tot;
-    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) s_w[wave] = v;
     __syncthreads();
-    if (q == 0) {
-        uint32_t add = 0;
-        for (uint32_t w = 0; w < wave; ++w) add += s_wsum[w];
-        s_start[d] += add;
-        plan->gstart[pass][d] = s_start[d];
-    }
+    uint32_t start = 0;
+    for (int w = 0; w < 1024 / WAVE; ++w) start += s_w[w];
+    if (tid == 0) plan->gstart[pass][d] = start;
     __syncthreads();
-    uint32_t run = s_start[d];
-    for (uint32_t k = 0; k < q; ++k) run += s_part[k][d];
+    // column scan: thread tid owns ranges [tid*per, (tid+1)*per)
+    const uint32_t per = (nranges + 1023) / 1024;
+    const uint32_t r0 = min(nranges, tid * per), r1 = min(nranges, r0 + per);
+    uint32_t local = 0;
+    for (uint32_t r = r0; r < r1; ++r) local += totals[size_t(r) * RADIX + d];
+    uint32_t x = local;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= uint32_t(off)) x += y;
+    }
+    if (lane == WAVE - 1) s_w[wave] = x;
+    __syncthreads();
+    uint32_t run = start + x - local;
+    for (uint32_t w = 0; w < wave; ++w) run += s_w[w];
     for (uint32_t r = r0; r < r1; ++r) {
-        const uint32_t c = totals[size_t(r) * RADIX + d];
-        totals[size_t(r) * RADIX + d] = run;
+        const size_t at = size_t(r) * RADIX + d;
+        const uint32_t c = totals[at];
+        totals[at] = run;
         run += c;
     }
 }
 
+// Per range r (thread = digit): the tiles' global digit offsets, written over
+// their counts.
 __global__ __launch_bounds__(RADIX) void k_range_offsets(uint32_t *__restrict__ counts, uint32_t tiles,
                                                          const uint32_t *__restrict__ totals, int pass,
                                                          const DevPlan *__restrict__ plan)
 {
-    if (plan->trivial[pass]) return;
+    if (plan->flags[pass] & PF_TRIVIAL) return;
     const uint32_t r = blockIdx.x, d = threadIdx.x;
     const uint32_t t0 = r * SCAN_RANGE, t1 = min(tiles, t0 + SCAN_RANGE);
+    uint32_t c[SCAN_RANGE];
+#pragma unroll
+    for (int k = 0; k < SCAN_RANGE; ++k) c[k] = t0 + k < t1 ? counts[size_t(t0 + k) * RADIX + d] : 0u;
     uint32_t run = totals[size_t(r) * RADIX + d];
-    for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t c = counts[size_t(t) * RADIX + d];
-        counts[size_t(t) * RADIX + d] = run;
-        run += c;
+#pragma unroll
+    for (int k = 0; k < SCAN_RANGE; ++k) {
+        if (t0 + k < t1) counts[size_t(t0 + k) * RADIX + d] = run;
+        run += c[k];
     }
 }
 
@@ -891,10 +945,11 @@ __global__ __launch_bounds__(RADIX) void k_range_offsets(uint32_t *__restrict__ 
 // ORSLOTS word pairs (by block) to keep atomics on one address few.
 // ---------------------------------------------------------------------------
 constexpr int ORSLOTS = 64;
+template <typename K> constexpr int OR_STRIDE = 128 / int(sizeof(K)); // one 128-byte line per slot
 
 template <typename K, bool FLOATK, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, uint32_t n,
-                                                   uint32_t *__restrict__ counts, K *__restrict__ orslots)
+                                                   uint32_t *__restrict__ counts, K *__restrict__ orslots, int vec)
 {
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int WAVES = BLOCK / WAVE;
@@ -906,11 +961,7 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
     const uint32_t base = blockIdx.x * uint32_t(TILE);
     const uint32_t cnt = min(uint32_t(TILE), n - base);
     K k[ITEMS];
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const uint32_t j = uint32_t(i * BLOCK + tid);
-        k[i] = j < cnt ? keys[base + j] : K(0);
-    }
+    load_tile_keys<K, BLOCK, ITEMS>(keys + base, cnt, vec, k);
     K vor = 0, vnor = 0;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
@@ -947,8 +998,8 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
             b |= s_nor[w];
         }
         const int slot = blockIdx.x % ORSLOTS;
-        atomicOr(&orslots[2 * slot], a);
-        atomicOr(&orslots[2 * slot + 1], b);
+        atomicOr(&orslots[slot * OR_STRIDE<K>], a);
+        atomicOr(&orslots[slot * OR_STRIDE<K> + 1], b);
     }
 }
 
@@ -962,8 +1013,8 @@ __global__ void k_plan_rts(const K *__restrict__ orslots, DevPlan *__restrict__ 
     constexpr int P = int(sizeof(K));
     K a = 0, b = 0;
     for (int s = 0; s < ORSLOTS; ++s) {
-        a |= orslots[2 * s];
-        b |= orslots[2 * s + 1];
+        a |= orslots[s * OR_STRIDE<K>];
+        b |= orslots[s * OR_STRIDE<K> + 1];
     }
     const K varying = a & b;
     uint32_t cur = 0;
@@ -983,6 +1034,7 @@ __global__ void k_plan_rts(const K *__restrict__ orslots, DevPlan *__restrict__ 
         plan->xout[p] = (floatk && p == last) ? 1u : 0u;
     }
     plan->copyback = cur;
+    pack_plan_flags(plan, P);
 }
 
 // ---------------------------------------------------------------------------

@@ -52,7 +52,7 @@ extern "C" {
 
 /* Tuning options for hpcs_ctx_set_option. */
 #define HPCS_OPT_RANK 1       /* 0: ballot match, 1: LDS-OR match (default) */
-#define HPCS_OPT_TILE_SHAPE 2 /* u32 keys: 0: 256x16, 1: 512x16 (default), 2: 256x32, 3: 512x32
+#define HPCS_OPT_TILE_SHAPE 2 /* u32 keys: 0: 256x16 (default), 1: 512x16, 2: 256x32, 3: 512x32
                                  (threads x keys per thread); other key types use 256x16 */
 #define HPCS_OPT_FALLBACK_ROUNDS 3 /* look-back rounds on an unpublished tile before counting it locally
                                       (default 24; 0 makes every wait use the local count, for tests) */

@@ -432,3 +432,18 @@ def test_segments_graph_replays_match_eager(oracle):
         torch.cuda.synchronize()
         assert np.array_equal(work.cpu().numpy().view(np.uint32), ref), rep
         _ = (src[:1000].to(torch.int64) * 3).sum().item()    # eager work between replays
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+def test_device_unaligned_subarray(oracle, offset):
+    """Device calls on a pointer that is not 16-byte aligned (a sub-array)."""
+    import torch
+    n = (1 << 20) + 77
+    a = datagen.keys_u32(n + 8, 300 + offset)
+    ref = a.copy()
+    oracle.sort_radix_lsd(ref, offset, n)
+    t = torch.from_numpy(a.view(np.int32)).cuda()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    N.check(N.lib().hpcs_dev_sort_radix_u32(N.context(), ctypes.c_void_p(t.data_ptr() + 4 * offset), n, st))
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy().view(np.uint32), ref)
