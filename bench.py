@@ -1,14 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark: stable LSD radix sort of 2^28 uint32 keys on one MI355X.
 
-One step = one full device-resident sort (histogram, plan, four onesweep
-passes) of a distinct copy of the same 2^28-key array (uniform 32-bit keys
-from splitmix64, seed 0x5EED0002), already resident in HBM.  W untimed
+One step = one full device-resident sort (upsweep, plan, then per 8-bit
+digit pass: tile counts, scan, scatter) of a distinct copy of the same
+2^28-key array (uniform 32-bit keys from splitmix64, seed 0x5EED0002),
+already resident in HBM.  W untimed
 warm-up sorts, then exactly K timed sorts bracketed by device
 synchronisation; the output of the last step is checked against an
 independent sort before anything is printed.
 
-Prints one JSON line: Gkeys/s, the onesweep kernel's roofline (algorithmic
+Prints one JSON line: Gkeys/s, the pass scatter kernel's roofline (algorithmic
 bytes = 8 B per key per pass, from HIP events on the launch stream) and the
 CPU baseline (the C restatement of ParallelAlgorithm.SortRadixPar, timed on
 this host's cores).  Per-distribution figures go to stderr and to the JSON
@@ -198,6 +199,15 @@ def main():
     per_key = {"scatter": 8, "onesweep": 8, "tile_counts": 4, "upsweep": 4, "histogram": 4}
     kernel_gbs = {k: round(per_key[k] * n / (v[1] / v[0] * 1e-3) / 1e9, 1) for k, v in stats.items() if k in per_key}
 
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes of this command (calibrated against a
+    # 1 GiB copy); null when no profile of this workload is present.
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "sort_u32_2p28_uniform_traffic.json")
+    if n == N_DEFAULT and os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = int(json.load(f)["traffic_bytes_per_launch"])
+
     extra = {"sort_ms": round(t_step * 1e3, 4),
              "sort_ideal_bytes": int(info.ideal_bytes),
              "sort_roofline_frac": round(sort_gbs / HBM_PEAK_GBS, 4),
@@ -249,7 +259,8 @@ def main():
         "config": {"workload": f"plain single-array stable LSD radix sort, uint32[2^{n.bit_length() - 1}] uniform",
                    "n": n, "digit_bits": info.digit_bits, "passes": info.passes, "tile_keys": info.tile_keys},
         "roofline": {"bound": "hbm", "achieved": round(d_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(d_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(d_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": d_bytes,
                      "kernel": f"k_onesweep as {dom} (one 8-bit pass: read + write 4 B per key)"},
         "cpu_baseline": cpu,
         "extra": extra,
