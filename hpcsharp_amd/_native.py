@@ -78,6 +78,15 @@ SIGNATURES = {
     "hpcs_dev_sort_radix_records": ([_p, _p, _i64, _i32, _i32, _i32, _i32, _p], _i32),
     "hpcs_dev_histogram_byte_components_u32": ([_p, _p, _i64, _p, _p], _i32),
     "hpcs_dev_histogram_byte_components_u64": ([_p, _p, _i64, _p, _p], _i32),
+    "hpcs_select_topk_u32": ([_p, _p, _i64, _i64, _i64], _i32),
+    "hpcs_topk_u32": ([_p, _p, _i64, _i64, _p], _i32),
+    "hpcs_topk_u64": ([_p, _p, _i64, _i64, _p], _i32),
+    "hpcs_topk_f32": ([_p, _p, _i64, _i64, _p], _i32),
+    "hpcs_topk_f64": ([_p, _p, _i64, _i64, _p], _i32),
+    "hpcs_dev_topk_u32": ([_p, _p, _i64, _i64, _p, _p], _i32),
+    "hpcs_dev_topk_u64": ([_p, _p, _i64, _i64, _p, _p], _i32),
+    "hpcs_dev_topk_f32": ([_p, _p, _i64, _i64, _p, _p], _i32),
+    "hpcs_dev_topk_f64": ([_p, _p, _i64, _i64, _p, _p], _i32),
 }
 
 _lib = None

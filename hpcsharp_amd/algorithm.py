@@ -185,6 +185,48 @@ class Algorithm:
         return out
 
 
+    # -- radix top-k -------------------------------------------------------------
+    @staticmethod
+    def SelectTopK(arrayToBeSelected: np.ndarray, start: int, length: int | None = None, k: int | None = None) -> None:
+        """SelectTopK(this uint[] a, int k) SelectionTopKRadix.cs:160-171 and
+        SelectTopK(this uint[] a, int start, int length, int k) :142-153.
+
+        Called as SelectTopK(a, k) or SelectTopK(a, start, length, k).  k is an
+        index: afterwards a[k:start+length] holds the largest keys of the range in
+        ascending order (a[k] is the k-th smallest) and a[start:k] the others."""
+        if arrayToBeSelected is None:
+            raise ArgumentNullException("arrayToBeSelected")
+        _require(arrayToBeSelected, "arrayToBeSelected", (np.uint32,))
+        if length is None and k is None:
+            start, length, k = 0, arrayToBeSelected.size, start
+            if length <= 0:
+                raise ArgumentOutOfRangeException("array length is invalid")
+        if start < 0 or length is None or length <= 0 or start + length > arrayToBeSelected.size:
+            raise ArgumentOutOfRangeException("start or length are invalid")
+        if k is None or k < start or k >= start + length:
+            raise ArgumentOutOfRangeException("k must be between start and (start + length)")
+        _call(N.lib().hpcs_select_topk_u32, _ptr(arrayToBeSelected), start, length, k)
+
+    @staticmethod
+    def TopK(inputArray: np.ndarray, k: int) -> np.ndarray:
+        """The k largest keys of a uint[]/ulong[]/float[]/double[] array in
+        descending order (the radix order of the sort: -0.0 below +0.0); the
+        input is not modified."""
+        _require(inputArray, "inputArray", tuple(_TOPK))
+        if k < 0 or k > inputArray.size:
+            raise ArgumentOutOfRangeException("k must lie in [0, n]")
+        out = np.empty(k, dtype=inputArray.dtype)
+        _call(getattr(N.lib(), _TOPK[inputArray.dtype]), _ptr(inputArray), inputArray.size, k, _ptr(out))
+        return out
+
+
+_TOPK = {
+    np.dtype(np.uint32): "hpcs_topk_u32",
+    np.dtype(np.uint64): "hpcs_topk_u64",
+    np.dtype(np.float32): "hpcs_topk_f32",
+    np.dtype(np.float64): "hpcs_topk_f64",
+}
+
 class ParallelAlgorithm:
     """Multicore-surface methods (namespace HPCsharp, class ParallelAlgorithm)."""
 

@@ -6,6 +6,7 @@
 // device-resident call, so the sequence can be captured in a HIP graph.
 #include "hpcs_kernels.hpp"
 #include "hpcs_segments.hpp"
+#include "hpcs_topk.hpp"
 #include "../../include/hpcsort.h"
 
 #include <hip/hip_runtime.h>
@@ -516,6 +517,123 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     }
 }
 
+// Radix top-k: the m largest keys of d_in[0, n) into d_out[0, m), in
+// descending order (desc) or ascending; with d_rest, the other n - m keys go
+// there in no particular order.  Scratch: [TopkState | cand 0 | cand 1].
+template <typename K, bool FLOATK>
+void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_rest, bool desc, hipStream_t st)
+{
+    using LV = TopkLevels<K>;
+    if (n < 0 || m < 0 || m > n) fail(HPCS_E_INVALID_ARG, "k must lie in [0, n]");
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    if (n > 0 && !d_in) fail(HPCS_E_INVALID_ARG, "keys is null");
+    if (m > 0 && !d_out) fail(HPCS_E_INVALID_ARG, "output is null");
+    if (reinterpret_cast<uintptr_t>(d_in) % sizeof(K) || reinterpret_cast<uintptr_t>(d_out) % sizeof(K) ||
+        reinterpret_cast<uintptr_t>(d_rest) % sizeof(K))
+        fail(HPCS_E_INVALID_ARG, "pointer is misaligned");
+    if (m == 0) {
+        if (d_rest && n > 0)
+            hip_check(hipMemcpyAsync(d_rest, d_in, size_t(n) * sizeof(K), hipMemcpyDeviceToDevice, st), "copy rest");
+        return;
+    }
+    const uint32_t un = uint32_t(n), um = uint32_t(m);
+    const int idesc = desc ? 1 : 0;
+    if (!d_rest && n <= TOPK_BLOCK_SORT) {
+        // the whole input fits one workgroup's LDS: sort it there
+        timed(c, st, "topk_sort", [&] {
+            hipLaunchKernelGGL((k_topk_block_sort<K, FLOATK>), dim3(1), dim3(topk_sort_threads(un)), 0, st, d_in, un,
+                               d_out, um, 0, nullptr, un, idesc);
+        });
+        return;
+    }
+    const size_t off_c0 = align_up(sizeof(TopkState));
+    const size_t off_c1 = align_up(off_c0 + size_t(n) * sizeof(K));
+    const size_t need = align_up(off_c1 + size_t(n) * sizeof(K));
+    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, need));
+    TopkState *ts = reinterpret_cast<TopkState *>(base);
+    K *cand[2] = {reinterpret_cast<K *>(base + off_c0), reinterpret_cast<K *>(base + off_c1)};
+    const int vec = (reinterpret_cast<uintptr_t>(d_in) % 16 == 0) ? 1 : 0;
+    constexpr int TILE = TopkTile<K>::KEYS;
+    launch_fill(c, ts, sizeof(TopkState), st);
+    timed(c, st, "topk_hist", [&] {
+        hipLaunchKernelGGL((k_topk_hist<K, FLOATK>),
+                           dim3(grid_for(uint64_t(n), TOPK_HIST_BLOCK * TopkTile<K>::ITEMS, c->num_cus * 2)),
+                           dim3(TOPK_HIST_BLOCK), 0, st, d_in, un, vec, ts);
+    });
+    const int cgrid = grid_for(uint64_t(n), TILE, c->num_cus * 6);
+    for (int l = 0; l < LV::LEVELS; ++l) {
+        timed(c, st, "topk_compact", [&] {
+            hipLaunchKernelGGL((k_topk_compact<K, FLOATK>), dim3(cgrid), dim3(TOPK_BLOCK), 0, st, d_in, un, vec,
+                               cand[l & 1], cand[(l + 1) & 1], l, ts, d_out, d_rest, um, idesc);
+        });
+    }
+    const uint64_t span = std::max<uint64_t>(uint64_t(m), uint64_t(n - m));
+    if (m <= TOPK_BLOCK_SORT) {
+        if (d_rest)
+            timed(c, st, "topk_finish", [&] {
+                hipLaunchKernelGGL((k_topk_finish<K, FLOATK>), dim3(grid_for(span, 256, c->num_cus * 4)), dim3(256),
+                                   0, st, ts, static_cast<K *>(nullptr), d_rest, um, un, idesc);
+            });
+        timed(c, st, "topk_sort", [&] {
+            hipLaunchKernelGGL((k_topk_block_sort<K, FLOATK>), dim3(1), dim3(topk_sort_threads(um)), 0, st, d_out,
+                               um, d_out, um, 1, ts, un, idesc);
+        });
+        return;
+    }
+    timed(c, st, "topk_finish", [&] {
+        hipLaunchKernelGGL((k_topk_finish<K, FLOATK>), dim3(grid_for(span, 256, c->num_cus * 4)), dim3(256), 0, st,
+                           ts, d_out, d_rest, um, un, idesc);
+    });
+    run_sort<K, false, false>(c, d_out, nullptr, m, st);
+    timed(c, st, "topk_values", [&] {
+        hipLaunchKernelGGL((k_topk_values<K, FLOATK>), dim3(grid_for(uint64_t(m), 256, c->num_cus * 8)), dim3(256),
+                           0, st, d_out, uint64_t(m), idesc);
+    });
+}
+
+// SelectTopK(uint[], start, length, k) in place: a[k, start + length) gets the
+// largest start + length - k keys of the range in ascending order, a[start, k)
+// the others in no particular order.
+template <typename K, bool FLOATK>
+void host_select_topk(hpcs_ctx *c, K *a, int64_t start, int64_t length, int64_t k)
+{
+    if (!a) fail(HPCS_E_INVALID_ARG, "arrayToBeSelected is null");
+    if (start < 0 || length <= 0) fail(HPCS_E_INVALID_ARG, "start or length is invalid");
+    if (k < start || k >= start + length) fail(HPCS_E_INVALID_ARG, "k must lie in [start, start + length)");
+    if (length >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "length must be below 2^30 in this version");
+    const int64_t m = start + length - k, r = length - m;
+    const size_t off_out = align_up(size_t(length) * sizeof(K));
+    const size_t off_rest = align_up(off_out + size_t(m) * sizeof(K));
+    char *io = static_cast<char *>(ensure(c->io, c->io_cap, off_rest + size_t(r) * sizeof(K) + sizeof(K)));
+    K *d_in = reinterpret_cast<K *>(io), *d_out = reinterpret_cast<K *>(io + off_out);
+    K *d_rest = reinterpret_cast<K *>(io + off_rest);
+    hipStream_t st = c->stream;
+    hip_check(hipMemcpyAsync(d_in, a + start, size_t(length) * sizeof(K), hipMemcpyHostToDevice, st), "H2D");
+    run_topk<K, FLOATK>(c, d_in, length, m, d_out, d_rest, false, st);
+    if (r > 0) hip_check(hipMemcpyAsync(a + start, d_rest, size_t(r) * sizeof(K), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipMemcpyAsync(a + k, d_out, size_t(m) * sizeof(K), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+// Host arrays: out[0, k) = the k largest keys of keys[0, n), descending.
+template <typename K, bool FLOATK>
+void host_topk(hpcs_ctx *c, const K *keys, int64_t n, int64_t k, K *out)
+{
+    if (n < 0 || k < 0 || k > n) fail(HPCS_E_INVALID_ARG, "k must lie in [0, n]");
+    if (n > 0 && !keys) fail(HPCS_E_INVALID_ARG, "keys is null");
+    if (k > 0 && !out) fail(HPCS_E_INVALID_ARG, "output is null");
+    if (k == 0) return;
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    const size_t off_out = align_up(size_t(n) * sizeof(K));
+    char *io = static_cast<char *>(ensure(c->io, c->io_cap, off_out + size_t(k) * sizeof(K)));
+    K *d_in = reinterpret_cast<K *>(io), *d_out = reinterpret_cast<K *>(io + off_out);
+    hipStream_t st = c->stream;
+    hip_check(hipMemcpyAsync(d_in, keys, size_t(n) * sizeof(K), hipMemcpyHostToDevice, st), "H2D");
+    run_topk<K, FLOATK>(c, d_in, n, k, d_out, nullptr, true, st);
+    hip_check(hipMemcpyAsync(out, d_out, size_t(k) * sizeof(K), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
 template <typename K>
 void run_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset, int32_t key_kind,
                  hipStream_t st)
@@ -959,4 +1077,66 @@ HPCS_API int32_t hpcs_dev_histogram_byte_components_u64(hpcs_ctx *ctx, const uin
                                                         uint32_t *d_counts, void *stream)
 {
     return guarded(ctx, [&] { dev_histogram<uint64_t, false>(ctx, d_keys, n, d_counts, pick_stream(ctx, stream)); });
+}
+
+// ---------------------------------------------------------------------------
+// Radix top-k
+// ---------------------------------------------------------------------------
+
+HPCS_API int32_t hpcs_select_topk_u32(hpcs_ctx *ctx, uint32_t *a, int64_t start, int64_t length, int64_t k)
+{
+    return guarded(ctx, [&] { host_select_topk<uint32_t, false>(ctx, a, start, length, k); });
+}
+
+HPCS_API int32_t hpcs_topk_u32(hpcs_ctx *ctx, const uint32_t *keys, int64_t n, int64_t k, uint32_t *out)
+{
+    return guarded(ctx, [&] { host_topk<uint32_t, false>(ctx, keys, n, k, out); });
+}
+
+HPCS_API int32_t hpcs_topk_u64(hpcs_ctx *ctx, const uint64_t *keys, int64_t n, int64_t k, uint64_t *out)
+{
+    return guarded(ctx, [&] { host_topk<uint64_t, false>(ctx, keys, n, k, out); });
+}
+
+HPCS_API int32_t hpcs_topk_f32(hpcs_ctx *ctx, const float *keys, int64_t n, int64_t k, float *out)
+{
+    return guarded(ctx, [&] {
+        host_topk<uint32_t, true>(ctx, reinterpret_cast<const uint32_t *>(keys), n, k, reinterpret_cast<uint32_t *>(out));
+    });
+}
+
+HPCS_API int32_t hpcs_topk_f64(hpcs_ctx *ctx, const double *keys, int64_t n, int64_t k, double *out)
+{
+    return guarded(ctx, [&] {
+        host_topk<uint64_t, true>(ctx, reinterpret_cast<const uint64_t *>(keys), n, k, reinterpret_cast<uint64_t *>(out));
+    });
+}
+
+HPCS_API int32_t hpcs_dev_topk_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n, int64_t k, uint32_t *d_out,
+                                   void *stream)
+{
+    return guarded(ctx, [&] { run_topk<uint32_t, false>(ctx, d_keys, n, k, d_out, nullptr, true, pick_stream(ctx, stream)); });
+}
+
+HPCS_API int32_t hpcs_dev_topk_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, int64_t k, uint64_t *d_out,
+                                   void *stream)
+{
+    return guarded(ctx, [&] { run_topk<uint64_t, false>(ctx, d_keys, n, k, d_out, nullptr, true, pick_stream(ctx, stream)); });
+}
+
+HPCS_API int32_t hpcs_dev_topk_f32(hpcs_ctx *ctx, const float *d_keys, int64_t n, int64_t k, float *d_out, void *stream)
+{
+    return guarded(ctx, [&] {
+        run_topk<uint32_t, true>(ctx, reinterpret_cast<const uint32_t *>(d_keys), n, k,
+                                 reinterpret_cast<uint32_t *>(d_out), nullptr, true, pick_stream(ctx, stream));
+    });
+}
+
+HPCS_API int32_t hpcs_dev_topk_f64(hpcs_ctx *ctx, const double *d_keys, int64_t n, int64_t k, double *d_out,
+                                   void *stream)
+{
+    return guarded(ctx, [&] {
+        run_topk<uint64_t, true>(ctx, reinterpret_cast<const uint64_t *>(d_keys), n, k,
+                                 reinterpret_cast<uint64_t *>(d_out), nullptr, true, pick_stream(ctx, stream));
+    });
 }

@@ -172,6 +172,38 @@ int32_t hpcs_dev_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *d_
 int32_t hpcs_dev_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, uint32_t *d_counts,
                                                void *stream);
 
+/* ------------------------------------------------------------------------
+ * Radix top-k
+ * ------------------------------------------------------------------------ */
+
+/* Replaces Algorithm.SelectTopK(this uint[] a, int start, int length, int k)
+ * (SelectionTopKRadix.cs:142-153; with start = 0 and length = a.Length the
+ * overload SelectTopK(this uint[] a, int k), :160-171).  k is an index into
+ * a, start <= k < start + length.  On return a[k, start + length) holds the
+ * largest start + length - k keys of the range in ascending order (so a[k] is
+ * the k-th smallest of the range counted from start), and a[start, k) the
+ * other keys in unspecified order.  The reference guarantees the multiset of
+ * a[k, start + length) only, and leaves a[start, k) holding moved-from
+ * copies; this call keeps the range a permutation of its input. */
+int32_t hpcs_select_topk_u32(hpcs_ctx *ctx, uint32_t *a, int64_t start, int64_t length, int64_t k);
+
+/* The k largest keys of keys[0, n) in descending order into out[0, k),
+ * 0 <= k <= n; keys is not modified.  Floating keys order as the radix sort
+ * orders them (-0.0 below +0.0).  Host arrays, synchronous. */
+int32_t hpcs_topk_u32(hpcs_ctx *ctx, const uint32_t *keys, int64_t n, int64_t k, uint32_t *out);
+int32_t hpcs_topk_u64(hpcs_ctx *ctx, const uint64_t *keys, int64_t n, int64_t k, uint64_t *out);
+int32_t hpcs_topk_f32(hpcs_ctx *ctx, const float *keys, int64_t n, int64_t k, float *out);
+int32_t hpcs_topk_f64(hpcs_ctx *ctx, const double *keys, int64_t n, int64_t k, double *out);
+/* The same on device buffers, stream-ordered; the launch sequence depends
+ * only on (n, k), so the call can be captured in a HIP graph after one
+ * warm-up call of the same size. */
+int32_t hpcs_dev_topk_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n, int64_t k, uint32_t *d_out,
+                          void *stream);
+int32_t hpcs_dev_topk_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, int64_t k, uint64_t *d_out,
+                          void *stream);
+int32_t hpcs_dev_topk_f32(hpcs_ctx *ctx, const float *d_keys, int64_t n, int64_t k, float *d_out, void *stream);
+int32_t hpcs_dev_topk_f64(hpcs_ctx *ctx, const double *d_keys, int64_t n, int64_t k, double *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -663,3 +663,127 @@ int hpcs_oracle_sort_segments_u32(uint32_t *a, const int64_t *seg_begin, int32_t
     }
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Radix top-k selection                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Algorithm.MoveRobElementsIntoKthBin, SelectionTopKRadix.cs:19-41: move the
+ * elements right of the k-th bin that belong in it into it, swapping out
+ * elements that belong to the right. */
+static int64_t topk_move_rob_into(uint32_t *a, int64_t sob_k, int64_t len_k, int64_t s_rob, int64_t len_rob, int shift,
+                                  uint32_t mask, uint32_t kb)
+{
+    int64_t end_k = sob_k + len_k - 1, end_rob = s_rob + len_rob - 1;
+    int64_t cr = s_rob, ci = sob_k;
+    for (;;) {
+        for (; cr <= end_rob; cr++)
+            if (((a[cr] >> shift) & mask) == kb) break;
+        if (cr <= end_rob)
+            for (; ci <= end_k; ci++)
+                if (((a[ci] >> shift) & mask) != kb) break;
+        if (cr > end_rob || ci > end_k) break;
+        if (((a[ci] >> shift) & mask) > kb) {
+            uint32_t t = a[ci];
+            a[ci++] = a[cr];
+            a[cr++] = t;
+        } else {
+            a[ci++] = a[cr++];
+        }
+    }
+    return ci;
+}
+
+/* Algorithm.MoveRobElementsOutOfKthBin, SelectionTopKRadix.cs:44-60. */
+static void topk_move_rob_out(uint32_t *a, int64_t sob_k, int64_t len_k, int64_t s_rob, int64_t len_rob, int shift,
+                              uint32_t mask, uint32_t kb)
+{
+    int64_t end_k = sob_k + len_k - 1, end_rob = s_rob + len_rob - 1;
+    int64_t cr = s_rob, ci = sob_k;
+    for (;;) {
+        for (; ci <= end_k; ci++)
+            if (((a[ci] >> shift) & mask) > kb) break;
+        if (ci <= end_k)
+            for (; cr <= end_rob; cr++)
+                if (((a[cr] >> shift) & mask) < kb) break;
+        if (cr > end_rob || ci > end_k) break;
+        a[cr++] = a[ci++];
+    }
+}
+
+/* Algorithm.MoveLobElements, SelectionTopKRadix.cs:64-93. */
+static void topk_move_lob(uint32_t *a, int64_t s_lob, int64_t len_lob, int64_t sob_k, int64_t len_k, int64_t s_kp1,
+                          int64_t len_rob, int shift, uint32_t mask, uint32_t kb)
+{
+    int64_t end_k = sob_k + len_k - 1, end_lob = s_lob + len_lob - 1, end_rob = s_kp1 + len_rob - 1;
+    int64_t cl = s_lob, ci = sob_k, cr = s_kp1;
+    for (;;) {
+        int found_ib = 0, found_rob = 0;
+        for (; cl <= end_lob; cl++) {
+            if (((a[cl] >> shift) & mask) == kb) { found_ib = 1; break; }
+            if (((a[cl] >> shift) & mask) > kb) { found_rob = 1; break; }
+        }
+        if (cl > end_lob) break;
+        if (found_ib) {
+            for (; ci <= end_k; ci++)
+                if (((a[ci] >> shift) & mask) != kb) break;
+            a[ci++] = a[cl++];
+        }
+        if (found_rob) {
+            for (; cr <= end_rob; cr++)
+                if (((a[cr] >> shift) & mask) <= kb) break;
+            a[cr++] = a[cl++];
+        }
+    }
+}
+
+/* Algorithm.RadixSelectionTopKWordInner, SelectionTopKRadix.cs:96-133:
+ * 16-bit MSD digits; the bin holding index k is gathered in place, elements
+ * of higher bins moved to its right, and the recursion continues in that bin
+ * with the next 16 bits. */
+static int topk_word_inner(uint32_t *a, int64_t first, int64_t length, int shift, int64_t k, int32_t *count,
+                           int64_t *sob)
+{
+    const int BINS = 65536;
+    const uint32_t mask = 0xFFFF;
+    int64_t last = first + length - 1;
+    memset(count, 0, sizeof(int32_t) * BINS);
+    for (int64_t i = first; i <= last; i++) count[(uint16_t)(a[i] >> shift)]++;
+    sob[0] = first;
+    sob[BINS] = last + 1;
+    for (int i = 1; i < BINS; i++) sob[i] = sob[i - 1] + count[i - 1];
+    int kb = 0;
+    for (; kb < BINS; kb++) {
+        int64_t bl = sob[kb + 1] - sob[kb];
+        if (bl == 0) continue;
+        if (k >= sob[kb] && k <= sob[kb + 1] - 1) break;
+    }
+    if (kb >= BINS) return HPCS_ORACLE_EINVAL;
+    int64_t len_k = sob[kb + 1] - sob[kb], s_kp1 = sob[kb + 1], s_k = sob[kb];
+    int64_t ci = topk_move_rob_into(a, s_k, len_k, s_kp1, last - s_kp1 + 1, shift, mask, (uint32_t)kb);
+    topk_move_rob_out(a, ci, s_kp1 - ci, s_kp1, last - s_kp1 + 1, shift, mask, (uint32_t)kb);
+    topk_move_lob(a, first, s_k - first, s_k, len_k, s_kp1, last - s_kp1 + 1, shift, mask, (uint32_t)kb);
+    if (shift > 0) {
+        shift = shift >= 16 ? shift - 16 : 0;
+        if (len_k > 1) return topk_word_inner(a, s_k, len_k, shift, k, count, sob);
+        if (len_k == 1) return HPCS_ORACLE_OK;
+        return HPCS_ORACLE_EINVAL; /* the reference throws here (:129) */
+    }
+    return HPCS_ORACLE_OK;
+}
+
+/* Algorithm.SelectTopK(this uint[] a, int k), SelectionTopKRadix.cs:160-171:
+ * afterwards a[k] is the k-th smallest key (0-based) and a[k..n) holds the
+ * n-k largest keys.  Elements are moved rather than swapped, so a[0..k) is
+ * left unspecified, as in the reference. */
+int hpcs_oracle_select_topk_u32(uint32_t *a, int64_t n, int64_t k)
+{
+    if (!a || n <= 0 || k < 0 || k >= n) return HPCS_ORACLE_EINVAL;
+    int32_t *count = (int32_t *)malloc(sizeof(int32_t) * 65536);
+    int64_t *sob = (int64_t *)malloc(sizeof(int64_t) * 65537);
+    if (!count || !sob) { free(count); free(sob); return HPCS_ORACLE_ENOMEM; }
+    int rc = topk_word_inner(a, 0, n, 16, k, count, sob);
+    free(count);
+    free(sob);
+    return rc;
+}

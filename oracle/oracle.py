@@ -55,6 +55,7 @@ def lib():
             "hpcs_oracle_sort_radix_pairs_u64_u32": (p, p, i64),
             "hpcs_oracle_sort_radix_pairs_u32_u32": (p, p, i64),
             "hpcs_oracle_sort_segments_u32": (p, p, i32),
+            "hpcs_oracle_select_topk_u32": (p, i64, i64),
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -167,3 +168,9 @@ def f32_key(bits: int) -> int:
 
 def f64_key(bits: int) -> int:
     return lib().hpcs_oracle_f64_key(bits)
+
+
+def select_topk_u32(a: np.ndarray, k: int) -> np.ndarray:
+    """SelectTopK(uint[], k): a[k] becomes the k-th smallest, a[k:] the n-k largest."""
+    _check(lib().hpcs_oracle_select_topk_u32(_ptr(a), a.size, k), "select topk")
+    return a

@@ -221,3 +221,15 @@ def test_planted_mistakes_fail_the_checks(oracle):
     eq = np.flatnonzero(k[1:] == k[:-1])[0]          # swap two equal keys' payloads: unstable
     v[[eq, eq + 1]] = v[[eq + 1, eq]]
     assert sha(v) != pc["perm_sha"]
+
+
+@pytest.mark.parametrize("n,k,mask", [(1000, 10, 0xFFFFFFFF), (100000, 99000, 0xFFFFFFFF), (70000, 123, 0xFFFF),
+                                      (50000, 25000, 0xFF00FF00), (5, 0, 0xFFFFFFFF), (40000, 39999, 0x7)])
+def test_select_topk_restatement(oracle, n, k, mask):
+    """SelectTopK leaves the n-k largest keys (as a multiset) in a[k:], with
+    a[k] the k-th smallest: checked against np.partition."""
+    a = datagen.keys_u32(n, n + k) & np.uint32(mask)
+    orig = a.copy()
+    oracle.select_topk_u32(a, k)
+    assert a[k] == np.partition(orig, k)[k]
+    assert np.array_equal(np.sort(a[k:]), np.sort(orig)[k:])
