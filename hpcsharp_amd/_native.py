@@ -26,6 +26,8 @@ HPCS_KEY_FLOAT = 1
 HPCS_DTYPE_U32, HPCS_DTYPE_U64, HPCS_DTYPE_F32, HPCS_DTYPE_F64 = 0, 1, 2, 3
 HPCS_OPT_RANK, HPCS_OPT_TILE_SHAPE, HPCS_OPT_FALLBACK_ROUNDS, HPCS_OPT_LOOKBACK_WINDOW, HPCS_OPT_PASS_ALGO = 1, 2, 3, 4, 5
 HPCS_OPT_PERSISTENT = 6
+HPCS_OP_SORT, HPCS_OP_TOPK = 0, 1
+HPCS_SERVE_REALTIME, HPCS_SERVE_ALONE = 0, 1
 
 # Every symbol include/hpcsort.h declares, with its argument types.
 _p, _i64, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
@@ -78,6 +80,9 @@ SIGNATURES = {
     "hpcs_dev_sort_radix_records": ([_p, _p, _i64, _i32, _i32, _i32, _i32, _p], _i32),
     "hpcs_dev_histogram_byte_components_u32": ([_p, _p, _i64, _p, _p], _i32),
     "hpcs_dev_histogram_byte_components_u64": ([_p, _p, _i64, _p, _p], _i32),
+    "hpcs_ctx_reserve": ([_p, _i32, _i64], _i32),
+    "hpcs_serve_run": ([_i32, _i32, _i64, _p, _i64, _i32, _p, _p], _i32),
+    "hpcs_serve_last_error": ([], ctypes.c_char_p),
     "hpcs_select_topk_u32": ([_p, _p, _i64, _i64, _i64], _i32),
     "hpcs_topk_u32": ([_p, _p, _i64, _i64, _p], _i32),
     "hpcs_topk_u64": ([_p, _p, _i64, _i64, _p], _i32),

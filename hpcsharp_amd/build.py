@@ -11,9 +11,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libhpcsort.so")
-SOURCES = [os.path.join(HERE, "csrc", "hpcs_api.hip")]
-HEADERS = [os.path.join(HERE, "csrc", h) for h in ("hpcs_kernels.hpp", "hpcs_segments.hpp")]
-DEPS = SOURCES + [os.path.join(HERE, "csrc", "hpcs_kernels.hpp"), os.path.join(REPO, "include", "hpcsort.h")]
+SOURCES = [os.path.join(HERE, "csrc", s) for s in ("hpcs_api.hip", "hpcs_serve.hip")]
+HEADERS = [os.path.join(HERE, "csrc", h) for h in ("hpcs_kernels.hpp", "hpcs_segments.hpp", "hpcs_topk.hpp")]
+DEPS = SOURCES + HEADERS + [os.path.join(REPO, "include", "hpcsort.h")]
 ARCH = os.environ.get("HPCS_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -46,6 +46,12 @@ void hip_check(hipError_t e, const char *what)
 
 constexpr int PASS_ONESWEEP = 0;
 constexpr int PASS_RTS = 1;
+constexpr int PASS_AUTO = 2;
+
+// Below these sizes the single-kernel onesweep pass beats reduce-then-scan's
+// five kernels per pass (launch-bound), measured on MI355X: 4-byte keys
+// below 2^19, 8-byte keys below 2^21.
+template <typename K> constexpr int64_t onesweep_below() { return sizeof(K) == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21); }
 
 // Onesweep tile shapes (threads x keys per thread).  Shape 0 serves every key
 // type; the larger u32 shapes halve the tiles (and look-back work) per pass.
@@ -77,7 +83,7 @@ struct hpcs_ctx {
     int tile_shape = 0;          // tuning knob for u32 keys, env HPCS_TILE_SHAPE=0..3
     int fallback_rounds = FALLBACK_ROUNDS;
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
-    int pass_algo = PASS_RTS;     // PASS_ONESWEEP or PASS_RTS (reduce-then-scan)
+    int pass_algo = PASS_AUTO;    // PASS_ONESWEEP, PASS_RTS (reduce-then-scan) or PASS_AUTO (by size)
     int persistent = 0;           // reduce-then-scan scatter: persistent workgroups with prefetch
     // kernel timing
     struct Pending {
@@ -389,7 +395,8 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
 
     const uint32_t tiles = uint32_t(L.tiles);
     const uint32_t nranges = (tiles + SCAN_RANGE - 1) / SCAN_RANGE;
-    if (c->pass_algo == PASS_RTS) {
+    const int algo = c->pass_algo != PASS_AUTO ? c->pass_algo : (n < onesweep_below<K>() ? PASS_ONESWEEP : PASS_RTS);
+    if (algo == PASS_RTS) {
         // the histogram area holds the OR slots; counts in look-back region 0
         launch_fill(c, base, L.off_lb0, st);
         K *orslots = reinterpret_cast<K *>(hist);
@@ -408,7 +415,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         });
     }
     for (int p = 0; p < P; ++p) {
-        if (c->pass_algo == PASS_RTS) {
+        if (algo == PASS_RTS) {
             // counts in look-back region 0, range sums in region 1
             uint32_t *counts = lb[0], *totals = lb[1];
             if (p > 0)
@@ -864,7 +871,8 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             ctx->lookback_window = int(value);
             break;
         case HPCS_OPT_PASS_ALGO:
-            if (value != PASS_ONESWEEP && value != PASS_RTS) fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0 or 1");
+            if (value != PASS_ONESWEEP && value != PASS_RTS && value != PASS_AUTO)
+                fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0, 1 or 2");
             ctx->pass_algo = int(value);
             break;
         case HPCS_OPT_PERSISTENT:
@@ -939,6 +947,31 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
     info->ideal_bytes = n * per_key;
     info->scratch_bytes = int64_t(L.total);
     return HPCS_OK;
+}
+
+HPCS_API int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n)
+{
+    return guarded(ctx, [&] {
+        size_t ks = 0;
+        switch (dtype) {
+        case HPCS_DTYPE_U32:
+        case HPCS_DTYPE_F32: ks = 4; break;
+        case HPCS_DTYPE_U64:
+        case HPCS_DTYPE_F64: ks = 8; break;
+        default: fail(HPCS_E_INVALID_ARG, "unknown dtype");
+        }
+        if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
+        if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+        size_t need = 0;
+        for (int shape = 0; shape < 4; ++shape) {
+            if (shape > 0 && ks != 4) break;
+            const int tk = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
+            need = std::max(need, make_layout(n, ks, false, tk).total);
+        }
+        const size_t topk = align_up(align_up(align_up(sizeof(TopkState)) + size_t(n) * ks) + size_t(n) * ks);
+        need = std::max(need, topk);
+        ensure(ctx->scratch, ctx->scratch_cap, need);
+    });
 }
 
 // host arrays

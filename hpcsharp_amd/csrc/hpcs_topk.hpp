@@ -374,15 +374,6 @@ template <typename K, bool FLOATK> __device__ inline K topk_value(K s, int desc)
     return value_of<K, FLOATK>(desc ? K(~s) : s);
 }
 
-// Bitonic compare-exchange of x with its partner value p: the element with
-// the lower index keeps the minimum in an ascending run (bit k of its index
-// clear), the maximum in a descending one.
-template <typename K> __device__ inline K bitonic_pick(K x, K p, uint32_t e, uint32_t j, uint32_t k)
-{
-    const bool keep_min = ((e & j) == 0u) == ((e & k) == 0u);
-    return keep_min ? (p < x ? p : x) : (p < x ? x : p);
-}
-
 // Steps with partner distance J < TOPK_SORT_ITEMS stay inside a thread.
 template <typename K, int J>
 __device__ inline void bitonic_in_thread(K (&v)[TOPK_SORT_ITEMS], uint32_t e0, uint32_t k)
@@ -441,23 +432,26 @@ __global__ __launch_bounds__(TOPK_SORT_BLOCK) void k_topk_block_sort(const K *__
     }
     for (uint32_t k = 2; k <= size; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            if (j >= WAVE_SPAN) {
-                __syncthreads();
+            if (j >= uint32_t(IT)) {
+                // all sixteen partner values are fetched before any is used,
+                // so the exchanges overlap instead of waiting one by one
+                K p[IT];
+                if (j >= WAVE_SPAN) {
+                    __syncthreads();
 #pragma unroll
-                for (int r = 0; r < IT; ++r) s[e0 + r] = v[r];
-                __syncthreads();
+                    for (int r = 0; r < IT; ++r) s[e0 + r] = v[r];
+                    __syncthreads();
 #pragma unroll
-                for (int r = 0; r < IT; ++r) {
-                    const uint32_t e = e0 + uint32_t(r);
-                    v[r] = bitonic_pick(v[r], s[e ^ j], e, j, k);
+                    for (int r = 0; r < IT; ++r) p[r] = s[(e0 + uint32_t(r)) ^ j];
+                } else {
+                    const int pl = int(lane ^ (j / uint32_t(IT)));
+#pragma unroll
+                    for (int r = 0; r < IT; ++r) p[r] = __shfl(v[r], pl);
                 }
-            } else if (j >= uint32_t(IT)) {
-                const int pl = int(lane ^ (j / uint32_t(IT)));
+                // the partner's direction bits are those of e0 for j >= 16
+                const bool keep_min = ((e0 & j) == 0u) == ((e0 & k) == 0u);
 #pragma unroll
-                for (int r = 0; r < IT; ++r) {
-                    const K p = __shfl(v[r], pl);
-                    v[r] = bitonic_pick(v[r], p, e0 + uint32_t(r), j, k);
-                }
+                for (int r = 0; r < IT; ++r) v[r] = keep_min ? (p[r] < v[r] ? p[r] : v[r]) : (p[r] < v[r] ? v[r] : p[r]);
             } else if (j == 8) {
                 bitonic_in_thread<K, 8>(v, e0, k);
             } else if (j == 4) {

@@ -57,7 +57,8 @@ extern "C" {
 #define HPCS_OPT_FALLBACK_ROUNDS 3 /* look-back rounds on an unpublished tile before counting it locally
                                       (default 24; 0 makes every wait use the local count, for tests) */
 #define HPCS_OPT_LOOKBACK_WINDOW 4 /* u32 keys: earlier tiles read per look-back round: 16, 32 (default), 64 */
-#define HPCS_OPT_PASS_ALGO 5       /* 0: single-pass onesweep with look-back, 1: reduce-then-scan (default) */
+#define HPCS_OPT_PASS_ALGO 5       /* 0: single-pass onesweep with look-back, 1: reduce-then-scan,
+                                      2: by size (default; onesweep for small arrays) */
 #define HPCS_OPT_PERSISTENT 6      /* reduce-then-scan scatter: 0 one workgroup per tile (default), 1 persistent
                                       workgroups that prefetch the next tile */
 
@@ -100,6 +101,10 @@ int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_t with_va
  * no counterpart in the reference (whose knobs are static properties such as
  * SortRadixParallelWorkQuanta, RadixSortLsdParallel.cs:763). */
 int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t value);
+/* Grow the context's device scratch now, so that later sorts and top-k calls
+ * of up to n keys of `dtype` allocate nothing (no counterpart in the
+ * reference; for latency-bound callers such as the serving loop). */
+int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n);
 /* Kernel timing (no counterpart in the reference, whose Stopwatch timing is
  * commented out, RadixSortLsd.cs:146-153): on != 0 clears the statistics and
  * brackets every later launch of this context with HIP events on its stream;
@@ -203,6 +208,57 @@ int32_t hpcs_dev_topk_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, int6
                           void *stream);
 int32_t hpcs_dev_topk_f32(hpcs_ctx *ctx, const float *d_keys, int64_t n, int64_t k, float *d_out, void *stream);
 int32_t hpcs_dev_topk_f64(hpcs_ctx *ctx, const double *d_keys, int64_t n, int64_t k, double *d_out, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference: the serving pattern built
+ * on the sort and top-k calls above)
+ * ------------------------------------------------------------------------ */
+
+#define HPCS_OP_SORT 0 /* ascending sort of the payload (radix order) */
+#define HPCS_OP_TOPK 1 /* the k largest keys of the payload, descending */
+
+#define HPCS_SERVE_REALTIME 0 /* replay the arrival times; slots run concurrently */
+#define HPCS_SERVE_ALONE 1    /* one request at a time, in list order */
+
+/* One request.  Its payload is n keys of `dtype`, key i being output i of
+ * splitmix64(seed) (for floating types the same bits with an all-ones
+ * exponent made finite); it is generated on the device at admission. */
+typedef struct hpcs_request {
+    int32_t op;       /* HPCS_OP_SORT or HPCS_OP_TOPK */
+    int32_t dtype;    /* HPCS_DTYPE_* */
+    int64_t n;        /* payload keys, 1 <= n < 2^30 */
+    int64_t k;        /* top-k: 1 <= k <= n */
+    uint64_t seed;    /* payload seed */
+    double arrival_s; /* arrival, seconds after the start (non-decreasing) */
+} hpcs_request;
+
+typedef struct hpcs_request_result {
+    double admit_s;    /* slot and memory granted */
+    double enqueued_s; /* all of the request's work enqueued on its slot's stream */
+    double done_s;     /* completion seen by the host */
+    uint64_t checksum; /* sum over i of mix64(bits_i + (i + 1) * 0x9E3779B97F4A7C15)
+                          of the result keys, mod 2^64 (mix64: the splitmix64
+                          finaliser) */
+    int32_t status;    /* HPCS_OK, or HPCS_E_BUDGET if the request alone exceeds the budget */
+    int32_t slot;      /* worker slot that ran it */
+} hpcs_request_result;
+
+typedef struct hpcs_serve_stats {
+    double wall_s;                  /* start to the last completion */
+    int64_t keys;                   /* payload keys of the completed requests */
+    int64_t peak_bytes;             /* most request bytes in flight at once */
+    int64_t request_budget_bytes;   /* budget left for requests after the slots' scratch */
+    int64_t scratch_bytes_per_slot; /* scratch reserved per slot */
+    int32_t slots;
+    int32_t pad;
+} hpcs_serve_stats;
+
+/* Run the requests on `device` with `slots` worker slots (one stream and one
+ * context each) within hbm_budget bytes of device memory (request buffers
+ * plus the slots' scratch).  results[i] belongs to reqs[i]. */
+int32_t hpcs_serve_run(int32_t device, int32_t slots, int64_t hbm_budget, const hpcs_request *reqs, int64_t nreq,
+                       int32_t mode, hpcs_request_result *results, hpcs_serve_stats *stats);
+const char *hpcs_serve_last_error(void);
 
 #ifdef __cplusplus
 }
