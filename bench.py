@@ -133,6 +133,96 @@ def batched_vs_plain(torch, N, datagen, nseg: int, seed: int, reps: int):
             "batched": "one hipGraph replay of the batched call (fixed 14-launch sequence)"}
 
 
+def timed_calls(torch, fn, reps: int) -> float:
+    """Seconds per call of fn() (device work on the current stream)."""
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def topk_figures(torch, N, datagen, reps: int):
+    """Radix top-k, k = 10^4 of 2^26 keys (the serving workload's largest
+    top-k requests), per key type; the u32 result is checked on the host."""
+    lib, ctx = N.lib(), N.context(0)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    n, k = 1 << 26, 10 ** 4
+    out = {}
+    for name, fn, gen in (("u32", "hpcs_dev_topk_u32", lambda: datagen.keys_u32(n, 0x5EED0009)),
+                          ("f32", "hpcs_dev_topk_f32", lambda: datagen.floats_finite(n, 0x5EED0009).view(np.uint32)),
+                          ("u64", "hpcs_dev_topk_u64", lambda: datagen.keys_u64(n, 0x5EED0009))):
+        h = gen()
+        x = torch.from_numpy(h.view(np.int32 if h.itemsize == 4 else np.int64)).cuda()
+        o = torch.empty(k, dtype=x.dtype, device="cuda")
+        f = getattr(lib, fn)
+        call = lambda: N.check(f(ctx, ctypes.c_void_p(x.data_ptr()), n, k, ctypes.c_void_p(o.data_ptr()), st))
+        t = timed_calls(torch, call, reps)
+        if name == "u32":
+            want = np.sort(np.partition(h, n - k)[n - k:])[::-1]
+            if not np.array_equal(o.cpu().numpy().view(np.uint32), want):
+                print("bench: top-k output differs from the host selection", file=sys.stderr)
+                sys.exit(3)
+        out[name] = {"ms": round(t * 1e3, 4), "gkeys_per_s": round(n / t / 1e9, 2),
+                     "read_GBs": round(n * h.itemsize / t / 1e9, 1)}
+        del x, o
+    return {"n": n, "k": k, "by_type": out}
+
+
+def cfg3_figures(torch, N, datagen, reps: int):
+    """Floating and record keys at 2^27: double[] with +-0.0, subnormals and
+    +-1e300 (plain sort), and {u64 key, u32 payload} records through the
+    key-offset record sort (stable; 8 passes over the keys)."""
+    lib, ctx = N.lib(), N.context(0)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    n = 1 << 27
+    out = {}
+    d = datagen.doubles_special(n, 0x5EED0006)
+    src = torch.from_numpy(d.view(np.int64)).cuda()
+    work = src.clone()
+
+    def f64():
+        work.copy_(src)
+        N.check(lib.hpcs_dev_sort_radix_f64(ctx, ctypes.c_void_p(work.data_ptr()), n, st))
+    t_copy = timed_calls(torch, lambda: work.copy_(src), reps)
+    t = timed_calls(torch, f64, reps) - t_copy
+    w = work.cpu().numpy().view(np.float64)
+    ok = bool(np.all(w[1:] >= w[:-1])) and bool(np.signbit(w[np.nonzero(w == 0)[0][0]]))
+    out["f64_sort"] = {"ms": round(t * 1e3, 3), "gkeys_per_s": round(n / t / 1e9, 2), "sorted": ok}
+    del src, work
+    rec = np.zeros(n, dtype=np.dtype({"names": ["key", "payload"], "formats": [np.uint64, np.uint32],
+                                      "offsets": [0, 8], "itemsize": 16}))
+    rec["key"] = datagen.keys_u64(n, 0x5EED0007)
+    rec["payload"] = np.arange(n, dtype=np.uint32)
+    rsrc = torch.from_numpy(rec.view(np.uint8).reshape(-1)).cuda()
+    rwork = rsrc.clone()
+
+    def recs():
+        rwork.copy_(rsrc)
+        N.check(lib.hpcs_dev_sort_radix_records(ctx, ctypes.c_void_p(rwork.data_ptr()), n, 16, 0, 64,
+                                                N.HPCS_KEY_UINT, st))
+    t_copy = timed_calls(torch, lambda: rwork.copy_(rsrc), reps)
+    t = timed_calls(torch, recs, reps) - t_copy
+    r = rwork.cpu().numpy().view(rec.dtype)
+    ok = bool(np.all(r["key"][1:] >= r["key"][:-1])) and bool(
+        np.array_equal(rec["key"][r["payload"]], r["key"]))
+    out["records_u64_key_u32_payload"] = {"ms": round(t * 1e3, 3), "gkeys_per_s": round(n / t / 1e9, 2),
+                                          "sorted_and_permutation": ok}
+    return {"n": n, **out}
+
+
+def serving_figures(nreq: int, slots: int):
+    """The serving workload: Poisson arrivals (2000/s), 70% sort / 30% top-k,
+    n log-uniform in [2^10, 2^26), u32/f32/u64, 200 GB budget."""
+    from hpcsharp_amd import serving as S
+    t = S.make_trace(nreq=nreq)
+    res, st = S.run(t, slots=slots, hbm_budget=200 * 10 ** 9)
+    s = S.summarize(t, res, st)
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
+
+
 def cpu_baseline(host_keys: np.ndarray, reps: int):
     """ParallelAlgorithm.SortRadixPar restated in C (oracle/), OpenMP threads."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -163,6 +253,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dists", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-topk", action="store_true")
+    ap.add_argument("--no-cfg3", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=20000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)
@@ -233,6 +327,17 @@ def main():
         if not extra["batched_vs_plain"]["identical_outputs"]:
             print("bench: batched and plain outputs differ", file=sys.stderr)
             sys.exit(3)
+
+    if not args.no_topk:
+        extra["topk"] = topk_figures(torch, N, datagen, 5)
+        log(f"[bench] top-k: {extra['topk']}")
+    if not args.no_cfg3:
+        extra["cfg3"] = cfg3_figures(torch, N, datagen, 3)
+        log(f"[bench] floating/record keys: {extra['cfg3']}")
+    if not args.no_serving:
+        log(f"[bench] serving: {args.serving_requests} requests")
+        extra["serving"] = serving_figures(args.serving_requests, 4)
+        log(f"[bench] serving: {extra['serving']}")
 
     cpu = None
     if not args.no_cpu:

@@ -23,6 +23,7 @@ HPCS_E_BUSY = -6
 
 HPCS_KEY_UINT = 0
 HPCS_KEY_FLOAT = 1
+HPCS_KEY_INT = 2
 HPCS_DTYPE_U32, HPCS_DTYPE_U64, HPCS_DTYPE_F32, HPCS_DTYPE_F64 = 0, 1, 2, 3
 HPCS_OPT_RANK, HPCS_OPT_TILE_SHAPE, HPCS_OPT_FALLBACK_ROUNDS, HPCS_OPT_LOOKBACK_WINDOW, HPCS_OPT_PASS_ALGO = 1, 2, 3, 4, 5
 HPCS_OPT_PERSISTENT = 6
@@ -83,6 +84,20 @@ SIGNATURES = {
     "hpcs_ctx_reserve": ([_p, _i32, _i64], _i32),
     "hpcs_serve_run": ([_i32, _i32, _i64, _p, _i64, _i32, _p, _p], _i32),
     "hpcs_serve_last_error": ([], ctypes.c_char_p),
+    "hpcs_histogram_byte_components_records": ([_p, _p, _i64, _i64, _i32, _i32, _i32, _p], _i32),
+    "hpcs_dev_histogram_byte_components_records": ([_p, _p, _i64, _i32, _i32, _i32, _p, _p], _i32),
+    "hpcs_sort_radix_i32": ([_p, _p, _i64], _i32),
+    "hpcs_sort_radix_i64": ([_p, _p, _i64], _i32),
+    "hpcs_dev_sort_radix_i32": ([_p, _p, _i64, _p], _i32),
+    "hpcs_dev_sort_radix_i64": ([_p, _p, _i64, _p], _i32),
+    "hpcs_histogram_byte_components_i32": ([_p, _p, _i64, _i64, _p], _i32),
+    "hpcs_histogram_byte_components_i64": ([_p, _p, _i64, _i64, _p], _i32),
+    "hpcs_dev_histogram_byte_components_i32": ([_p, _p, _i64, _p, _p], _i32),
+    "hpcs_dev_histogram_byte_components_i64": ([_p, _p, _i64, _p, _p], _i32),
+    "hpcs_topk_i32": ([_p, _p, _i64, _i64, _p], _i32),
+    "hpcs_topk_i64": ([_p, _p, _i64, _i64, _p], _i32),
+    "hpcs_dev_topk_i32": ([_p, _p, _i64, _i64, _p, _p], _i32),
+    "hpcs_dev_topk_i64": ([_p, _p, _i64, _i64, _p, _p], _i32),
     "hpcs_select_topk_u32": ([_p, _p, _i64, _i64, _i64], _i32),
     "hpcs_topk_u32": ([_p, _p, _i64, _i64, _p], _i32),
     "hpcs_topk_u64": ([_p, _p, _i64, _i64, _p], _i32),

@@ -69,6 +69,8 @@ _SORT = {
     np.dtype(np.uint64): "hpcs_sort_radix_u64",
     np.dtype(np.float32): "hpcs_sort_radix_f32",
     np.dtype(np.float64): "hpcs_sort_radix_f64",
+    np.dtype(np.int32): "hpcs_sort_radix_i32",
+    np.dtype(np.int64): "hpcs_sort_radix_i64",
 }
 
 
@@ -94,8 +96,8 @@ class Algorithm:
     def SortRadixLsd(inOutArray: np.ndarray, startIndex: int | None = None, length: int | None = None) -> None:
         """SortRadixLsd(this uint[]) RadixSortLsd.cs:255-260; (start, length) :242-247
         with the Experimental overload's sub-range semantics (:1409-1457);
-        ulong[] :424-442."""
-        _require(inOutArray, "inOutArray", (np.uint32, np.uint64))
+        ulong[] :424-442; int[] / long[] :267-357, 444-532 (signed order)."""
+        _require(inOutArray, "inOutArray", (np.uint32, np.uint64, np.int32, np.int64))
         if startIndex is None:
             _sort_whole(inOutArray)
         else:
@@ -172,18 +174,34 @@ class Algorithm:
 
     # -- histograms ------------------------------------------------------------
     @staticmethod
-    def HistogramByteComponents(inArray: np.ndarray, l: int, r: int) -> np.ndarray:
+    def HistogramByteComponents(inArray: np.ndarray, l: int, r: int, getKey: str | None = None) -> np.ndarray:
         """HistogramByteComponents(uint[]/ulong[], int l, int r) Histogram.cs:261-296, 957-990:
-        int32[digits][256] of [l, r] inclusive."""
-        _require(inArray, "inArray", (np.uint32, np.uint64))
+        int32[digits][256] of [l, r] inclusive; int[] / long[] :992-1052 count
+        the top byte ^ 128.  With getKey (a field name of a structured array,
+        the key extractor Func<T, UInt32/UInt64>, Histogram.cs:760-844) the
+        bytes of each record's key are counted."""
+        if inArray is None:
+            raise ArgumentNullException("inArray")
+        if getKey is not None:
+            if not isinstance(inArray, np.ndarray) or inArray.dtype.names is None or getKey not in inArray.dtype.names:
+                raise TypeError("getKey must name a field of a structured array")
+            kt, off = inArray.dtype.fields[getKey][:2]
+            if kt not in (np.dtype(np.uint32), np.dtype(np.uint64)):
+                raise TypeError("the key field must be uint32 or uint64")
+            if l < 0 or (r >= l and r >= inArray.size):
+                raise IndexError("l..r outside the array")
+            out = np.zeros((kt.itemsize, 256), dtype=np.int32)
+            _call(N.lib().hpcs_histogram_byte_components_records, _ptr(np.ascontiguousarray(inArray)), l, r,
+                  inArray.dtype.itemsize, off, 8 * kt.itemsize, _ptr(out))
+            return out
+        _require(inArray, "inArray", (np.uint32, np.uint64, np.int32, np.int64))
         if l < 0 or (r >= l and r >= inArray.size):
             raise IndexError("l..r outside the array")
         P = inArray.dtype.itemsize
         out = np.zeros((P, 256), dtype=np.int32)
-        fn = N.lib().hpcs_histogram_byte_components_u32 if P == 4 else N.lib().hpcs_histogram_byte_components_u64
+        fn = getattr(N.lib(), "hpcs_histogram_byte_components_" + ("i" if inArray.dtype.kind == "i" else "u") + str(8 * P))
         _call(fn, _ptr(inArray), l, r, _ptr(out))
         return out
-
 
     # -- radix top-k -------------------------------------------------------------
     @staticmethod
@@ -225,6 +243,8 @@ _TOPK = {
     np.dtype(np.uint64): "hpcs_topk_u64",
     np.dtype(np.float32): "hpcs_topk_f32",
     np.dtype(np.float64): "hpcs_topk_f64",
+    np.dtype(np.int32): "hpcs_topk_i32",
+    np.dtype(np.int64): "hpcs_topk_i64",
 }
 
 class ParallelAlgorithm:
@@ -232,12 +252,12 @@ class ParallelAlgorithm:
 
     @staticmethod
     def SortRadixPar(inputArray: np.ndarray, ParallelWorkQuantum: int = 64 * 1024) -> np.ndarray:
-        """SortRadixPar(this uint[], int ParallelWorkQuantum = 64Ki) RadixSortLsdParallel.cs:145-232
-        and (this ulong[]) :690-734.  The work quantum is accepted and unused:
-        the GPU tiles are fixed by the kernel."""
+        """SortRadixPar(this uint[], int ParallelWorkQuantum = 64Ki) RadixSortLsdParallel.cs:145-232,
+        (this ulong[]) :690-734 and the int[] / long[] forms :446-668.  The work
+        quantum is accepted and unused: the GPU tiles are fixed by the kernel."""
         if inputArray is None:
             raise ArgumentNullException("inputArray")
-        _require(inputArray, "inputArray", (np.uint32, np.uint64))
+        _require(inputArray, "inputArray", (np.uint32, np.uint64, np.int32, np.int64))
         _sort_whole(inputArray)
         return inputArray
 

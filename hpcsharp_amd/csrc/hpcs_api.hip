@@ -221,7 +221,7 @@ void launch_fill(hpcs_ctx *c, void *p, uint64_t bytes, hipStream_t st)
     });
 }
 
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 void launch_histogram(hpcs_ctx *c, const K *d_keys, uint64_t n, uint32_t *d_hist, hipStream_t st)
 {
     constexpr int VEC = 16 / int(sizeof(K));
@@ -237,7 +237,7 @@ void launch_histogram(hpcs_ctx *c, const K *d_keys, uint64_t n, uint32_t *d_hist
     });
 }
 
-template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int W>
+template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int W>
 void launch_onesweep_w(int rank_mode, int fb, dim3 grid, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb,
                        uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
 {
@@ -249,7 +249,7 @@ void launch_onesweep_w(int rank_mode, int fb, dim3 grid, hipStream_t st, K *a, K
                            a, b, va, vb, n, pass, plan, lb, next, fb);
 }
 
-template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS>
+template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS>
 void launch_onesweep_shape(int rank_mode, int window, int fb, dim3 grid, hipStream_t st, K *a, K *b, uint32_t *va,
                            uint32_t *vb, uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
 {
@@ -268,7 +268,7 @@ void launch_onesweep_shape(int rank_mode, int window, int fb, dim3 grid, hipStre
                                                              lb, next);
 }
 
-template <typename K, bool HAS_VALS, bool FLOATK>
+template <typename K, bool HAS_VALS, int FLOATK>
 void launch_onesweep(int rank_mode, int shape, int window, int fb, dim3 grid, hipStream_t st, K *a, K *b,
                      uint32_t *va, uint32_t *vb, uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
 {
@@ -289,7 +289,7 @@ void launch_onesweep(int rank_mode, int shape, int window, int fb, dim3 grid, hi
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const K *b, uint32_t n, int pass,
                         const DevPlan *plan, uint32_t *counts)
 {
@@ -309,7 +309,7 @@ void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const 
                            vec);
 }
 
-template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int RANK>
+template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int RANK>
 void launch_scatter_r(int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb, uint32_t n, int pass,
                       DevPlan *plan, const uint32_t *offsets, uint32_t tiles)
 {
@@ -329,7 +329,7 @@ void launch_scatter_r(int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uin
                        vb, n, pass, plan, offsets, tiles);
 }
 
-template <typename K, bool HAS_VALS, bool FLOATK>
+template <typename K, bool HAS_VALS, int FLOATK>
 void launch_scatter(int rank_mode, int shape, int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb,
                     uint32_t n, int pass, DevPlan *plan, const uint32_t *offsets, uint32_t tiles)
 {
@@ -356,7 +356,7 @@ void launch_scatter(int rank_mode, int shape, int num_cus, hipStream_t st, K *a,
                                                                 tiles);
 }
 
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n, uint32_t *counts, K *orslots)
 {
     const int block = TILE_SHAPES[shape].block, items = TILE_SHAPES[shape].items;
@@ -372,7 +372,7 @@ void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n
 }
 
 // The device-resident sort: keys (and optional uint32 values) in place.
-template <typename K, bool HAS_VALS, bool FLOATK>
+template <typename K, bool HAS_VALS, int FLOATK>
 void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
 {
     if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
@@ -527,7 +527,7 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
 // Radix top-k: the m largest keys of d_in[0, n) into d_out[0, m), in
 // descending order (desc) or ascending; with d_rest, the other n - m keys go
 // there in no particular order.  Scratch: [TopkState | cand 0 | cand 1].
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_rest, bool desc, hipStream_t st)
 {
     using LV = TopkLevels<K>;
@@ -601,7 +601,7 @@ void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_r
 // SelectTopK(uint[], start, length, k) in place: a[k, start + length) gets the
 // largest start + length - k keys of the range in ascending order, a[start, k)
 // the others in no particular order.
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 void host_select_topk(hpcs_ctx *c, K *a, int64_t start, int64_t length, int64_t k)
 {
     if (!a) fail(HPCS_E_INVALID_ARG, "arrayToBeSelected is null");
@@ -623,7 +623,7 @@ void host_select_topk(hpcs_ctx *c, K *a, int64_t start, int64_t length, int64_t 
 }
 
 // Host arrays: out[0, k) = the k largest keys of keys[0, n), descending.
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 void host_topk(hpcs_ctx *c, const K *keys, int64_t n, int64_t k, K *out)
 {
     if (n < 0 || k < 0 || k > n) fail(HPCS_E_INVALID_ARG, "k must lie in [0, n]");
@@ -659,7 +659,7 @@ void run_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t
     const int grid = grid_for(uint64_t(n), 256, c->num_cus * 8);
     hipLaunchKernelGGL((k_extract_keys<K>), dim3(grid), dim3(256), 0, st,
                        static_cast<const unsigned char *>(d_recs), uint64_t(n), rec_size, key_offset,
-                       key_kind == HPCS_KEY_FLOAT ? 1 : 0, keys, idx);
+                       key_kind, keys, idx);
     hip_check(hipGetLastError(), "k_extract_keys launch");
     run_sort<K, true, false>(c, keys, idx, n, st);
     const int words = ((rec_size & 3) == 0 && (reinterpret_cast<uintptr_t>(d_recs) & 3) == 0) ? 1 : 0;
@@ -676,7 +676,8 @@ void check_records_args(int64_t n, int32_t rec_size, int32_t key_offset, int32_t
 {
     if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
     if (key_bits != 32 && key_bits != 64) fail(HPCS_E_INVALID_ARG, "key_bits must be 32 or 64");
-    if (key_kind != HPCS_KEY_UINT && key_kind != HPCS_KEY_FLOAT) fail(HPCS_E_INVALID_ARG, "unknown key_kind");
+    if (key_kind != HPCS_KEY_UINT && key_kind != HPCS_KEY_FLOAT && key_kind != HPCS_KEY_INT)
+        fail(HPCS_E_INVALID_ARG, "unknown key_kind");
     if (rec_size <= 0 || key_offset < 0 || key_offset + key_bits / 8 > rec_size)
         fail(HPCS_E_INVALID_ARG, "key does not fit inside the record");
     if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
@@ -694,7 +695,7 @@ void dev_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t
         run_records<uint64_t>(c, d_recs, n, rec_size, key_offset, key_kind, st);
 }
 
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 void dev_histogram(hpcs_ctx *c, const K *d_keys, int64_t n, uint32_t *d_counts, hipStream_t st)
 {
     if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
@@ -708,8 +709,39 @@ void dev_histogram(hpcs_ctx *c, const K *d_keys, int64_t n, uint32_t *d_counts, 
 // Device-resident calls run on the caller's stream; NULL is HIP's null stream.
 hipStream_t pick_stream(hpcs_ctx *, void *stream) { return static_cast<hipStream_t>(stream); }
 
+// Byte histogram of the key inside each record (the key extracted into the
+// context's scratch, then the plain histogram).
+void dev_histogram_records(hpcs_ctx *c, const void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset,
+                           int32_t key_bits, uint32_t *d_counts, hipStream_t st)
+{
+    check_records_args(n, rec_size, key_offset, key_bits, HPCS_KEY_UINT);
+    if (!d_counts) fail(HPCS_E_INVALID_ARG, "counts is null");
+    if (n > 0 && !d_recs) fail(HPCS_E_INVALID_ARG, "records is null");
+    const size_t ks = size_t(key_bits / 8);
+    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, align_up(size_t(std::max<int64_t>(n, 1)) * ks)));
+    const int grid = grid_for(uint64_t(std::max<int64_t>(n, 1)), 256, c->num_cus * 8);
+    auto recs = static_cast<const unsigned char *>(d_recs);
+    if (key_bits == 32) {
+        if (n > 0)
+            timed(c, st, "extract", [&] {
+                hipLaunchKernelGGL((k_extract_keys<uint32_t>), dim3(grid), dim3(256), 0, st, recs, uint64_t(n), rec_size,
+                                   key_offset, int(HPCS_KEY_UINT), reinterpret_cast<uint32_t *>(base),
+                                   static_cast<uint32_t *>(nullptr));
+            });
+        dev_histogram<uint32_t, KIND_UINT>(c, reinterpret_cast<const uint32_t *>(base), n, d_counts, st);
+    } else {
+        if (n > 0)
+            timed(c, st, "extract", [&] {
+                hipLaunchKernelGGL((k_extract_keys<uint64_t>), dim3(grid), dim3(256), 0, st, recs, uint64_t(n), rec_size,
+                                   key_offset, int(HPCS_KEY_UINT), reinterpret_cast<uint64_t *>(base),
+                                   static_cast<uint32_t *>(nullptr));
+            });
+        dev_histogram<uint64_t, KIND_UINT>(c, reinterpret_cast<const uint64_t *>(base), n, d_counts, st);
+    }
+}
+
 // Host-array call: copy in, sort on the device, copy out, wait.
-template <typename K, bool HAS_VALS, bool FLOATK>
+template <typename K, bool HAS_VALS, int FLOATK>
 void host_sort(hpcs_ctx *c, K *keys, uint32_t *vals, int64_t n)
 {
     if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
@@ -732,7 +764,7 @@ void host_sort(hpcs_ctx *c, K *keys, uint32_t *vals, int64_t n)
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
-template <typename K>
+template <typename K, int KIND = KIND_UINT>
 void host_histogram(hpcs_ctx *c, const K *a, int64_t l, int64_t r, int32_t *counts)
 {
     if (!a) fail(HPCS_E_INVALID_ARG, "inArray is null");
@@ -747,7 +779,7 @@ void host_histogram(hpcs_ctx *c, const K *a, int64_t l, int64_t r, int32_t *coun
     uint32_t *d_counts = reinterpret_cast<uint32_t *>(io + hoff);
     hipStream_t st = c->stream;
     if (n > 0) hip_check(hipMemcpyAsync(d_keys, a + l, size_t(n) * sizeof(K), hipMemcpyHostToDevice, st), "H2D");
-    dev_histogram<K, false>(c, d_keys, n, d_counts, st);
+    dev_histogram<K, KIND>(c, d_keys, n, d_counts, st);
     uint32_t tmp[8 * RADIX];
     hip_check(hipMemcpyAsync(tmp, d_counts, P * RADIX * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "D2H");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -1171,5 +1203,145 @@ HPCS_API int32_t hpcs_dev_topk_f64(hpcs_ctx *ctx, const double *d_keys, int64_t 
     return guarded(ctx, [&] {
         run_topk<uint64_t, true>(ctx, reinterpret_cast<const uint64_t *>(d_keys), n, k,
                                  reinterpret_cast<uint64_t *>(d_out), nullptr, true, pick_stream(ctx, stream));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Signed keys: int[] / long[] (RadixSortLsd.cs:267-357, 444-532;
+// RadixSortLsdParallel.cs:446-668; Histogram.cs:992-1052): the sign bit is
+// flipped on the way in and out, so only the top digit differs from the
+// unsigned sort, as in the reference (its top digit ^ 128).
+// ---------------------------------------------------------------------------
+
+HPCS_API int32_t hpcs_sort_radix_i32(hpcs_ctx *ctx, int32_t *keys, int64_t n)
+{
+    return guarded(ctx, [&] { host_sort<uint32_t, false, KIND_INT>(ctx, reinterpret_cast<uint32_t *>(keys), nullptr, n); });
+}
+
+HPCS_API int32_t hpcs_sort_radix_i64(hpcs_ctx *ctx, int64_t *keys, int64_t n)
+{
+    return guarded(ctx, [&] { host_sort<uint64_t, false, KIND_INT>(ctx, reinterpret_cast<uint64_t *>(keys), nullptr, n); });
+}
+
+HPCS_API int32_t hpcs_dev_sort_radix_i32(hpcs_ctx *ctx, int32_t *d_keys, int64_t n, void *stream)
+{
+    return guarded(ctx, [&] {
+        run_sort<uint32_t, false, KIND_INT>(ctx, reinterpret_cast<uint32_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
+    });
+}
+
+HPCS_API int32_t hpcs_dev_sort_radix_i64(hpcs_ctx *ctx, int64_t *d_keys, int64_t n, void *stream)
+{
+    return guarded(ctx, [&] {
+        run_sort<uint64_t, false, KIND_INT>(ctx, reinterpret_cast<uint64_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
+    });
+}
+
+HPCS_API int32_t hpcs_histogram_byte_components_i32(hpcs_ctx *ctx, const int32_t *a, int64_t l, int64_t r,
+                                                    int32_t *counts)
+{
+    return guarded(ctx, [&] {
+        host_histogram<uint32_t, KIND_INT>(ctx, reinterpret_cast<const uint32_t *>(a), l, r, counts);
+    });
+}
+
+HPCS_API int32_t hpcs_histogram_byte_components_i64(hpcs_ctx *ctx, const int64_t *a, int64_t l, int64_t r,
+                                                    int32_t *counts)
+{
+    return guarded(ctx, [&] {
+        host_histogram<uint64_t, KIND_INT>(ctx, reinterpret_cast<const uint64_t *>(a), l, r, counts);
+    });
+}
+
+HPCS_API int32_t hpcs_dev_histogram_byte_components_i32(hpcs_ctx *ctx, const int32_t *d_keys, int64_t n,
+                                                        uint32_t *d_counts, void *stream)
+{
+    return guarded(ctx, [&] {
+        dev_histogram<uint32_t, KIND_INT>(ctx, reinterpret_cast<const uint32_t *>(d_keys), n, d_counts,
+                                          pick_stream(ctx, stream));
+    });
+}
+
+HPCS_API int32_t hpcs_dev_histogram_byte_components_i64(hpcs_ctx *ctx, const int64_t *d_keys, int64_t n,
+                                                        uint32_t *d_counts, void *stream)
+{
+    return guarded(ctx, [&] {
+        dev_histogram<uint64_t, KIND_INT>(ctx, reinterpret_cast<const uint64_t *>(d_keys), n, d_counts,
+                                          pick_stream(ctx, stream));
+    });
+}
+
+HPCS_API int32_t hpcs_topk_i32(hpcs_ctx *ctx, const int32_t *keys, int64_t n, int64_t k, int32_t *out)
+{
+    return guarded(ctx, [&] {
+        host_topk<uint32_t, KIND_INT>(ctx, reinterpret_cast<const uint32_t *>(keys), n, k, reinterpret_cast<uint32_t *>(out));
+    });
+}
+
+HPCS_API int32_t hpcs_topk_i64(hpcs_ctx *ctx, const int64_t *keys, int64_t n, int64_t k, int64_t *out)
+{
+    return guarded(ctx, [&] {
+        host_topk<uint64_t, KIND_INT>(ctx, reinterpret_cast<const uint64_t *>(keys), n, k, reinterpret_cast<uint64_t *>(out));
+    });
+}
+
+HPCS_API int32_t hpcs_dev_topk_i32(hpcs_ctx *ctx, const int32_t *d_keys, int64_t n, int64_t k, int32_t *d_out,
+                                   void *stream)
+{
+    return guarded(ctx, [&] {
+        run_topk<uint32_t, KIND_INT>(ctx, reinterpret_cast<const uint32_t *>(d_keys), n, k,
+                                     reinterpret_cast<uint32_t *>(d_out), nullptr, true, pick_stream(ctx, stream));
+    });
+}
+
+HPCS_API int32_t hpcs_dev_topk_i64(hpcs_ctx *ctx, const int64_t *d_keys, int64_t n, int64_t k, int64_t *d_out,
+                                   void *stream)
+{
+    return guarded(ctx, [&] {
+        run_topk<uint64_t, KIND_INT>(ctx, reinterpret_cast<const uint64_t *>(d_keys), n, k,
+                                     reinterpret_cast<uint64_t *>(d_out), nullptr, true, pick_stream(ctx, stream));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Key-extractor histograms: HistogramByteComponents<T>(T[], l, r,
+// Func<T, UInt32 / UInt64>) (Histogram.cs:760-844) with the key at a fixed
+// offset inside each record.
+// ---------------------------------------------------------------------------
+
+HPCS_API int32_t hpcs_histogram_byte_components_records(hpcs_ctx *ctx, const void *recs, int64_t l, int64_t r,
+                                                        int32_t rec_size, int32_t key_offset, int32_t key_bits,
+                                                        int32_t *counts)
+{
+    return guarded(ctx, [&] {
+        if (!recs) fail(HPCS_E_INVALID_ARG, "inArray is null");
+        if (!counts) fail(HPCS_E_INVALID_ARG, "counts is null");
+        if (l < 0) fail(HPCS_E_INVALID_ARG, "l must be non-negative");
+        const int64_t n = r >= l ? r - l + 1 : 0;
+        check_records_args(n, rec_size, key_offset, key_bits, HPCS_KEY_UINT);
+        if (n >= (int64_t(1) << 31)) fail(HPCS_E_UNSUPPORTED, "int32 counts limit the range to 2^31 - 1 keys");
+        const int P = key_bits / 8;
+        const size_t hoff = align_up(size_t(n) * size_t(rec_size));
+        char *io = static_cast<char *>(ensure(ctx->io, ctx->io_cap, hoff + P * RADIX * sizeof(uint32_t)));
+        uint32_t *d_counts = reinterpret_cast<uint32_t *>(io + hoff);
+        hipStream_t st = ctx->stream;
+        if (n > 0)
+            hip_check(hipMemcpyAsync(io, static_cast<const unsigned char *>(recs) + size_t(l) * size_t(rec_size),
+                                     size_t(n) * size_t(rec_size), hipMemcpyHostToDevice, st),
+                      "H2D");
+        dev_histogram_records(ctx, io, n, rec_size, key_offset, key_bits, d_counts, st);
+        uint32_t tmp[8 * RADIX];
+        hip_check(hipMemcpyAsync(tmp, d_counts, P * RADIX * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "D2H");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        for (int i = 0; i < P * RADIX; ++i) counts[i] = int32_t(tmp[i]);
+    });
+}
+
+HPCS_API int32_t hpcs_dev_histogram_byte_components_records(hpcs_ctx *ctx, const void *d_recs, int64_t n,
+                                                            int32_t rec_size, int32_t key_offset, int32_t key_bits,
+                                                            uint32_t *d_counts, void *stream)
+{
+    return guarded(ctx, [&] {
+        dev_histogram_records(ctx, d_recs, n, rec_size, key_offset, key_bits, d_counts, pick_stream(ctx, stream));
     });
 }

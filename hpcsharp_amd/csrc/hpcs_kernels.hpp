@@ -83,6 +83,20 @@ template <> struct KeyOps<uint64_t> {
     __device__ static inline uint32_t digit(uint64_t k, int shift) { return uint32_t(k >> shift) & (RADIX - 1); }
 };
 
+// Key kinds: 0 unsigned (the bits are the key), 1 IEEE float (sign flip for
+// non-negatives, all bits flipped for negatives: Histogram.cs:1210-1288,
+// RadixSortMsd.cs:1379-1383), 2 two's-complement signed (sign bit flipped:
+// the reference's top digit ^ 128, RadixSortLsd.cs:293, Histogram.cs:1019).
+constexpr int KIND_UINT = 0, KIND_FLOAT = 1, KIND_INT = 2;
+template <typename K, int KIND> __device__ inline K key_enc(K b)
+{
+    return KIND == KIND_FLOAT ? KeyOps<K>::to_key(b) : KIND == KIND_INT ? K(b ^ KeyOps<K>::SIGN) : b;
+}
+template <typename K, int KIND> __device__ inline K key_dec(K k)
+{
+    return KIND == KIND_FLOAT ? KeyOps<K>::from_key(k) : KIND == KIND_INT ? K(k ^ KeyOps<K>::SIGN) : k;
+}
+
 __device__ inline uint32_t lb_load(const uint32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -148,7 +162,7 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t words, uint32_t va
 // ---------------------------------------------------------------------------
 // `head` (< VEC) keys precede the first 16-byte aligned key; they are counted
 // lane by lane together with the tail.
-template <typename K, bool FLOATK, int BLOCK>
+template <typename K, int FLOATK, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys, uint64_t n, uint32_t head,
                                                      uint32_t *__restrict__ hist)
 {
@@ -174,7 +188,7 @@ __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys,
 #pragma unroll
         for (int j = 0; j < VEC; ++j) {
             const K raw = v[j];
-            K key = FLOATK ? KeyOps<K>::to_key(raw) : raw;
+            K key = FLOATK ? key_enc<K, FLOATK>(raw) : raw;
 #pragma unroll
             for (int d = 0; d < P; ++d) {
                 const uint32_t dg = KeyOps<K>::digit(key, 8 * d);
@@ -193,7 +207,7 @@ __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys,
         const uint64_t extra = (n < head ? n : uint64_t(head)) + (n > tail0 ? n - tail0 : 0);
         for (uint64_t t = threadIdx.x; t < extra; t += BLOCK) {
             const uint64_t i = t < head ? t : tail0 + (t - head);
-            K key = FLOATK ? KeyOps<K>::to_key(keys[i]) : keys[i];
+            K key = FLOATK ? key_enc<K, FLOATK>(keys[i]) : keys[i];
 #pragma unroll
             for (int d = 0; d < P; ++d) atomicAdd(&sh[d * RADIX + KeyOps<K>::digit(key, 8 * d)], 1u);
         }
@@ -295,7 +309,7 @@ constexpr int RANK_LDSOR = 1;
 // kernel takes the value as an argument so tests can force the fallback.
 constexpr int FALLBACK_ROUNDS = 24;
 
-template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int RANK, int LOOKBACK_WINDOW>
+template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int RANK, int LOOKBACK_WINDOW>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__restrict__ keysB,
                                                     uint32_t *__restrict__ valsA, uint32_t *__restrict__ valsB,
                                                     uint32_t n, int pass, DevPlan *__restrict__ plan,
@@ -370,7 +384,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     if (xin) {
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i)
-            if (i * WAVE + lane < wave_valid) keys[i] = KeyOps<K>::to_key(keys[i]);
+            if (i * WAVE + lane < wave_valid) keys[i] = key_enc<K, FLOATK>(keys[i]);
     }
 
     // ---- 1. per-wave digit counts ----
@@ -532,7 +546,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
                 const uint32_t qn = min(uint32_t(TILE), n - qbase);
                 for (uint32_t j = tid; j < qn; j += BLOCK) {
                     const K k = kin[qbase + j];
-                    atomicAdd(&s_fb[KeyOps<K>::digit(xin ? KeyOps<K>::to_key(k) : k, shift)], 1u);
+                    atomicAdd(&s_fb[KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k) : k, shift)], 1u);
                 }
                 __syncthreads();
                 if (!done && uint32_t(p) == q) {
@@ -562,7 +576,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
         if (j < tile_valid) {
             const K k = s_keys[j];
             const uint32_t dst = s_base[KeyOps<K>::digit(k, shift)] + uint32_t(j);
-            kout[dst] = xout ? KeyOps<K>::from_key(k) : k;
+            kout[dst] = xout ? key_dec<K, FLOATK>(k) : k;
             if constexpr (HAS_VALS) vout[dst] = s_vals[j];
         }
     }
@@ -585,7 +599,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
 // per-wave counts, tile digit starts by a block scan, keys staged in LDS in
 // sorted order, then written in order so each digit's run is contiguous.
 // ---------------------------------------------------------------------------
-template <typename K, bool HAS_VALS, bool FLOATK, int BLOCK, int ITEMS, int RANK>
+template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int RANK>
 __global__ __launch_bounds__(BLOCK) void k_scatter(K *__restrict__ keysA, K *__restrict__ keysB,
                                                    uint32_t *__restrict__ valsA, uint32_t *__restrict__ valsB,
                                                    uint32_t n, int pass, const DevPlan *__restrict__ plan,
@@ -653,7 +667,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(K *__restrict__ keysA, K *__r
         K keys[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
-            keys[j] = (xin && j * WAVE + lane < wave_valid) ? KeyOps<K>::to_key(cur[j]) : cur[j];
+            keys[j] = (xin && j * WAVE + lane < wave_valid) ? key_enc<K, FLOATK>(cur[j]) : cur[j];
 
         for (int q = tid; q < WAVES * RADIX; q += BLOCK) (&s_cnt[0][0])[q] = 0u;
         if constexpr (RANK == RANK_LDSOR)
@@ -759,7 +773,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(K *__restrict__ keysA, K *__r
             if (jj < tile_valid) {
                 const K k = s_keys[jj];
                 const uint32_t dst = s_base[KeyOps<K>::digit(k, shift)] + uint32_t(jj);
-                kout[dst] = xout ? KeyOps<K>::from_key(k) : k;
+                kout[dst] = xout ? key_dec<K, FLOATK>(k) : k;
                 if constexpr (HAS_VALS) vout[dst] = s_vals[jj];
             }
         }
@@ -815,7 +829,7 @@ __device__ inline void load_tile_keys(const K *__restrict__ p, uint32_t cnt, boo
     }
 }
 
-template <typename K, bool FLOATK, int BLOCK, int ITEMS>
+template <typename K, int FLOATK, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ keysA, const K *__restrict__ keysB,
                                                        uint32_t n, int pass, const DevPlan *__restrict__ plan,
                                                        uint32_t *__restrict__ counts, int vec)
@@ -838,7 +852,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t j = uint32_t(i * BLOCK + tid);
         const bool valid = j < cnt;
-        const uint32_t d = KeyOps<K>::digit(xin ? KeyOps<K>::to_key(k[i]) : k[i], shift);
+        const uint32_t d = KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[i]) : k[i], shift);
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
         if (__all(valid && d == d0)) {
             if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
@@ -947,7 +961,7 @@ __global__ __launch_bounds__(RADIX) void k_range_offsets(uint32_t *__restrict__ 
 constexpr int ORSLOTS = 64;
 template <typename K> constexpr int OR_STRIDE = 128 / int(sizeof(K)); // one 128-byte line per slot
 
-template <typename K, bool FLOATK, int BLOCK, int ITEMS>
+template <typename K, int FLOATK, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, uint32_t n,
                                                    uint32_t *__restrict__ counts, K *__restrict__ orslots, int vec)
 {
@@ -967,7 +981,7 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t j = uint32_t(i * BLOCK + tid);
         const bool valid = j < cnt;
-        const K key = FLOATK ? KeyOps<K>::to_key(k[i]) : k[i];
+        const K key = FLOATK ? key_enc<K, FLOATK>(k[i]) : k[i];
         if (valid) {
             vor |= key;
             vnor |= ~key;
@@ -1057,7 +1071,7 @@ __global__ void k_copyback(K *__restrict__ keysA, const K *__restrict__ keysB, u
 // ---------------------------------------------------------------------------
 template <typename K>
 __global__ void k_extract_keys(const unsigned char *__restrict__ recs, uint64_t n, int rec_size, int key_offset,
-                               int floatk, K *__restrict__ keys, uint32_t *__restrict__ idx)
+                               int kind, K *__restrict__ keys, uint32_t *__restrict__ idx)
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -1069,8 +1083,8 @@ __global__ void k_extract_keys(const unsigned char *__restrict__ recs, uint64_t 
             k = 0;
             for (int b = 0; b < int(sizeof(K)); ++b) k |= K(p[b]) << (8 * b);
         }
-        keys[i] = floatk ? KeyOps<K>::to_key(k) : k;
-        idx[i] = uint32_t(i);
+        keys[i] = kind == KIND_FLOAT ? KeyOps<K>::to_key(k) : kind == KIND_INT ? K(k ^ KeyOps<K>::SIGN) : k;
+        if (idx) idx[i] = uint32_t(i);
     }
 }
 

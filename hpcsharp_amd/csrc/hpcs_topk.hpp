@@ -97,15 +97,15 @@ struct TopkState {
     TopkSel sel[TOPK_MAX_LEVELS + 1]; // sel[l + 1] written by level l
 };
 
-// Order key of a value: the float transform (sign flip / all-bits flip), or
-// the value itself.
-template <typename K, bool FLOATK> __device__ inline K order_key(K raw)
+// Order key of a value: the key-kind transform (float: sign flip / all-bits
+// flip; signed: sign flip; unsigned: the value itself).
+template <typename K, int FLOATK> __device__ inline K order_key(K raw)
 {
-    return FLOATK ? KeyOps<K>::to_key(raw) : raw;
+    return key_enc<K, FLOATK>(raw);
 }
-template <typename K, bool FLOATK> __device__ inline K value_of(K u)
+template <typename K, int FLOATK> __device__ inline K value_of(K u)
 {
-    return FLOATK ? KeyOps<K>::from_key(u) : u;
+    return key_dec<K, FLOATK>(u);
 }
 // Sort key of an order key: ascending sort keys give the requested order.
 template <typename K> __device__ inline K sort_key(K u, int desc)
@@ -130,7 +130,7 @@ __device__ inline void wave_count(uint32_t *s_hist, uint32_t d, bool valid)
 }
 
 // Level-0 counts over the whole input (grid-stride over tiles).
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 __global__ __launch_bounds__(TOPK_HIST_BLOCK) void k_topk_hist(const K *__restrict__ in, uint32_t n, int vec,
                                                                TopkState *__restrict__ st)
 {
@@ -235,7 +235,7 @@ template <typename K> struct WaveStage {
 // level's candidates, their next digit counted on the way; lower keys go to
 // `rest` as values.  Each wave stages its appends in LDS, so the tiles need
 // no workgroup barrier.
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 __global__ __launch_bounds__(TOPK_BLOCK) void k_topk_compact(const K *__restrict__ in, uint32_t n_in, int in_vec,
                                                              const K *__restrict__ cand_src, K *__restrict__ cand_dst,
                                                              int level, TopkState *__restrict__ st,
@@ -351,7 +351,7 @@ __device__ inline TopkTail<K> topk_tail(const TopkState *__restrict__ st, uint32
 }
 
 // Copies of T at the ends of the output (positions [m - need, m)) and rest.
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 __global__ void k_topk_finish(const TopkState *__restrict__ st, K *__restrict__ out, K *__restrict__ rest,
                               uint32_t m, uint32_t n, int desc)
 {
@@ -369,7 +369,7 @@ __global__ void k_topk_finish(const TopkState *__restrict__ st, K *__restrict__ 
 
 // Sort keys back to values: ~s for descending order, then the inverse float
 // transform.
-template <typename K, bool FLOATK> __device__ inline K topk_value(K s, int desc)
+template <typename K, int FLOATK> __device__ inline K topk_value(K s, int desc)
 {
     return value_of<K, FLOATK>(desc ? K(~s) : s);
 }
@@ -398,7 +398,7 @@ __device__ inline void bitonic_in_thread(K (&v)[TOPK_SORT_ITEMS], uint32_t e0, u
 //   mode 1: src[0, m - need) are sort keys appended by the compaction, and
 //           the need copies of the threshold key come from the state.
 // Equal sort keys are equal values, so the order among them is immaterial.
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 __global__ __launch_bounds__(TOPK_SORT_BLOCK) void k_topk_block_sort(const K *__restrict__ src, uint32_t len,
                                                                      K *__restrict__ dst, uint32_t m, int mode,
                                                                      const TopkState *__restrict__ st, uint32_t n,
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(TOPK_SORT_BLOCK) void k_topk_block_sort(const K *__
     }
 }
 
-template <typename K, bool FLOATK>
+template <typename K, int FLOATK>
 __global__ void k_topk_values(K *__restrict__ keys, uint64_t len, int desc)
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;

@@ -49,6 +49,7 @@ extern "C" {
 
 #define HPCS_KEY_UINT 0  /* record key is an unsigned integer */
 #define HPCS_KEY_FLOAT 1 /* record key is an IEEE float/double (radix key transform) */
+#define HPCS_KEY_INT 2   /* two's-complement signed key (int / long) */
 
 /* Tuning options for hpcs_ctx_set_option. */
 #define HPCS_OPT_RANK 1       /* 0: ballot match, 1: LDS-OR match (default) */
@@ -177,6 +178,37 @@ int32_t hpcs_dev_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *d_
 int32_t hpcs_dev_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, uint32_t *d_counts,
                                                void *stream);
 
+/* Replace HistogramByteComponents<T>(T[] inArray, int l, int r,
+ * Func<T, UInt32> / Func<T, UInt64> getKey) (Histogram.cs:760-844) for a
+ * key stored at key_offset inside records of rec_size bytes: counts is
+ * int32[key_bits / 8][256] of the key bytes of records l..r (inclusive). */
+int32_t hpcs_histogram_byte_components_records(hpcs_ctx *ctx, const void *recs, int64_t l, int64_t r, int32_t rec_size,
+                                               int32_t key_offset, int32_t key_bits, int32_t *counts);
+int32_t hpcs_dev_histogram_byte_components_records(hpcs_ctx *ctx, const void *d_recs, int64_t n, int32_t rec_size,
+                                                   int32_t key_offset, int32_t key_bits, uint32_t *d_counts,
+                                                   void *stream);
+
+/* ------------------------------------------------------------------------
+ * Signed keys: int[] and long[]
+ * ------------------------------------------------------------------------ */
+
+/* Replace Algorithm.SortRadixLsd(this int[]) / (this long[])
+ * (RadixSortLsd.cs:267-357, 444-532) and ParallelAlgorithm.SortRadixPar(int[]
+ * / long[]) (RadixSortLsdParallel.cs:446-668): ascending, in place. */
+int32_t hpcs_sort_radix_i32(hpcs_ctx *ctx, int32_t *keys, int64_t n);
+int32_t hpcs_sort_radix_i64(hpcs_ctx *ctx, int64_t *keys, int64_t n);
+int32_t hpcs_dev_sort_radix_i32(hpcs_ctx *ctx, int32_t *d_keys, int64_t n, void *stream);
+int32_t hpcs_dev_sort_radix_i64(hpcs_ctx *ctx, int64_t *d_keys, int64_t n, void *stream);
+/* Replace Algorithm.HistogramByteComponents(int[] / long[], l, r)
+ * (Histogram.cs:992-1052): as the unsigned histograms, except that the top
+ * digit counts (byte ^ 128), the reference's order of signed keys. */
+int32_t hpcs_histogram_byte_components_i32(hpcs_ctx *ctx, const int32_t *a, int64_t l, int64_t r, int32_t *counts);
+int32_t hpcs_histogram_byte_components_i64(hpcs_ctx *ctx, const int64_t *a, int64_t l, int64_t r, int32_t *counts);
+int32_t hpcs_dev_histogram_byte_components_i32(hpcs_ctx *ctx, const int32_t *d_keys, int64_t n, uint32_t *d_counts,
+                                               void *stream);
+int32_t hpcs_dev_histogram_byte_components_i64(hpcs_ctx *ctx, const int64_t *d_keys, int64_t n, uint32_t *d_counts,
+                                               void *stream);
+
 /* ------------------------------------------------------------------------
  * Radix top-k
  * ------------------------------------------------------------------------ */
@@ -199,6 +231,8 @@ int32_t hpcs_topk_u32(hpcs_ctx *ctx, const uint32_t *keys, int64_t n, int64_t k,
 int32_t hpcs_topk_u64(hpcs_ctx *ctx, const uint64_t *keys, int64_t n, int64_t k, uint64_t *out);
 int32_t hpcs_topk_f32(hpcs_ctx *ctx, const float *keys, int64_t n, int64_t k, float *out);
 int32_t hpcs_topk_f64(hpcs_ctx *ctx, const double *keys, int64_t n, int64_t k, double *out);
+int32_t hpcs_topk_i32(hpcs_ctx *ctx, const int32_t *keys, int64_t n, int64_t k, int32_t *out);
+int32_t hpcs_topk_i64(hpcs_ctx *ctx, const int64_t *keys, int64_t n, int64_t k, int64_t *out);
 /* The same on device buffers, stream-ordered; the launch sequence depends
  * only on (n, k), so the call can be captured in a HIP graph after one
  * warm-up call of the same size. */
@@ -208,6 +242,10 @@ int32_t hpcs_dev_topk_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, int6
                           void *stream);
 int32_t hpcs_dev_topk_f32(hpcs_ctx *ctx, const float *d_keys, int64_t n, int64_t k, float *d_out, void *stream);
 int32_t hpcs_dev_topk_f64(hpcs_ctx *ctx, const double *d_keys, int64_t n, int64_t k, double *d_out, void *stream);
+int32_t hpcs_dev_topk_i32(hpcs_ctx *ctx, const int32_t *d_keys, int64_t n, int64_t k, int32_t *d_out,
+                          void *stream);
+int32_t hpcs_dev_topk_i64(hpcs_ctx *ctx, const int64_t *d_keys, int64_t n, int64_t k, int64_t *d_out,
+                          void *stream);
 
 /* ------------------------------------------------------------------------
  * Serving loop (no counterpart in the reference: the serving pattern built

@@ -532,17 +532,18 @@ int hpcs_oracle_sort_radix_f64(double *a, int64_t n)
 
 #define HPCS_KEY_UINT 0
 #define HPCS_KEY_FLOAT 1
+#define HPCS_KEY_INT 2
 
 static uint64_t record_key(const unsigned char *rec, int key_offset, int key_bits, int key_kind)
 {
     if (key_bits == 32) {
         uint32_t k;
         memcpy(&k, rec + key_offset, 4);
-        return key_kind == HPCS_KEY_FLOAT ? hpcs_oracle_f32_key(k) : k;
+        return key_kind == HPCS_KEY_FLOAT ? hpcs_oracle_f32_key(k) : key_kind == HPCS_KEY_INT ? (k ^ 0x80000000u) : k;
     } else {
         uint64_t k;
         memcpy(&k, rec + key_offset, 8);
-        return key_kind == HPCS_KEY_FLOAT ? hpcs_oracle_f64_key(k) : k;
+        return key_kind == HPCS_KEY_FLOAT ? hpcs_oracle_f64_key(k) : key_kind == HPCS_KEY_INT ? (k ^ 0x8000000000000000ull) : k;
     }
 }
 
@@ -556,7 +557,7 @@ int hpcs_oracle_sort_radix_records(void *recs, int64_t n, int32_t rec_size, int3
     if (!recs && n > 0) return HPCS_ORACLE_EINVAL;
     if (key_bits != 32 && key_bits != 64) return HPCS_ORACLE_EINVAL;
     if (key_offset < 0 || key_offset + key_bits / 8 > rec_size) return HPCS_ORACLE_EINVAL;
-    if (key_kind != HPCS_KEY_UINT && key_kind != HPCS_KEY_FLOAT) return HPCS_ORACLE_EINVAL;
+    if (key_kind != HPCS_KEY_UINT && key_kind != HPCS_KEY_FLOAT && key_kind != HPCS_KEY_INT) return HPCS_ORACLE_EINVAL;
     if (n <= 1) return HPCS_ORACLE_OK;
     int ndig = key_bits / 8;
     unsigned char *tmp = (unsigned char *)malloc((size_t)rec_size * (size_t)n);
@@ -786,4 +787,88 @@ int hpcs_oracle_select_topk_u32(uint32_t *a, int64_t n, int64_t k)
     free(count);
     free(sob);
     return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Signed keys                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Algorithm.HistogramByteComponents(int[], l, r), Histogram.cs:992-1017: the
+ * low three bytes as they are, the top byte ^ 128. */
+int hpcs_oracle_histogram_byte_components_i32(const int32_t *a, int64_t l, int64_t r, int32_t *count)
+{
+    if (!a || !count) return HPCS_ORACLE_EINVAL;
+    memset(count, 0, sizeof(int32_t) * 4 * 256);
+    for (int64_t i = l; i <= r; i++) {
+        uint32_t u = (uint32_t)a[i];
+        count[0 * 256 + (u & 0xff)]++;
+        count[1 * 256 + ((u >> 8) & 0xff)]++;
+        count[2 * 256 + ((u >> 16) & 0xff)]++;
+        count[3 * 256 + ((u >> 24) ^ 128)]++;
+    }
+    return HPCS_ORACLE_OK;
+}
+
+/* HistogramByteComponents(long[], l, r), Histogram.cs:1019-1052. */
+int hpcs_oracle_histogram_byte_components_i64(const int64_t *a, int64_t l, int64_t r, int32_t *count)
+{
+    if (!a || !count) return HPCS_ORACLE_EINVAL;
+    memset(count, 0, sizeof(int32_t) * 8 * 256);
+    for (int64_t i = l; i <= r; i++) {
+        uint64_t u = (uint64_t)a[i];
+        for (int d = 0; d < 7; d++) count[d * 256 + ((u >> (8 * d)) & 0xff)]++;
+        count[7 * 256 + ((u >> 56) ^ 128)]++;
+    }
+    return HPCS_ORACLE_OK;
+}
+
+/* SortRadixLsdInner(this int[], start, length), RadixSortLsd.cs:267-300 (the
+ * long[] form :444-532): four (eight) stable byte passes whose last digit is
+ * the top byte ^ 128; the result is back in a after an even pass count. */
+int hpcs_oracle_sort_radix_lsd_i32(int32_t *a, int64_t n)
+{
+    if (!a && n > 0) return HPCS_ORACLE_EINVAL;
+    if (n <= 1) return HPCS_ORACLE_OK;
+    int32_t count[4 * 256];
+    int64_t sob[4 * 256];
+    hpcs_oracle_histogram_byte_components_i32(a, 0, n - 1, count);
+    start_of_bins(4, 256, count, 0, sob);
+    int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    if (!tmp) return HPCS_ORACLE_ENOMEM;
+    int32_t *in = a, *out = tmp;
+    for (int d = 0; d < 4; d++) {
+        int64_t *s = sob + d * 256;
+        for (int64_t i = 0; i < n; i++) {
+            uint32_t u = (uint32_t)in[i];
+            uint32_t dig = d < 3 ? ((u >> (8 * d)) & 0xff) : ((u >> 24) ^ 128);
+            out[s[dig]++] = in[i];
+        }
+        int32_t *t = in; in = out; out = t;
+    }
+    free(tmp);
+    return HPCS_ORACLE_OK;
+}
+
+int hpcs_oracle_sort_radix_lsd_i64(int64_t *a, int64_t n)
+{
+    if (!a && n > 0) return HPCS_ORACLE_EINVAL;
+    if (n <= 1) return HPCS_ORACLE_OK;
+    int32_t count[8 * 256];
+    int64_t sob[8 * 256];
+    hpcs_oracle_histogram_byte_components_i64(a, 0, n - 1, count);
+    start_of_bins(8, 256, count, 0, sob);
+    int64_t *tmp = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    if (!tmp) return HPCS_ORACLE_ENOMEM;
+    int64_t *in = a, *out = tmp;
+    for (int d = 0; d < 8; d++) {
+        int64_t *s = sob + d * 256;
+        for (int64_t i = 0; i < n; i++) {
+            uint64_t u = (uint64_t)in[i];
+            uint64_t dig = d < 7 ? ((u >> (8 * d)) & 0xff) : ((u >> 56) ^ 128);
+            out[s[dig]++] = in[i];
+        }
+        int64_t *t = in; in = out; out = t;
+    }
+    free(tmp);
+    return HPCS_ORACLE_OK;
 }

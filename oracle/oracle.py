@@ -21,6 +21,7 @@ _lib = None
 
 KEY_UINT = 0
 KEY_FLOAT = 1
+KEY_INT = 2
 
 
 def build() -> str:
@@ -56,6 +57,10 @@ def lib():
             "hpcs_oracle_sort_radix_pairs_u32_u32": (p, p, i64),
             "hpcs_oracle_sort_segments_u32": (p, p, i32),
             "hpcs_oracle_select_topk_u32": (p, i64, i64),
+            "hpcs_oracle_histogram_byte_components_i32": (p, i64, i64, p),
+            "hpcs_oracle_histogram_byte_components_i64": (p, i64, i64, p),
+            "hpcs_oracle_sort_radix_lsd_i32": (p, i64),
+            "hpcs_oracle_sort_radix_lsd_i64": (p, i64),
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -173,4 +178,22 @@ def f64_key(bits: int) -> int:
 def select_topk_u32(a: np.ndarray, k: int) -> np.ndarray:
     """SelectTopK(uint[], k): a[k] becomes the k-th smallest, a[k:] the n-k largest."""
     _check(lib().hpcs_oracle_select_topk_u32(_ptr(a), a.size, k), "select topk")
+    return a
+
+
+def histogram_byte_components_signed(a: np.ndarray, l: int, r: int) -> np.ndarray:
+    """HistogramByteComponents(int[] / long[], l, r): the top digit is byte ^ 128."""
+    if a.dtype == np.int32:
+        out = np.zeros((4, 256), dtype=np.int32)
+        _check(lib().hpcs_oracle_histogram_byte_components_i32(_ptr(a), l, r, _ptr(out)), "histogram i32")
+    else:
+        out = np.zeros((8, 256), dtype=np.int32)
+        _check(lib().hpcs_oracle_histogram_byte_components_i64(_ptr(a), l, r, _ptr(out)), "histogram i64")
+    return out
+
+
+def sort_radix_signed(a: np.ndarray) -> np.ndarray:
+    """SortRadixLsd(this int[] / long[]) in place."""
+    f = lib().hpcs_oracle_sort_radix_lsd_i32 if a.dtype == np.int32 else lib().hpcs_oracle_sort_radix_lsd_i64
+    _check(f(_ptr(a), a.size), "sort signed")
     return a

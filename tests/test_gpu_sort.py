@@ -447,3 +447,75 @@ def test_device_unaligned_subarray(oracle, offset):
     N.check(N.lib().hpcs_dev_sort_radix_u32(N.context(), ctypes.c_void_p(t.data_ptr() + 4 * offset), n, st))
     torch.cuda.synchronize()
     assert np.array_equal(t.cpu().numpy().view(np.uint32), ref)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("n", [1000, 70001, (1 << 20) + 3])
+def test_signed_keys_vs_oracle(oracle, dtype, n):
+    """int[] / long[]: sort, parallel sort, histogram (top byte ^ 128) and
+    top-k against the oracle's restatements (small n exercises the onesweep
+    pass, the largest the reduce-then-scan pass)."""
+    x = datagen.keys_u64(n, 900 + n).view(np.int64)
+    x = (x >> np.int64(32)).astype(np.int32) if dtype == np.int32 else x.copy()
+    x[:4] = [0, -1, np.iinfo(dtype).min, np.iinfo(dtype).max]
+    ref = x.copy()
+    oracle.sort_radix_signed(ref)
+    h = Algorithm.HistogramByteComponents(x, 0, n - 1)
+    assert np.array_equal(h, oracle.histogram_byte_components_signed(x, 0, n - 1))
+    top = Algorithm.TopK(x, 777)
+    assert np.array_equal(top, ref[::-1][:777])
+    a = x.copy()
+    Algorithm.SortRadixLsd(a)
+    assert np.array_equal(a, ref)
+    b = x.copy()
+    assert ParallelAlgorithm.SortRadixPar(b) is b
+    assert np.array_equal(b, ref)
+
+
+def test_signed_device_and_records(oracle):
+    import torch
+    n = 300007
+    x = (datagen.keys_u64(n, 77).view(np.int64) >> np.int64(40)).astype(np.int32)
+    ref = x.copy()
+    oracle.sort_radix_signed(ref)
+    t = torch.from_numpy(x.copy()).cuda()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    N.check(N.lib().hpcs_dev_sort_radix_i32(N.context(), ctypes.c_void_p(t.data_ptr()), n, st))
+    counts = torch.empty(4 * 256, dtype=torch.int32, device="cuda")
+    N.check(N.lib().hpcs_dev_histogram_byte_components_i32(N.context(), ctypes.c_void_p(t.data_ptr()), n,
+                                                           ctypes.c_void_p(counts.data_ptr()), st))
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy(), ref)
+    assert np.array_equal(counts.cpu().numpy().reshape(4, 256), oracle.histogram_byte_components_signed(x, 0, n - 1))
+    rec = np.zeros(n, dtype=record_dtype(np.uint64))
+    keys = (datagen.keys_u64(n, 78) & np.uint64(0xFFFFF)).astype(np.int64) - (1 << 19)
+    rec["key"] = keys.view(np.uint64)
+    rec["payload"] = np.arange(n, dtype=np.uint32)
+    want = rec.copy()
+    oracle.sort_radix_records(want, 0, 64, oracle.KEY_INT)
+    rc = N.lib().hpcs_sort_radix_records(N.context(), rec.ctypes.data_as(ctypes.c_void_p), n, rec.dtype.itemsize,
+                                         0, 64, N.HPCS_KEY_INT)
+    N.check(rc)
+    assert np.array_equal(rec["key"], want["key"]) and np.array_equal(rec["payload"], want["payload"])
+
+
+@pytest.mark.parametrize("key_type", [np.uint32, np.uint64])
+def test_key_extractor_histogram(oracle, key_type):
+    """HistogramByteComponents<T>(T[], l, r, getKey): the byte counts of the
+    records' keys over [l, r] (brute force), host and device forms."""
+    import torch
+    n = 100003
+    rec = np.zeros(n, dtype=np.dtype({"names": ["pad", "key", "payload"], "formats": [np.uint8, key_type, np.uint32],
+                                      "offsets": [0, 4 if key_type == np.uint32 else 8, 16], "itemsize": 24}))
+    rec["key"] = datagen.keys_u64(n, 515).astype(key_type)
+    l, r = 17, n - 30
+    h = Algorithm.HistogramByteComponents(rec, l, r, "key")
+    assert np.array_equal(h, hist_bytes(np.ascontiguousarray(rec["key"][l:r + 1])))
+    d = torch.from_numpy(rec.view(np.uint8).reshape(-1).copy()).cuda()
+    counts = torch.empty(np.dtype(key_type).itemsize * 256, dtype=torch.int32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    N.check(N.lib().hpcs_dev_histogram_byte_components_records(
+        N.context(), ctypes.c_void_p(d.data_ptr()), n, 24, rec.dtype.fields["key"][1], 8 * np.dtype(key_type).itemsize,
+        ctypes.c_void_p(counts.data_ptr()), st))
+    torch.cuda.synchronize()
+    assert np.array_equal(counts.cpu().numpy().reshape(-1, 256), hist_bytes(np.ascontiguousarray(rec["key"])))

@@ -233,3 +233,35 @@ def test_select_topk_restatement(oracle, n, k, mask):
     oracle.select_topk_u32(a, k)
     assert a[k] == np.partition(orig, k)[k]
     assert np.array_equal(np.sort(a[k:]), np.sort(orig)[k:])
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_signed_sort_and_histogram_closed_forms(oracle, dtype):
+    """SortRadixLsd(int[]/long[]) equals np.sort; the histogram's top digit is
+    the top byte ^ 128, the other digits the plain bytes (bincount)."""
+    n = 40007
+    x = datagen.keys_u64(n, 4 + dtype().itemsize).view(np.int64)
+    x = (x >> np.int64(32)).astype(np.int32) if dtype == np.int32 else x.copy()
+    x[:5] = [0, -1, np.iinfo(dtype).min, np.iinfo(dtype).max, 1]
+    ref = np.sort(x)
+    h = oracle.histogram_byte_components_signed(x, 0, n - 1)
+    u = x.view(np.uint32 if dtype == np.int32 else np.uint64)
+    want = hist_bytes(u)
+    top = u.dtype.itemsize - 1
+    want[top] = np.bincount(((u >> np.array(8 * top, dtype=u.dtype)).astype(np.int64) ^ 128), minlength=256)
+    assert np.array_equal(h, want)
+    oracle.sort_radix_signed(x)
+    assert np.array_equal(x, ref)
+    bad = x.copy()
+    bad[[0, -1]] = bad[[-1, 0]]                       # planted mistake
+    assert not np.array_equal(bad, ref)
+
+
+def test_signed_record_keys(oracle):
+    n = 5000
+    rec = np.zeros(n, dtype=record_dtype(np.uint64))
+    keys = (datagen.keys_u64(n, 61) & np.uint64(0xFFFF)).astype(np.int64) - 30000
+    rec["key"] = keys.view(np.uint64)
+    rec["payload"] = np.arange(n, dtype=np.uint32)
+    oracle.sort_radix_records(rec, 0, 64, oracle.KEY_INT)
+    assert np.array_equal(rec["payload"], np.argsort(keys, kind="stable").astype(np.uint32))
