@@ -213,6 +213,23 @@ def cfg3_figures(torch, N, datagen, reps: int):
     return {"n": n, **out}
 
 
+def host_array_figures(N, keys: np.ndarray, reps: int):
+    """The host-array drop-in call (hpcs_sort_radix_u32 on pageable host
+    memory): PCIe copy in, sort, copy out; never the headline value."""
+    lib, ctx = N.lib(), N.context(0)
+    times = []
+    for r in range(reps + 1):
+        a = keys.copy()
+        t0 = time.perf_counter()
+        N.check(lib.hpcs_sort_radix_u32(ctx, a.ctypes.data_as(ctypes.c_void_p), a.size))
+        if r:
+            times.append(time.perf_counter() - t0)
+    ok = bool((a[1:] >= a[:-1]).all())
+    t = float(np.median(times))
+    return {"ms": round(t * 1e3, 2), "gkeys_per_s": round(keys.size / t / 1e9, 3), "sorted": ok,
+            "note": "includes host-to-device and device-to-host copies of the whole array"}
+
+
 def serving_figures(nreq: int, slots: int):
     """The serving workload: Poisson arrivals (2000/s), 70% sort / 30% top-k,
     n log-uniform in [2^10, 2^26), u32/f32/u64, 200 GB budget."""
@@ -254,6 +271,7 @@ def main():
     ap.add_argument("--no-dists", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-topk", action="store_true")
+    ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-cfg3", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=20000)
@@ -328,6 +346,9 @@ def main():
             print("bench: batched and plain outputs differ", file=sys.stderr)
             sys.exit(3)
 
+    if not args.no_host:
+        extra["host_array"] = host_array_figures(N, keys, 2)
+        log(f"[bench] host-array call: {extra['host_array']}")
     if not args.no_topk:
         extra["topk"] = topk_figures(torch, N, datagen, 5)
         log(f"[bench] top-k: {extra['topk']}")
