@@ -82,6 +82,8 @@ SIGNATURES = {
     "hpcs_dev_histogram_byte_components_u32": ([_p, _p, _i64, _p, _p], _i32),
     "hpcs_dev_histogram_byte_components_u64": ([_p, _p, _i64, _p, _p], _i32),
     "hpcs_ctx_reserve": ([_p, _i32, _i64], _i32),
+    "hpcs_ctx_load_tuning": ([_p, ctypes.c_char_p], _i32),
+    "hpcs_tuning_check": ([ctypes.c_char_p, _p], _i32),
     "hpcs_serve_run": ([_i32, _i32, _i64, _p, _i64, _i32, _p, _p], _i32),
     "hpcs_serve_last_error": ([], ctypes.c_char_p),
     "hpcs_histogram_byte_components_records": ([_p, _p, _i64, _i64, _i32, _i32, _i32, _p], _i32),
@@ -120,6 +122,11 @@ class HpcsError(RuntimeError):
         self.code = code
 
 
+# The tuning table measured on MI355X by hpcsharp_amd.autotune; the
+# process-wide contexts load it unless HPCS_TUNE_FILE names another.
+TUNE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_gfx950.txt")
+
+
 def lib() -> ctypes.CDLL:
     """Load libhpcsort.so; raise if it has not been built."""
     global _lib
@@ -148,8 +155,15 @@ def context(device: int = 0) -> ctypes.c_void_p:
         if ctx is None:
             ctx = ctypes.c_void_p()
             check(lib().hpcs_ctx_create(device, ctypes.byref(ctx)))
+            if not os.environ.get("HPCS_TUNE_FILE") and os.path.exists(TUNE_PATH):
+                check(lib().hpcs_ctx_load_tuning(ctx, TUNE_PATH.encode()))
             _contexts[device] = ctx
         return ctx
+
+
+def load_tuning(path: str | None, ctx=None) -> None:
+    """Replace a context's tuning table (None clears it)."""
+    check(lib().hpcs_ctx_load_tuning(ctx if ctx is not None else context(), (path or "").encode()))
 
 
 def new_context(device: int = 0) -> ctypes.c_void_p:

@@ -1,0 +1,129 @@
+"""Autotuner: measure every sort variant per key type and size class on this
+GPU and write the tuning table libhpcsort.so reads (hpcs_ctx_load_tuning).
+
+Variants: pass algorithm (onesweep with look-back, reduce-then-scan), tile
+shape (256x16, 512x16, 256x32, 512x32 threads x keys; 4-byte keys without
+values) and ranking (ballot, LDS-OR).  Digits are 8-bit in this version.
+Size classes are powers of two; a class's row applies to every n up to it.
+Each measured cell is appended to a JSON log as it finishes, and a rerun
+skips the cells the log already holds (the sweep resumes).
+
+Run on the GPU:  python -m hpcsharp_amd.autotune [--classes 12,16,20,24,28]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import itertools
+import json
+import os
+import sys
+import time
+
+from hpcsharp_amd import _native as N
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+# (table dtype, key bytes, with u32 values, entry point)
+KINDS = {
+    "u32": (4, False, "hpcs_dev_sort_radix_u32"),
+    "u64": (8, False, "hpcs_dev_sort_radix_u64"),
+    "pairs64": (8, True, "hpcs_dev_sort_radix_pairs_u64_u32"),
+}
+
+
+def candidates(key_bytes: int, vals: bool):
+    shapes = (0, 1, 2, 3) if key_bytes == 4 and not vals else (0,)
+    return [{"algo": a, "shape": s, "rank": r} for a, s, r in itertools.product((0, 1), shapes, (1, 0))]
+
+
+def time_variant(torch, ctx, kind: str, lg: int, cand: dict, reps: int) -> float:
+    """Median seconds per sort of 2^lg uniform keys (device-resident)."""
+    key_bytes, vals, fn = KINDS[kind]
+    n = 1 << lg
+    lib = N.lib()
+    N.set_option(N.HPCS_OPT_PASS_ALGO, cand["algo"], ctx)
+    N.set_option(N.HPCS_OPT_TILE_SHAPE, cand["shape"], ctx)
+    N.set_option(N.HPCS_OPT_RANK, cand["rank"], ctx)
+    g = torch.Generator(device="cuda").manual_seed(lg * 1000 + key_bytes)
+    src = torch.randint(-2 ** 31, 2 ** 31 - 1, (n * key_bytes // 4,), dtype=torch.int32, device="cuda", generator=g)
+    keys = src.clone()
+    v = torch.arange(n, dtype=torch.int32, device="cuda") if vals else None
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    f = getattr(lib, fn)
+
+    def call():
+        if vals:
+            N.check(f(ctx, ctypes.c_void_p(keys.data_ptr()), ctypes.c_void_p(v.data_ptr()), n, sp))
+        else:
+            N.check(f(ctx, ctypes.c_void_p(keys.data_ptr()), n, sp))
+    times = []
+    for r in range(reps + 1):
+        keys.copy_(src)
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(st)
+        call()
+        end.record(st)
+        end.synchronize()
+        if r:
+            times.append(start.elapsed_time(end) * 1e-3)
+    times.sort()
+    return times[len(times) // 2]
+
+
+def table_text(results: dict) -> str:
+    lines = ["# libhpcsort tuning table (hpcs_ctx_load_tuning), measured on "
+             + results.get("device", "this GPU") + " by hpcsharp_amd.autotune",
+             "# dtype max_n algo shape rank   # median ms per sort of the best variant"]
+    alias = {"u32": ("u32", "f32", "i32"), "u64": ("u64", "f64", "i64"), "pairs64": ("pairs64",)}
+    for kind in KINDS:
+        for lg in sorted({int(k.split(":")[1]) for k in results["cells"] if k.startswith(kind + ":")}):
+            cells = [c for k, c in results["cells"].items() if k.startswith(f"{kind}:{lg}:")]
+            best = min(cells, key=lambda c: c["s"])
+            for name in alias[kind]:
+                lines.append(f"{name} {1 << lg} {best['algo']} {best['shape']} {best['rank']}"
+                             f"   # {best['s'] * 1e3:.4f} ms")
+    return "\n".join(lines) + "\n"
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--classes", default="12,16,20,24,28")
+    ap.add_argument("--kinds", default="u32,u64,pairs64")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--log", default=os.path.join(HERE, "tune_gfx950.json"))
+    ap.add_argument("--out", default=N.TUNE_PATH)
+    args = ap.parse_args(argv)
+    import torch
+    if not torch.cuda.is_available():
+        print("autotune needs a HIP device", file=sys.stderr)
+        return 1
+    prop = torch.cuda.get_device_properties(0)
+    results = {"device": f"{prop.name} ({getattr(prop, 'gcnArchName', 'unknown arch')})", "cells": {}}
+    if os.path.exists(args.log):
+        with open(args.log) as f:
+            results = json.load(f)
+    ctx = N.new_context(0)
+    try:
+        for kind in args.kinds.split(","):
+            key_bytes, vals, _ = KINDS[kind]
+            for lg in (int(x) for x in args.classes.split(",")):
+                for cand in candidates(key_bytes, vals):
+                    key = f"{kind}:{lg}:{cand['algo']}:{cand['shape']}:{cand['rank']}"
+                    if key in results["cells"]:
+                        continue                     # measured in an earlier run
+                    s = time_variant(torch, ctx, kind, lg, cand, args.reps)
+                    results["cells"][key] = dict(cand, s=s)
+                    print(f"[autotune] {key}: {s * 1e3:.4f} ms", flush=True)
+                    with open(args.log, "w") as f:
+                        json.dump(results, f, indent=1, sort_keys=True)
+    finally:
+        N.destroy_context(ctx)
+    with open(args.out, "w") as f:
+        f.write(table_text(results))
+    print(f"[autotune] wrote {args.out}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

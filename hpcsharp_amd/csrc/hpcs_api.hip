@@ -48,10 +48,6 @@ constexpr int PASS_ONESWEEP = 0;
 constexpr int PASS_RTS = 1;
 constexpr int PASS_AUTO = 2;
 
-// Below these sizes the single-kernel onesweep pass beats reduce-then-scan's
-// five kernels per pass (launch-bound), measured on MI355X: 4-byte keys
-// below 2^19, 8-byte keys below 2^21.
-template <typename K> constexpr int64_t onesweep_below() { return sizeof(K) == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21); }
 
 // Onesweep tile shapes (threads x keys per thread).  Shape 0 serves every key
 // type; the larger u32 shapes halve the tiles (and look-back work) per pass.
@@ -60,6 +56,78 @@ struct TileShape {
 };
 constexpr TileShape TILE_SHAPES[] = {{256, 16}, {512, 16}, {256, 32}, {512, 32}};
 constexpr int NUM_TILE_SHAPES = 4;
+
+// One row of a tuning table: for sorts of up to max_n keys of key_bytes
+// (with or without u32 values), the pass algorithm, tile shape and ranking.
+struct TuneEntry {
+    int key_bytes;
+    int vals;
+    int64_t max_n;
+    int algo, shape, rank;
+};
+
+// Parse a tuning table: one row per line, `#` starts a comment,
+//   <dtype> <max_n> <algo> <shape> <rank>
+// dtype is u32, f32, i32 (4-byte keys), u64, f64, i64 (8-byte keys), pairs32
+// or pairs64 (keys with u32 values); algo 0 onesweep / 1 reduce-then-scan;
+// shape 0-3 (4-byte keys only, else 0); rank 0 ballot / 1 LDS-OR.
+bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, std::string &err)
+{
+    out.clear();
+    size_t pos = 0;
+    int line_no = 0;
+    while (pos <= text.size()) {
+        size_t eol = text.find('\n', pos);
+        if (eol == std::string::npos) eol = text.size();
+        std::string line = text.substr(pos, eol - pos);
+        pos = eol + 1;
+        ++line_no;
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        char dt[16] = {0};
+        long long max_n = 0;
+        int algo = 0, shape = 0, rank = 0, consumed = 0;
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        if (std::sscanf(line.c_str(), " %15s %lld %d %d %d %n", dt, &max_n, &algo, &shape, &rank, &consumed) != 5 ||
+            line.find_first_not_of(" \t\r", size_t(consumed)) != std::string::npos) {
+            err = "line " + std::to_string(line_no) + ": expected <dtype> <max_n> <algo> <shape> <rank>";
+            return false;
+        }
+        const std::string d(dt);
+        TuneEntry e{};
+        if (d == "u32" || d == "f32" || d == "i32") e.key_bytes = 4, e.vals = 0;
+        else if (d == "u64" || d == "f64" || d == "i64") e.key_bytes = 8, e.vals = 0;
+        else if (d == "pairs32") e.key_bytes = 4, e.vals = 1;
+        else if (d == "pairs64") e.key_bytes = 8, e.vals = 1;
+        else {
+            err = "line " + std::to_string(line_no) + ": unknown dtype " + d;
+            return false;
+        }
+        if (max_n < 1 || algo < 0 || algo > 1 || rank < 0 || rank > 1 || shape < 0 || shape >= NUM_TILE_SHAPES ||
+            (shape != 0 && (e.key_bytes != 4 || e.vals))) {
+            err = "line " + std::to_string(line_no) + ": value out of range";
+            return false;
+        }
+        e.max_n = max_n;
+        e.algo = algo;
+        e.shape = shape;
+        e.rank = rank;
+        out.push_back(e);
+    }
+    return true;
+}
+
+bool read_file(const char *path, std::string &text)
+{
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return false;
+    char buf[4096];
+    size_t got;
+    text.clear();
+    while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+    std::fclose(f);
+    return true;
+}
 constexpr int SORT_BLOCK = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
@@ -85,6 +153,10 @@ struct hpcs_ctx {
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
     int pass_algo = PASS_AUTO;    // PASS_ONESWEEP, PASS_RTS (reduce-then-scan) or PASS_AUTO (by size)
     int persistent = 0;           // reduce-then-scan scatter: persistent workgroups with prefetch
+    // tuning table (hpcs_ctx_load_tuning / HPCS_TUNE_FILE); options set
+    // explicitly with hpcs_ctx_set_option or the environment take precedence
+    std::vector<TuneEntry> tune;
+    bool set_algo = false, set_shape = false, set_rank = false;
     // kernel timing
     struct Pending {
         const char *name;
@@ -372,6 +444,34 @@ void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n
 }
 
 // The device-resident sort: keys (and optional uint32 values) in place.
+struct SortParams {
+    int algo, shape, rank;
+};
+
+// The pass algorithm, tile shape and ranking for a sort of n keys: options
+// set explicitly win, then the tuning table's row for the smallest size class
+// that holds n (the largest class beyond it), then the built-in defaults.
+SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n)
+{
+    SortParams p{c->pass_algo, (key_bytes == 4 && !vals) ? c->tile_shape : 0, c->rank_mode};
+    const TuneEntry *best = nullptr, *largest = nullptr;
+    for (const TuneEntry &e : c->tune) {
+        if (e.key_bytes != key_bytes || e.vals != int(vals)) continue;
+        if (!largest || e.max_n > largest->max_n) largest = &e;
+        if (e.max_n >= n && (!best || e.max_n < best->max_n)) best = &e;
+    }
+    if (!best) best = largest;
+    if (best) {
+        if (!c->set_algo) p.algo = best->algo;
+        if (!c->set_shape && key_bytes == 4 && !vals) p.shape = best->shape;
+        if (!c->set_rank) p.rank = best->rank;
+    }
+    // Below these sizes the single-kernel onesweep pass beats reduce-then-
+    // scan's five kernels per pass (launch-bound), measured on MI355X.
+    if (p.algo == PASS_AUTO) p.algo = n < (key_bytes == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21)) ? PASS_ONESWEEP : PASS_RTS;
+    return p;
+}
+
 template <typename K, bool HAS_VALS, int FLOATK>
 void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
 {
@@ -382,7 +482,8 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
     if (reinterpret_cast<uintptr_t>(d_keys) % sizeof(K)) fail(HPCS_E_INVALID_ARG, "keys pointer is misaligned");
     constexpr int P = int(sizeof(K));
-    const int shape = (sizeof(K) == 4 && !HAS_VALS) ? c->tile_shape : 0;
+    const SortParams prm = resolve_params(c, int(sizeof(K)), HAS_VALS, n);
+    const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
     const Layout L = make_layout(n, sizeof(K), HAS_VALS, tile_keys);
     char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
@@ -395,7 +496,8 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
 
     const uint32_t tiles = uint32_t(L.tiles);
     const uint32_t nranges = (tiles + SCAN_RANGE - 1) / SCAN_RANGE;
-    const int algo = c->pass_algo != PASS_AUTO ? c->pass_algo : (n < onesweep_below<K>() ? PASS_ONESWEEP : PASS_RTS);
+    const int algo = prm.algo;
+    const int rank_mode = prm.rank;
     if (algo == PASS_RTS) {
         // the histogram area holds the OR slots; counts in look-back region 0
         launch_fill(c, base, L.off_lb0, st);
@@ -434,10 +536,10 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
             timed(c, st, "scatter", [&] {
                 // the persistent kernel keeps two tiles in registers: not for 512x32 tiles
                 if (c->persistent && shape != 3)
-                    launch_scatter<K, HAS_VALS, FLOATK>(c->rank_mode, shape, c->num_cus, st, d_keys, keysB, d_vals,
+                    launch_scatter<K, HAS_VALS, FLOATK>(rank_mode, shape, c->num_cus, st, d_keys, keysB, d_vals,
                                                         valsB, uint32_t(n), p, plan, counts, tiles);
                 else
-                    launch_onesweep<K, HAS_VALS, FLOATK>(c->rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
+                    launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
                                                          d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
                                                          nullptr);
             });
@@ -445,7 +547,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         }
         uint32_t *next = (p + 1 < P) ? lb[(p + 1) & 1] : nullptr;
         timed(c, st, "onesweep", [&] {
-            launch_onesweep<K, HAS_VALS, FLOATK>(c->rank_mode, shape, c->lookback_window, c->fallback_rounds,
+            launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, c->lookback_window, c->fallback_rounds,
                                                  dim3(unsigned(L.tiles)), st, d_keys, keysB, d_vals, valsB,
                                                  uint32_t(n), p, plan, lb[p & 1], next);
         });
@@ -838,10 +940,19 @@ HPCS_API int32_t hpcs_ctx_create(int32_t device, hpcs_ctx **out)
         hipDeviceProp_t prop;
         hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-        if (const char *r = std::getenv("HPCS_RANK")) c->rank_mode = std::strcmp(r, "ballot") == 0 ? RANK_BALLOT : RANK_LDSOR;
+        if (const char *r = std::getenv("HPCS_RANK")) {
+            c->rank_mode = std::strcmp(r, "ballot") == 0 ? RANK_BALLOT : RANK_LDSOR;
+            c->set_rank = true;
+        }
         if (const char *t = std::getenv("HPCS_TILE_SHAPE")) {
             const int v = std::atoi(t);
             c->tile_shape = (v >= 0 && v < NUM_TILE_SHAPES) ? v : 0;
+            c->set_shape = true;
+        }
+        if (const char *tf = std::getenv("HPCS_TUNE_FILE")) {
+            std::string text, err;
+            if (!read_file(tf, text)) fail(HPCS_E_INVALID_ARG, std::string("cannot read HPCS_TUNE_FILE ") + tf);
+            if (!parse_tuning(text, c->tune, err)) fail(HPCS_E_INVALID_ARG, "HPCS_TUNE_FILE: " + err);
         }
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         *out = c;
@@ -893,10 +1004,12 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
         case HPCS_OPT_RANK:
             if (value != RANK_BALLOT && value != RANK_LDSOR) fail(HPCS_E_INVALID_ARG, "rank mode must be 0 or 1");
             ctx->rank_mode = int(value);
+            ctx->set_rank = true;
             break;
         case HPCS_OPT_TILE_SHAPE:
             if (value < 0 || value >= NUM_TILE_SHAPES) fail(HPCS_E_INVALID_ARG, "unknown tile shape");
             ctx->tile_shape = int(value);
+            ctx->set_shape = true;
             break;
         case HPCS_OPT_LOOKBACK_WINDOW:
             if (value != 16 && value != 32 && value != 64) fail(HPCS_E_INVALID_ARG, "window must be 16, 32 or 64");
@@ -906,6 +1019,7 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             if (value != PASS_ONESWEEP && value != PASS_RTS && value != PASS_AUTO)
                 fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0, 1 or 2");
             ctx->pass_algo = int(value);
+            ctx->set_algo = value != PASS_AUTO;
             break;
         case HPCS_OPT_PERSISTENT:
             if (value != 0 && value != 1) fail(HPCS_E_INVALID_ARG, "persistent must be 0 or 1");
@@ -967,7 +1081,7 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
         return HPCS_E_INVALID_ARG;
     }
     const int passes = int(ks * 8 / RADIX_BITS);
-    const int shape = (ctx && ks == 4 && !with_values) ? ctx->tile_shape : 0;
+    const int shape = ctx ? resolve_params(ctx, int(ks), with_values != 0, n).shape : 0;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
     const Layout L = make_layout(n, ks, with_values != 0, tile_keys);
     const int64_t per_key = int64_t(ks) + passes * 2 * (int64_t(ks) + (with_values ? 4 : 0));
@@ -1344,4 +1458,43 @@ HPCS_API int32_t hpcs_dev_histogram_byte_components_records(hpcs_ctx *ctx, const
     return guarded(ctx, [&] {
         dev_histogram_records(ctx, d_recs, n, rec_size, key_offset, key_bits, d_counts, pick_stream(ctx, stream));
     });
+}
+
+// ---------------------------------------------------------------------------
+// Tuning tables
+// ---------------------------------------------------------------------------
+
+HPCS_API int32_t hpcs_ctx_load_tuning(hpcs_ctx *ctx, const char *path)
+{
+    return guarded(ctx, [&] {
+        if (!path || !*path) {
+            ctx->tune.clear();
+            return;
+        }
+        std::string text, err;
+        std::vector<TuneEntry> rows;
+        if (!read_file(path, text)) fail(HPCS_E_INVALID_ARG, std::string("cannot read ") + path);
+        if (!parse_tuning(text, rows, err)) fail(HPCS_E_INVALID_ARG, err);
+        ctx->tune.swap(rows);
+    });
+}
+
+HPCS_API int32_t hpcs_tuning_check(const char *path, int32_t *rows)
+{
+    if (!path) {
+        g_last_error = "path is null";
+        return HPCS_E_INVALID_ARG;
+    }
+    std::string text, err;
+    std::vector<TuneEntry> out;
+    if (!read_file(path, text)) {
+        g_last_error = std::string("cannot read ") + path;
+        return HPCS_E_INVALID_ARG;
+    }
+    if (!parse_tuning(text, out, err)) {
+        g_last_error = err;
+        return HPCS_E_INVALID_ARG;
+    }
+    if (rows) *rows = int32_t(out.size());
+    return HPCS_OK;
 }

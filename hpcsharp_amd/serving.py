@@ -11,6 +11,7 @@ reads the results.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -57,6 +58,8 @@ def run(trace: np.ndarray, slots: int = 4, hbm_budget: int = 200 * 10 ** 9,
         mode: int = N.HPCS_SERVE_REALTIME, device: int = 0):
     """Replay the trace on one GPU; returns (results, stats)."""
     trace = np.ascontiguousarray(trace, dtype=REQUEST)
+    if not os.environ.get("HPCS_TUNE_FILE") and os.path.exists(N.TUNE_PATH):
+        os.environ["HPCS_TUNE_FILE"] = N.TUNE_PATH   # the slots' contexts load the measured table
     res = np.zeros(trace.size, dtype=RESULT)
     st = ServeStats()
     rc = N.lib().hpcs_serve_run(device, slots, int(hbm_budget), trace.ctypes.data_as(ctypes.c_void_p), trace.size,

@@ -106,6 +106,20 @@ int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t value);
  * of up to n keys of `dtype` allocate nothing (no counterpart in the
  * reference; for latency-bound callers such as the serving loop). */
 int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n);
+/* Tuning tables (no counterpart in the reference, whose tuning knobs are
+ * static properties, RadixSortLsdParallel.cs:763, SystemAttributes.cs:9).
+ * A table is text, one row per line (`#` comments):
+ *   <dtype> <max_n> <algo> <shape> <rank>
+ * dtype u32|f32|i32|u64|f64|i64|pairs32|pairs64; a row applies to sorts of at
+ * most max_n keys (the row with the smallest such max_n; beyond every row,
+ * the largest); algo 0 onesweep, 1 reduce-then-scan; shape 0-3 (4-byte keys
+ * without values only, else 0); rank 0 ballot, 1 LDS-OR.  Options set with
+ * hpcs_ctx_set_option (or HPCS_RANK / HPCS_TILE_SHAPE) take precedence.
+ * hpcs_ctx_load_tuning replaces the context's table (NULL or "" clears it);
+ * a context also loads $HPCS_TUNE_FILE when created.  hpcs_tuning_check
+ * parses a table without a device and reports its row count. */
+int32_t hpcs_ctx_load_tuning(hpcs_ctx *ctx, const char *path);
+int32_t hpcs_tuning_check(const char *path, int32_t *rows);
 /* Kernel timing (no counterpart in the reference, whose Stopwatch timing is
  * commented out, RadixSortLsd.cs:146-153): on != 0 clears the statistics and
  * brackets every later launch of this context with HIP events on its stream;

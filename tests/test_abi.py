@@ -58,3 +58,32 @@ def test_null_context_is_rejected_without_a_gpu():
     assert info.passes == 4 and info.digit_bits == 8
     assert info.ideal_bytes == (1 << 28) * 36     # one histogram read + read and write per pass
     assert lib.hpcs_plan_query(None, 99, 10, 0, ctypes.byref(info)) == N.HPCS_E_INVALID_ARG
+
+
+def _table(tmp_path, text):
+    p = tmp_path / "tune.txt"
+    p.write_text(text)
+    return str(p).encode()
+
+
+def test_tuning_table_parser(tmp_path):
+    if not os.path.exists(N.LIB_PATH):
+        pytest.skip("libhpcsort.so not built")
+    lib = N.lib()
+    rows = ctypes.c_int32()
+    good = "# comment\nu32 4096 0 0 1   # best\n\nf64 268435456 1 0 1\npairs64 1024 0 0 0\n"
+    assert lib.hpcs_tuning_check(_table(tmp_path, good), ctypes.byref(rows)) == 0
+    assert rows.value == 3
+    for bad in ("u16 4096 0 0 1\n", "u64 4096 0 2 1\n", "u32 4096 0 0 1 7\n", "u32 0 0 0 1\n",
+                "u32 4096 3 0 1\n", "pairs32 100 0 1 0\n", "u32 4096 0 0\n"):
+        assert lib.hpcs_tuning_check(_table(tmp_path, bad), ctypes.byref(rows)) == N.HPCS_E_INVALID_ARG, bad
+        assert b"line 1" in lib.hpcs_last_error()
+    assert lib.hpcs_tuning_check(b"/nonexistent/tune.txt", None) == N.HPCS_E_INVALID_ARG
+
+
+def test_committed_tuning_table_parses():
+    if not os.path.exists(N.TUNE_PATH):
+        pytest.skip("no tuning table committed")
+    rows = ctypes.c_int32()
+    assert N.lib().hpcs_tuning_check(N.TUNE_PATH.encode(), ctypes.byref(rows)) == 0
+    assert rows.value > 0

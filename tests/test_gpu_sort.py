@@ -519,3 +519,35 @@ def test_key_extractor_histogram(oracle, key_type):
         ctypes.c_void_p(counts.data_ptr()), st))
     torch.cuda.synchronize()
     assert np.array_equal(counts.cpu().numpy().reshape(-1, 256), hist_bytes(np.ascontiguousarray(rec["key"])))
+
+
+def test_tuning_table_selects_the_variant(oracle, tmp_path):
+    """A tuning row picks the tile shape, pass algorithm and ranking for its
+    size class; an option set explicitly wins; the output never changes."""
+    ctx = N.new_context(0)
+    try:
+        p = tmp_path / "tune.txt"
+        p.write_text("u32 2097152 0 3 0\nu32 268435456 1 0 1\n")
+        N.load_tuning(str(p), ctx)
+        info = N.PlanInfo()
+        N.check(N.lib().hpcs_plan_query(ctx, N.HPCS_DTYPE_U32, 1 << 20, 0, ctypes.byref(info)))
+        assert info.tile_keys == 512 * 32
+        N.check(N.lib().hpcs_plan_query(ctx, N.HPCS_DTYPE_U32, 1 << 24, 0, ctypes.byref(info)))
+        assert info.tile_keys == 256 * 16
+        n = (1 << 20) + 999
+        a = datagen.keys_u32(n, 4321)
+        ref = a.copy()
+        oracle.sort_radix_lsd(ref)
+        N.check(N.lib().hpcs_sort_radix_u32(ctx, a.ctypes.data_as(ctypes.c_void_p), n))
+        assert np.array_equal(a, ref)
+        N.set_option(N.HPCS_OPT_TILE_SHAPE, 1, ctx)       # explicit option beats the table
+        N.check(N.lib().hpcs_plan_query(ctx, N.HPCS_DTYPE_U32, 1 << 20, 0, ctypes.byref(info)))
+        assert info.tile_keys == 512 * 16
+        N.load_tuning(None, ctx)
+        b = datagen.keys_u32(n, 4322)
+        ref = b.copy()
+        oracle.sort_radix_lsd(ref)
+        N.check(N.lib().hpcs_sort_radix_u32(ctx, b.ctypes.data_as(ctypes.c_void_p), n))
+        assert np.array_equal(b, ref)
+    finally:
+        N.destroy_context(ctx)
