@@ -203,6 +203,29 @@ class Algorithm:
         _call(fn, _ptr(inArray), l, r, _ptr(out))
         return out
 
+    @staticmethod
+    def Histogram11bitComponents(inArray: np.ndarray, l: int, r: int) -> np.ndarray:
+        """Histogram11bitComponents(uint[], l, r) Histogram.cs:298-331: int32[3][2048]."""
+        return Algorithm.HistogramNbitComponents(inArray, l, r, 11)
+
+    @staticmethod
+    def HistogramNbitComponents(inArray: np.ndarray, l: int, r: int, numberOfBitPerComponent: int) -> np.ndarray:
+        """HistogramNbitComponents(uint[]/ulong[], l, r, bits) Histogram.cs:381-427:
+        int32[ceil(width / bits)][2^bits] (bits in [1, 16] here)."""
+        if inArray is None:
+            raise ArgumentNullException("inArray")
+        _require(inArray, "inArray", (np.uint32, np.uint64))
+        if l < 0 or (r >= l and r >= inArray.size):
+            raise IndexError("l..r outside the array")
+        bits = int(numberOfBitPerComponent)
+        if not 1 <= bits <= 16:
+            raise ArgumentOutOfRangeException("numberOfBitPerComponent")
+        width = 8 * inArray.dtype.itemsize
+        out = np.zeros(((width + bits - 1) // bits, 1 << bits), dtype=np.int32)
+        fn = N.lib().hpcs_histogram_nbit_components_u32 if width == 32 else N.lib().hpcs_histogram_nbit_components_u64
+        _call(fn, _ptr(inArray), l, r, bits, _ptr(out))
+        return out
+
     # -- radix top-k -------------------------------------------------------------
     @staticmethod
     def SelectTopK(arrayToBeSelected: np.ndarray, start: int, length: int | None = None, k: int | None = None) -> None:

@@ -808,6 +808,53 @@ void dev_histogram(hpcs_ctx *c, const K *d_keys, int64_t n, uint32_t *d_counts, 
     if (n > 0) launch_histogram<K, FLOATK>(c, d_keys, uint64_t(n), d_counts, st);
 }
 
+template <typename K>
+int nbit_digits(int bits)
+{
+    if (bits < 1 || bits > 16) fail(HPCS_E_INVALID_ARG, "bits per digit must lie in [1, 16]");
+    return (int(8 * sizeof(K)) + bits - 1) / bits;
+}
+
+template <typename K>
+void dev_histogram_nbit(hpcs_ctx *c, const K *d_keys, int64_t n, int bits, uint32_t *d_counts, hipStream_t st)
+{
+    const int digits = nbit_digits<K>(bits);
+    if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
+    if (!d_counts) fail(HPCS_E_INVALID_ARG, "counts is null");
+    if (n > 0 && !d_keys) fail(HPCS_E_INVALID_ARG, "keys is null");
+    if (n >= (int64_t(1) << 32)) fail(HPCS_E_UNSUPPORTED, "histogram counts are 32-bit");
+    const size_t table = size_t(digits) << bits;
+    launch_fill(c, d_counts, table * sizeof(uint32_t), st);
+    if (n == 0) return;
+    const size_t lds = table * sizeof(uint32_t);
+    const int in_lds = lds <= 64 * 1024 ? 1 : 0;
+    timed(c, st, "histogram_nbit", [&] {
+        hipLaunchKernelGGL((k_histogram_nbit<K>), dim3(grid_for(uint64_t(n), 256, c->num_cus * (in_lds ? 2 : 8))),
+                           dim3(256), in_lds ? lds : 0, st, d_keys, uint64_t(n), bits, digits, in_lds, d_counts);
+    });
+}
+
+template <typename K>
+void host_histogram_nbit(hpcs_ctx *c, const K *a, int64_t l, int64_t r, int bits, int32_t *counts)
+{
+    const int digits = nbit_digits<K>(bits);
+    if (!a) fail(HPCS_E_INVALID_ARG, "inArray is null");
+    if (!counts) fail(HPCS_E_INVALID_ARG, "counts is null");
+    if (l < 0) fail(HPCS_E_INVALID_ARG, "l must be non-negative");
+    const int64_t n = r >= l ? r - l + 1 : 0;
+    if (n >= (int64_t(1) << 31)) fail(HPCS_E_UNSUPPORTED, "int32 counts limit the range to 2^31 - 1 keys");
+    const size_t table = size_t(digits) << bits;
+    const size_t hoff = align_up(size_t(n) * sizeof(K));
+    char *io = static_cast<char *>(ensure(c->io, c->io_cap, hoff + table * sizeof(uint32_t)));
+    K *d_keys = reinterpret_cast<K *>(io);
+    uint32_t *d_counts = reinterpret_cast<uint32_t *>(io + hoff);
+    hipStream_t st = c->stream;
+    if (n > 0) hip_check(hipMemcpyAsync(d_keys, a + l, size_t(n) * sizeof(K), hipMemcpyHostToDevice, st), "H2D");
+    dev_histogram_nbit<K>(c, d_keys, n, bits, d_counts, st);
+    hip_check(hipMemcpyAsync(counts, d_counts, table * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
 // Device-resident calls run on the caller's stream; NULL is HIP's null stream.
 hipStream_t pick_stream(hpcs_ctx *, void *stream) { return static_cast<hipStream_t>(stream); }
 
@@ -1497,4 +1544,34 @@ HPCS_API int32_t hpcs_tuning_check(const char *path, int32_t *rows)
     }
     if (rows) *rows = int32_t(out.size());
     return HPCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// N-bit histograms: Histogram11bitComponents(uint[], l, r) (Histogram.cs:
+// 298-331, bits = 11) and HistogramNbitComponents(uint[] / ulong[], l, r,
+// bits) (:381-427)
+// ---------------------------------------------------------------------------
+
+HPCS_API int32_t hpcs_histogram_nbit_components_u32(hpcs_ctx *ctx, const uint32_t *a, int64_t l, int64_t r,
+                                                    int32_t bits, int32_t *counts)
+{
+    return guarded(ctx, [&] { host_histogram_nbit<uint32_t>(ctx, a, l, r, bits, counts); });
+}
+
+HPCS_API int32_t hpcs_histogram_nbit_components_u64(hpcs_ctx *ctx, const uint64_t *a, int64_t l, int64_t r,
+                                                    int32_t bits, int32_t *counts)
+{
+    return guarded(ctx, [&] { host_histogram_nbit<uint64_t>(ctx, a, l, r, bits, counts); });
+}
+
+HPCS_API int32_t hpcs_dev_histogram_nbit_components_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n,
+                                                        int32_t bits, uint32_t *d_counts, void *stream)
+{
+    return guarded(ctx, [&] { dev_histogram_nbit<uint32_t>(ctx, d_keys, n, bits, d_counts, pick_stream(ctx, stream)); });
+}
+
+HPCS_API int32_t hpcs_dev_histogram_nbit_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n,
+                                                        int32_t bits, uint32_t *d_counts, void *stream)
+{
+    return guarded(ctx, [&] { dev_histogram_nbit<uint64_t>(ctx, d_keys, n, bits, d_counts, pick_stream(ctx, stream)); });
 }

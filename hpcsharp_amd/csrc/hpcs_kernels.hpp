@@ -1067,6 +1067,51 @@ __global__ void k_copyback(K *__restrict__ keysA, const K *__restrict__ keysB, u
 }
 
 // ---------------------------------------------------------------------------
+// N-bit digit histograms (Histogram11bitComponents, HistogramNbitComponents):
+// digit d of a key is (key >> (d * bits)) & (2^bits - 1).  The tables of all
+// digits sit in dynamic LDS when they fit (`in_lds`), else the counts go
+// straight to the global result.  Grid-stride; a wave whose valid lanes share
+// a digit adds once.
+// ---------------------------------------------------------------------------
+template <typename K>
+__global__ __launch_bounds__(256) void k_histogram_nbit(const K *__restrict__ keys, uint64_t n, int bits, int digits,
+                                                        int in_lds, uint32_t *__restrict__ counts)
+{
+    extern __shared__ uint32_t sh_nbit[];
+    const uint32_t bins = 1u << bits, mask = bins - 1u;
+    const uint32_t table = uint32_t(digits) * bins;
+    uint32_t *tab = in_lds ? sh_nbit : counts;
+    if (in_lds) {
+        for (uint32_t i = threadIdx.x; i < table; i += blockDim.x) sh_nbit[i] = 0u;
+        __syncthreads();
+    }
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t start = uint64_t(blockIdx.x) * blockDim.x;
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (uint64_t base = start; base < n; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        const bool valid = i < n;
+        const K k = valid ? keys[i] : K(0);
+        const uint64_t vm = __ballot(valid);
+        const int first = __builtin_ctzll(vm);
+        for (int d = 0; d < digits; ++d) {
+            const uint32_t v = uint32_t(k >> (d * bits)) & mask;
+            const uint32_t v0 = __shfl(v, first);
+            if (__ballot(valid && v == v0) == vm) {
+                if (lane == first) atomicAdd(&tab[uint32_t(d) * bins + v0], uint32_t(__popcll(vm)));
+            } else if (valid) {
+                atomicAdd(&tab[uint32_t(d) * bins + v], 1u);
+            }
+        }
+    }
+    if (in_lds) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < table; i += blockDim.x)
+            if (sh_nbit[i]) atomicAdd(&counts[i], sh_nbit[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Records: extract (radix key, index) pairs, and gather records by index.
 // ---------------------------------------------------------------------------
 template <typename K>

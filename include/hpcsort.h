@@ -192,6 +192,19 @@ int32_t hpcs_dev_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *d_
 int32_t hpcs_dev_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, uint32_t *d_counts,
                                                void *stream);
 
+/* Replace Histogram11bitComponents(uint[], l, r) (Histogram.cs:298-331; bits
+ * = 11) and HistogramNbitComponents(uint[] / ulong[], l, r, bits)
+ * (Histogram.cs:381-427): digit d is (key >> (d * bits)) & (2^bits - 1), d <
+ * ceil(key width / bits); counts is int32[digits][2^bits] row-major; 1 <=
+ * bits <= 16. */
+int32_t hpcs_histogram_nbit_components_u32(hpcs_ctx *ctx, const uint32_t *a, int64_t l, int64_t r, int32_t bits,
+                                           int32_t *counts);
+int32_t hpcs_histogram_nbit_components_u64(hpcs_ctx *ctx, const uint64_t *a, int64_t l, int64_t r, int32_t bits,
+                                           int32_t *counts);
+int32_t hpcs_dev_histogram_nbit_components_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n, int32_t bits,
+                                               uint32_t *d_counts, void *stream);
+int32_t hpcs_dev_histogram_nbit_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, int32_t bits,
+                                               uint32_t *d_counts, void *stream);
 /* Replace HistogramByteComponents<T>(T[] inArray, int l, int r,
  * Func<T, UInt32> / Func<T, UInt64> getKey) (Histogram.cs:760-844) for a
  * key stored at key_offset inside records of rec_size bytes: counts is

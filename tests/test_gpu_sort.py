@@ -551,3 +551,18 @@ def test_tuning_table_selects_the_variant(oracle, tmp_path):
         assert np.array_equal(b, ref)
     finally:
         N.destroy_context(ctx)
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.uint64])
+@pytest.mark.parametrize("bits", [1, 5, 8, 11, 12, 16])
+def test_nbit_histograms_vs_oracle(oracle, dtype, bits):
+    """Histogram11bitComponents / HistogramNbitComponents against the oracle's
+    restatement, over a sub-range (LDS tables up to 64 KiB, global beyond)."""
+    n = 200003
+    a = datagen.keys_u64(n, 70 + bits).astype(dtype)
+    a[:1000] = a[0]                                    # a run of equal keys (one add per wave)
+    l, r = 3, n - 11
+    got = Algorithm.HistogramNbitComponents(a, l, r, bits)
+    assert np.array_equal(got, oracle.histogram_nbit_components(a, l, r, bits))
+    if dtype == np.uint32 and bits == 11:
+        assert np.array_equal(Algorithm.Histogram11bitComponents(a, l, r), got)
