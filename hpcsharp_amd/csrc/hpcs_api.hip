@@ -629,6 +629,22 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
 // Radix top-k: the m largest keys of d_in[0, n) into d_out[0, m), in
 // descending order (desc) or ascending; with d_rest, the other n - m keys go
 // there in no particular order.  Scratch: [TopkState | cand 0 | cand 1].
+// One-workgroup sort of the selection: counting ranks for small sets, the
+// bitonic network for larger ones.
+template <typename K, int FLOATK>
+void launch_topk_sort(const K *src, uint32_t len, K *dst, uint32_t m, int mode, const TopkState *ts, uint32_t n,
+                      int desc, hipStream_t st)
+{
+    if (len <= uint32_t(TOPK_RANK_SORT)) {
+        uint32_t t = 64;
+        while (t < len) t <<= 1;
+        hipLaunchKernelGGL((k_topk_rank_sort<K, FLOATK>), dim3(1), dim3(t), 0, st, src, len, dst, m, mode, ts, n, desc);
+    } else {
+        hipLaunchKernelGGL((k_topk_block_sort<K, FLOATK>), dim3(1), dim3(topk_sort_threads(len)), 0, st, src, len, dst,
+                           m, mode, ts, n, desc);
+    }
+}
+
 template <typename K, int FLOATK>
 void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_rest, bool desc, hipStream_t st)
 {
@@ -648,11 +664,8 @@ void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_r
     const uint32_t un = uint32_t(n), um = uint32_t(m);
     const int idesc = desc ? 1 : 0;
     if (!d_rest && n <= TOPK_BLOCK_SORT) {
-        // the whole input fits one workgroup's LDS: sort it there
-        timed(c, st, "topk_sort", [&] {
-            hipLaunchKernelGGL((k_topk_block_sort<K, FLOATK>), dim3(1), dim3(topk_sort_threads(un)), 0, st, d_in, un,
-                               d_out, um, 0, nullptr, un, idesc);
-        });
+        // the whole input fits one workgroup: sort it there
+        timed(c, st, "topk_sort", [&] { launch_topk_sort<K, FLOATK>(d_in, un, d_out, um, 0, nullptr, un, idesc, st); });
         return;
     }
     const size_t off_c0 = align_up(sizeof(TopkState));
@@ -669,7 +682,17 @@ void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_r
                            dim3(grid_for(uint64_t(n), TOPK_HIST_BLOCK * TopkTile<K>::ITEMS, c->num_cus * 2)),
                            dim3(TOPK_HIST_BLOCK), 0, st, d_in, un, vec, ts);
     });
-    const int cgrid = grid_for(uint64_t(n), TILE, c->num_cus * 6);
+    // one resident wave of workgroups (a second, partial round would leave
+    // most CUs idle at the end of the grid-stride loop)
+    static int occ = -1;
+    if (occ < 0) {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_topk_compact<K, FLOATK>, TOPK_BLOCK, 0) != hipSuccess ||
+            o < 1)
+            o = 1;
+        occ = o;
+    }
+    const int cgrid = grid_for(uint64_t(n), TILE, c->num_cus * occ);
     for (int l = 0; l < LV::LEVELS; ++l) {
         timed(c, st, "topk_compact", [&] {
             hipLaunchKernelGGL((k_topk_compact<K, FLOATK>), dim3(cgrid), dim3(TOPK_BLOCK), 0, st, d_in, un, vec,
@@ -683,10 +706,7 @@ void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_r
                 hipLaunchKernelGGL((k_topk_finish<K, FLOATK>), dim3(grid_for(span, 256, c->num_cus * 4)), dim3(256),
                                    0, st, ts, static_cast<K *>(nullptr), d_rest, um, un, idesc);
             });
-        timed(c, st, "topk_sort", [&] {
-            hipLaunchKernelGGL((k_topk_block_sort<K, FLOATK>), dim3(1), dim3(topk_sort_threads(um)), 0, st, d_out,
-                               um, d_out, um, 1, ts, un, idesc);
-        });
+        timed(c, st, "topk_sort", [&] { launch_topk_sort<K, FLOATK>(d_out, um, d_out, um, 1, ts, un, idesc, st); });
         return;
     }
     timed(c, st, "topk_finish", [&] {

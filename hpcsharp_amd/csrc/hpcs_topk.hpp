@@ -18,8 +18,9 @@
 //                   k_rem copies of T close the output, the surplus copies of
 //                   T go to `rest`
 // Output positions come from atomics, so the selected keys are then sorted
-// (one workgroup up to TOPK_BLOCK_SORT keys, else the full radix sort)
-// and written back as values, descending or ascending.
+// (one workgroup: by counting ranks up to TOPK_RANK_SORT keys, by a bitonic
+// network up to TOPK_BLOCK_SORT keys; beyond, the full radix sort) and
+// written back as values, descending or ascending.
 #pragma once
 #include "hpcs_kernels.hpp"
 #include "hpcs_segments.hpp"
@@ -35,6 +36,7 @@ constexpr int TOPK_MAX_LEVELS = 6;
 constexpr int TOPK_SORT_BLOCK = 1024;
 constexpr int TOPK_SORT_ITEMS = 16;
 constexpr int TOPK_BLOCK_SORT = TOPK_SORT_BLOCK * TOPK_SORT_ITEMS; // largest selection sorted in one workgroup
+constexpr int TOPK_RANK_SORT = 256; // up to this many keys, ranked by counting instead
 
 // Threads of the one-workgroup sort for len keys: a power of two, at least
 // one wave.
@@ -282,11 +284,20 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_topk_compact(const K *__restrict
     if (keep_cand)
         for (int i = tid; i < TOPK_BINS; i += TOPK_BLOCK) s_hist[i] = 0u;
     __syncthreads();
-    for (uint32_t base = blockIdx.x * uint32_t(TILE); base < n; base += gridDim.x * uint32_t(TILE)) {
+    // the next tile's loads are issued before this tile is classified, so
+    // each wave keeps reads in flight while it works
+    const uint32_t step = gridDim.x * uint32_t(TILE);
+    K nx[ITEMS];
+    uint32_t base = blockIdx.x * uint32_t(TILE);
+    if (base < n) load_tile_keys<K, TOPK_BLOCK, ITEMS>(src + base, min(uint32_t(TILE), n - base), vec, nx);
+    for (; base < n; base += step) {
         const uint32_t cnt = min(uint32_t(TILE), n - base);
         const bool full = vec && cnt == uint32_t(TILE);
         K u[ITEMS];
-        load_tile_keys<K, TOPK_BLOCK, ITEMS>(src + base, cnt, vec, u);
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) u[j] = nx[j];
+        if (base + step < n)
+            load_tile_keys<K, TOPK_BLOCK, ITEMS>(src + base + step, min(uint32_t(TILE), n - base - step), vec, nx);
         uint32_t n_rest = 0;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
@@ -416,19 +427,22 @@ __global__ __launch_bounds__(TOPK_SORT_BLOCK) void k_topk_block_sort(const K *__
         have = m - t.need;
         len = m;
     }
-    const uint32_t size = blockDim.x * uint32_t(IT); // a power of two >= len
+    const uint32_t nt = blockDim.x;
+    const uint32_t size = nt * uint32_t(IT); // a power of two >= len
     const uint32_t e0 = tid * uint32_t(IT);
     K v[IT];
-    // unconditional loads (the index clamped into the buffer, which holds at
-    // least one key), so that all sixteen are in flight together
+    // Element 16t + r of the network starts as input key r * nt + t (any
+    // bijection will do for unsorted input; this one makes the loads
+    // coalesced).  The loads are unconditional (the index clamped into the
+    // buffer, which holds at least one key), so all sixteen are in flight.
     const uint32_t last = have > 0u ? have - 1u : 0u;
 #pragma unroll
-    for (int r = 0; r < IT; ++r) v[r] = src[min(e0 + uint32_t(r), last)];
+    for (int r = 0; r < IT; ++r) v[r] = src[min(uint32_t(r) * nt + tid, last)];
 #pragma unroll
     for (int r = 0; r < IT; ++r) {
-        const uint32_t e = e0 + uint32_t(r);
+        const uint32_t q = uint32_t(r) * nt + tid;
         const K x = mode == 0 ? sort_key(order_key<K, FLOATK>(v[r]), desc) : v[r];
-        v[r] = e < have ? x : (e < len ? fill : K(~K(0))); // padding sorts last
+        v[r] = q < have ? x : (q < len ? fill : K(~K(0))); // padding sorts last
     }
     for (uint32_t k = 2; k <= size; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -437,12 +451,16 @@ __global__ __launch_bounds__(TOPK_SORT_BLOCK) void k_topk_block_sort(const K *__
                 // so the exchanges overlap instead of waiting one by one
                 K p[IT];
                 if (j >= WAVE_SPAN) {
+                    // element 16t + r sits at s[r * nt + t]: a wave's lanes
+                    // touch consecutive words (no bank conflicts), and the
+                    // partner 16t' + r, t' = t ^ (j / 16), is in the same row
                     __syncthreads();
 #pragma unroll
-                    for (int r = 0; r < IT; ++r) s[e0 + r] = v[r];
+                    for (int r = 0; r < IT; ++r) s[uint32_t(r) * nt + tid] = v[r];
                     __syncthreads();
+                    const uint32_t pt = tid ^ (j / uint32_t(IT));
 #pragma unroll
-                    for (int r = 0; r < IT; ++r) p[r] = s[(e0 + uint32_t(r)) ^ j];
+                    for (int r = 0; r < IT; ++r) p[r] = s[uint32_t(r) * nt + pt];
                 } else {
                     const int pl = int(lane ^ (j / uint32_t(IT)));
 #pragma unroll
@@ -468,6 +486,42 @@ __global__ __launch_bounds__(TOPK_SORT_BLOCK) void k_topk_block_sort(const K *__
         const uint32_t e = e0 + uint32_t(r);
         if (e < m) dst[e] = topk_value<K, FLOATK>(v[r], desc);
     }
+}
+
+// Up to TOPK_RANK_SORT keys: one thread per key counts the keys that order
+// before it ((key, index) pairs, so ranks are distinct) against the whole
+// set in LDS and writes itself to its rank.  Same sources and modes as
+// k_topk_block_sort; blockDim.x is a power of two >= len.
+template <typename K, int FLOATK>
+__global__ __launch_bounds__(TOPK_RANK_SORT) void k_topk_rank_sort(const K *__restrict__ src, uint32_t len,
+                                                                   K *__restrict__ dst, uint32_t m, int mode,
+                                                                   const TopkState *__restrict__ st, uint32_t n,
+                                                                   int desc)
+{
+    __shared__ K s[TOPK_RANK_SORT];
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    uint32_t have = len;
+    K fill = K(0);
+    if (mode == 1) {
+        const TopkTail<K> t = topk_tail<K>(st, m, n);
+        fill = sort_key(t.T, desc);
+        have = m - t.need;
+        len = m;
+    }
+    K x = src[min(tid, have > 0u ? have - 1u : 0u)];
+    x = mode == 0 ? sort_key(order_key<K, FLOATK>(x), desc) : x;
+    x = tid < have ? x : (tid < len ? fill : K(~K(0))); // padding sorts last
+    s[tid] = x;
+    __syncthreads();
+    uint32_t rank = 0;
+    for (uint32_t i = 0; i < nt; i += 4) {
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+            const K y = s[i + e];
+            rank += (y < x || (y == x && i + e < tid)) ? 1u : 0u;
+        }
+    }
+    if (tid < len && rank < m) dst[rank] = topk_value<K, FLOATK>(x, desc);
 }
 
 template <typename K, int FLOATK>

@@ -61,7 +61,8 @@ def _keys(dtype, n, seed, dist="uniform"):
 
 @pytest.mark.parametrize("n,k", [(1000, 10), (65537, 1), (65537, 4096), (1 << 20, 10000), (300001, 16384),
                                  (300001, 16385), (200003, 200003), (5, 5), (1, 1), (12345, 0),
-                                 (16384, 16384), (9000, 4500), (2049, 1025), (20000, 1024)])
+                                 (16384, 16384), (9000, 4500), (2049, 1025), (20000, 1024), (256, 256),
+                                 (257, 200), (100000, 256), (100000, 255), (70000, 1)])
 def test_topk_u32_vs_oracle(oracle, n, k):
     a = datagen.keys_u32(n, n ^ k)
     got = Algorithm.TopK(a, k)
