@@ -115,18 +115,25 @@ class Algorithm:
         - SortRadix(this uint[]) -- NuGet 3.17.0's name for SortRadixLsdFunc
           (HPCsharpExamples Program.cs:26): sorts in place, returns the array;
         - SortRadix<T>(T[] items, UInt32[]/UInt64[] keys) RadixSortLsd.cs:
-          1183-1299: items sorted stably by a parallel key array, returned as
-          (items, keys); here items are uint32 (e.g. indices or handles)."""
+          1183-1299: items (any fixed-size dtype, e.g. a structured array)
+          sorted stably by a parallel key array, returned as (items, keys)."""
         if keys is None:
             _require(inputArray, "inputArray", tuple(_SORT))
             _sort_whole(inputArray)
             return inputArray
-        _require(inputArray, "inputArray", (np.uint32,))
+        if inputArray is None:
+            raise ArgumentNullException("inputArray")
         _require(keys, "inKeys", (np.uint32, np.uint64))
+        if not isinstance(inputArray, np.ndarray) or inputArray.ndim != 1 or not inputArray.flags["C_CONTIGUOUS"]:
+            raise TypeError("inputArray must be a contiguous one-dimensional numpy array")
         if keys.size != inputArray.size:
             raise ArgumentOutOfRangeException("inKeys must have one key per item")
-        fn = N.lib().hpcs_sort_radix_pairs_u64_u32 if keys.dtype == np.uint64 else N.lib().hpcs_sort_radix_pairs_u32_u32
-        _call(fn, _ptr(keys), _ptr(inputArray), keys.size)
+        if inputArray.dtype == np.uint32:
+            fn = N.lib().hpcs_sort_radix_pairs_u64_u32 if keys.dtype == np.uint64 else N.lib().hpcs_sort_radix_pairs_u32_u32
+            _call(fn, _ptr(keys), _ptr(inputArray), keys.size)
+        else:                                   # items of any fixed size
+            _call(N.lib().hpcs_sort_radix_keys_items, _ptr(keys), 8 * keys.dtype.itemsize, _ptr(inputArray),
+                  inputArray.dtype.itemsize, keys.size)
         return inputArray, keys
 
     # -- float[] / double[] ----------------------------------------------------

@@ -794,6 +794,50 @@ void run_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t
     hip_check(hipFreeAsync(work, st), "hipFreeAsync");
 }
 
+// Items sorted by a separate key array (the keys are sorted alongside):
+// (key, index) pairs through the stable sort, then the items gathered by
+// index into a work buffer and copied back.
+template <typename K>
+void run_keys_items(hpcs_ctx *c, K *d_keys, void *d_items, int64_t n, int32_t item_size, hipStream_t st)
+{
+    const size_t off_out = align_up(size_t(n) * sizeof(uint32_t));
+    const size_t need = align_up(off_out + size_t(n) * size_t(item_size));
+    void *work = nullptr;
+    hip_check(hipMallocAsync(&work, need, st), "hipMallocAsync");
+    uint32_t *idx = static_cast<uint32_t *>(work);
+    unsigned char *out = static_cast<unsigned char *>(work) + off_out;
+    hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(uint64_t(n), 256, c->num_cus * 8)), dim3(256), 0, st, idx,
+                       uint64_t(n));
+    hip_check(hipGetLastError(), "k_iota_u32 launch");
+    run_sort<K, true, KIND_UINT>(c, d_keys, idx, n, st);
+    const int words = ((item_size & 3) == 0 && (reinterpret_cast<uintptr_t>(d_items) & 3) == 0) ? 1 : 0;
+    hipLaunchKernelGGL(k_gather_records,
+                       dim3(grid_for(uint64_t(n) * uint64_t(item_size) / 4 + 1, 256, c->num_cus * 8)), dim3(256), 0,
+                       st, static_cast<const unsigned char *>(d_items), out, idx, uint64_t(n), item_size, words);
+    hip_check(hipGetLastError(), "k_gather_records launch");
+    hip_check(hipMemcpyAsync(d_items, out, size_t(n) * size_t(item_size), hipMemcpyDeviceToDevice, st),
+              "hipMemcpyAsync items");
+    hip_check(hipFreeAsync(work, st), "hipFreeAsync");
+}
+
+void check_keys_items_args(int64_t n, int32_t key_bits, int32_t item_size)
+{
+    if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
+    if (key_bits != 32 && key_bits != 64) fail(HPCS_E_INVALID_ARG, "key_bits must be 32 or 64");
+    if (item_size <= 0) fail(HPCS_E_INVALID_ARG, "item_size must be positive");
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+}
+
+void dev_keys_items(hpcs_ctx *c, void *d_keys, int32_t key_bits, void *d_items, int32_t item_size, int64_t n,
+                    hipStream_t st)
+{
+    check_keys_items_args(n, key_bits, item_size);
+    if (n <= 1) return;
+    if (!d_keys || !d_items) fail(HPCS_E_INVALID_ARG, "keys or items is null");
+    if (key_bits == 32) run_keys_items<uint32_t>(c, static_cast<uint32_t *>(d_keys), d_items, n, item_size, st);
+    else run_keys_items<uint64_t>(c, static_cast<uint64_t *>(d_keys), d_items, n, item_size, st);
+}
+
 void check_records_args(int64_t n, int32_t rec_size, int32_t key_offset, int32_t key_bits, int32_t key_kind)
 {
     if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
@@ -1594,4 +1638,35 @@ HPCS_API int32_t hpcs_dev_histogram_nbit_components_u64(hpcs_ctx *ctx, const uin
                                                         int32_t bits, uint32_t *d_counts, void *stream)
 {
     return guarded(ctx, [&] { dev_histogram_nbit<uint64_t>(ctx, d_keys, n, bits, d_counts, pick_stream(ctx, stream)); });
+}
+
+// ---------------------------------------------------------------------------
+// Items by a separate key array: SortRadix<T>(this T[] inputArray, UInt32[] /
+// UInt64[] inKeys) (RadixSortLsd.cs:1183-1299) for items of any fixed size
+// ---------------------------------------------------------------------------
+
+HPCS_API int32_t hpcs_sort_radix_keys_items(hpcs_ctx *ctx, void *keys, int32_t key_bits, void *items, int32_t item_size,
+                                            int64_t n)
+{
+    return guarded(ctx, [&] {
+        check_keys_items_args(n, key_bits, item_size);
+        if (n <= 1) return;
+        if (!keys || !items) fail(HPCS_E_INVALID_ARG, "keys or items is null");
+        const size_t kb = size_t(n) * size_t(key_bits / 8);
+        const size_t ioff = align_up(kb);
+        char *io = static_cast<char *>(ensure(ctx->io, ctx->io_cap, ioff + size_t(n) * size_t(item_size)));
+        hipStream_t st = ctx->stream;
+        hip_check(hipMemcpyAsync(io, keys, kb, hipMemcpyHostToDevice, st), "H2D");
+        hip_check(hipMemcpyAsync(io + ioff, items, size_t(n) * size_t(item_size), hipMemcpyHostToDevice, st), "H2D");
+        dev_keys_items(ctx, io, key_bits, io + ioff, item_size, n, st);
+        hip_check(hipMemcpyAsync(keys, io, kb, hipMemcpyDeviceToHost, st), "D2H");
+        hip_check(hipMemcpyAsync(items, io + ioff, size_t(n) * size_t(item_size), hipMemcpyDeviceToHost, st), "D2H");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    });
+}
+
+HPCS_API int32_t hpcs_dev_sort_radix_keys_items(hpcs_ctx *ctx, void *d_keys, int32_t key_bits, void *d_items,
+                                                int32_t item_size, int64_t n, void *stream)
+{
+    return guarded(ctx, [&] { dev_keys_items(ctx, d_keys, key_bits, d_items, item_size, n, pick_stream(ctx, stream)); });
 }

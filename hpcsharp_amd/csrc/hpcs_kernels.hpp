@@ -1158,4 +1158,10 @@ __global__ void k_gather_records(const unsigned char *__restrict__ in, unsigned 
     }
 }
 
+__global__ void k_iota_u32(uint32_t *__restrict__ out, uint64_t n)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = uint32_t(i);
+}
+
 } // namespace hpcs

@@ -192,6 +192,16 @@ int32_t hpcs_dev_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *d_
 int32_t hpcs_dev_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, uint32_t *d_counts,
                                                void *stream);
 
+/* Replace SortRadix<T>(this T[] inputArray, UInt32[] / UInt64[] inKeys)
+ * (RadixSortLsd.cs:1183-1299) for items of any fixed size: keys (key_bits 32
+ * or 64, unsigned) are sorted stably in place and items[i] (item_size bytes
+ * each) moves with keys[i].  The full key is sorted (the reference's UInt64
+ * overload stops after four passes, :1258, :1284). */
+int32_t hpcs_sort_radix_keys_items(hpcs_ctx *ctx, void *keys, int32_t key_bits, void *items, int32_t item_size,
+                                   int64_t n);
+int32_t hpcs_dev_sort_radix_keys_items(hpcs_ctx *ctx, void *d_keys, int32_t key_bits, void *d_items,
+                                       int32_t item_size, int64_t n, void *stream);
+
 /* Replace Histogram11bitComponents(uint[], l, r) (Histogram.cs:298-331; bits
  * = 11) and HistogramNbitComponents(uint[] / ulong[], l, r, bits)
  * (Histogram.cs:381-427): digit d is (key >> (d * bits)) & (2^bits - 1), d <

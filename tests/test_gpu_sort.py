@@ -566,3 +566,22 @@ def test_nbit_histograms_vs_oracle(oracle, dtype, bits):
     assert np.array_equal(got, oracle.histogram_nbit_components(a, l, r, bits))
     if dtype == np.uint32 and bits == 11:
         assert np.array_equal(Algorithm.Histogram11bitComponents(a, l, r), got)
+
+
+@pytest.mark.parametrize("key_type", [np.uint32, np.uint64])
+def test_items_by_separate_keys(oracle, key_type):
+    """SortRadix<T>(T[] items, keys[]): items of any size move with their
+    keys, stably (narrow keys give many ties; payload = original index)."""
+    n = 150001
+    keys = (datagen.keys_u64(n, 31) & np.uint64(0x3FF)).astype(key_type)
+    items = np.zeros(n, dtype=np.dtype({"names": ["idx", "tag"], "formats": [np.uint32, "S9"], "itemsize": 13}))
+    items["idx"] = np.arange(n, dtype=np.uint32)
+    items["tag"] = [b"x%08d" % i for i in range(n)]
+    want = np.argsort(keys, kind="stable")
+    k2, v2 = keys.copy(), np.arange(n, dtype=np.uint32)
+    oracle.sort_radix_pairs(k2.astype(np.uint64) if key_type == np.uint64 else k2, v2)
+    assert np.array_equal(v2, want.astype(np.uint32))              # the oracle agrees with the closed form
+    got_items, got_keys = Algorithm.SortRadix(items, keys)
+    assert np.array_equal(got_keys, np.sort(keys.copy()))
+    assert np.array_equal(got_items["idx"], want.astype(np.uint32))
+    assert np.array_equal(got_items["tag"], np.array([b"x%08d" % i for i in want]))
