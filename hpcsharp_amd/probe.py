@@ -1,0 +1,70 @@
+"""Short, fixed workloads for profiling runs (rocprofv3) of the top-k and
+serving paths.
+
+  python -m hpcsharp_amd.probe topk [--lg 26] [--k 10000] [--reps 5]
+      hpcs_dev_topk_u32 then hpcs_dev_topk_u64 on 2^lg uniform keys,
+      3 warm-up and `reps` timed calls each; prints ms per call.
+  python -m hpcsharp_amd.probe serving [--requests 20000] [--slots 4]
+      the serving workload (hpcsharp_amd.serving.make_trace) through
+      hpcs_serve_run; prints the summary as JSON.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import sys
+
+
+def topk(lg: int, k: int, reps: int) -> None:
+    import torch
+    from hpcsharp_amd import _native as N
+    ctx, lib = N.context(0), N.lib()
+    n = 1 << lg
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    g = torch.Generator(device="cuda").manual_seed(lg)
+    for fn, words, dt in (("hpcs_dev_topk_u32", 1, torch.int32), ("hpcs_dev_topk_u64", 2, torch.int64)):
+        x = torch.randint(-2 ** 31, 2 ** 31 - 1, (n * words,), dtype=torch.int32, device="cuda", generator=g).view(dt)
+        out = torch.empty(k, dtype=dt, device="cuda")
+        f = getattr(lib, fn)
+        call = lambda: N.check(f(ctx, ctypes.c_void_p(x.data_ptr()), n, k, ctypes.c_void_p(out.data_ptr()), sp))
+        for _ in range(3):
+            call()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        for _ in range(reps):
+            call()
+        b.record(st)
+        b.synchronize()
+        print(f"{fn} n=2^{lg} k={k}: {a.elapsed_time(b) / reps:.4f} ms per call", flush=True)
+
+
+def serving(requests: int, slots: int) -> None:
+    from hpcsharp_amd import serving as S
+    t = S.make_trace(nreq=requests)
+    res, st = S.run(t, slots=slots)
+    print(json.dumps(S.summarize(t, res, st)), flush=True)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="what", required=True)
+    a = sub.add_parser("topk")
+    a.add_argument("--lg", type=int, default=26)
+    a.add_argument("--k", type=int, default=10000)
+    a.add_argument("--reps", type=int, default=5)
+    b = sub.add_parser("serving")
+    b.add_argument("--requests", type=int, default=20000)
+    b.add_argument("--slots", type=int, default=4)
+    args = ap.parse_args(argv)
+    if args.what == "topk":
+        topk(args.lg, args.k, args.reps)
+    else:
+        serving(args.requests, args.slots)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
