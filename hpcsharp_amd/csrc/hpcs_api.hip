@@ -479,7 +479,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     if (n > 0 && !d_keys) fail(HPCS_E_INVALID_ARG, "keys is null");
     if (HAS_VALS && n > 0 && !d_vals) fail(HPCS_E_INVALID_ARG, "values is null");
     if (n <= 1) return;
-    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
     if (reinterpret_cast<uintptr_t>(d_keys) % sizeof(K)) fail(HPCS_E_INVALID_ARG, "keys pointer is misaligned");
     constexpr int P = int(sizeof(K));
     const SortParams prm = resolve_params(c, int(sizeof(K)), HAS_VALS, n);
@@ -496,7 +496,9 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
 
     const uint32_t tiles = uint32_t(L.tiles);
     const uint32_t nranges = (tiles + SCAN_RANGE - 1) / SCAN_RANGE;
-    const int algo = prm.algo;
+    // look-back words carry 30-bit counts: larger sorts take the
+    // reduce-then-scan passes whatever the option or table says
+    const int algo = (prm.algo == PASS_ONESWEEP && n >= LOOKBACK_MAX_KEYS) ? PASS_RTS : prm.algo;
     const int rank_mode = prm.rank;
     if (algo == PASS_RTS) {
         // the histogram area holds the OR slots; counts in look-back region 0
@@ -650,7 +652,7 @@ void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_r
 {
     using LV = TopkLevels<K>;
     if (n < 0 || m < 0 || m > n) fail(HPCS_E_INVALID_ARG, "k must lie in [0, n]");
-    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
     if (n > 0 && !d_in) fail(HPCS_E_INVALID_ARG, "keys is null");
     if (m > 0 && !d_out) fail(HPCS_E_INVALID_ARG, "output is null");
     if (reinterpret_cast<uintptr_t>(d_in) % sizeof(K) || reinterpret_cast<uintptr_t>(d_out) % sizeof(K) ||
@@ -729,7 +731,7 @@ void host_select_topk(hpcs_ctx *c, K *a, int64_t start, int64_t length, int64_t 
     if (!a) fail(HPCS_E_INVALID_ARG, "arrayToBeSelected is null");
     if (start < 0 || length <= 0) fail(HPCS_E_INVALID_ARG, "start or length is invalid");
     if (k < start || k >= start + length) fail(HPCS_E_INVALID_ARG, "k must lie in [start, start + length)");
-    if (length >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "length must be below 2^30 in this version");
+    if (length >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "length must be below 2^31");
     const int64_t m = start + length - k, r = length - m;
     const size_t off_out = align_up(size_t(length) * sizeof(K));
     const size_t off_rest = align_up(off_out + size_t(m) * sizeof(K));
@@ -752,7 +754,7 @@ void host_topk(hpcs_ctx *c, const K *keys, int64_t n, int64_t k, K *out)
     if (n > 0 && !keys) fail(HPCS_E_INVALID_ARG, "keys is null");
     if (k > 0 && !out) fail(HPCS_E_INVALID_ARG, "output is null");
     if (k == 0) return;
-    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
     const size_t off_out = align_up(size_t(n) * sizeof(K));
     char *io = static_cast<char *>(ensure(c->io, c->io_cap, off_out + size_t(k) * sizeof(K)));
     K *d_in = reinterpret_cast<K *>(io), *d_out = reinterpret_cast<K *>(io + off_out);
@@ -825,7 +827,7 @@ void check_keys_items_args(int64_t n, int32_t key_bits, int32_t item_size)
     if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
     if (key_bits != 32 && key_bits != 64) fail(HPCS_E_INVALID_ARG, "key_bits must be 32 or 64");
     if (item_size <= 0) fail(HPCS_E_INVALID_ARG, "item_size must be positive");
-    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
 }
 
 void dev_keys_items(hpcs_ctx *c, void *d_keys, int32_t key_bits, void *d_items, int32_t item_size, int64_t n,
@@ -846,7 +848,7 @@ void check_records_args(int64_t n, int32_t rec_size, int32_t key_offset, int32_t
         fail(HPCS_E_INVALID_ARG, "unknown key_kind");
     if (rec_size <= 0 || key_offset < 0 || key_offset + key_bits / 8 > rec_size)
         fail(HPCS_E_INVALID_ARG, "key does not fit inside the record");
-    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
 }
 
 void dev_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset, int32_t key_bits,
@@ -961,7 +963,7 @@ void host_sort(hpcs_ctx *c, K *keys, uint32_t *vals, int64_t n)
     if (n > 0 && !keys) fail(HPCS_E_INVALID_ARG, "keys is null");
     if (HAS_VALS && n > 0 && !vals) fail(HPCS_E_INVALID_ARG, "values is null");
     if (n <= 1) return;
-    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+    if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
     const size_t kbytes = size_t(n) * sizeof(K);
     const size_t voff = align_up(kbytes);
     const size_t need = voff + (HAS_VALS ? size_t(n) * sizeof(uint32_t) : 0);
@@ -1218,7 +1220,7 @@ HPCS_API int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n)
         default: fail(HPCS_E_INVALID_ARG, "unknown dtype");
         }
         if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
-        if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^30 in this version");
+        if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
         size_t need = 0;
         for (int shape = 0; shape < 4; ++shape) {
             if (shape > 0 && ks != 4) break;

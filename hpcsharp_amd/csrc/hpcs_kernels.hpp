@@ -35,7 +35,11 @@ constexpr int WAVE = 64;
 constexpr uint32_t LB_AGG = 1u << 30;
 constexpr uint32_t LB_INC = 2u << 30;
 constexpr uint32_t LB_VALUE = (1u << 30) - 1;
-constexpr int64_t MAX_KEYS = int64_t(1) << 30;
+// A look-back word holds counts below 2^30, so the onesweep pass serves
+// fewer keys; every other path takes up to the reference's largest array
+// (2^31 - 1 elements) with 32-bit counts and offsets.
+constexpr int64_t LOOKBACK_MAX_KEYS = int64_t(1) << 30;
+constexpr int64_t MAX_KEYS = int64_t(1) << 31;
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));

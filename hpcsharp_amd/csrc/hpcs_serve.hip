@@ -418,7 +418,7 @@ void validate(const hpcs_request *reqs, int64_t nreq)
         const hpcs_request &r = reqs[i];
         if (r.op != HPCS_OP_SORT && r.op != HPCS_OP_TOPK) throw HipFail{HPCS_E_INVALID_ARG, "unknown op"};
         if (r.dtype < HPCS_DTYPE_U32 || r.dtype > HPCS_DTYPE_F64) throw HipFail{HPCS_E_INVALID_ARG, "unknown dtype"};
-        if (r.n < 1 || r.n >= (int64_t(1) << 30)) throw HipFail{HPCS_E_INVALID_ARG, "n must lie in [1, 2^30)"};
+        if (r.n < 1 || r.n >= (int64_t(1) << 31)) throw HipFail{HPCS_E_INVALID_ARG, "n must lie in [1, 2^31)"};
         if (r.op == HPCS_OP_TOPK && (r.k < 1 || r.k > r.n)) throw HipFail{HPCS_E_INVALID_ARG, "k must lie in [1, n]"};
         if (r.arrival_s < prev) throw HipFail{HPCS_E_INVALID_ARG, "arrivals must be non-decreasing"};
         prev = r.arrival_s;

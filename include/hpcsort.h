@@ -26,8 +26,8 @@
  *  - Floating-point keys use the reference's radix key (non-negative values
  *    flip the sign bit, negative values flip every bit, RadixSortMsd.cs:
  *    1379-1383, 1484-1488): ascending order with -0.0 before +0.0.
- *  - Element counts are int64_t; n must be below 2^30 elements per array in
- *    this version (HPCS_E_UNSUPPORTED otherwise).
+ *  - Element counts are int64_t; n must be below 2^31 elements per array, the
+ *    reference's largest array (HPCS_E_UNSUPPORTED otherwise).
  */
 #ifndef HPCSORT_H
 #define HPCSORT_H
@@ -301,7 +301,7 @@ int32_t hpcs_dev_topk_i64(hpcs_ctx *ctx, const int64_t *d_keys, int64_t n, int64
 typedef struct hpcs_request {
     int32_t op;       /* HPCS_OP_SORT or HPCS_OP_TOPK */
     int32_t dtype;    /* HPCS_DTYPE_* */
-    int64_t n;        /* payload keys, 1 <= n < 2^30 */
+    int64_t n;        /* payload keys, 1 <= n < 2^31 */
     int64_t k;        /* top-k: 1 <= k <= n */
     uint64_t seed;    /* payload seed */
     double arrival_s; /* arrival, seconds after the start (non-decreasing) */

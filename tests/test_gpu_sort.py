@@ -585,3 +585,30 @@ def test_items_by_separate_keys(oracle, key_type):
     assert np.array_equal(got_keys, np.sort(keys.copy()))
     assert np.array_equal(got_items["idx"], want.astype(np.uint32))
     assert np.array_equal(got_items["tag"], np.array([b"x%08d" % i for i in want]))
+
+
+def test_beyond_2pow30_keys():
+    """n = 2^30 + 12345 (past the look-back words' 30-bit counts): sortedness
+    and the multiset (byte histograms, key sum) of a device-generated array."""
+    import torch
+    n = (1 << 30) + 12345
+    g = torch.Generator(device="cuda").manual_seed(30)
+    t = torch.randint(-2 ** 31, 2 ** 31 - 1, (n,), dtype=torch.int32, device="cuda", generator=g)
+    ctx = N.context()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    h0 = torch.empty(4 * 256, dtype=torch.int32, device="cuda")
+    N.check(N.lib().hpcs_dev_histogram_byte_components_u32(ctx, ctypes.c_void_p(t.data_ptr()), n,
+                                                           ctypes.c_void_p(h0.data_ptr()), st))
+    s0 = int((t.to(torch.int64) & 0xFFFFFFFF).sum().item())
+    N.check(N.lib().hpcs_dev_sort_radix_u32(ctx, ctypes.c_void_p(t.data_ptr()), n, st))
+    h1 = torch.empty(4 * 256, dtype=torch.int32, device="cuda")
+    N.check(N.lib().hpcs_dev_histogram_byte_components_u32(ctx, ctypes.c_void_p(t.data_ptr()), n,
+                                                           ctypes.c_void_p(h1.data_ptr()), st))
+    torch.cuda.synchronize()
+    assert torch.equal(h0, h1)
+    u = t.to(torch.int64) & 0xFFFFFFFF
+    assert int(u.sum().item()) == s0
+    assert bool((u[1:] >= u[:-1]).all().item())
+    del u, t
+    torch.cuda.empty_cache()
+    N.check(N.lib().hpcs_ctx_trim(ctx))
