@@ -1,8 +1,11 @@
 // hpcs_api.hip -- host engine and C ABI of libhpcsort.so.
 //
-// The engine owns, per context, a device scratch pool and a stream.  A sort
-// is a fixed sequence of stream-ordered launches (fill, histogram, plan, P
-// onesweep passes, copy-back); no host synchronisation happens inside a
+// The engine owns, per context, a device scratch pool, a stream, tuning
+// options and a tuning table.  A sort is a fixed sequence of stream-ordered
+// launches chosen from (dtype, n): for large arrays fill, upsweep, plan and
+// per pass tile counts, three scan kernels and the scatter (reduce-then-
+// scan); for small ones fill, histogram, plan and one onesweep kernel per
+// pass; then a copy-back.  No host synchronisation happens inside a
 // device-resident call, so the sequence can be captured in a HIP graph.
 #include "hpcs_kernels.hpp"
 #include "hpcs_segments.hpp"

@@ -1,6 +1,14 @@
 // hpcs_kernels.hpp -- gfx950 kernels of the stable LSD radix sort.
 //
-// Pipeline of one sort of n keys (8-bit digits, P = sizeof(key) passes):
+// Two pipelines sort n keys (8-bit digits, P = sizeof(key) passes).
+//
+// Reduce-then-scan (large arrays): k_upsweep (pass-0 tile counts and the
+// OR / AND of all keys in one read), k_plan_rts (which passes move data),
+// then per pass k_tile_counts (passes >= 1), k_range_totals /
+// k_scan_columns / k_range_offsets (every tile's digit offsets) and
+// k_onesweep in its prescanned form as the scatter (tiles XCD-contiguous).
+//
+// Onesweep (small arrays):
 //   k_fill        zero the histogram, the plan and look-back region 0
 //   k_histogram   one read of the keys, every digit's 256-bin count at once
 //                 (the GPU form of HistogramByteComponents, Histogram.cs:261-296)
