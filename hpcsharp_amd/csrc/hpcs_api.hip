@@ -10,6 +10,7 @@
 #include "hpcs_kernels.hpp"
 #include "hpcs_segments.hpp"
 #include "hpcs_topk.hpp"
+#include "hpcs_tuning.hpp"
 #include "../../include/hpcsort.h"
 
 #include <hip/hip_runtime.h>
@@ -59,66 +60,7 @@ struct TileShape {
 };
 constexpr TileShape TILE_SHAPES[] = {{256, 16}, {512, 16}, {256, 32}, {512, 32}};
 constexpr int NUM_TILE_SHAPES = 4;
-
-// One row of a tuning table: for sorts of up to max_n keys of key_bytes
-// (with or without u32 values), the pass algorithm, tile shape and ranking.
-struct TuneEntry {
-    int key_bytes;
-    int vals;
-    int64_t max_n;
-    int algo, shape, rank;
-};
-
-// Parse a tuning table: one row per line, `#` starts a comment,
-//   <dtype> <max_n> <algo> <shape> <rank>
-// dtype is u32, f32, i32 (4-byte keys), u64, f64, i64 (8-byte keys), pairs32
-// or pairs64 (keys with u32 values); algo 0 onesweep / 1 reduce-then-scan;
-// shape 0-3 (4-byte keys only, else 0); rank 0 ballot / 1 LDS-OR.
-bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, std::string &err)
-{
-    out.clear();
-    size_t pos = 0;
-    int line_no = 0;
-    while (pos <= text.size()) {
-        size_t eol = text.find('\n', pos);
-        if (eol == std::string::npos) eol = text.size();
-        std::string line = text.substr(pos, eol - pos);
-        pos = eol + 1;
-        ++line_no;
-        const size_t hash = line.find('#');
-        if (hash != std::string::npos) line.resize(hash);
-        char dt[16] = {0};
-        long long max_n = 0;
-        int algo = 0, shape = 0, rank = 0, consumed = 0;
-        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
-        if (std::sscanf(line.c_str(), " %15s %lld %d %d %d %n", dt, &max_n, &algo, &shape, &rank, &consumed) != 5 ||
-            line.find_first_not_of(" \t\r", size_t(consumed)) != std::string::npos) {
-            err = "line " + std::to_string(line_no) + ": expected <dtype> <max_n> <algo> <shape> <rank>";
-            return false;
-        }
-        const std::string d(dt);
-        TuneEntry e{};
-        if (d == "u32" || d == "f32" || d == "i32") e.key_bytes = 4, e.vals = 0;
-        else if (d == "u64" || d == "f64" || d == "i64") e.key_bytes = 8, e.vals = 0;
-        else if (d == "pairs32") e.key_bytes = 4, e.vals = 1;
-        else if (d == "pairs64") e.key_bytes = 8, e.vals = 1;
-        else {
-            err = "line " + std::to_string(line_no) + ": unknown dtype " + d;
-            return false;
-        }
-        if (max_n < 1 || algo < 0 || algo > 1 || rank < 0 || rank > 1 || shape < 0 || shape >= NUM_TILE_SHAPES ||
-            (shape != 0 && (e.key_bytes != 4 || e.vals))) {
-            err = "line " + std::to_string(line_no) + ": value out of range";
-            return false;
-        }
-        e.max_n = max_n;
-        e.algo = algo;
-        e.shape = shape;
-        e.rank = rank;
-        out.push_back(e);
-    }
-    return true;
-}
+static_assert(NUM_TILE_SHAPES == TUNE_TILE_SHAPES, "the tuning parser knows every tile shape");
 
 bool read_file(const char *path, std::string &text)
 {

@@ -20,6 +20,7 @@
 // sort and top-k surface (SortRadix, RadixSortLsd.cs; SelectTopK,
 // SelectionTopKRadix.cs:142-171).
 #include "../../include/hpcsort.h"
+#include "hpcs_arena.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -122,50 +123,6 @@ size_t key_size(int32_t dtype) { return (dtype == HPCS_DTYPE_U32 || dtype == HPC
 
 size_t round_up(size_t x) { return (x + 255) / 256 * 256; }
 
-// Host-managed first-fit allocator over one device region.
-class Arena {
-  public:
-    void reset(char *base, size_t size)
-    {
-        base_ = base;
-        free_.clear();
-        if (size) free_[0] = size;
-    }
-    // Offset of a block of `bytes` (a multiple of 256), or SIZE_MAX.
-    size_t alloc(size_t bytes)
-    {
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < bytes) continue;
-            const size_t off = it->first, left = it->second - bytes;
-            free_.erase(it);
-            if (left) free_[off + bytes] = left;
-            return off;
-        }
-        return SIZE_MAX;
-    }
-    void release(size_t off, size_t bytes)
-    {
-        auto it = free_.emplace(off, bytes).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-    char *at(size_t off) const { return base_ + off; }
-
-  private:
-    char *base_ = nullptr;
-    std::map<size_t, size_t> free_;
-};
-
 // Device bytes a request holds while in flight: its payload and, for top-k,
 // the output.
 size_t request_bytes(const hpcs_request &r)
@@ -219,7 +176,7 @@ struct Server {
     int64_t budget = 0;  // bytes for request buffers (after slot scratch)
     int64_t scratch = 0; // bytes reserved per slot
     char *arena_mem = nullptr;
-    Arena arena; // guarded by mu
+    hpcs::Arena arena; // guarded by mu
     std::vector<std::unique_ptr<Slot>> slots;
     unsigned long long *d_sums = nullptr;
 
