@@ -385,6 +385,15 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
         const int off = i * WAVE + lane;
         keys[i] = off < wave_valid ? kin[wave_base + off] : ~K(0);
     }
+    // values load with the keys, so their latency hides behind the ranking
+    uint32_t vals[HAS_VALS ? ITEMS : 1];
+    if constexpr (HAS_VALS) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int off = i * WAVE + lane;
+            vals[i] = off < wave_valid ? vin[wave_base + off] : 0u;
+        }
+    }
     uint32_t pre_off = 0;
     if constexpr (PRESCANNED)
         if (tid < RADIX) pre_off = lb_cur[size_t(tile) * RADIX + tid];
@@ -492,7 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
             const int off = i * WAVE + lane;
-            if (off < wave_valid) s_vals[pos[i]] = vin[wave_base + off];
+            if (off < wave_valid) s_vals[pos[i]] = vals[i];
         }
     }
 
