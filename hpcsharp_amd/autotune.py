@@ -2,8 +2,8 @@
 GPU and write the tuning table libhpcsort.so reads (hpcs_ctx_load_tuning).
 
 Variants: pass algorithm (onesweep with look-back, reduce-then-scan), tile
-shape (256x16, 512x16, 256x32, 512x32 threads x keys; 4-byte keys without
-values) and ranking (ballot, LDS-OR).  Digits are 8-bit in this version.
+shape (256x16 and 256x8 threads x keys for every sort; 512x16, 256x32 and
+512x32 for 4-byte keys without values) and ranking (ballot, LDS-OR).  Digits are 8-bit in this version.
 Size classes are powers of two; a class's row applies to every n up to it.
 Each measured cell is appended to a JSON log as it finishes, and a rerun
 skips the cells the log already holds (the sweep resumes).
@@ -32,7 +32,7 @@ KINDS = {
 
 
 def candidates(key_bytes: int, vals: bool):
-    shapes = (0, 1, 2, 3) if key_bytes == 4 and not vals else (0,)
+    shapes = (0, 1, 2, 3, 4) if key_bytes == 4 and not vals else (0, 4)
     return [{"algo": a, "shape": s, "rank": r} for a, s, r in itertools.product((0, 1), shapes, (1, 0))]
 
 

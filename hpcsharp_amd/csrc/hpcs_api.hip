@@ -58,8 +58,14 @@ constexpr int PASS_AUTO = 2;
 struct TileShape {
     int block, items;
 };
-constexpr TileShape TILE_SHAPES[] = {{256, 16}, {512, 16}, {256, 32}, {512, 32}};
-constexpr int NUM_TILE_SHAPES = 4;
+constexpr TileShape TILE_SHAPES[] = {{256, 16}, {512, 16}, {256, 32}, {512, 32}, {256, 8}};
+constexpr int NUM_TILE_SHAPES = 5;
+
+// Shapes 1-3 serve 4-byte keys without values; 0 and 4 serve every sort.
+bool shape_allowed(int shape, int key_bytes, bool vals)
+{
+    return shape == 0 || shape == 4 || (key_bytes == 4 && !vals);
+}
 static_assert(NUM_TILE_SHAPES == TUNE_TILE_SHAPES, "the tuning parser knows every tile shape");
 
 bool read_file(const char *path, std::string &text)
@@ -93,7 +99,7 @@ struct hpcs_ctx {
     void *io = nullptr;
     size_t io_cap = 0;
     int rank_mode = RANK_LDSOR;  // tuning knob, env HPCS_RANK=ballot|ldsor
-    int tile_shape = 0;          // tuning knob for u32 keys, env HPCS_TILE_SHAPE=0..3
+    int tile_shape = 0;          // tuning knob, env HPCS_TILE_SHAPE=0..4 (1-3: 4-byte keys without values)
     int fallback_rounds = FALLBACK_ROUNDS;
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
     int pass_algo = PASS_AUTO;    // PASS_ONESWEEP, PASS_RTS (reduce-then-scan) or PASS_AUTO (by size)
@@ -300,6 +306,9 @@ void launch_onesweep(int rank_mode, int shape, int window, int fb, dim3 grid, hi
             return launch_onesweep_shape<K, HAS_VALS, FLOATK, 512, 32>(rank_mode, window, fb, grid, st, a, b, va, vb,
                                                                        n, pass, plan, lb, next);
     }
+    if (shape == 4)
+        return launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 8>(rank_mode, window, fb, grid, st, a, b, va, vb, n, pass,
+                                                                  plan, lb, next);
     launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 16>(rank_mode, window, fb, grid, st, a, b, va, vb, n, pass, plan,
                                                         lb, next);
 }
@@ -320,6 +329,9 @@ void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const 
                            vec);
     else if (block == 256 && items == 32)
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
+                           vec);
+    else if (block == 256 && items == 8)
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 8>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
                            vec);
     else
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts,
@@ -384,6 +396,8 @@ void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n
         hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, n, counts, orslots, vec);
     else if (block == 256 && items == 32)
         hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, n, counts, orslots, vec);
+    else if (block == 256 && items == 8)
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 8>), grid, dim3(256), 0, st, a, n, counts, orslots, vec);
     else
         hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, n, counts, orslots, vec);
 }
@@ -398,7 +412,7 @@ struct SortParams {
 // that holds n (the largest class beyond it), then the built-in defaults.
 SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n)
 {
-    SortParams p{c->pass_algo, (key_bytes == 4 && !vals) ? c->tile_shape : 0, c->rank_mode};
+    SortParams p{c->pass_algo, shape_allowed(c->tile_shape, key_bytes, vals) ? c->tile_shape : 0, c->rank_mode};
     const TuneEntry *best = nullptr, *largest = nullptr;
     for (const TuneEntry &e : c->tune) {
         if (e.key_bytes != key_bytes || e.vals != int(vals)) continue;
@@ -408,7 +422,7 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
     if (!best) best = largest;
     if (best) {
         if (!c->set_algo) p.algo = best->algo;
-        if (!c->set_shape && key_bytes == 4 && !vals) p.shape = best->shape;
+        if (!c->set_shape && shape_allowed(best->shape, key_bytes, vals)) p.shape = best->shape;
         if (!c->set_rank) p.rank = best->rank;
     }
     // Below these sizes the single-kernel onesweep pass beats reduce-then-
@@ -482,7 +496,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
             });
             timed(c, st, "scatter", [&] {
                 // the persistent kernel keeps two tiles in registers: not for 512x32 tiles
-                if (c->persistent && shape != 3)
+                if (c->persistent && shape != 3 && shape != 4)
                     launch_scatter<K, HAS_VALS, FLOATK>(rank_mode, shape, c->num_cus, st, d_keys, keysB, d_vals,
                                                         valsB, uint32_t(n), p, plan, counts, tiles);
                 else
@@ -1167,8 +1181,8 @@ HPCS_API int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n)
         if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
         if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
         size_t need = 0;
-        for (int shape = 0; shape < 4; ++shape) {
-            if (shape > 0 && ks != 4) break;
+        for (int shape = 0; shape < NUM_TILE_SHAPES; ++shape) {
+            if (!shape_allowed(shape, int(ks), false)) continue;
             const int tk = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
             need = std::max(need, make_layout(n, ks, false, tk).total);
         }

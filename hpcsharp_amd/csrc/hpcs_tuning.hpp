@@ -10,7 +10,7 @@
 
 namespace hpcs {
 
-constexpr int TUNE_TILE_SHAPES = 4; // tile shapes 0-3 (threads x keys per thread)
+constexpr int TUNE_TILE_SHAPES = 5; // tile shapes 0-4 (threads x keys per thread)
 
 // One row of a tuning table: for sorts of up to max_n keys of key_bytes
 // (with or without u32 values), the pass algorithm, tile shape and ranking.
@@ -25,7 +25,8 @@ struct TuneEntry {
 //   <dtype> <max_n> <algo> <shape> <rank>
 // dtype is u32, f32, i32 (4-byte keys), u64, f64, i64 (8-byte keys), pairs32
 // or pairs64 (keys with u32 values); algo 0 onesweep / 1 reduce-then-scan;
-// shape 0-3 (4-byte keys only, else 0); rank 0 ballot / 1 LDS-OR.
+// shape 0-4 (1-3 for 4-byte keys without values only); rank 0 ballot / 1
+// LDS-OR.
 inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, std::string &err)
 {
     out.clear();
@@ -59,7 +60,7 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
             return false;
         }
         if (max_n < 1 || algo < 0 || algo > 1 || rank < 0 || rank > 1 || shape < 0 || shape >= TUNE_TILE_SHAPES ||
-            (shape != 0 && (e.key_bytes != 4 || e.vals))) {
+            (shape != 0 && shape != 4 && (e.key_bytes != 4 || e.vals))) {
             err = "line " + std::to_string(line_no) + ": value out of range";
             return false;
         }

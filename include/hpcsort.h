@@ -53,8 +53,8 @@ extern "C" {
 
 /* Tuning options for hpcs_ctx_set_option. */
 #define HPCS_OPT_RANK 1       /* 0: ballot match, 1: LDS-OR match (default) */
-#define HPCS_OPT_TILE_SHAPE 2 /* u32 keys: 0: 256x16 (default), 1: 512x16, 2: 256x32, 3: 512x32
-                                 (threads x keys per thread); other key types use 256x16 */
+#define HPCS_OPT_TILE_SHAPE 2 /* 0: 256x16 (default), 4: 256x8 (threads x keys per thread), any sort;
+                                 4-byte keys without values also 1: 512x16, 2: 256x32, 3: 512x32 */
 #define HPCS_OPT_FALLBACK_ROUNDS 3 /* look-back rounds on an unpublished tile before counting it locally
                                       (default 24; 0 makes every wait use the local count, for tests) */
 #define HPCS_OPT_LOOKBACK_WINDOW 4 /* u32 keys: earlier tiles read per look-back round: 16, 32 (default), 64 */
@@ -112,8 +112,9 @@ int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n);
  *   <dtype> <max_n> <algo> <shape> <rank>
  * dtype u32|f32|i32|u64|f64|i64|pairs32|pairs64; a row applies to sorts of at
  * most max_n keys (the row with the smallest such max_n; beyond every row,
- * the largest); algo 0 onesweep, 1 reduce-then-scan; shape 0-3 (4-byte keys
- * without values only, else 0); rank 0 ballot, 1 LDS-OR.  Options set with
+ * the largest); algo 0 onesweep, 1 reduce-then-scan; shape 0-4
+ * (HPCS_OPT_TILE_SHAPE; 1-3 for 4-byte keys without values only); rank 0
+ * ballot, 1 LDS-OR.  Options set with
  * hpcs_ctx_set_option (or HPCS_RANK / HPCS_TILE_SHAPE) take precedence.
  * hpcs_ctx_load_tuning replaces the context's table (NULL or "" clears it);
  * a context also loads $HPCS_TUNE_FILE when created.  hpcs_tuning_check

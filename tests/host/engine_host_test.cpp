@@ -77,7 +77,8 @@ static int tuning_test()
     CHECK(rows[0].key_bytes == 4 && rows[0].vals == 0 && rows[0].max_n == 4096 && rows[0].shape == 1);
     CHECK(rows[1].key_bytes == 8 && rows[1].vals == 1 && rows[1].algo == 1);
     CHECK(rows[2].max_n == 268435456);
-    const char *bad[] = {"u8 1 0 0 0", "u32 0 0 0 0", "u32 1 2 0 0", "u32 1 0 4 0", "u32 1 0 0 2", "u64 1 0 1 0",
+    CHECK(hpcs::parse_tuning("u64 1 0 4 1\npairs32 7 1 4 0", rows, err) && rows.size() == 2 && rows[1].shape == 4);
+    const char *bad[] = {"u8 1 0 0 0", "u32 0 0 0 0", "u32 1 2 0 0", "u32 1 0 5 0", "u32 1 0 0 2", "u64 1 0 1 0",
                          "pairs32 1 0 3 1", "u32 1 0 0 1 9", "u32 1 0", "u32 -5 0 0 1",
                          "aaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaa 1 0 0 1"};
     for (const char *b : bad) {

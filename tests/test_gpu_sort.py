@@ -304,7 +304,7 @@ def test_forced_lookback_fallback(oracle, shape):
 
 
 @pytest.mark.parametrize("algo", [0, 1])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 def test_tile_shapes_u32(oracle, shape, algo):
     ctx = N.new_context(0)
     try:
@@ -316,6 +316,30 @@ def test_tile_shapes_u32(oracle, shape, algo):
             oracle.sort_radix_lsd(ref)
             N.check(N.lib().hpcs_sort_radix_u32(ctx, a.ctypes.data_as(ctypes.c_void_p), a.size))
             assert np.array_equal(a, ref), (n, dist)
+    finally:
+        N.destroy_context(ctx)
+
+
+@pytest.mark.parametrize("algo", [0, 1])
+def test_small_tile_shape_eight_byte_keys(oracle, algo):
+    """The 256x8 tile shape is accepted for 8-byte keys and for pairs, and
+    gives the oracle's output in both pass algorithms."""
+    ctx = N.new_context(0)
+    try:
+        N.set_option(N.HPCS_OPT_TILE_SHAPE, 4, ctx)
+        N.set_option(N.HPCS_OPT_PASS_ALGO, algo, ctx)
+        for n in (2047, 2048 * 3 + 1, (1 << 21) + 5):
+            a = datagen.keys_u64(n, 7 + n)
+            ref = a.copy()
+            oracle.sort_radix_lsd(ref)
+            N.check(N.lib().hpcs_sort_radix_u64(ctx, a.ctypes.data_as(ctypes.c_void_p), a.size))
+            assert np.array_equal(a, ref), n
+            k = datagen.keys_u64(n, 11 + n) & np.uint64(0xFF00FF)
+            v = np.arange(n, dtype=np.uint32)
+            order = np.argsort(k, kind="stable")
+            N.check(N.lib().hpcs_sort_radix_pairs_u64_u32(ctx, k.ctypes.data_as(ctypes.c_void_p),
+                                                          v.ctypes.data_as(ctypes.c_void_p), n))
+            assert np.array_equal(v, order.astype(np.uint32)), n
     finally:
         N.destroy_context(ctx)
 
