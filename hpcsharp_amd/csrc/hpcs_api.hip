@@ -104,6 +104,8 @@ struct hpcs_ctx {
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
     int pass_algo = PASS_AUTO;    // PASS_ONESWEEP, PASS_RTS (reduce-then-scan) or PASS_AUTO (by size)
     int persistent = 0;           // reduce-then-scan scatter: persistent workgroups with prefetch
+    int digit_bytes = 2;          // reduce-then-scan: tile counts from next-digit bytes written by the
+                                  // scatter: 0 off, 1 on, 2 by type (on for 8-byte keys with values)
     // tuning table (hpcs_ctx_load_tuning / HPCS_TUNE_FILE); options set
     // explicitly with hpcs_ctx_set_option or the environment take precedence
     std::vector<TuneEntry> tune;
@@ -154,9 +156,9 @@ void *ensure(void *&buf, size_t &cap, size_t need)
 
 // Scratch layout of one sort:
 //   [hist P x 256 u32 | DevPlan | look-back region 0]  zero-filled first
-//   [look-back region 1] [keys B] [values B]
+//   [look-back region 1] [keys B] [values B] [next-pass digit bytes, n]
 struct Layout {
-    size_t off_hist, off_plan, off_dtot, off_lb0, zero_bytes, off_lb1, off_keysB, off_valsB, total;
+    size_t off_hist, off_plan, off_dtot, off_lb0, zero_bytes, off_lb1, off_keysB, off_valsB, off_dig, total;
     int64_t tiles;
 };
 
@@ -175,7 +177,8 @@ Layout make_layout(int64_t n, size_t key_size, bool vals, int tile_keys = SORT_T
     L.off_lb1 = align_up(L.zero_bytes);
     L.off_keysB = align_up(L.off_lb1 + lb_bytes);
     L.off_valsB = align_up(L.off_keysB + size_t(n) * key_size);
-    L.total = align_up(L.off_valsB + (vals ? size_t(n) * sizeof(uint32_t) : 0));
+    L.off_dig = align_up(L.off_valsB + (vals ? size_t(n) * sizeof(uint32_t) : 0));
+    L.total = align_up(L.off_dig + size_t(n));
     return L;
 }
 
@@ -262,80 +265,83 @@ void launch_histogram(hpcs_ctx *c, const K *d_keys, uint64_t n, uint32_t *d_hist
 
 template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int W>
 void launch_onesweep_w(int rank_mode, int fb, dim3 grid, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb,
-                       uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
+                       uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next,
+                       uint8_t *dig)
 {
     if (rank_mode == RANK_BALLOT)
         hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_BALLOT, W>), grid, dim3(BLOCK), 0, st,
-                           a, b, va, vb, n, pass, plan, lb, next, fb);
+                           a, b, va, vb, n, pass, plan, lb, next, fb, dig);
     else
         hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_LDSOR, W>), grid, dim3(BLOCK), 0, st,
-                           a, b, va, vb, n, pass, plan, lb, next, fb);
+                           a, b, va, vb, n, pass, plan, lb, next, fb, dig);
 }
 
 template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS>
 void launch_onesweep_shape(int rank_mode, int window, int fb, dim3 grid, hipStream_t st, K *a, K *b, uint32_t *va,
-                           uint32_t *vb, uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
+                           uint32_t *vb, uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next,
+                           uint8_t *dig)
 {
     if (window == 0)
         return launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 0>(rank_mode, fb, grid, st, a, b, va, vb, n,
-                                                                       pass, plan, lb, next);
+                                                                       pass, plan, lb, next, dig);
     if constexpr (sizeof(K) == 4 && !HAS_VALS) {
         if (window <= 16)
             return launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 16>(rank_mode, fb, grid, st, a, b, va, vb, n,
-                                                                            pass, plan, lb, next);
+                                                                            pass, plan, lb, next, dig);
         if (window >= 64)
             return launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 64>(rank_mode, fb, grid, st, a, b, va, vb, n,
-                                                                            pass, plan, lb, next);
+                                                                            pass, plan, lb, next, dig);
     }
     launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 32>(rank_mode, fb, grid, st, a, b, va, vb, n, pass, plan,
-                                                             lb, next);
+                                                             lb, next, dig);
 }
 
 template <typename K, bool HAS_VALS, int FLOATK>
 void launch_onesweep(int rank_mode, int shape, int window, int fb, dim3 grid, hipStream_t st, K *a, K *b,
-                     uint32_t *va, uint32_t *vb, uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next)
+                     uint32_t *va, uint32_t *vb, uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next,
+                     uint8_t *dig)
 {
     if constexpr (sizeof(K) == 4 && !HAS_VALS) {
         if (shape == 1)
             return launch_onesweep_shape<K, HAS_VALS, FLOATK, 512, 16>(rank_mode, window, fb, grid, st, a, b, va, vb,
-                                                                       n, pass, plan, lb, next);
+                                                                       n, pass, plan, lb, next, dig);
         if (shape == 2)
             return launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 32>(rank_mode, window, fb, grid, st, a, b, va, vb,
-                                                                       n, pass, plan, lb, next);
+                                                                       n, pass, plan, lb, next, dig);
         if (shape == 3)
             return launch_onesweep_shape<K, HAS_VALS, FLOATK, 512, 32>(rank_mode, window, fb, grid, st, a, b, va, vb,
-                                                                       n, pass, plan, lb, next);
+                                                                       n, pass, plan, lb, next, dig);
     }
     if (shape == 4)
         return launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 8>(rank_mode, window, fb, grid, st, a, b, va, vb, n, pass,
-                                                                  plan, lb, next);
+                                                                  plan, lb, next, dig);
     launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 16>(rank_mode, window, fb, grid, st, a, b, va, vb, n, pass, plan,
-                                                        lb, next);
+                                                        lb, next, dig);
 }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 template <typename K, int FLOATK>
 void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const K *b, uint32_t n, int pass,
-                        const DevPlan *plan, uint32_t *counts)
+                        const DevPlan *plan, uint32_t *counts, const uint8_t *dig)
 {
     const int block = TILE_SHAPES[shape].block, items = TILE_SHAPES[shape].items;
     const int vec = aligned16(a) && aligned16(b);
     if (block == 256 && items == 16)
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
-                           vec);
+                           vec, dig);
     else if (block == 512 && items == 16)
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts,
-                           vec);
+                           vec, dig);
     else if (block == 256 && items == 32)
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
-                           vec);
+                           vec, dig);
     else if (block == 256 && items == 8)
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 8>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
-                           vec);
+                           vec, dig);
     else
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts,
-                           vec);
+                           vec, dig);
 }
 
 template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int RANK>
@@ -459,6 +465,14 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     // reduce-then-scan passes whatever the option or table says
     const int algo = (prm.algo == PASS_ONESWEEP && n >= LOOKBACK_MAX_KEYS) ? PASS_RTS : prm.algo;
     const int rank_mode = prm.rank;
+    // reduce-then-scan: the scatter writes the next pass's digit bytes for
+    // its tile counts (not the persistent scatter)
+    const bool persistent = c->persistent && shape != 3 && shape != 4;
+    // Measured at 2^28 on MI355X: the byte stores cost the scatter about what
+    // the byte counts save for u64 keys, more for u32 keys; with u32 values
+    // beside u64 keys the sort gains 5%.
+    const bool want_dig = c->digit_bytes == 1 || (c->digit_bytes == 2 && sizeof(K) == 8 && HAS_VALS);
+    uint8_t *dig = (want_dig && !persistent) ? reinterpret_cast<uint8_t *>(base + L.off_dig) : nullptr;
     if (algo == PASS_RTS) {
         // the histogram area holds the OR slots; counts in look-back region 0
         launch_fill(c, base, L.off_lb0, st);
@@ -484,7 +498,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
             if (p > 0)
                 timed(c, st, "tile_counts", [&] {
                     launch_tile_counts<K, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan,
-                                                  counts);
+                                                  counts, dig);
                 });
             timed(c, st, "scan", [&] {
                 uint32_t *dtot_p = dtot + size_t(p) * RADIX * LINE_WORDS;
@@ -496,13 +510,13 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
             });
             timed(c, st, "scatter", [&] {
                 // the persistent kernel keeps two tiles in registers: not for 512x32 tiles
-                if (c->persistent && shape != 3 && shape != 4)
+                if (persistent)
                     launch_scatter<K, HAS_VALS, FLOATK>(rank_mode, shape, c->num_cus, st, d_keys, keysB, d_vals,
                                                         valsB, uint32_t(n), p, plan, counts, tiles);
                 else
                     launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
                                                          d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
-                                                         nullptr);
+                                                         nullptr, dig);
             });
             continue;
         }
@@ -510,7 +524,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         timed(c, st, "onesweep", [&] {
             launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, c->lookback_window, c->fallback_rounds,
                                                  dim3(unsigned(L.tiles)), st, d_keys, keysB, d_vals, valsB,
-                                                 uint32_t(n), p, plan, lb[p & 1], next);
+                                                 uint32_t(n), p, plan, lb[p & 1], next, nullptr);
         });
     }
     timed(c, st, "copyback", [&] {
@@ -1092,6 +1106,10 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
                 fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0, 1 or 2");
             ctx->pass_algo = int(value);
             ctx->set_algo = value != PASS_AUTO;
+            break;
+        case HPCS_OPT_DIGIT_BYTES:
+            if (value < 0 || value > 2) fail(HPCS_E_INVALID_ARG, "digit bytes must be 0, 1 or 2");
+            ctx->digit_bytes = int(value);
             break;
         case HPCS_OPT_PERSISTENT:
             if (value != 0 && value != 1) fail(HPCS_E_INVALID_ARG, "persistent must be 0 or 1");

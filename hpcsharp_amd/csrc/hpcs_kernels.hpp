@@ -68,6 +68,12 @@ struct DevPlan {
     uint32_t pad[47];
 };
 constexpr uint32_t PF_TRIVIAL = 1u, PF_SRC_B = 2u, PF_XIN = 4u, PF_XOUT = 8u;
+// Reduce-then-scan plans only: PF_FIRST marks the first pass that moves keys,
+// and bits PF_NEXT_SHIFT.. of a pass's flags hold the next pass that moves
+// keys (PF_NEXT_NONE after the last one).
+constexpr uint32_t PF_FIRST = 16u;
+constexpr int PF_NEXT_SHIFT = 8;
+constexpr uint32_t PF_NEXT_NONE = 0xFu;
 
 // Pack the per-pass fields into flags[] (one thread, after they are set).
 __device__ inline void pack_plan_flags(DevPlan *plan, int P)
@@ -326,7 +332,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
                                                     uint32_t *__restrict__ valsA, uint32_t *__restrict__ valsB,
                                                     uint32_t n, int pass, DevPlan *__restrict__ plan,
                                                     uint32_t *__restrict__ lb_cur, uint32_t *__restrict__ lb_next,
-                                                    int fb_rounds)
+                                                    int fb_rounds, uint8_t *__restrict__ dig)
 {
     static_assert(BLOCK >= RADIX && BLOCK % WAVE == 0, "one thread per digit is required");
     constexpr int WAVES = BLOCK / WAVE;
@@ -590,6 +596,12 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     __syncthreads();
 
     // ---- 5. write the tile out, each digit's run contiguous ----
+    // With `dig` (reduce-then-scan form) each key's digit of the next pass
+    // that moves keys is also written, one byte at the key's new position, so
+    // that pass counts its tiles from n bytes instead of re-reading the keys.
+    const uint32_t next_pass = (flags >> PF_NEXT_SHIFT) & PF_NEXT_NONE;
+    const bool wdig = PRESCANNED && dig != nullptr && next_pass != PF_NEXT_NONE;
+    const int next_shift = int(next_pass) * RADIX_BITS;
     const int tile_valid = n > tile_base ? int(min(uint32_t(TILE), n - tile_base)) : 0;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
@@ -599,6 +611,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
             const uint32_t dst = s_base[KeyOps<K>::digit(k, shift)] + uint32_t(j);
             kout[dst] = xout ? key_dec<K, FLOATK>(k) : k;
             if constexpr (HAS_VALS) vout[dst] = s_vals[j];
+            if (wdig) dig[dst] = uint8_t(KeyOps<K>::digit(k, next_shift));
         }
     }
 }
@@ -850,11 +863,29 @@ __device__ inline void load_tile_keys(const K *__restrict__ p, uint32_t cnt, boo
     }
 }
 
+// Count one digit per LDS atomic; a wave whose lanes all hold the same digit
+// adds 64 once.
+__device__ inline void count_digit(uint32_t *sh, uint32_t d, bool valid, int lane)
+{
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__all(valid && d == d0)) {
+        if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
+    } else if (valid) {
+        atomicAdd(&sh[d], 1u);
+    }
+}
+
+// Per-tile digit counts of a pass.  When the previous pass that moved keys
+// wrote this pass's digits as bytes (`dig`, every pass after the first that
+// moves keys), the tile's TILE bytes are read (thread t: bytes t*ITEMS ..,
+// in 4-byte words); otherwise the keys are.
 template <typename K, int FLOATK, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ keysA, const K *__restrict__ keysB,
                                                        uint32_t n, int pass, const DevPlan *__restrict__ plan,
-                                                       uint32_t *__restrict__ counts, int vec)
+                                                       uint32_t *__restrict__ counts, int vec,
+                                                       const uint8_t *__restrict__ dig)
 {
+    static_assert(ITEMS % 4 == 0, "digit bytes are read in 4-byte words");
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ uint32_t sh[RADIX];
     const uint32_t flags = plan->flags[pass];
@@ -867,19 +898,39 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
     __syncthreads();
     const uint32_t base = blockIdx.x * uint32_t(TILE);
     const uint32_t cnt = min(uint32_t(TILE), n - base);
+    if (dig != nullptr && !(flags & PF_FIRST)) {
+        const uint32_t first = uint32_t(tid * ITEMS); // this thread's bytes [first, first + ITEMS)
+        uint32_t w[ITEMS / 4];
+        if (cnt == uint32_t(TILE)) {
+            const uint32_t *pw = reinterpret_cast<const uint32_t *>(dig + base + first);
+#pragma unroll
+            for (int i = 0; i < ITEMS / 4; ++i) w[i] = pw[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < ITEMS / 4; ++i) {
+                uint32_t x = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t j = first + uint32_t(4 * i + b);
+                    if (j < cnt) x |= uint32_t(dig[base + j]) << (8 * b);
+                }
+                w[i] = x;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+            count_digit(sh, (w[i / 4] >> (8 * (i % 4))) & 0xFFu, first + uint32_t(i) < cnt, lane);
+        __syncthreads();
+        for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
+        return;
+    }
     K k[ITEMS];
     load_tile_keys<K, BLOCK, ITEMS>(kin + base, cnt, vec, k);
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t j = uint32_t(i * BLOCK + tid);
         const bool valid = j < cnt;
-        const uint32_t d = KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[i]) : k[i], shift);
-        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        if (__all(valid && d == d0)) {
-            if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
-        } else if (valid) {
-            atomicAdd(&sh[d], 1u);
-        }
+        count_digit(sh, KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[i]) : k[i], shift), valid, lane);
     }
     __syncthreads();
     for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
@@ -1070,6 +1121,12 @@ __global__ void k_plan_rts(const K *__restrict__ orslots, DevPlan *__restrict__ 
     }
     plan->copyback = cur;
     pack_plan_flags(plan, P);
+    uint32_t next = PF_NEXT_NONE;
+    for (int p = P - 1; p >= 0; --p) {
+        plan->flags[p] |= next << PF_NEXT_SHIFT;
+        if (!plan->trivial[p]) next = uint32_t(p);
+    }
+    if (first >= 0) plan->flags[first] |= PF_FIRST;
 }
 
 // ---------------------------------------------------------------------------

@@ -62,6 +62,10 @@ extern "C" {
                                       2: by size (default; onesweep for small arrays) */
 #define HPCS_OPT_PERSISTENT 6      /* reduce-then-scan scatter: 0 one workgroup per tile (default), 1 persistent
                                       workgroups that prefetch the next tile */
+#define HPCS_OPT_DIGIT_BYTES 7     /* reduce-then-scan: 1 the scatter also writes each key's next digit as a
+                                      byte and the next pass counts its tiles from those n bytes; 0 the next
+                                      pass re-reads the keys; 2 (default) 1 for 8-byte keys with values, else 0
+                                      (scratch holds the n bytes either way) */
 
 #define HPCS_DTYPE_U32 0
 #define HPCS_DTYPE_U64 1

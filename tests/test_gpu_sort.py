@@ -344,6 +344,47 @@ def test_small_tile_shape_eight_byte_keys(oracle, algo):
         N.destroy_context(ctx)
 
 
+@pytest.mark.parametrize("mode", ["bytes", "keys", "persistent"])
+def test_rts_next_digit_bytes(oracle, mode):
+    """Reduce-then-scan passes count their tiles from the digit bytes the
+    previous moving pass wrote (or from the keys): the same output as the
+    oracle when the first pass, or passes in the middle, move nothing."""
+    ctx = N.new_context(0)
+    P = ctypes.c_void_p
+    try:
+        N.set_option(N.HPCS_OPT_PASS_ALGO, 1, ctx)
+        with pytest.raises(N.HpcsError):
+            N.set_option(N.HPCS_OPT_DIGIT_BYTES, 3, ctx)
+        N.set_option(N.HPCS_OPT_DIGIT_BYTES, 0 if mode == "keys" else 1, ctx)
+        N.set_option(N.HPCS_OPT_PERSISTENT, 1 if mode == "persistent" else 0, ctx)
+        n = 3 * (1 << 20) + 4097                    # a ragged last tile
+        base = datagen.keys_u32(n, 808)
+        for mask in (0xFFFFFFFF, 0xFFFFFF00, 0x00FF00FF, 0xFF000000, 0x0000FF00):
+            a = base & np.uint32(mask)
+            ra = a.copy()
+            oracle.sort_radix_lsd(ra)
+            N.check(N.lib().hpcs_sort_radix_u32(ctx, a.ctypes.data_as(P), n))
+            assert np.array_equal(a, ra), hex(mask)
+        for mask in (0xFFFFFFFFFFFFFFFF, 0xFF00FF00FF00FF00, 0x00000000FFFF0000):
+            b = datagen.keys_u64(n, 809) & np.uint64(mask)
+            rb = b.copy()
+            oracle.sort_radix_lsd(rb)
+            N.check(N.lib().hpcs_sort_radix_u64(ctx, b.ctypes.data_as(P), n))
+            assert np.array_equal(b, rb), hex(mask)
+        x = datagen.doubles_special(n, 810)
+        rx = x.copy()
+        oracle.sort_radix_float(rx)
+        N.check(N.lib().hpcs_sort_radix_f64(ctx, x.ctypes.data_as(P), n))
+        assert np.array_equal(x.view(np.uint64), rx.view(np.uint64))
+        k = datagen.keys_u64(n, 811) & np.uint64(0xFFFF00FF00)
+        v = np.arange(n, dtype=np.uint32)
+        order = np.argsort(k, kind="stable").astype(np.uint32)
+        N.check(N.lib().hpcs_sort_radix_pairs_u64_u32(ctx, k.ctypes.data_as(P), v.ctypes.data_as(P), n))
+        assert np.array_equal(v, order)
+    finally:
+        N.destroy_context(ctx)
+
+
 def test_both_pass_algorithms_all_types(oracle):
     """The single-pass look-back form and the reduce-then-scan form give the
     oracle's output for every key type and for key-value pairs."""
