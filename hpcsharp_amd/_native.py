@@ -28,6 +28,7 @@ HPCS_DTYPE_U32, HPCS_DTYPE_U64, HPCS_DTYPE_F32, HPCS_DTYPE_F64 = 0, 1, 2, 3
 HPCS_OPT_RANK, HPCS_OPT_TILE_SHAPE, HPCS_OPT_FALLBACK_ROUNDS, HPCS_OPT_LOOKBACK_WINDOW, HPCS_OPT_PASS_ALGO = 1, 2, 3, 4, 5
 HPCS_OPT_PERSISTENT = 6
 HPCS_OPT_DIGIT_BYTES = 7
+HPCS_OPT_DIGIT_BITS = 8
 HPCS_OP_SORT, HPCS_OP_TOPK = 0, 1
 HPCS_SERVE_REALTIME, HPCS_SERVE_ALONE = 0, 1
 

@@ -8,6 +8,7 @@
 // pass; then a copy-back.  No host synchronisation happens inside a
 // device-resident call, so the sequence can be captured in a HIP graph.
 #include "hpcs_kernels.hpp"
+#include "hpcs_digit11.hpp"
 #include "hpcs_segments.hpp"
 #include "hpcs_topk.hpp"
 #include "hpcs_tuning.hpp"
@@ -108,8 +109,9 @@ struct hpcs_ctx {
                                   // scatter: 0 off, 1 on, 2 by type (on for 8-byte keys with values)
     // tuning table (hpcs_ctx_load_tuning / HPCS_TUNE_FILE); options set
     // explicitly with hpcs_ctx_set_option or the environment take precedence
+    int digit_bits = 8;           // 8, or 11 for onesweep sorts of keys without values
     std::vector<TuneEntry> tune;
-    bool set_algo = false, set_shape = false, set_rank = false;
+    bool set_algo = false, set_shape = false, set_rank = false, set_bits = false;
     // kernel timing
     struct Pending {
         const char *name;
@@ -410,7 +412,7 @@ void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n
 
 // The device-resident sort: keys (and optional uint32 values) in place.
 struct SortParams {
-    int algo, shape, rank;
+    int algo, shape, rank, bits;
 };
 
 // The pass algorithm, tile shape and ranking for a sort of n keys: options
@@ -418,7 +420,8 @@ struct SortParams {
 // that holds n (the largest class beyond it), then the built-in defaults.
 SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n)
 {
-    SortParams p{c->pass_algo, shape_allowed(c->tile_shape, key_bytes, vals) ? c->tile_shape : 0, c->rank_mode};
+    SortParams p{c->pass_algo, shape_allowed(c->tile_shape, key_bytes, vals) ? c->tile_shape : 0, c->rank_mode,
+                 c->digit_bits};
     const TuneEntry *best = nullptr, *largest = nullptr;
     for (const TuneEntry &e : c->tune) {
         if (e.key_bytes != key_bytes || e.vals != int(vals)) continue;
@@ -430,11 +433,73 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
         if (!c->set_algo) p.algo = best->algo;
         if (!c->set_shape && shape_allowed(best->shape, key_bytes, vals)) p.shape = best->shape;
         if (!c->set_rank) p.rank = best->rank;
+        if (!c->set_bits) p.bits = best->bits;
     }
     // Below these sizes the single-kernel onesweep pass beats reduce-then-
     // scan's five kernels per pass (launch-bound), measured on MI355X.
     if (p.algo == PASS_AUTO) p.algo = n < (key_bytes == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21)) ? PASS_ONESWEEP : PASS_RTS;
+    // 11-bit digits exist for onesweep passes over keys without values (their
+    // look-back words carry 30-bit counts); other sorts keep 8-bit digits
+    if (p.bits == 11 && (vals || p.algo != PASS_ONESWEEP || n >= LOOKBACK_MAX_KEYS)) p.bits = 8;
+    if (p.bits == 11) {
+        p.shape = 0;
+        p.rank = RANK_BALLOT;
+    }
     return p;
+}
+
+// Scratch of an 11-bit sort:
+//   [hist passes x 2048 u32 | DevPlan11 | look-back region 0]  zero-filled
+//   [look-back region 1] [keys B]
+struct Layout11 {
+    size_t off_plan, off_lb0, zero_bytes, off_lb1, off_keysB, total;
+    int64_t tiles;
+};
+
+Layout11 make_layout11(int64_t n, size_t key_size)
+{
+    Layout11 L{};
+    L.tiles = (n + D11_TILE - 1) / D11_TILE;
+    const size_t lb_bytes = size_t(L.tiles) * D11_RADIX * sizeof(uint32_t);
+    L.off_plan = align_up(D11_MAX_PASSES * D11_RADIX * sizeof(uint32_t));
+    L.off_lb0 = align_up(L.off_plan + sizeof(DevPlan11));
+    L.zero_bytes = L.off_lb0 + lb_bytes;
+    L.off_lb1 = align_up(L.zero_bytes);
+    L.off_keysB = align_up(L.off_lb1 + lb_bytes);
+    L.total = align_up(L.off_keysB + size_t(n) * key_size);
+    return L;
+}
+
+// The onesweep sort with 11-bit digits (keys only; 1 < n < 2^30).
+template <typename K, int FLOATK>
+void run_sort11(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st)
+{
+    constexpr int P = d11_passes<K>();
+    const Layout11 L = make_layout11(n, sizeof(K));
+    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
+    uint32_t *hist = reinterpret_cast<uint32_t *>(base);
+    DevPlan11 *plan = reinterpret_cast<DevPlan11 *>(base + L.off_plan);
+    uint32_t *lb[2] = {reinterpret_cast<uint32_t *>(base + L.off_lb0), reinterpret_cast<uint32_t *>(base + L.off_lb1)};
+    K *keysB = reinterpret_cast<K *>(base + L.off_keysB);
+    launch_fill(c, base, L.zero_bytes, st);
+    timed(c, st, "histogram", [&] {
+        hipLaunchKernelGGL((k_hist11<K, FLOATK>), dim3(grid_for(uint64_t(n), 256, c->num_cus * 4)), dim3(256), 0, st,
+                           d_keys, uint64_t(n), hist);
+    });
+    timed(c, st, "plan", [&] {
+        hipLaunchKernelGGL((k_plan11<K>), dim3(1), dim3(1024), 0, st, hist, plan, uint64_t(n), FLOATK ? 1 : 0);
+    });
+    for (int p = 0; p < P; ++p) {
+        uint32_t *next = (p + 1 < P) ? lb[(p + 1) & 1] : nullptr;
+        timed(c, st, "onesweep", [&] {
+            hipLaunchKernelGGL((k_onesweep11<K, FLOATK>), dim3(unsigned(L.tiles)), dim3(D11_BLOCK), 0, st, d_keys,
+                               keysB, uint32_t(n), p, plan, lb[p & 1], next, c->fallback_rounds);
+        });
+    }
+    timed(c, st, "copyback", [&] {
+        hipLaunchKernelGGL((k_copyback11<K>), dim3(grid_for(uint64_t(n), 256, c->num_cus * 8)), dim3(256), 0, st,
+                           d_keys, keysB, uint64_t(n), plan);
+    });
 }
 
 template <typename K, bool HAS_VALS, int FLOATK>
@@ -448,6 +513,9 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     if (reinterpret_cast<uintptr_t>(d_keys) % sizeof(K)) fail(HPCS_E_INVALID_ARG, "keys pointer is misaligned");
     constexpr int P = int(sizeof(K));
     const SortParams prm = resolve_params(c, int(sizeof(K)), HAS_VALS, n);
+    if constexpr (!HAS_VALS) {
+        if (prm.bits == 11) return run_sort11<K, FLOATK>(c, d_keys, n, st);
+    }
     const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
     const Layout L = make_layout(n, sizeof(K), HAS_VALS, tile_keys);
@@ -1107,6 +1175,11 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             ctx->pass_algo = int(value);
             ctx->set_algo = value != PASS_AUTO;
             break;
+        case HPCS_OPT_DIGIT_BITS:
+            if (value != 8 && value != 11) fail(HPCS_E_INVALID_ARG, "digit bits must be 8 or 11");
+            ctx->digit_bits = int(value);
+            ctx->set_bits = true;
+            break;
         case HPCS_OPT_DIGIT_BYTES:
             if (value < 0 || value > 2) fail(HPCS_E_INVALID_ARG, "digit bytes must be 0, 1 or 2");
             ctx->digit_bytes = int(value);
@@ -1170,8 +1243,21 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
         g_last_error = "n must be non-negative";
         return HPCS_E_INVALID_ARG;
     }
+    const SortParams prm = ctx ? resolve_params(ctx, int(ks), with_values != 0, n) : SortParams{PASS_AUTO, 0, 0, 8};
+    if (prm.bits == 11) {
+        const int passes = ks == 4 ? d11_passes<uint32_t>() : d11_passes<uint64_t>();
+        const Layout11 L = make_layout11(n, ks);
+        info->digit_bits = D11_BITS;
+        info->passes = passes;
+        info->tile_keys = D11_TILE;
+        info->block_threads = D11_BLOCK;
+        info->tiles = L.tiles;
+        info->ideal_bytes = n * (int64_t(ks) + passes * 2 * int64_t(ks));
+        info->scratch_bytes = int64_t(L.total);
+        return HPCS_OK;
+    }
     const int passes = int(ks * 8 / RADIX_BITS);
-    const int shape = ctx ? resolve_params(ctx, int(ks), with_values != 0, n).shape : 0;
+    const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
     const Layout L = make_layout(n, ks, with_values != 0, tile_keys);
     const int64_t per_key = int64_t(ks) + passes * 2 * (int64_t(ks) + (with_values ? 4 : 0));
@@ -1204,6 +1290,7 @@ HPCS_API int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n)
             const int tk = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
             need = std::max(need, make_layout(n, ks, false, tk).total);
         }
+        if (n < LOOKBACK_MAX_KEYS) need = std::max(need, make_layout11(n, ks).total);
         const size_t topk = align_up(align_up(align_up(sizeof(TopkState)) + size_t(n) * ks) + size_t(n) * ks);
         need = std::max(need, topk);
         ensure(ctx->scratch, ctx->scratch_cap, need);

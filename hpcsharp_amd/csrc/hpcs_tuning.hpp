@@ -1,5 +1,5 @@
 // hpcs_tuning.hpp -- tuning tables of libhpcsort.so: one row per line,
-//   <dtype> <max_n> <algo> <shape> <rank>
+//   <dtype> <max_n> <algo> <shape> <rank> [<digit_bits>]
 // Plain C++ (no device code), so the parser is tested on the host with
 // sanitizers.
 #pragma once
@@ -13,20 +13,24 @@ namespace hpcs {
 constexpr int TUNE_TILE_SHAPES = 5; // tile shapes 0-4 (threads x keys per thread)
 
 // One row of a tuning table: for sorts of up to max_n keys of key_bytes
-// (with or without u32 values), the pass algorithm, tile shape and ranking.
+// (with or without u32 values), the pass algorithm, tile shape, ranking and
+// digit width.
 struct TuneEntry {
     int key_bytes;
     int vals;
     int64_t max_n;
     int algo, shape, rank;
+    int bits;
 };
 
 // Parse a tuning table: one row per line, `#` starts a comment,
-//   <dtype> <max_n> <algo> <shape> <rank>
+//   <dtype> <max_n> <algo> <shape> <rank> [<digit_bits>]
 // dtype is u32, f32, i32 (4-byte keys), u64, f64, i64 (8-byte keys), pairs32
 // or pairs64 (keys with u32 values); algo 0 onesweep / 1 reduce-then-scan;
 // shape 0-4 (1-3 for 4-byte keys without values only); rank 0 ballot / 1
-// LDS-OR.
+// LDS-OR; digit_bits 8 (default) or 11.  11-bit digits exist in one form
+// only: onesweep passes, 256x16 tiles, ballot ranking, keys without values,
+// so a row with digit_bits 11 reads `<dtype> <max_n> 0 0 0 11`.
 inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, std::string &err)
 {
     out.clear();
@@ -42,11 +46,13 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
         if (hash != std::string::npos) line.resize(hash);
         char dt[16] = {0};
         long long max_n = 0;
-        int algo = 0, shape = 0, rank = 0, consumed = 0;
+        int algo = 0, shape = 0, rank = 0, bits = 8, consumed = 0, consumed6 = 0;
         if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
-        if (std::sscanf(line.c_str(), " %15s %lld %d %d %d %n", dt, &max_n, &algo, &shape, &rank, &consumed) != 5 ||
-            line.find_first_not_of(" \t\r", size_t(consumed)) != std::string::npos) {
-            err = "line " + std::to_string(line_no) + ": expected <dtype> <max_n> <algo> <shape> <rank>";
+        const int got = std::sscanf(line.c_str(), " %15s %lld %d %d %d %n%d %n", dt, &max_n, &algo, &shape, &rank,
+                                    &consumed, &bits, &consumed6);
+        if (got == 6) consumed = consumed6;
+        if (got < 5 || line.find_first_not_of(" \t\r", size_t(consumed)) != std::string::npos) {
+            err = "line " + std::to_string(line_no) + ": expected <dtype> <max_n> <algo> <shape> <rank> [<digit_bits>]";
             return false;
         }
         const std::string d(dt);
@@ -60,7 +66,8 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
             return false;
         }
         if (max_n < 1 || algo < 0 || algo > 1 || rank < 0 || rank > 1 || shape < 0 || shape >= TUNE_TILE_SHAPES ||
-            (shape != 0 && shape != 4 && (e.key_bytes != 4 || e.vals))) {
+            (shape != 0 && shape != 4 && (e.key_bytes != 4 || e.vals)) || (bits != 8 && bits != 11) ||
+            (bits == 11 && (algo != 0 || shape != 0 || rank != 0 || e.vals))) {
             err = "line " + std::to_string(line_no) + ": value out of range";
             return false;
         }
@@ -68,6 +75,7 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
         e.algo = algo;
         e.shape = shape;
         e.rank = rank;
+        e.bits = bits;
         out.push_back(e);
     }
     return true;

@@ -62,6 +62,9 @@ extern "C" {
                                       2: by size (default; onesweep for small arrays) */
 #define HPCS_OPT_PERSISTENT 6      /* reduce-then-scan scatter: 0 one workgroup per tile (default), 1 persistent
                                       workgroups that prefetch the next tile */
+#define HPCS_OPT_DIGIT_BITS 8      /* 8 (default) or 11: 11-bit digits (3 passes for 4-byte keys, 6 for 8-byte
+                                      keys) in the onesweep form, for sorts of keys without values below 2^30
+                                      keys; other sorts keep 8-bit digits */
 #define HPCS_OPT_DIGIT_BYTES 7     /* reduce-then-scan: 1 the scatter also writes each key's next digit as a
                                       byte and the next pass counts its tiles from those n bytes; 0 the next
                                       pass re-reads the keys; 2 (default) 1 for 8-byte keys with values, else 0
@@ -113,12 +116,14 @@ int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n);
 /* Tuning tables (no counterpart in the reference, whose tuning knobs are
  * static properties, RadixSortLsdParallel.cs:763, SystemAttributes.cs:9).
  * A table is text, one row per line (`#` comments):
- *   <dtype> <max_n> <algo> <shape> <rank>
+ *   <dtype> <max_n> <algo> <shape> <rank> [<digit_bits>]
  * dtype u32|f32|i32|u64|f64|i64|pairs32|pairs64; a row applies to sorts of at
  * most max_n keys (the row with the smallest such max_n; beyond every row,
  * the largest); algo 0 onesweep, 1 reduce-then-scan; shape 0-4
  * (HPCS_OPT_TILE_SHAPE; 1-3 for 4-byte keys without values only); rank 0
- * ballot, 1 LDS-OR.  Options set with
+ * ballot, 1 LDS-OR; digit_bits 8 (default) or 11 (HPCS_OPT_DIGIT_BITS; an
+ * 11-bit row reads `<dtype> <max_n> 0 0 0 11`, keys without values only).
+ * Options set with
  * hpcs_ctx_set_option (or HPCS_RANK / HPCS_TILE_SHAPE) take precedence.
  * hpcs_ctx_load_tuning replaces the context's table (NULL or "" clears it);
  * a context also loads $HPCS_TUNE_FILE when created.  hpcs_tuning_check

@@ -78,8 +78,13 @@ static int tuning_test()
     CHECK(rows[1].key_bytes == 8 && rows[1].vals == 1 && rows[1].algo == 1);
     CHECK(rows[2].max_n == 268435456);
     CHECK(hpcs::parse_tuning("u64 1 0 4 1\npairs32 7 1 4 0", rows, err) && rows.size() == 2 && rows[1].shape == 4);
+    CHECK(rows[0].bits == 8 && rows[1].bits == 8);
+    CHECK(hpcs::parse_tuning("u32 65536 0 0 0 11  # eleven\nf64 9 1 0 1 8", rows, err) && rows.size() == 2);
+    CHECK(rows[0].bits == 11 && rows[0].algo == 0 && rows[1].bits == 8 && rows[1].algo == 1);
     const char *bad[] = {"u8 1 0 0 0", "u32 0 0 0 0", "u32 1 2 0 0", "u32 1 0 5 0", "u32 1 0 0 2", "u64 1 0 1 0",
                          "pairs32 1 0 3 1", "u32 1 0 0 1 9", "u32 1 0", "u32 -5 0 0 1",
+                         "u32 1 1 0 0 11", "u32 1 0 4 0 11", "u32 1 0 0 1 11", "pairs64 1 0 0 0 11",
+                         "u32 1 0 0 0 11 8", "u32 1 0 0 0 x",
                          "aaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaaa 1 0 0 1"};
     for (const char *b : bad) {
         CHECK(!hpcs::parse_tuning(b, rows, err));
