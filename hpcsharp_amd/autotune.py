@@ -1,14 +1,16 @@
 """Autotuner: measure every sort variant per key type and size class on this
 GPU and write the tuning table libhpcsort.so reads (hpcs_ctx_load_tuning).
 
-Variants: pass algorithm (onesweep with look-back, reduce-then-scan), tile
-shape (256x16 and 256x8 threads x keys for every sort; 512x16, 256x32 and
-512x32 for 4-byte keys without values) and ranking (ballot, LDS-OR).  Digits are 8-bit in this version.
+Variants: digit width (8 bits; 11 bits for keys without values), pass
+algorithm (onesweep with look-back, reduce-then-scan, one workgroup in LDS
+for small arrays), tile shape (256x16 and
+256x8 threads x keys for every sort; 512x16, 256x32 and 512x32 for 4-byte
+keys without values) and ranking (ballot, LDS-OR).
 Size classes are powers of two; a class's row applies to every n up to it.
 Each measured cell is appended to a JSON log as it finishes, and a rerun
 skips the cells the log already holds (the sweep resumes).
 
-Run on the GPU:  python -m hpcsharp_amd.autotune [--classes 12,16,20,24,28]
+Run on the GPU:  python -m hpcsharp_amd.autotune [--classes 12,13,14,16,18,20,22,24,26,28]
 """
 from __future__ import annotations
 
@@ -31,9 +33,26 @@ KINDS = {
 }
 
 
-def candidates(key_bytes: int, vals: bool):
+# largest arrays the one-workgroup sort (pass algorithm 3) takes
+BLOCK_SORT_MAX = {4: 16384, 8: 8192}
+
+
+def candidates(key_bytes: int, vals: bool, n: int | None = None):
+    """8-bit digits: every pass algorithm x tile shape x ranking, and the
+    one-workgroup sort where n fits it; 11-bit digits (keys without values):
+    their one form, onesweep passes with 256x16 tiles and ballot ranking."""
     shapes = (0, 1, 2, 3, 4) if key_bytes == 4 and not vals else (0, 4)
-    return [{"algo": a, "shape": s, "rank": r} for a, s, r in itertools.product((0, 1), shapes, (1, 0))]
+    out = [{"algo": a, "shape": s, "rank": r, "bits": 8} for a, s, r in itertools.product((0, 1), shapes, (1, 0))]
+    if not vals:
+        out.append({"algo": 0, "shape": 0, "rank": 0, "bits": 11})
+    if n is not None and n <= BLOCK_SORT_MAX[key_bytes]:
+        out.append({"algo": 3, "shape": 0, "rank": 1, "bits": 8})
+    return out
+
+
+def cell_key(kind: str, lg: int, cand: dict) -> str:
+    key = f"{kind}:{lg}:{cand['algo']}:{cand['shape']}:{cand['rank']}"
+    return key + ":11" if cand.get("bits", 8) == 11 else key
 
 
 def time_variant(torch, ctx, kind: str, lg: int, cand: dict, reps: int) -> float:
@@ -44,6 +63,7 @@ def time_variant(torch, ctx, kind: str, lg: int, cand: dict, reps: int) -> float
     N.set_option(N.HPCS_OPT_PASS_ALGO, cand["algo"], ctx)
     N.set_option(N.HPCS_OPT_TILE_SHAPE, cand["shape"], ctx)
     N.set_option(N.HPCS_OPT_RANK, cand["rank"], ctx)
+    N.set_option(N.HPCS_OPT_DIGIT_BITS, cand.get("bits", 8), ctx)
     g = torch.Generator(device="cuda").manual_seed(lg * 1000 + key_bytes)
     src = torch.randint(-2 ** 31, 2 ** 31 - 1, (n * key_bytes // 4,), dtype=torch.int32, device="cuda", generator=g)
     keys = src.clone()
@@ -74,21 +94,21 @@ def time_variant(torch, ctx, kind: str, lg: int, cand: dict, reps: int) -> float
 def table_text(results: dict) -> str:
     lines = ["# libhpcsort tuning table (hpcs_ctx_load_tuning), measured on "
              + results.get("device", "this GPU") + " by hpcsharp_amd.autotune",
-             "# dtype max_n algo shape rank   # median ms per sort of the best variant"]
+             "# dtype max_n algo shape rank digit_bits   # median ms per sort of the best variant"]
     alias = {"u32": ("u32", "f32", "i32"), "u64": ("u64", "f64", "i64"), "pairs64": ("pairs64",)}
     for kind in KINDS:
         for lg in sorted({int(k.split(":")[1]) for k in results["cells"] if k.startswith(kind + ":")}):
             cells = [c for k, c in results["cells"].items() if k.startswith(f"{kind}:{lg}:")]
             best = min(cells, key=lambda c: c["s"])
             for name in alias[kind]:
-                lines.append(f"{name} {1 << lg} {best['algo']} {best['shape']} {best['rank']}"
+                lines.append(f"{name} {1 << lg} {best['algo']} {best['shape']} {best['rank']} {best.get('bits', 8)}"
                              f"   # {best['s'] * 1e3:.4f} ms")
     return "\n".join(lines) + "\n"
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--classes", default="12,16,20,24,28")
+    ap.add_argument("--classes", default="12,13,14,16,18,20,22,24,26,28")
     ap.add_argument("--kinds", default="u32,u64,pairs64")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--log", default=os.path.join(HERE, "tune_gfx950.json"))
@@ -108,8 +128,8 @@ def main(argv=None):
         for kind in args.kinds.split(","):
             key_bytes, vals, _ = KINDS[kind]
             for lg in (int(x) for x in args.classes.split(",")):
-                for cand in candidates(key_bytes, vals):
-                    key = f"{kind}:{lg}:{cand['algo']}:{cand['shape']}:{cand['rank']}"
+                for cand in candidates(key_bytes, vals, 1 << lg):
+                    key = cell_key(kind, lg, cand)
                     if key in results["cells"]:
                         continue                     # measured in an earlier run
                     s = time_variant(torch, ctx, kind, lg, cand, args.reps)

@@ -52,6 +52,10 @@ void hip_check(hipError_t e, const char *what)
 constexpr int PASS_ONESWEEP = 0;
 constexpr int PASS_RTS = 1;
 constexpr int PASS_AUTO = 2;
+constexpr int PASS_BLOCK = 3; // one workgroup sorts the whole array in LDS
+// largest arrays the one-workgroup sort takes (LDS: keys, values, counters)
+constexpr int64_t BLOCK_SORT_MAX4 = 1024 * 16; // 4-byte keys
+constexpr int64_t BLOCK_SORT_MAX8 = 1024 * 8;  // 8-byte keys
 
 
 // Onesweep tile shapes (threads x keys per thread).  Shape 0 serves every key
@@ -437,7 +441,15 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
     }
     // Below these sizes the single-kernel onesweep pass beats reduce-then-
     // scan's five kernels per pass (launch-bound), measured on MI355X.
-    if (p.algo == PASS_AUTO) p.algo = n < (key_bytes == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21)) ? PASS_ONESWEEP : PASS_RTS;
+    const int64_t block_max = key_bytes == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8;
+    if (p.algo == PASS_BLOCK && n > block_max) p.algo = PASS_AUTO;
+    // by size, measured on MI355X: one workgroup up to 4096 4-byte / 8192
+    // 8-byte keys (beyond 4096 4-byte keys the 1024-thread workgroup on one
+    // CU loses to onesweep passes over many CUs)
+    if (p.algo == PASS_AUTO)
+        p.algo = n <= (key_bytes == 4 ? 4096 : BLOCK_SORT_MAX8) ? PASS_BLOCK
+                 : n < (key_bytes == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21)) ? PASS_ONESWEEP
+                                                                                   : PASS_RTS;
     // 11-bit digits exist for onesweep passes over keys without values (their
     // look-back words carry 30-bit counts); other sorts keep 8-bit digits
     if (p.bits == 11 && (vals || p.algo != PASS_ONESWEEP || n >= LOOKBACK_MAX_KEYS)) p.bits = 8;
@@ -446,6 +458,21 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
         p.rank = RANK_BALLOT;
     }
     return p;
+}
+
+// The one-workgroup sort (n <= BLOCK_SORT_MAX4 / BLOCK_SORT_MAX8): no scratch.
+template <typename K, bool HAS_VALS, int FLOATK>
+void run_sort_block(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
+{
+    constexpr int ITEMS = sizeof(K) == 4 ? 16 : 8;
+    timed(c, st, "block_sort", [&] {
+        if (n <= 256 * ITEMS)
+            hipLaunchKernelGGL((k_sort_block<K, FLOATK, HAS_VALS, 256, ITEMS>), dim3(1), dim3(256), 0, st, d_keys,
+                               d_vals, uint32_t(n));
+        else
+            hipLaunchKernelGGL((k_sort_block<K, FLOATK, HAS_VALS, 1024, ITEMS>), dim3(1), dim3(1024), 0, st, d_keys,
+                               d_vals, uint32_t(n));
+    });
 }
 
 // Scratch of an 11-bit sort:
@@ -513,6 +540,10 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     if (reinterpret_cast<uintptr_t>(d_keys) % sizeof(K)) fail(HPCS_E_INVALID_ARG, "keys pointer is misaligned");
     constexpr int P = int(sizeof(K));
     const SortParams prm = resolve_params(c, int(sizeof(K)), HAS_VALS, n);
+    if (prm.algo == PASS_BLOCK) {
+        static_assert(BLOCK_SORT_MAX4 == 1024 * 16 && BLOCK_SORT_MAX8 == 1024 * 8, "k_sort_block sizes");
+        return run_sort_block<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
+    }
     if constexpr (!HAS_VALS) {
         if (prm.bits == 11) return run_sort11<K, FLOATK>(c, d_keys, n, st);
     }
@@ -1170,8 +1201,8 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             ctx->lookback_window = int(value);
             break;
         case HPCS_OPT_PASS_ALGO:
-            if (value != PASS_ONESWEEP && value != PASS_RTS && value != PASS_AUTO)
-                fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0, 1 or 2");
+            if (value != PASS_ONESWEEP && value != PASS_RTS && value != PASS_AUTO && value != PASS_BLOCK)
+                fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0, 1, 2 or 3");
             ctx->pass_algo = int(value);
             ctx->set_algo = value != PASS_AUTO;
             break;
@@ -1257,6 +1288,17 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
         return HPCS_OK;
     }
     const int passes = int(ks * 8 / RADIX_BITS);
+    if (prm.algo == PASS_BLOCK) {
+        const int items = ks == 4 ? 16 : 8;
+        info->digit_bits = RADIX_BITS;
+        info->passes = passes;
+        info->block_threads = n <= 256 * items ? 256 : 1024;
+        info->tile_keys = info->block_threads * items;
+        info->tiles = 1;
+        info->ideal_bytes = n * (int64_t(ks) + passes * 2 * (int64_t(ks) + (with_values ? 4 : 0)));
+        info->scratch_bytes = 0;
+        return HPCS_OK;
+    }
     const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
     const Layout L = make_layout(n, ks, with_values != 0, tile_keys);

@@ -36,12 +36,14 @@ struct SegPlan {
 };
 
 // Ballot ranking of one tile held lane-striped in registers (wave w, item i,
-// lane l = key w*64*ITEMS + i*64 + l): on return every valid key has been
-// stored to s_keys at its stable in-tile position sorted by the digit at
-// `shift`, and s_start[d] holds the tile's first position of digit d.
-template <typename K, int BLOCK, int ITEMS>
+// lane l = key w*64*ITEMS + i*64 + l): on return every valid key (and, with
+// HAS_VALS, its value) has been stored to s_keys (s_vals) at its stable
+// in-tile position sorted by the digit at `shift`, and s_start[d] holds the
+// tile's first position of digit d.
+template <typename K, int BLOCK, int ITEMS, bool HAS_VALS = false>
 __device__ inline void tile_rank_stage(const K (&keys)[ITEMS], int wave_valid, int shift,
-                                       uint32_t (*s_cnt)[RADIX], uint32_t *s_scan, uint32_t *s_start, K *s_keys)
+                                       uint32_t (*s_cnt)[RADIX], uint32_t *s_scan, uint32_t *s_start, K *s_keys,
+                                       const uint32_t *vals = nullptr, uint32_t *s_vals = nullptr)
 {
     constexpr int WAVES = BLOCK / WAVE;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
@@ -87,8 +89,13 @@ __device__ inline void tile_rank_stage(const K (&keys)[ITEMS], int wave_valid, i
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j)
-        if (j * WAVE + lane < wave_valid) s_keys[cnt[KeyOps<K>::digit(keys[j], shift)] + rank[j]] = keys[j];
+    for (int j = 0; j < ITEMS; ++j) {
+        if (j * WAVE + lane < wave_valid) {
+            const uint32_t at = cnt[KeyOps<K>::digit(keys[j], shift)] + rank[j];
+            s_keys[at] = keys[j];
+            if constexpr (HAS_VALS) s_vals[at] = vals[j];
+        }
+    }
     __syncthreads();
 }
 
@@ -358,6 +365,85 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_scatter(uint32_t *__restrict_
             }
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_sort_block: one workgroup sorts a whole array of n <= BLOCK * ITEMS keys
+// (and u32 values) in LDS: one read, the LSD passes over the varying digit
+// positions (tile_rank_stage per pass, the float / signed key transform on
+// load and its inverse on store), one write.  One launch replaces the
+// count / plan / pass / copy-back sequence of the multi-tile sorts.
+// ---------------------------------------------------------------------------
+template <typename K, int FLOATK, bool HAS_VALS, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_sort_block(K *__restrict__ keys, uint32_t *__restrict__ vals, uint32_t n)
+{
+    constexpr int WAVES = BLOCK / WAVE;
+    constexpr int WAVE_TILE = WAVE * ITEMS;
+    constexpr int TILE = BLOCK * ITEMS;
+    constexpr int P = int(sizeof(K));
+    __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint32_t s_scan[RADIX / WAVE];
+    __shared__ uint32_t s_start[RADIX];
+    __shared__ K s_keys[TILE];
+    __shared__ uint32_t s_vals[HAS_VALS ? TILE : 1];
+    __shared__ K s_or[WAVES], s_nor[WAVES];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    const int len = int(min(n, uint32_t(TILE)));
+    const int wave_valid = len > wave * WAVE_TILE ? min(WAVE_TILE, len - wave * WAVE_TILE) : 0;
+    K v[ITEMS];
+    uint32_t pv[HAS_VALS ? ITEMS : 1];
+    K vor = 0, vnor = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const int o = j * WAVE + lane;
+        v[j] = ~K(0);
+        if (o < wave_valid) {
+            v[j] = FLOATK ? key_enc<K, FLOATK>(keys[wave * WAVE_TILE + o]) : keys[wave * WAVE_TILE + o];
+            vor |= v[j];
+            vnor |= ~v[j];
+        }
+        if constexpr (HAS_VALS) pv[j] = o < wave_valid ? vals[wave * WAVE_TILE + o] : 0u;
+    }
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        vor |= __shfl_xor(vor, off);
+        vnor |= __shfl_xor(vnor, off);
+    }
+    if (lane == 0) {
+        s_or[wave] = vor;
+        s_nor[wave] = vnor;
+    }
+    __syncthreads();
+    K a = 0, c = 0;
+    for (int w = 0; w < WAVES; ++w) {
+        a |= s_or[w];
+        c |= s_nor[w];
+    }
+    const K varying = a & c; // bits that differ between keys
+    bool staged = false;     // keys sit in s_keys in sorted order
+    for (int pass = 0; pass < P; ++pass) {
+        if (((varying >> (8 * pass)) & K(0xFF)) == 0) continue;
+        if (staged) {
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                const int o = j * WAVE + lane;
+                if (o < wave_valid) {
+                    v[j] = s_keys[wave * WAVE_TILE + o];
+                    if constexpr (HAS_VALS) pv[j] = s_vals[wave * WAVE_TILE + o];
+                }
+            }
+            __syncthreads();
+        }
+        tile_rank_stage<K, BLOCK, ITEMS, HAS_VALS>(v, wave_valid, 8 * pass, s_cnt, s_scan, s_start, s_keys, pv,
+                                                   s_vals);
+        staged = true;
+    }
+    if (staged) {
+        for (int j = tid; j < len; j += BLOCK) {
+            keys[j] = FLOATK ? key_dec<K, FLOATK>(s_keys[j]) : s_keys[j];
+            if constexpr (HAS_VALS) vals[j] = s_vals[j];
+        }
     }
 }
 

@@ -59,7 +59,9 @@ extern "C" {
                                       (default 24; 0 makes every wait use the local count, for tests) */
 #define HPCS_OPT_LOOKBACK_WINDOW 4 /* u32 keys: earlier tiles read per look-back round: 16, 32 (default), 64 */
 #define HPCS_OPT_PASS_ALGO 5       /* 0: single-pass onesweep with look-back, 1: reduce-then-scan,
-                                      2: by size (default; onesweep for small arrays) */
+                                      2: by size (default: one workgroup up to 4096 4-byte / 8192 8-byte
+                                      keys, onesweep for small arrays, reduce-then-scan beyond),
+                                      3: one workgroup sorts the array in LDS (larger arrays: by size) */
 #define HPCS_OPT_PERSISTENT 6      /* reduce-then-scan scatter: 0 one workgroup per tile (default), 1 persistent
                                       workgroups that prefetch the next tile */
 #define HPCS_OPT_DIGIT_BITS 8      /* 8 (default) or 11: 11-bit digits (3 passes for 4-byte keys, 6 for 8-byte
@@ -119,7 +121,8 @@ int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n);
  *   <dtype> <max_n> <algo> <shape> <rank> [<digit_bits>]
  * dtype u32|f32|i32|u64|f64|i64|pairs32|pairs64; a row applies to sorts of at
  * most max_n keys (the row with the smallest such max_n; beyond every row,
- * the largest); algo 0 onesweep, 1 reduce-then-scan; shape 0-4
+ * the largest); algo 0 onesweep, 1 reduce-then-scan, 3 one workgroup (see
+ * HPCS_OPT_PASS_ALGO); shape 0-4
  * (HPCS_OPT_TILE_SHAPE; 1-3 for 4-byte keys without values only); rank 0
  * ballot, 1 LDS-OR; digit_bits 8 (default) or 11 (HPCS_OPT_DIGIT_BITS; an
  * 11-bit row reads `<dtype> <max_n> 0 0 0 11`, keys without values only).

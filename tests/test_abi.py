@@ -75,7 +75,7 @@ def test_tuning_table_parser(tmp_path):
     assert lib.hpcs_tuning_check(_table(tmp_path, good), ctypes.byref(rows)) == 0
     assert rows.value == 4
     for bad in ("u16 4096 0 0 1\n", "u64 4096 0 2 1\n", "u32 4096 0 0 1 7\n", "u32 0 0 0 1\n",
-                "u32 4096 3 0 1\n", "pairs32 100 0 1 0\n", "u32 4096 0 0\n", "u32 4096 1 0 0 11\n",
+                "u32 4096 2 0 1\n", "u32 4096 3 0 0 11\n", "pairs32 100 0 1 0\n", "u32 4096 0 0\n", "u32 4096 1 0 0 11\n",
                 "pairs64 4096 0 0 0 11\n"):
         assert lib.hpcs_tuning_check(_table(tmp_path, bad), ctypes.byref(rows)) == N.HPCS_E_INVALID_ARG, bad
         assert b"line 1" in lib.hpcs_last_error()
