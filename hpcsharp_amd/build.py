@@ -44,7 +44,26 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libhpcsort.so")
     os.replace(tmp, LIB)
+    build_tools(verbose)
     return LIB
+
+
+# Stand-alone measurement programs (no library API): source -> binary in the package.
+TOOLS = {"scatter_ceiling": os.path.join(HERE, "csrc", "scatter_ceiling.hip")}
+
+
+def build_tools(verbose: bool = False) -> None:
+    for name, src in TOOLS.items():
+        out = os.path.join(HERE, name)
+        cmd = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", src, "-o", out + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed building {name}")
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":
