@@ -632,23 +632,23 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     });
 }
 
-// Batched segments: scratch [SegPlan | seg_tile0 | small | large | tile_seg |
-// counts | keys B].  The plan's word `bad` counts rejected segments.
+// Batched segments: scratch [SegPlan | seg_tile0 | small | large | tile
+// records | counts | keys B].  The plan's word `bad` counts rejected segments.
 struct SegLayout {
-    size_t off_plan, off_tile0, off_small, off_large, off_tileseg, off_counts, off_keysB, total;
+    size_t off_plan, off_tile0, off_small, off_large, off_tinfo, off_counts, off_keysB, total;
     uint64_t max_tiles;
 };
 
 SegLayout seg_layout(int64_t n, int nseg)
 {
     SegLayout L{};
-    L.max_tiles = uint64_t(n) / SEG_TILE + uint64_t(nseg) + 1;
+    L.max_tiles = uint64_t(n) / SEG_TILE + uint64_t(nseg) + 1; // tiles of the large segments, at most
     L.off_plan = 0;
     L.off_tile0 = align_up(sizeof(SegPlan));
     L.off_small = align_up(L.off_tile0 + size_t(nseg) * 4);
     L.off_large = align_up(L.off_small + size_t(nseg) * 4);
-    L.off_tileseg = align_up(L.off_large + size_t(nseg) * 4);
-    L.off_counts = align_up(L.off_tileseg + L.max_tiles * 4);
+    L.off_tinfo = align_up(L.off_large + size_t(nseg) * 4);
+    L.off_counts = align_up(L.off_tinfo + L.max_tiles * sizeof(uint4));
     L.off_keysB = align_up(L.off_counts + L.max_tiles * RADIX * 4);
     L.total = align_up(L.off_keysB + size_t(n) * 4);
     return L;
@@ -666,36 +666,32 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     uint32_t *tile0 = reinterpret_cast<uint32_t *>(base + L.off_tile0);
     uint32_t *small = reinterpret_cast<uint32_t *>(base + L.off_small);
     uint32_t *large = reinterpret_cast<uint32_t *>(base + L.off_large);
-    uint32_t *tileseg = reinterpret_cast<uint32_t *>(base + L.off_tileseg);
+    uint4 *tinfo = reinterpret_cast<uint4 *>(base + L.off_tinfo);
     uint32_t *counts = reinterpret_cast<uint32_t *>(base + L.off_counts);
     uint32_t *keysB = reinterpret_cast<uint32_t *>(base + L.off_keysB);
     timed(c, st, "seg_plan", [&] {
-        hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, d_begin, nseg, n, sp, tile0, tileseg, small, large);
+        hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, d_begin, nseg, n, sp, tile0, small, large);
     });
-    const int small_grid = std::min(nseg, c->num_cus * 8);
+    const int list_grid = std::min(nseg, c->num_cus * 8);
+    timed(c, st, "seg_tiles", [&] {
+        hipLaunchKernelGGL(k_seg_tiles, dim3(list_grid), dim3(256), 0, st, d_begin, sp, tile0, large, tinfo);
+    });
     timed(c, st, "seg_small", [&] {
-        hipLaunchKernelGGL(k_seg_small, dim3(small_grid), dim3(SEG_BLOCK), 0, st, d_keys, d_begin, sp, small);
+        hipLaunchKernelGGL(k_seg_small, dim3(list_grid), dim3(SEG_BLOCK), 0, st, d_keys, d_begin, sp, small);
     });
-    const int count_grid = c->num_cus * 8;
-    const int scan_grid = std::min(nseg, c->num_cus * 8);
-    static int occ = -1;
-    if (occ < 0) {
-        int o = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_seg_scatter, SEG_BLOCK, 0) != hipSuccess || o < 1) o = 1;
-        occ = o;
-    }
-    const int scatter_grid = std::max(int(NUM_XCD), c->num_cus * occ / int(NUM_XCD) * int(NUM_XCD));
+    // one workgroup per tile of the large segments (the device knows how
+    // many; the grid is the bound and spare workgroups return)
+    const dim3 tile_grid(unsigned(L.max_tiles));
     for (int p = 0; p < 4; ++p) {
         timed(c, st, "seg_counts", [&] {
-            hipLaunchKernelGGL(k_seg_counts, dim3(count_grid), dim3(SEG_BLOCK), 0, st, d_keys, keysB, p, d_begin, sp,
-                               tile0, tileseg, counts);
+            hipLaunchKernelGGL(k_seg_counts, tile_grid, dim3(SEG_BLOCK), 0, st, d_keys, keysB, p, sp, tinfo, counts);
         });
         timed(c, st, "seg_scan", [&] {
-            hipLaunchKernelGGL(k_seg_scan, dim3(scan_grid), dim3(RADIX), 0, st, d_begin, sp, tile0, large, counts);
+            hipLaunchKernelGGL(k_seg_scan, dim3(list_grid), dim3(SEG_SCAN_GROUPS * RADIX), 0, st, d_begin, sp, tile0,
+                               large, counts);
         });
         timed(c, st, "seg_scatter", [&] {
-            hipLaunchKernelGGL(k_seg_scatter, dim3(scatter_grid), dim3(SEG_BLOCK), 0, st, d_keys, keysB, p, d_begin,
-                               sp, tile0, tileseg, counts);
+            hipLaunchKernelGGL(k_seg_scatter, tile_grid, dim3(SEG_BLOCK), 0, st, d_keys, keysB, p, sp, tinfo, counts);
         });
     }
 }

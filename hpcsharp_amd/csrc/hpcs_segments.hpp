@@ -9,6 +9,7 @@
 //   k_seg_plan     classify: segments of at most SEG_SMALL keys go to a list
 //                  sorted by k_seg_small; larger ones get a range of tiles of
 //                  TILE keys (a tile never crosses a segment)
+//   k_seg_tiles    one 16-byte record per tile (its keys, its segment)
 //   k_seg_small    one workgroup per small segment: four LSD passes entirely
 //                  in LDS (one HBM read and one write per key)
 //   per pass p of the large segments:
@@ -99,10 +100,78 @@ __device__ inline void tile_rank_stage(const K (&keys)[ITEMS], int wave_valid, i
     __syncthreads();
 }
 
-// One block of 1024 threads.  begin has nseg + 1 entries.
+// As tile_rank_stage (keys only), ranking with LDS-OR match masks: per-wave
+// counts by LDS atomics first, then each lane ORs its bit into its digit's
+// mask, reads the mask back and stages its key at base + lanes below it.
+template <typename K, int BLOCK, int ITEMS>
+__device__ inline void tile_rank_stage_ldsor(const K (&keys)[ITEMS], int wave_valid, int shift,
+                                             uint32_t (*s_cnt)[RADIX], uint64_t (*s_match)[RADIX],
+                                             uint32_t *s_scan, uint32_t *s_start, K *s_keys)
+{
+    constexpr int WAVES = BLOCK / WAVE;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    for (int q = tid; q < WAVES * RADIX; q += BLOCK) {
+        (&s_cnt[0][0])[q] = 0u;
+        (&s_match[0][0])[q] = 0ull;
+    }
+    __syncthreads();
+    uint32_t *cnt = s_cnt[wave];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+        if (j * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[j], shift)], 1u);
+    __syncthreads();
+    uint32_t tile_cnt = 0, tile_start = 0;
+    if (tid < RADIX) {
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = s_cnt[w][tid];
+            s_cnt[w][tid] = tile_cnt;
+            tile_cnt += c;
+        }
+        uint32_t xs = tile_cnt;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(xs, off);
+            if (lane >= off) xs += y;
+        }
+        if (lane == WAVE - 1) s_scan[wave] = xs;
+        tile_start = xs - tile_cnt;
+    }
+    __syncthreads();
+    if (tid < RADIX) {
+        for (int w = 0; w < wave; ++w) tile_start += s_scan[w];
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) s_cnt[w][tid] += tile_start;
+        s_start[tid] = tile_start;
+    }
+    __syncthreads();
+    uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
+    const uint32_t lanebit = 1u << (lane & 31);
+    const int half = lane >> 5;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t d = KeyOps<K>::digit(keys[j], shift);
+        const bool valid = j * WAVE + lane < wave_valid;
+        if (valid) atomicOr(&match32[2 * d + half], lanebit);
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t peers = s_match[wave][d];
+        const uint32_t below = count_below(peers);
+        const uint32_t old = cnt[d];
+        if (valid && below == 0) {
+            cnt[d] = old + uint32_t(__popcll(peers));
+            s_match[wave][d] = 0ull;
+        }
+        if (valid) s_keys[old + below] = keys[j];
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+}
+
+// One block of 1024 threads.  begin has nseg + 1 entries.  Per segment: its
+// first tile (seg_tile0) and its place in the small or large list.
 __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ begin, int nseg, int64_t n,
                                                    SegPlan *__restrict__ sp, uint32_t *__restrict__ seg_tile0,
-                                                   uint32_t *__restrict__ tile_seg, uint32_t *__restrict__ small_list,
+                                                   uint32_t *__restrict__ small_list,
                                                    uint32_t *__restrict__ large_list)
 {
     __shared__ uint32_t s_wsum[16];
@@ -146,7 +215,6 @@ __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ b
         }
         if (s < nseg) {
             seg_tile0[s] = excl[0];
-            for (uint32_t t = 0; t < tiles; ++t) tile_seg[excl[0] + t] = uint32_t(s);
             if (small) small_list[excl[1]] = uint32_t(s);
             if (large) large_list[excl[2]] = uint32_t(s);
         }
@@ -157,6 +225,28 @@ __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ b
         sp->num_small = s_carry[1];
         sp->num_large = s_carry[2];
         sp->bad = s_bad;
+    }
+}
+
+// One record per tile of the large segments: {first key, segment's first key,
+// keys in the tile, segment}, so the pass kernels find a tile with one load.
+// Grid-stride over the large segments, the segment's tiles across the threads.
+__global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restrict__ begin,
+                                                   const SegPlan *__restrict__ sp,
+                                                   const uint32_t *__restrict__ seg_tile0,
+                                                   const uint32_t *__restrict__ large_list,
+                                                   uint4 *__restrict__ tinfo)
+{
+    const uint32_t nlarge = sp->num_large;
+    for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
+        const uint32_t s = large_list[k];
+        const int64_t b = begin[s], len = begin[s + 1] - b;
+        const uint32_t t0 = seg_tile0[s];
+        const uint32_t nt = uint32_t((len + SEG_TILE - 1) / SEG_TILE);
+        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+            const int64_t tb = int64_t(t) * SEG_TILE;
+            tinfo[t0 + t] = make_uint4(uint32_t(b + tb), uint32_t(b), uint32_t(min(int64_t(SEG_TILE), len - tb)), s);
+        }
     }
 }
 
@@ -230,79 +320,93 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_small(uint32_t *__restrict__ 
     }
 }
 
-// Grid-stride over the large segments' tiles: digit counts of pass `pass`.
+// One workgroup per tile of the large segments (the grid may exceed the tile
+// count; spare workgroups return): digit counts of pass `pass`.
 __global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
                                                           const uint32_t *__restrict__ keysB, int pass,
-                                                          const int64_t *__restrict__ begin,
                                                           const SegPlan *__restrict__ sp,
-                                                          const uint32_t *__restrict__ seg_tile0,
-                                                          const uint32_t *__restrict__ tile_seg,
+                                                          const uint4 *__restrict__ tinfo,
                                                           uint32_t *__restrict__ counts)
 {
     __shared__ uint32_t sh[RADIX];
+    const uint32_t t = blockIdx.x;
+    if (t >= sp->num_tiles) return;
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
     const int shift = 8 * pass;
-    const uint32_t ntiles = sp->num_tiles;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint32_t s = tile_seg[t];
-        const int64_t tb = begin[s] + int64_t(t - seg_tile0[s]) * SEG_TILE;
-        const int cnt = int(min(int64_t(SEG_TILE), begin[s + 1] - tb));
-        sh[tid] = 0u;
-        __syncthreads();
-        uint32_t k[SEG_ITEMS];
+    const uint4 ti = tinfo[t];
+    const uint32_t tb = ti.x;
+    const int cnt = int(ti.z);
+    sh[tid] = 0u;
+    __syncthreads();
+    uint32_t k[SEG_ITEMS];
 #pragma unroll
-        for (int j = 0; j < SEG_ITEMS; ++j) {
-            const int o = j * SEG_BLOCK + tid;
-            k[j] = o < cnt ? kin[tb + o] : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < SEG_ITEMS; ++j) {
-            const bool valid = j * SEG_BLOCK + tid < cnt;
-            const uint32_t d = (k[j] >> shift) & 0xFFu;
-            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-            if (__all(valid && d == d0)) {
-                if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
-            } else if (valid) {
-                atomicAdd(&sh[d], 1u);
-            }
-        }
-        __syncthreads();
-        counts[size_t(t) * RADIX + tid] = sh[tid];
-        __syncthreads();
+    for (int j = 0; j < SEG_ITEMS; ++j) {
+        const int o = j * SEG_BLOCK + tid;
+        k[j] = o < cnt ? kin[size_t(tb) + o] : 0u;
     }
+#pragma unroll
+    for (int j = 0; j < SEG_ITEMS; ++j) {
+        const bool valid = j * SEG_BLOCK + tid < cnt;
+        const uint32_t d = (k[j] >> shift) & 0xFFu;
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        if (__all(valid && d == d0)) {
+            if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
+        } else if (valid) {
+            atomicAdd(&sh[d], 1u);
+        }
+    }
+    __syncthreads();
+    counts[size_t(t) * RADIX + tid] = sh[tid];
 }
 
-// Grid-stride over the large segments, one workgroup (thread = digit) each:
-// digit starts of the segment and the per-tile offsets, in place of counts.
-__global__ __launch_bounds__(RADIX) void k_seg_scan(const int64_t *__restrict__ begin,
-                                                    const SegPlan *__restrict__ sp,
-                                                    const uint32_t *__restrict__ seg_tile0,
-                                                    const uint32_t *__restrict__ large_list,
-                                                    uint32_t *__restrict__ counts)
+// Grid-stride over the large segments, one workgroup each: digit starts of
+// the segment and the per-tile offsets, written in place of the counts.  The
+// SEG_SCAN_GROUPS x RADIX threads split the segment's tiles into as many
+// consecutive parts; thread (g, d) owns digit d of part g.
+constexpr int SEG_SCAN_GROUPS = 4;
+__global__ __launch_bounds__(SEG_SCAN_GROUPS * RADIX) void k_seg_scan(const int64_t *__restrict__ begin,
+                                                                     const SegPlan *__restrict__ sp,
+                                                                     const uint32_t *__restrict__ seg_tile0,
+                                                                     const uint32_t *__restrict__ large_list,
+                                                                     uint32_t *__restrict__ counts)
 {
+    __shared__ uint32_t s_part[SEG_SCAN_GROUPS][RADIX];
     __shared__ uint32_t s_wsum[RADIX / WAVE];
-    const int d = threadIdx.x, lane = d & (WAVE - 1), wave = d >> 6;
+    const int d = threadIdx.x & (RADIX - 1), g = threadIdx.x / RADIX;
+    const int lane = d & (WAVE - 1), wave = d >> 6;
     const uint32_t nlarge = sp->num_large;
     for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
         const uint32_t s = large_list[k];
         const uint32_t t0 = seg_tile0[s];
         const uint32_t nt = uint32_t((begin[s + 1] - begin[s] + SEG_TILE - 1) / SEG_TILE);
-        uint32_t tot = 0;
+        const uint32_t per = (nt + SEG_SCAN_GROUPS - 1) / SEG_SCAN_GROUPS;
+        const uint32_t ta = min(nt, uint32_t(g) * per), tz = min(nt, ta + per); // this part's tiles
+        uint32_t part = 0;
 #pragma unroll 8
-        for (uint32_t t = 0; t < nt; ++t) tot += counts[size_t(t0 + t) * RADIX + d];
+        for (uint32_t t = ta; t < tz; ++t) part += counts[size_t(t0 + t) * RADIX + d];
+        s_part[g][d] = part;
+        __syncthreads();
+        // digit d's total over the segment, exclusive scan over digits (group 0)
+        uint32_t tot = 0, before = 0;
+#pragma unroll
+        for (int h = 0; h < SEG_SCAN_GROUPS; ++h) {
+            const uint32_t v = s_part[h][d];
+            tot += v;
+            if (h < g) before += v;
+        }
         uint32_t x = tot;
 #pragma unroll
         for (int off = 1; off < WAVE; off <<= 1) {
             const uint32_t y = __shfl_up(x, off);
             if (lane >= off) x += y;
         }
-        if (lane == WAVE - 1) s_wsum[wave] = x;
+        if (g == 0 && lane == WAVE - 1) s_wsum[wave] = x;
         __syncthreads();
-        uint32_t run = x - tot;
+        uint32_t run = x - tot + before;
         for (int w = 0; w < wave; ++w) run += s_wsum[w];
-        // offsets are positions within the segment (added to begin[s] at use)
-        for (uint32_t t = 0; t < nt; ++t) {
+        // offsets are positions within the segment (added to its first key at use)
+        for (uint32_t t = ta; t < tz; ++t) {
             const size_t at = size_t(t0 + t) * RADIX + d;
             const uint32_t c = counts[at];
             counts[at] = run;
@@ -312,59 +416,53 @@ __global__ __launch_bounds__(RADIX) void k_seg_scan(const int64_t *__restrict__ 
     }
 }
 
-// Scatter of pass `pass` for the large segments' tiles.  Workgroup b walks
-// the tiles of XCD (b % 8)'s contiguous share of [0, num_tiles) so that
-// neighbouring tiles (adjacent digit runs) meet in one L2.
+// Scatter of pass `pass` for the large segments' tiles, one workgroup per
+// tile (spare workgroups of the grid return).  Workgroup b < num_tiles takes
+// tile xcd_contiguous(b, num_tiles): XCD b % 8 walks a contiguous share of the
+// tiles, so neighbouring tiles (adjacent digit runs) meet in one L2.
 __global__ __launch_bounds__(SEG_BLOCK) void k_seg_scatter(uint32_t *__restrict__ keysA, uint32_t *__restrict__ keysB,
-                                                           int pass, const int64_t *__restrict__ begin,
-                                                           const SegPlan *__restrict__ sp,
-                                                           const uint32_t *__restrict__ seg_tile0,
-                                                           const uint32_t *__restrict__ tile_seg,
+                                                           int pass, const SegPlan *__restrict__ sp,
+                                                           const uint4 *__restrict__ tinfo,
                                                            const uint32_t *__restrict__ offsets)
 {
     constexpr int WAVES = SEG_BLOCK / WAVE;
     constexpr int WAVE_TILE = WAVE * SEG_ITEMS;
     __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint64_t s_match[WAVES][RADIX];
     __shared__ uint32_t s_scan[RADIX / WAVE];
     __shared__ uint32_t s_start[RADIX];
     __shared__ uint32_t s_base[RADIX];
     __shared__ uint32_t s_keys[SEG_TILE];
+    const uint32_t ntiles = sp->num_tiles;
+    if (blockIdx.x >= ntiles) return;
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
     uint32_t *__restrict__ kout = (pass & 1) ? keysA : keysB;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
     const int shift = 8 * pass;
-    const uint32_t ntiles = sp->num_tiles;
-    const uint32_t x = blockIdx.x % NUM_XCD;
-    const uint32_t per = ntiles / NUM_XCD, rem = ntiles % NUM_XCD;
-    const uint32_t x_start = x * per + min(x, rem), x_count = per + (x < rem ? 1u : 0u);
-    const uint32_t stride = (gridDim.x - x + NUM_XCD - 1) / NUM_XCD;
-    for (uint32_t i = blockIdx.x / NUM_XCD; i < x_count; i += stride) {
-        const uint32_t t = x_start + i;
-        const uint32_t s = tile_seg[t];
-        const int64_t sb = begin[s];
-        const int64_t tb = sb + int64_t(t - seg_tile0[s]) * SEG_TILE;
-        const int cnt = int(min(int64_t(SEG_TILE), begin[s + 1] - tb));
-        const int wave_valid = cnt > wave * WAVE_TILE ? min(WAVE_TILE, cnt - wave * WAVE_TILE) : 0;
-        uint32_t v[SEG_ITEMS];
+    const uint32_t t = xcd_contiguous(blockIdx.x, ntiles);
+    const uint4 ti = tinfo[t];
+    const uint32_t tb = ti.x, sb = ti.y;
+    const int cnt = int(ti.z);
+    const int wave_valid = cnt > wave * WAVE_TILE ? min(WAVE_TILE, cnt - wave * WAVE_TILE) : 0;
+    uint32_t v[SEG_ITEMS];
 #pragma unroll
-        for (int j = 0; j < SEG_ITEMS; ++j) {
-            const int o = j * WAVE + lane;
-            v[j] = o < wave_valid ? kin[tb + wave * WAVE_TILE + o] : ~0u;
-        }
-        const uint32_t off = offsets[size_t(t) * RADIX + (tid & (RADIX - 1))];
-        tile_rank_stage<uint32_t, SEG_BLOCK, SEG_ITEMS>(v, wave_valid, shift, s_cnt, s_scan, s_start, s_keys);
-        if (tid < RADIX) s_base[tid] = off - s_start[tid];
-        __syncthreads();
-        uint32_t *out = kout + sb;
+    for (int j = 0; j < SEG_ITEMS; ++j) {
+        const int o = j * WAVE + lane;
+        v[j] = o < wave_valid ? kin[size_t(tb) + wave * WAVE_TILE + o] : ~0u;
+    }
+    const uint32_t off = offsets[size_t(t) * RADIX + (tid & (RADIX - 1))];
+    tile_rank_stage_ldsor<uint32_t, SEG_BLOCK, SEG_ITEMS>(v, wave_valid, shift, s_cnt, s_match, s_scan, s_start,
+                                                          s_keys);
+    if (tid < RADIX) s_base[tid] = off - s_start[tid];
+    __syncthreads();
+    uint32_t *out = kout + sb;
 #pragma unroll
-        for (int j = 0; j < SEG_ITEMS; ++j) {
-            const int jj = j * SEG_BLOCK + tid;
-            if (jj < cnt) {
-                const uint32_t k = s_keys[jj];
-                out[s_base[(k >> shift) & 0xFFu] + uint32_t(jj)] = k;
-            }
+    for (int j = 0; j < SEG_ITEMS; ++j) {
+        const int jj = j * SEG_BLOCK + tid;
+        if (jj < cnt) {
+            const uint32_t k = s_keys[jj];
+            out[s_base[(k >> shift) & 0xFFu] + uint32_t(jj)] = k;
         }
-        __syncthreads();
     }
 }
 
