@@ -1,9 +1,13 @@
-"""Short, fixed workloads for profiling runs (rocprofv3) of the top-k and
-serving paths.
+"""Short, fixed workloads for profiling runs (rocprofv3) of the top-k,
+batched-segment and serving paths.
 
   python -m hpcsharp_amd.probe topk [--lg 26] [--k 10000] [--reps 5]
       hpcs_dev_topk_u32 then hpcs_dev_topk_u64 on 2^lg uniform keys,
       3 warm-up and `reps` timed calls each; prints ms per call.
+  python -m hpcsharp_amd.probe segments [--nseg 4096] [--reps 5]
+      hpcs_dev_sort_segments_u32 on nseg uint32 segments with lengths
+      log-uniform in [16, 2^20] (seed 0x5EED0008), 1 warm-up and `reps` timed
+      calls; prints ms per call.
   python -m hpcsharp_amd.probe serving [--requests 20000] [--slots 4]
       the serving workload (hpcsharp_amd.serving.make_trace) through
       hpcs_serve_run; prints the summary as JSON.
@@ -41,6 +45,39 @@ def topk(lg: int, k: int, reps: int) -> None:
         print(f"{fn} n=2^{lg} k={k}: {a.elapsed_time(b) / reps:.4f} ms per call", flush=True)
 
 
+def segments(nseg: int, reps: int) -> None:
+    import numpy as np
+    import torch
+    from hpcsharp_amd import _native as N
+    from hpcsharp_amd import datagen
+    seed = 0x5EED0008
+    lens = datagen.segment_lengths(nseg, seed, 16, 1 << 20)
+    begin = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(begin[-1])
+    keys = datagen.keys_u32(total, seed + 1, "uniform")
+    ctx, lib = N.context(0), N.lib()
+    src = torch.from_numpy(keys.view(np.int32)).cuda()
+    d_begin = torch.from_numpy(begin).cuda()
+    work = src.clone()
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    call = lambda: N.check(lib.hpcs_dev_sort_segments_u32(ctx, ctypes.c_void_p(work.data_ptr()), total,
+                                                          ctypes.c_void_p(d_begin.data_ptr()), nseg, sp))
+    call()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        work.copy_(src)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        call()
+        b.record(st)
+        b.synchronize()
+        ms.append(a.elapsed_time(b))
+    print(f"hpcs_dev_sort_segments_u32 {nseg} segments, {total} keys: {sorted(ms)[len(ms) // 2]:.4f} ms per call "
+          f"(median of {reps})", flush=True)
+
+
 def serving(requests: int, slots: int) -> None:
     from hpcsharp_amd import serving as S
     t = S.make_trace(nreq=requests)
@@ -58,9 +95,14 @@ def main(argv=None) -> int:
     b = sub.add_parser("serving")
     b.add_argument("--requests", type=int, default=20000)
     b.add_argument("--slots", type=int, default=4)
+    c = sub.add_parser("segments")
+    c.add_argument("--nseg", type=int, default=4096)
+    c.add_argument("--reps", type=int, default=5)
     args = ap.parse_args(argv)
     if args.what == "topk":
         topk(args.lg, args.k, args.reps)
+    elif args.what == "segments":
+        segments(args.nseg, args.reps)
     else:
         serving(args.requests, args.slots)
     return 0
