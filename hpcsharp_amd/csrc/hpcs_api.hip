@@ -699,8 +699,8 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
 // Radix top-k: the m largest keys of d_in[0, n) into d_out[0, m), in
 // descending order (desc) or ascending; with d_rest, the other n - m keys go
 // there in no particular order.  Scratch: [TopkState | cand 0 | cand 1].
-// One-workgroup sort of the selection: counting ranks for small sets, the
-// bitonic network for larger ones.
+// One-workgroup sort of the selection: counting ranks for small sets, an LSD
+// radix sort in LDS for larger ones.
 template <typename K, int FLOATK>
 void launch_topk_sort(const K *src, uint32_t len, K *dst, uint32_t m, int mode, const TopkState *ts, uint32_t n,
                       int desc, hipStream_t st)
@@ -709,9 +709,12 @@ void launch_topk_sort(const K *src, uint32_t len, K *dst, uint32_t m, int mode, 
         uint32_t t = 64;
         while (t < len) t <<= 1;
         hipLaunchKernelGGL((k_topk_rank_sort<K, FLOATK>), dim3(1), dim3(t), 0, st, src, len, dst, m, mode, ts, n, desc);
+    } else if (len <= uint32_t(256 * TOPK_SORT_ITEMS)) {
+        hipLaunchKernelGGL((k_topk_radix_sort<K, FLOATK, 256>), dim3(1), dim3(256), 0, st, src, len, dst, m, mode, ts,
+                           n, desc);
     } else {
-        hipLaunchKernelGGL((k_topk_block_sort<K, FLOATK>), dim3(1), dim3(topk_sort_threads(len)), 0, st, src, len, dst,
-                           m, mode, ts, n, desc);
+        hipLaunchKernelGGL((k_topk_radix_sort<K, FLOATK, TOPK_SORT_BLOCK>), dim3(1), dim3(TOPK_SORT_BLOCK), 0, st, src,
+                           len, dst, m, mode, ts, n, desc);
     }
 }
 

@@ -18,9 +18,9 @@
 //                   k_rem copies of T close the output, the surplus copies of
 //                   T go to `rest`
 // Output positions come from atomics, so the selected keys are then sorted
-// (one workgroup: by counting ranks up to TOPK_RANK_SORT keys, by a bitonic
-// network up to TOPK_BLOCK_SORT keys; beyond, the full radix sort) and
-// written back as values, descending or ascending.
+// (one workgroup: by counting ranks up to TOPK_RANK_SORT keys, by an LSD
+// radix sort in LDS up to TOPK_BLOCK_SORT keys; beyond, the full radix sort)
+// and written back as values, descending or ascending.
 #pragma once
 #include "hpcs_kernels.hpp"
 #include "hpcs_segments.hpp"
@@ -37,15 +37,6 @@ constexpr int TOPK_SORT_BLOCK = 1024;
 constexpr int TOPK_SORT_ITEMS = 16;
 constexpr int TOPK_BLOCK_SORT = TOPK_SORT_BLOCK * TOPK_SORT_ITEMS; // largest selection sorted in one workgroup
 constexpr int TOPK_RANK_SORT = 256; // up to this many keys, ranked by counting instead
-
-// Threads of the one-workgroup sort for len keys: a power of two, at least
-// one wave.
-inline int topk_sort_threads(uint32_t len)
-{
-    int t = 64;
-    while (uint32_t(t) * TOPK_SORT_ITEMS < len) t <<= 1;
-    return t;
-}
 
 // Keys per thread and tile: 64 bytes per thread (four 16-byte vectors).
 template <typename K> struct TopkTile {
@@ -385,40 +376,30 @@ template <typename K, int FLOATK> __device__ inline K topk_value(K s, int desc)
     return value_of<K, FLOATK>(desc ? K(~s) : s);
 }
 
-// Steps with partner distance J < TOPK_SORT_ITEMS stay inside a thread.
-template <typename K, int J>
-__device__ inline void bitonic_in_thread(K (&v)[TOPK_SORT_ITEMS], uint32_t e0, uint32_t k)
+// One workgroup: LSD radix sort in LDS of up to BLOCK * TOPK_SORT_ITEMS sort
+// keys (the passes over the digit positions that vary; tile_rank_stage per
+// pass), then the first m written to dst as values.  Sources: mode 0 reads
+// values src[0, len) (the whole input of a small top-k), mode 1 the sort keys
+// src[0, m - need) appended by the compaction plus need copies of the
+// threshold key (from the state).  Equal sort keys are equal values, so the
+// order among them is immaterial.  src may be dst: every key is loaded
+// before the first barrier and written after the last.
+template <typename K, int FLOATK, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_topk_radix_sort(const K *__restrict__ src, uint32_t len,
+                                                          K *__restrict__ dst, uint32_t m, int mode,
+                                                          const TopkState *__restrict__ st, uint32_t n, int desc)
 {
-#pragma unroll
-    for (int r = 0; r < TOPK_SORT_ITEMS; ++r) {
-        if (r & J) continue;
-        const K a = v[r], c = v[r | J];
-        const bool up = ((e0 + uint32_t(r)) & k) == 0u;
-        const bool swap = up ? (c < a) : (a < c);
-        v[r] = swap ? c : a;
-        v[r | J] = swap ? a : c;
-    }
-}
-
-// One workgroup: bitonic sort of up to TOPK_BLOCK_SORT sort keys, then the
-// first m written to dst as values.  Thread t holds elements [16t, 16t + 16)
-// in registers; partners at distance below 16 are in the same thread, below
-// 1024 in the same wave (exchanged with shuffles), farther ones go through
-// LDS.  Keys come from
-//   mode 0: src[0, len) are values (the whole input of a small top-k);
-//   mode 1: src[0, m - need) are sort keys appended by the compaction, and
-//           the need copies of the threshold key come from the state.
-// Equal sort keys are equal values, so the order among them is immaterial.
-template <typename K, int FLOATK>
-__global__ __launch_bounds__(TOPK_SORT_BLOCK) void k_topk_block_sort(const K *__restrict__ src, uint32_t len,
-                                                                     K *__restrict__ dst, uint32_t m, int mode,
-                                                                     const TopkState *__restrict__ st, uint32_t n,
-                                                                     int desc)
-{
-    constexpr int IT = TOPK_SORT_ITEMS;
-    constexpr uint32_t WAVE_SPAN = uint32_t(IT) * WAVE; // elements held by one wave
-    __shared__ K s[TOPK_BLOCK_SORT];
-    const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1);
+    constexpr int ITEMS = TOPK_SORT_ITEMS;
+    constexpr int WAVES = BLOCK / WAVE;
+    constexpr int WAVE_TILE = WAVE * ITEMS;
+    constexpr int TILE = BLOCK * ITEMS;
+    constexpr int P = int(sizeof(K));
+    __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint32_t s_scan[RADIX / WAVE];
+    __shared__ uint32_t s_start[RADIX];
+    __shared__ K s_keys[TILE];
+    __shared__ K s_or[WAVES], s_nor[WAVES];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
     uint32_t have = len;
     K fill = K(0);
     if (mode == 1) {
@@ -427,71 +408,69 @@ __global__ __launch_bounds__(TOPK_SORT_BLOCK) void k_topk_block_sort(const K *__
         have = m - t.need;
         len = m;
     }
-    const uint32_t nt = blockDim.x;
-    const uint32_t size = nt * uint32_t(IT); // a power of two >= len
-    const uint32_t e0 = tid * uint32_t(IT);
-    K v[IT];
-    // Element 16t + r of the network starts as input key r * nt + t (any
-    // bijection will do for unsorted input; this one makes the loads
-    // coalesced).  The loads are unconditional (the index clamped into the
-    // buffer, which holds at least one key), so all sixteen are in flight.
-    const uint32_t last = have > 0u ? have - 1u : 0u;
+    const int cnt = int(min(len, uint32_t(TILE)));
+    const int wave_valid = cnt > wave * WAVE_TILE ? min(WAVE_TILE, cnt - wave * WAVE_TILE) : 0;
+    K v[ITEMS];
+    K vor = 0, vnor = 0;
 #pragma unroll
-    for (int r = 0; r < IT; ++r) v[r] = src[min(uint32_t(r) * nt + tid, last)];
-#pragma unroll
-    for (int r = 0; r < IT; ++r) {
-        const uint32_t q = uint32_t(r) * nt + tid;
-        const K x = mode == 0 ? sort_key(order_key<K, FLOATK>(v[r]), desc) : v[r];
-        v[r] = q < have ? x : (q < len ? fill : K(~K(0))); // padding sorts last
-    }
-    for (uint32_t k = 2; k <= size; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            if (j >= uint32_t(IT)) {
-                // all sixteen partner values are fetched before any is used,
-                // so the exchanges overlap instead of waiting one by one
-                K p[IT];
-                if (j >= WAVE_SPAN) {
-                    // element 16t + r sits at s[r * nt + t]: a wave's lanes
-                    // touch consecutive words (no bank conflicts), and the
-                    // partner 16t' + r, t' = t ^ (j / 16), is in the same row
-                    __syncthreads();
-#pragma unroll
-                    for (int r = 0; r < IT; ++r) s[uint32_t(r) * nt + tid] = v[r];
-                    __syncthreads();
-                    const uint32_t pt = tid ^ (j / uint32_t(IT));
-#pragma unroll
-                    for (int r = 0; r < IT; ++r) p[r] = s[uint32_t(r) * nt + pt];
-                } else {
-                    const int pl = int(lane ^ (j / uint32_t(IT)));
-#pragma unroll
-                    for (int r = 0; r < IT; ++r) p[r] = __shfl(v[r], pl);
-                }
-                // the partner's direction bits are those of e0 for j >= 16
-                const bool keep_min = ((e0 & j) == 0u) == ((e0 & k) == 0u);
-#pragma unroll
-                for (int r = 0; r < IT; ++r) v[r] = keep_min ? (p[r] < v[r] ? p[r] : v[r]) : (p[r] < v[r] ? v[r] : p[r]);
-            } else if (j == 8) {
-                bitonic_in_thread<K, 8>(v, e0, k);
-            } else if (j == 4) {
-                bitonic_in_thread<K, 4>(v, e0, k);
-            } else if (j == 2) {
-                bitonic_in_thread<K, 2>(v, e0, k);
-            } else {
-                bitonic_in_thread<K, 1>(v, e0, k);
-            }
+    for (int i = 0; i < ITEMS; ++i) {
+        const int o = i * WAVE + lane;
+        const uint32_t q = uint32_t(wave * WAVE_TILE + o);
+        v[i] = ~K(0);
+        if (o < wave_valid) {
+            K x = fill;
+            if (q < have) x = mode == 0 ? sort_key(order_key<K, FLOATK>(src[q]), desc) : src[q];
+            v[i] = x;
+            vor |= x;
+            vnor |= ~x;
         }
     }
 #pragma unroll
-    for (int r = 0; r < IT; ++r) {
-        const uint32_t e = e0 + uint32_t(r);
-        if (e < m) dst[e] = topk_value<K, FLOATK>(v[r], desc);
+    for (int off = 1; off < WAVE; off <<= 1) {
+        vor |= __shfl_xor(vor, off);
+        vnor |= __shfl_xor(vnor, off);
     }
+    if (lane == 0) {
+        s_or[wave] = vor;
+        s_nor[wave] = vnor;
+    }
+    __syncthreads();
+    K a = 0, c = 0;
+    for (int w = 0; w < WAVES; ++w) {
+        a |= s_or[w];
+        c |= s_nor[w];
+    }
+    const K varying = a & c;
+    bool staged = false;
+    for (int pass = 0; pass < P; ++pass) {
+        if (((varying >> (8 * pass)) & K(0xFF)) == 0) continue;
+        if (staged) {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) {
+                const int o = i * WAVE + lane;
+                if (o < wave_valid) v[i] = s_keys[wave * WAVE_TILE + o];
+            }
+            __syncthreads();
+        }
+        tile_rank_stage<K, BLOCK, ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_scan, s_start, s_keys);
+        staged = true;
+    }
+    if (!staged) { // every key equal: any order is sorted
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int o = i * WAVE + lane;
+            if (o < wave_valid) s_keys[wave * WAVE_TILE + o] = v[i];
+        }
+        __syncthreads();
+    }
+    const uint32_t out = min(m, uint32_t(cnt));
+    for (uint32_t j = tid; j < out; j += BLOCK) dst[j] = topk_value<K, FLOATK>(s_keys[j], desc);
 }
 
 // Up to TOPK_RANK_SORT keys: one thread per key counts the keys that order
 // before it ((key, index) pairs, so ranks are distinct) against the whole
 // set in LDS and writes itself to its rank.  Same sources and modes as
-// k_topk_block_sort; blockDim.x is a power of two >= len.
+// k_topk_radix_sort; blockDim.x is a power of two >= len.
 template <typename K, int FLOATK>
 __global__ __launch_bounds__(TOPK_RANK_SORT) void k_topk_rank_sort(const K *__restrict__ src, uint32_t len,
                                                                    K *__restrict__ dst, uint32_t m, int mode,
