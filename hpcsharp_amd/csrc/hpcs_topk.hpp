@@ -289,11 +289,19 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_topk_compact(const K *__restrict
         for (int j = 0; j < ITEMS; ++j) u[j] = nx[j];
         if (base + step < n)
             load_tile_keys<K, TOPK_BLOCK, ITEMS>(src + base + step, min(uint32_t(TILE), n - base - step), vec, nx);
+        // most keys lie below the bin: a wave with none at or above it
+        // (and no `rest` output to fill) has nothing to append in this tile
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            if (level == 0) u[j] = order_key<K, FLOATK>(u[j]);
+            any |= (full || j * TOPK_BLOCK + tid < int(cnt)) && LV::digit(u[j], level) >= b;
+        }
+        if (!rest && __ballot(any) == 0ull) continue;
         uint32_t n_rest = 0;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             const bool valid = full || j * TOPK_BLOCK + tid < int(cnt);
-            if (level == 0) u[j] = order_key<K, FLOATK>(u[j]);
             const uint32_t d = LV::digit(u[j], level);
             const bool to_out = valid && (d > b || (d == b && take_bin));
             const bool to_cand = valid && keep_cand && d == b;
