@@ -836,6 +836,15 @@ void host_topk(hpcs_ctx *c, const K *keys, int64_t n, int64_t k, K *out)
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
+// k_gather_records access width: 2 for 16-byte vectors (16-byte aligned
+// records whose size is a multiple of 16), 1 for words, 0 for bytes.
+int gather_mode(const void *recs, int32_t rec_size)
+{
+    const uintptr_t a = reinterpret_cast<uintptr_t>(recs);
+    if ((rec_size & 15) == 0 && (a & 15) == 0) return 2;
+    return ((rec_size & 3) == 0 && (a & 3) == 0) ? 1 : 0;
+}
+
 template <typename K>
 void run_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset, int32_t key_kind,
                  hipStream_t st)
@@ -857,7 +866,7 @@ void run_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t
                        key_kind, keys, idx);
     hip_check(hipGetLastError(), "k_extract_keys launch");
     run_sort<K, true, false>(c, keys, idx, n, st);
-    const int words = ((rec_size & 3) == 0 && (reinterpret_cast<uintptr_t>(d_recs) & 3) == 0) ? 1 : 0;
+    const int words = gather_mode(d_recs, rec_size);
     hipLaunchKernelGGL(k_gather_records, dim3(grid_for(uint64_t(n) * uint64_t(rec_size) / 4 + 1, 256, c->num_cus * 8)),
                        dim3(256), 0, st, static_cast<const unsigned char *>(d_recs), out, idx, uint64_t(n), rec_size,
                        words);
@@ -883,7 +892,7 @@ void run_keys_items(hpcs_ctx *c, K *d_keys, void *d_items, int64_t n, int32_t it
                        uint64_t(n));
     hip_check(hipGetLastError(), "k_iota_u32 launch");
     run_sort<K, true, KIND_UINT>(c, d_keys, idx, n, st);
-    const int words = ((item_size & 3) == 0 && (reinterpret_cast<uintptr_t>(d_items) & 3) == 0) ? 1 : 0;
+    const int words = gather_mode(d_items, item_size);
     hipLaunchKernelGGL(k_gather_records,
                        dim3(grid_for(uint64_t(n) * uint64_t(item_size) / 4 + 1, 256, c->num_cus * 8)), dim3(256), 0,
                        st, static_cast<const unsigned char *>(d_items), out, idx, uint64_t(n), item_size, words);

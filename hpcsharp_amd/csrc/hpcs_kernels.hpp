@@ -1217,7 +1217,17 @@ __global__ void k_gather_records(const unsigned char *__restrict__ in, unsigned 
                                  const uint32_t *__restrict__ idx, uint64_t n, int rec_size, int words4)
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    if (words4) {
+    if (words4 == 2) {
+        // 16-byte aligned records of 16 * v bytes: one 16-byte vector per thread
+        const uint32_t v = uint32_t(rec_size) >> 4;
+        const uint64_t total = n * v;
+        const u32x4 *in4 = reinterpret_cast<const u32x4 *>(in);
+        u32x4 *out4 = reinterpret_cast<u32x4 *>(out);
+        for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
+            const uint64_t r = v == 1u ? t : t / v;
+            out4[t] = in4[uint64_t(idx[r]) * v + (t - r * v)];
+        }
+    } else if (words4) {
         const int words = rec_size >> 2;
         const uint64_t total = n * uint64_t(words);
         for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride) {
