@@ -7,8 +7,8 @@
  * NULL = HIP's null stream).
  *
  * Each entry point names the reference interface it replaces (paths relative
- * to the HPCsharp source tree).  The managed binding a .NET caller would use
- * ([LibraryImport] over pinned spans) is given in INTEGRATION.md.
+ * to the HPCsharp source tree).  A .NET caller binds these entry points with
+ * [LibraryImport] over pinned spans.
  *
  * Conventions shared by every entry point:
  *  - Return HPCS_OK (0) or a negative HPCS_E_* status; hpcs_last_error()
