@@ -171,6 +171,66 @@ def topk_figures(torch, N, datagen, reps: int):
     return {"n": n, "k": k, "by_type": out}
 
 
+def event_ms(torch, prep, fn, reps: int) -> float:
+    """Median milliseconds of fn() between HIP events on the current stream,
+    prep() run before each call outside the timed interval."""
+    st = torch.cuda.current_stream()
+    prep()
+    fn()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        prep()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        fn()
+        b.record(st)
+        b.synchronize()
+        ms.append(a.elapsed_time(b))
+    return float(np.median(ms))
+
+
+def histogram_figures(torch, N, datagen, keys_host: np.ndarray, reps: int):
+    """HistogramByteComponents (device-resident): on the headline array
+    (GB/s against the one-read roofline, counts checked with torch.bincount)
+    and on the 2^20-key array of the reference's CPU-runnable workload (seed
+    0x5EED0001), together with that array's sort latency."""
+    lib, ctx = N.lib(), N.context(0)
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    P = ctypes.c_void_p
+    out = {}
+    for tag, h in (("n_2p28", keys_host), ("n_2p20", datagen.keys_u32(1 << 20, 0x5EED0001))):
+        n = h.size
+        x = torch.from_numpy(h.view(np.int32)).cuda()
+        counts = torch.empty(4 * 256, dtype=torch.int32, device="cuda")
+        call = lambda: N.check(lib.hpcs_dev_histogram_byte_components_u32(ctx, P(x.data_ptr()), n,
+                                                                          P(counts.data_ptr()), sp))
+        ms = event_ms(torch, lambda: None, call, reps)
+        xl = x.to(torch.int64) & 0xFFFFFFFF
+        want = torch.stack([torch.bincount((xl >> (8 * d)) & 255, minlength=256) for d in range(4)]).reshape(-1)
+        if not torch.equal(counts.to(torch.int64), want):
+            print("bench: histogram differs from torch.bincount", file=sys.stderr)
+            sys.exit(3)
+        r = {"n": n, "histogram_us": round(ms * 1e3, 2), "histogram_gkeys_per_s": round(n / ms / 1e6, 2),
+             "histogram_GBs": round(4 * n / ms / 1e6, 1)}
+        if tag == "n_2p28":
+            r["histogram_roofline_frac"] = round(4 * n / ms / 1e6 / HBM_PEAK_GBS, 4)
+        else:
+            work = x.clone()
+            sort = lambda: N.check(lib.hpcs_dev_sort_radix_u32(ctx, P(work.data_ptr()), n, sp))
+            sms = event_ms(torch, lambda: work.copy_(x), sort, reps)
+            ref, _ = torch.sort(xl)
+            if not torch.equal(work.to(torch.int64) & 0xFFFFFFFF, ref):
+                print("bench: 2^20 sort differs from torch.sort", file=sys.stderr)
+                sys.exit(3)
+            r["sort_us"] = round(sms * 1e3, 2)
+            r["sort_gkeys_per_s"] = round(n / sms / 1e6, 2)
+            del work
+        out[tag] = r
+        del x, counts, xl, want
+    return out
+
+
 def cfg3_figures(torch, N, datagen, reps: int):
     """Floating and record keys at 2^27: double[] with +-0.0, subnormals and
     +-1e300 (plain sort), and {u64 key, u32 payload} records through the
@@ -274,6 +334,7 @@ def main():
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-cfg3", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-hist", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=20000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -346,6 +407,9 @@ def main():
             print("bench: batched and plain outputs differ", file=sys.stderr)
             sys.exit(3)
 
+    if not args.no_hist:
+        extra["histogram"] = histogram_figures(torch, N, datagen, keys, 10)
+        log(f"[bench] histograms and the 2^20 workload: {extra['histogram']}")
     if not args.no_host:
         extra["host_array"] = host_array_figures(N, keys, 2)
         log(f"[bench] host-array call: {extra['host_array']}")

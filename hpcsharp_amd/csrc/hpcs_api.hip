@@ -262,7 +262,7 @@ void launch_histogram(hpcs_ctx *c, const K *d_keys, uint64_t n, uint32_t *d_hist
     const uint32_t head = uint32_t(((16 - (addr & 15)) & 15) / sizeof(K));
     const uint64_t chunks = (n > head ? n - head : 0) / (WAVE * VEC);
     const int waves_per_block = HIST_BLOCK / WAVE;
-    const int grid = grid_for(chunks, waves_per_block, c->num_cus * 4);
+    const int grid = grid_for(chunks, waves_per_block * HIST_UNROLL, c->num_cus * 8);
     timed(c, st, "histogram", [&] {
         hipLaunchKernelGGL((k_histogram<K, FLOATK, HIST_BLOCK>), dim3(grid), dim3(HIST_BLOCK), 0, st, d_keys, n,
                            head, d_hist);

@@ -180,6 +180,7 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t words, uint32_t va
 // ---------------------------------------------------------------------------
 // `head` (< VEC) keys precede the first 16-byte aligned key; they are counted
 // lane by lane together with the tail.
+constexpr int HIST_UNROLL = 4; // 1 KiB chunks per wave in flight
 template <typename K, int FLOATK, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys, uint64_t n, uint32_t head,
                                                      uint32_t *__restrict__ hist)
@@ -201,20 +202,29 @@ __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys,
     const uint64_t nwaves = uint64_t(gridDim.x) * WAVES;
     const Vec *vkeys = reinterpret_cast<const Vec *>(keys + head);
 
-    for (uint64_t c = wave_id; c < nchunks; c += nwaves) {
-        Vec v = __builtin_nontemporal_load(&vkeys[c * WAVE + lane]);
+    // HIST_UNROLL chunks per wave and round, all loads issued before any is
+    // counted (one 1 KiB load per wave in flight left HBM mostly idle)
+    for (uint64_t c0 = wave_id * HIST_UNROLL; c0 < nchunks; c0 += nwaves * HIST_UNROLL) {
+        Vec v[HIST_UNROLL];
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) {
-            const K raw = v[j];
-            K key = FLOATK ? key_enc<K, FLOATK>(raw) : raw;
+        for (int u = 0; u < HIST_UNROLL; ++u)
+            if (c0 + u < nchunks) v[u] = __builtin_nontemporal_load(&vkeys[(c0 + u) * WAVE + lane]);
 #pragma unroll
-            for (int d = 0; d < P; ++d) {
-                const uint32_t dg = KeyOps<K>::digit(key, 8 * d);
-                const uint32_t d0 = __builtin_amdgcn_readfirstlane(dg);
-                if (__all(dg == d0)) {
-                    if (lane == 0) atomicAdd(&sh[d * RADIX + d0], uint32_t(WAVE));
-                } else {
-                    atomicAdd(&sh[d * RADIX + dg], 1u);
+        for (int u = 0; u < HIST_UNROLL; ++u) {
+            if (c0 + u >= nchunks) break; // wave-uniform
+#pragma unroll
+            for (int j = 0; j < VEC; ++j) {
+                const K raw = v[u][j];
+                K key = FLOATK ? key_enc<K, FLOATK>(raw) : raw;
+#pragma unroll
+                for (int d = 0; d < P; ++d) {
+                    const uint32_t dg = KeyOps<K>::digit(key, 8 * d);
+                    const uint32_t d0 = __builtin_amdgcn_readfirstlane(dg);
+                    if (__all(dg == d0)) {
+                        if (lane == 0) atomicAdd(&sh[d * RADIX + d0], uint32_t(WAVE));
+                    } else {
+                        atomicAdd(&sh[d * RADIX + dg], 1u);
+                    }
                 }
             }
         }
