@@ -42,9 +42,9 @@ def _raise(err: N.HpcsError):
     raise InvalidOperationException(str(err)) from None
 
 
-def _call(fn, *args):
+def _call(fn, *args, ctx=None):
     try:
-        N.check(fn(N.context(), *args))
+        N.check(fn(N.context() if ctx is None else ctx, *args))
     except N.HpcsError as e:
         _raise(e)
 
@@ -74,18 +74,32 @@ _SORT = {
 }
 
 
-def _sort_whole(a: np.ndarray) -> None:
-    _call(getattr(N.lib(), _SORT[a.dtype]), _ptr(a), a.size)
+_DIGIT_CTX: dict[int, ctypes.c_void_p] = {}
 
 
-def _sort_range(a: np.ndarray, startIndex: int, length: int) -> None:
+def _digit_context(bits: int):
+    """A context whose sorts use `bits`-bit digits where the library has that
+    form (11: the single-pass form of keys without values), created once."""
+    ctx = _DIGIT_CTX.get(bits)
+    if ctx is None:
+        ctx = N.new_context(0)
+        N.set_option(N.HPCS_OPT_DIGIT_BITS, bits, ctx)
+        _DIGIT_CTX[bits] = ctx
+    return ctx
+
+
+def _sort_whole(a: np.ndarray, ctx=None) -> None:
+    _call(getattr(N.lib(), _SORT[a.dtype]), _ptr(a), a.size, ctx=ctx)
+
+
+def _sort_range(a: np.ndarray, startIndex: int, length: int, ctx=None) -> None:
     if startIndex < 0 or length < 0 or startIndex + length > a.size:
         raise ArgumentOutOfRangeException("startIndex/length outside the array")
     if a.dtype == np.uint32:
-        _call(N.lib().hpcs_sort_radix_range_u32, _ptr(a), startIndex, length)
+        _call(N.lib().hpcs_sort_radix_range_u32, _ptr(a), startIndex, length, ctx=ctx)
     else:
         sub = a[startIndex:startIndex + length]
-        _call(getattr(N.lib(), _SORT[a.dtype]), _ptr(sub), sub.size)
+        _call(getattr(N.lib(), _SORT[a.dtype]), _ptr(sub), sub.size, ctx=ctx)
 
 
 class Algorithm:
@@ -102,6 +116,38 @@ class Algorithm:
             _sort_whole(inOutArray)
         else:
             _sort_range(inOutArray, startIndex, inOutArray.size - startIndex if length is None else length)
+
+    @staticmethod
+    def SortRadixLsd11bit(inOutArray: np.ndarray, startIndex: int | None = None,
+                          length: int | None = None) -> np.ndarray:
+        """SortRadixLsd11bit(this uint[]) RadixSortLsd.cs:1540-1546 and
+        (startIndex, length) :1484-1532: sorted with 11-bit digits (three
+        passes) where the library's single-pass form serves the size, with
+        8-bit digits otherwise (a stable LSD sort's output does not depend on
+        the digit width).  Returns the array, sorted; the reference's
+        whole-array overload returns its pass-2 state (odd pass count,
+        :1540-1546), a quirk not reproduced."""
+        _require(inOutArray, "inOutArray", (np.uint32,))
+        ctx = _digit_context(11)
+        if startIndex is None:
+            _sort_whole(inOutArray, ctx)
+        else:
+            _sort_range(inOutArray, startIndex, inOutArray.size - startIndex if length is None else length, ctx)
+        return inOutArray
+
+    @staticmethod
+    def SortRadixLsdNbit(inOutArray: np.ndarray, startIndex: int, length: int, bitsPerDigit: int) -> None:
+        """SortRadixLsdNbit(this uint[] / ulong[], startIndex, length,
+        bitsPerDigit) RadixSortLsd.cs:1616-1661, 1839-1882:
+        ArgumentOutOfRangeException when bitsPerDigit exceeds the key width
+        (:1621-1622, :1844-1845) or is below 1.  The range is sorted in place
+        (the Experimental overload's sub-range semantics); 11-bit digits are
+        used as in SortRadixLsd11bit, any other width sorts with the library's
+        8-bit passes (same output)."""
+        _require(inOutArray, "inOutArray", (np.uint32, np.uint64))
+        if bitsPerDigit < 1 or bitsPerDigit > 8 * inOutArray.itemsize:
+            raise ArgumentOutOfRangeException("bitsPerDigit")
+        _sort_range(inOutArray, startIndex, length, _digit_context(11) if bitsPerDigit == 11 else None)
 
     @staticmethod
     def SortRadixLsdFunc(inOutArray: np.ndarray, startIndex: int | None = None, length: int | None = None) -> np.ndarray:

@@ -88,3 +88,18 @@ def test_committed_tuning_table_parses():
     rows = ctypes.c_int32()
     assert N.lib().hpcs_tuning_check(N.TUNE_PATH.encode(), ctypes.byref(rows)) == 0
     assert rows.value > 0
+
+
+def test_nbit_sort_argument_checks():
+    """SortRadixLsdNbit rejects digit widths beyond the key (RadixSortLsd.cs:
+    1621-1622, 1844-1845) or below 1, and a null array, before any device
+    work."""
+    import numpy as np
+    from hpcsharp_amd.algorithm import Algorithm, ArgumentNullException, ArgumentOutOfRangeException
+    with pytest.raises(ArgumentNullException):
+        Algorithm.SortRadixLsdNbit(None, 0, 0, 8)
+    for arr, bad in ((np.zeros(4, np.uint32), 33), (np.zeros(4, np.uint32), 0), (np.zeros(4, np.uint64), 65)):
+        with pytest.raises(ArgumentOutOfRangeException):
+            Algorithm.SortRadixLsdNbit(arr, 0, 4, bad)
+    with pytest.raises(ArgumentNullException):
+        Algorithm.SortRadixLsd11bit(None)

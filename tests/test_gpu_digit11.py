@@ -140,3 +140,35 @@ def test_tuning_row_selects_eleven_bits(oracle, tmp_path):
         assert np.array_equal(a, ra)
     finally:
         N.destroy_context(ctx)
+
+
+def test_python_mirror_11bit_and_nbit_sorts(oracle):
+    """Algorithm.SortRadixLsd11bit / SortRadixLsdNbit (RadixSortLsd.cs:1484-1546,
+    1616-1661, 1839-1882) against the oracle's SortRadixLsdNbit restatement:
+    whole arrays, sub-ranges (only the range changes), several widths."""
+    from hpcsharp_amd.algorithm import Algorithm
+    a = datagen.keys_u32(200003, 3100)
+    ra = a.copy()
+    oracle.sort_radix_lsd_nbit(ra, 11)
+    assert Algorithm.SortRadixLsd11bit(a) is a
+    assert np.array_equal(a, ra)
+    b = datagen.keys_u32(100000, 3101)
+    rb = b.copy()
+    rb[1000:61000] = np.sort(rb[1000:61000])
+    Algorithm.SortRadixLsd11bit(b, 1000, 60000)
+    assert np.array_equal(b, rb)
+    for bits in (1, 5, 8, 11, 16, 32):
+        c = datagen.keys_u32(50001, 3200 + bits)
+        rc = c.copy()
+        rc[7:50001 - 9] = np.sort(rc[7:50001 - 9])
+        Algorithm.SortRadixLsdNbit(c, 7, 50001 - 16, bits)
+        assert np.array_equal(c, rc), bits
+    for bits in (11, 13, 64):
+        d = datagen.keys_u64(40009, 3300 + bits)
+        rd = d.copy()
+        if bits <= 16:
+            oracle.sort_radix_lsd_nbit(rd, bits)
+        else:
+            rd.sort()
+        Algorithm.SortRadixLsdNbit(d, 0, d.size, bits)
+        assert np.array_equal(d, rd), bits
