@@ -572,6 +572,36 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     // beside u64 keys the sort gains 5%.
     const bool want_dig = c->digit_bytes == 1 || (c->digit_bytes == 2 && sizeof(K) == 8 && HAS_VALS);
     uint8_t *dig = (want_dig && !persistent) ? reinterpret_cast<uint8_t *>(base + L.off_dig) : nullptr;
+    if (algo == PASS_RTS && tiles <= SCAN_DIGIT_MAX_TILES) {
+        // few tiles: launches, not bytes, set the time.  One histogram of
+        // every digit gives the plan (digit starts, trivial passes); per
+        // pass the tile counts, one scan kernel and the scatter.
+        launch_fill(c, base, L.off_lb0, st);
+        launch_histogram<K, FLOATK>(c, d_keys, uint64_t(n), hist, st);
+        timed(c, st, "plan", [&] {
+            hipLaunchKernelGGL((k_plan<P>), dim3(1), dim3(RADIX), 0, st, hist, plan, uint64_t(n), FLOATK ? 1 : 0);
+        });
+        uint32_t *counts = lb[0];
+        for (int p = 0; p < P; ++p) {
+            timed(c, st, "tile_counts", [&] {
+                launch_tile_counts<K, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan, counts,
+                                              nullptr);
+            });
+            timed(c, st, "scan", [&] {
+                hipLaunchKernelGGL(k_scan_digit, dim3(RADIX), dim3(1024), 0, st, counts, tiles, p, plan);
+            });
+            timed(c, st, "scatter", [&] {
+                launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
+                                                     d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
+                                                     nullptr, nullptr);
+            });
+        }
+        timed(c, st, "copyback", [&] {
+            hipLaunchKernelGGL((k_copyback<K>), dim3(grid_for(uint64_t(n), 256, c->num_cus * 8)), dim3(256), 0, st,
+                               d_keys, keysB, HAS_VALS ? d_vals : nullptr, valsB, uint64_t(n), plan);
+        });
+        return;
+    }
     if (algo == PASS_RTS) {
         // the histogram area holds the OR slots; counts in look-back region 0
         launch_fill(c, base, L.off_lb0, st);

@@ -1032,6 +1032,42 @@ __global__ __launch_bounds__(RADIX) void k_range_offsets(uint32_t *__restrict__ 
     }
 }
 
+// The three scan kernels in one, for sorts of few tiles (the plan's digit
+// starts come from a full histogram, k_histogram + k_plan): workgroup d
+// scans digit d's column of the tile counts, the tiles split over its
+// threads, and writes the tiles' global offsets over the counts.
+// Measured on MI355X: faster than the five-launch pass up to 512 tiles (u32
+// 2^20: 0.109 -> 0.092 ms, u64 2^20: 0.240 -> 0.199 ms), slower from 1024
+// (the all-digit histogram and the column loads outgrow the launches saved).
+constexpr uint32_t SCAN_DIGIT_MAX_TILES = 512;
+__global__ __launch_bounds__(1024) void k_scan_digit(uint32_t *__restrict__ counts, uint32_t tiles, int pass,
+                                                     const DevPlan *__restrict__ plan)
+{
+    __shared__ uint32_t s_w[1024 / WAVE];
+    if (plan->flags[pass] & PF_TRIVIAL) return;
+    const uint32_t d = blockIdx.x, tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    const uint32_t per = (tiles + 1023) / 1024;
+    const uint32_t t0 = min(tiles, tid * per), t1 = min(tiles, t0 + per);
+    uint32_t local = 0;
+    for (uint32_t t = t0; t < t1; ++t) local += counts[size_t(t) * RADIX + d];
+    uint32_t x = local;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= uint32_t(off)) x += y;
+    }
+    if (lane == WAVE - 1) s_w[wave] = x;
+    __syncthreads();
+    uint32_t run = plan->gstart[pass][d] + x - local;
+    for (uint32_t w = 0; w < wave; ++w) run += s_w[w];
+    for (uint32_t t = t0; t < t1; ++t) {
+        const size_t at = size_t(t) * RADIX + d;
+        const uint32_t c = counts[at];
+        counts[at] = run;
+        run += c;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_upsweep: the first read of a reduce-then-scan sort.  Per tile it writes
 // the digit-0 counts (pass 0's reduce step, [tile][digit]) and folds the keys
