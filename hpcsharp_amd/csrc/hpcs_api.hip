@@ -354,14 +354,15 @@ template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int RANK>
 void launch_scatter_r(int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb, uint32_t n, int pass,
                       DevPlan *plan, const uint32_t *offsets, uint32_t tiles)
 {
-    static int occ = -1; // resident workgroups per CU for this instantiation
-    if (occ < 0) {
+    // resident workgroups per CU for this instantiation (a function-local
+    // static: initialised once, safely, by the first of concurrent callers)
+    static const int occ = [] {
         int o = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK>, BLOCK,
                                                          0) != hipSuccess || o < 1)
             o = 1;
-        occ = o;
-    }
+        return o;
+    }();
     uint32_t grid = uint32_t(num_cus) * uint32_t(occ);
     grid = grid / NUM_XCD * NUM_XCD;
     if (grid < NUM_XCD) grid = NUM_XCD;
@@ -787,14 +788,13 @@ void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_r
     });
     // one resident wave of workgroups (a second, partial round would leave
     // most CUs idle at the end of the grid-stride loop)
-    static int occ = -1;
-    if (occ < 0) {
+    static const int occ = [] { // resident compaction workgroups per CU, initialised once
         int o = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_topk_compact<K, FLOATK>, TOPK_BLOCK, 0) != hipSuccess ||
             o < 1)
             o = 1;
-        occ = o;
-    }
+        return o;
+    }();
     const int cgrid = grid_for(uint64_t(n), TILE, c->num_cus * occ);
     for (int l = 0; l < LV::LEVELS; ++l) {
         timed(c, st, "topk_compact", [&] {
