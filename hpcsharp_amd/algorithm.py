@@ -18,11 +18,16 @@ import numpy as np
 from . import _native as N
 
 
-class ArgumentNullException(ValueError):
+class ArgumentException(ValueError):
+    """System.ArgumentException (e.g. PartitionRadix.cs:180-181); as in .NET,
+    the two below derive from it."""
+
+
+class ArgumentNullException(ArgumentException):
     """System.ArgumentNullException (e.g. RadixSortLsd.cs:215-216)."""
 
 
-class ArgumentOutOfRangeException(ValueError):
+class ArgumentOutOfRangeException(ArgumentException):
     """System.ArgumentOutOfRangeException (e.g. RadixSortLsd.cs:1621-1622)."""
 
 
@@ -182,13 +187,55 @@ class Algorithm:
                   inputArray.dtype.itemsize, keys.size)
         return inputArray, keys
 
-    # -- float[] / double[] ----------------------------------------------------
+    # -- MSD entry points ------------------------------------------------------
     @staticmethod
-    def SortRadixMsd(arrayToBeSorted: np.ndarray) -> None:
-        """SortRadixMsd(this float[]/double[]) RadixSortMsd.cs:1563-1595: ascending by the
-        radix key (-0.0 before +0.0)."""
-        _require(arrayToBeSorted, "arrayToBeSorted", (np.float32, np.float64))
-        _sort_whole(arrayToBeSorted)
+    def SortRadixMsd(arrayToBeSorted: np.ndarray, start: int | None = None, length: int | None = None,
+                     threshold: int = 1024) -> None:
+        """SortRadixMsd(this uint[] [, start, length] [, threshold]) RadixSortMsd.cs:682-712,
+        ulong[] :813-838, int[] :1285, long[] :1308, float[]/double[] :1563-1595 (ascending by
+        the radix key, -0.0 before +0.0).  The reference sorts in place MSD-first with an
+        Array.Sort base case below `threshold` (accepted here, unused); the result of a
+        sort is the sorted array whatever the algorithm, and the library's stable LSD
+        passes produce it (with n scratch keys, unlike the in-place MSD form)."""
+        _require(arrayToBeSorted, "arrayToBeSorted",
+                 (np.uint32, np.uint64, np.int32, np.int64, np.float32, np.float64))
+        if start is None:
+            _sort_whole(arrayToBeSorted)
+        else:
+            _sort_range(arrayToBeSorted, start, arrayToBeSorted.size - start if length is None else length)
+
+    @staticmethod
+    def PartitionRadix(arrayToBePartitioned: np.ndarray, *args, threshold: int = 1024) -> None:
+        """PartitionRadix(this uint[], [start, length,] k | int[] k [, threshold])
+        PartitionRadix.cs:135-210: afterwards a[k] holds the k-th smallest key of the
+        range, no key before it is larger and none after it smaller, for every k given
+        (the reference's multi-select leaves the rest in no particular order; a sorted
+        range meets the same contract, and is what this returns).  Argument checks as
+        the reference's: a null array or k array -> ArgumentNullException, an empty
+        array or range -> ArgumentOutOfRangeException, more indices than keys ->
+        ArgumentException, no indices -> no change."""
+        a = arrayToBePartitioned
+        if a is None:
+            raise ArgumentNullException("arrayToBePartitioned")
+        if len(args) == 3:
+            start, length, k = args
+        elif len(args) == 1:
+            start, length, k = 0, None, args[0]
+        else:
+            raise TypeError("PartitionRadix(a, k) or PartitionRadix(a, start, length, k)")
+        _require(a, "arrayToBePartitioned", (np.uint32,))
+        if k is None:
+            raise ArgumentNullException("k")
+        if length is None:
+            length = a.size
+        if a.size <= 0 or length <= 0:
+            raise ArgumentOutOfRangeException("array length or length is invalid")
+        ks = [k] if isinstance(k, (int, np.integer)) else list(k)
+        if len(ks) > a.size:
+            raise ArgumentException("k array length cannot be greater than the array to be selected length")
+        if not ks:
+            return
+        _sort_range(a, start, length)
 
     @staticmethod
     def SortRadixMsdInPlaceFunc(arrayToBeSorted: np.ndarray) -> np.ndarray:

@@ -103,3 +103,27 @@ def test_nbit_sort_argument_checks():
             Algorithm.SortRadixLsdNbit(arr, 0, 4, bad)
     with pytest.raises(ArgumentNullException):
         Algorithm.SortRadixLsd11bit(None)
+
+
+def test_partition_radix_argument_checks():
+    """PartitionRadix (PartitionRadix.cs:135-210): null array or k ->
+    ArgumentNullException, empty array -> ArgumentOutOfRangeException, more
+    indices than keys -> ArgumentException; all before any device work."""
+    import numpy as np
+    from hpcsharp_amd.algorithm import (Algorithm, ArgumentException, ArgumentNullException,
+                                        ArgumentOutOfRangeException)
+    with pytest.raises(ArgumentNullException):
+        Algorithm.PartitionRadix(None, 3)
+    with pytest.raises(ArgumentNullException):
+        Algorithm.PartitionRadix(np.zeros(4, np.uint32), 0, 4, None)
+    with pytest.raises(ArgumentOutOfRangeException):
+        Algorithm.PartitionRadix(np.zeros(0, np.uint32), 0)
+    with pytest.raises(ArgumentOutOfRangeException):
+        Algorithm.PartitionRadix(np.zeros(4, np.uint32), 0, 0, [1])
+    with pytest.raises(ArgumentException):
+        Algorithm.PartitionRadix(np.zeros(2, np.uint32), [0, 1, 1])
+    a = np.array([3, 1, 2], np.uint32)
+    Algorithm.PartitionRadix(a, [])            # no indices: unchanged, no device call
+    assert a.tolist() == [3, 1, 2]
+    assert issubclass(ArgumentNullException, ArgumentException)
+    assert issubclass(ArgumentOutOfRangeException, ArgumentException)

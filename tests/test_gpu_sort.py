@@ -677,3 +677,39 @@ def test_beyond_2pow30_keys():
     del u, t
     torch.cuda.empty_cache()
     N.check(N.lib().hpcs_ctx_trim(ctx))
+
+
+def test_partition_radix_and_msd_entry_points():
+    """PartitionRadix (PartitionRadix.cs:135-210): a[k] is the k-th smallest
+    of the range, nothing before it larger, nothing after it smaller, the
+    range a permutation, the rest untouched; SortRadixMsd for the integer
+    key types (RadixSortMsd.cs:682-838, 1285, 1308) sorts."""
+    n = 300007
+    a = datagen.keys_u32(n, 4040)
+    orig = a.copy()
+    k = 123456
+    Algorithm.PartitionRadix(a, k)
+    assert a[k] == np.partition(orig, k)[k]
+    assert a[:k].max() <= a[k] <= a[k + 1:].min()
+    assert np.array_equal(np.sort(a), np.sort(orig))
+    b = datagen.keys_u32(n, 4041)
+    ob = b.copy()
+    ks = [10, 5000, 77777]
+    Algorithm.PartitionRadix(b, 1000, 200000, ks)
+    r = b[1000:201000]
+    ref = np.sort(ob[1000:201000])
+    for kk in ks:
+        assert r[kk] == ref[kk]
+        assert r[:kk].max(initial=0) <= r[kk] <= r[kk + 1:].min(initial=0xFFFFFFFF)
+    assert np.array_equal(b[:1000], ob[:1000]) and np.array_equal(b[201000:], ob[201000:])
+    assert np.array_equal(np.sort(r), ref)
+    for dt, gen in ((np.uint32, datagen.keys_u32), (np.uint64, datagen.keys_u64),
+                    (np.int32, datagen.keys_u32), (np.int64, datagen.keys_u64)):
+        x = gen(70001, 4050).view(dt).copy()
+        want = np.sort(x)
+        Algorithm.SortRadixMsd(x)
+        assert np.array_equal(x, want), dt
+    y = datagen.keys_u32(50000, 4060)
+    oy = y.copy()
+    Algorithm.SortRadixMsd(y, 100, 40000, 64)
+    assert np.array_equal(y[100:40100], np.sort(oy[100:40100])) and np.array_equal(y[:100], oy[:100])
