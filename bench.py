@@ -273,6 +273,32 @@ def cfg3_figures(torch, N, datagen, reps: int):
     return {"n": n, **out}
 
 
+def cfg3_cpu_baseline(datagen, lg: int = 24):
+    """The floating and record workloads on the host: single-threaded C
+    restatements of SortRadixMsd(double[]) and SortRadixLsd2 over records
+    (oracle/), on a 2^lg sample of the same generators; best of 2."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    n = 1 << lg
+    d = datagen.doubles_special(n, 0x5EED0006)
+    rec = np.zeros(n, dtype=np.dtype({"names": ["key", "payload"], "formats": [np.uint64, np.uint32],
+                                      "offsets": [0, 8], "itemsize": 16}))
+    rec["key"] = datagen.keys_u64(n, 0x5EED0007)
+    rec["payload"] = np.arange(n, dtype=np.uint32)
+    out = {"sample": f"2^{lg} elements of the same generators, 1 thread, best of 2, on {cpu_model()}",
+           "cores": 1, "kind": "port"}
+    for name, src, fn in (("f64_sort", d, O.sort_radix_float),
+                          ("records_u64_key_u32_payload", rec, lambda r: O.sort_radix_records(r, 0, 64))):
+        best = float("inf")
+        for _ in range(2):
+            a = src.copy()
+            t0 = time.perf_counter()
+            fn(a)
+            best = min(best, time.perf_counter() - t0)
+        out[name] = {"ms": round(best * 1e3, 1), "gkeys_per_s": round(n / best / 1e9, 4)}
+    return out
+
+
 def host_array_figures(N, keys: np.ndarray, reps: int):
     """The host-array drop-in call (hpcs_sort_radix_u32 on pageable host
     memory): PCIe copy in, sort, copy out; never the headline value."""
@@ -298,6 +324,18 @@ def serving_figures(nreq: int, slots: int):
     res, st = S.run(t, slots=slots, hbm_budget=200 * 10 ** 9)
     s = S.summarize(t, res, st)
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
+
+
+def cpu_model() -> str:
+    """Model name of the host CPU, from /proc/cpuinfo ("unknown CPU" elsewhere)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def cpu_baseline(host_keys: np.ndarray, reps: int):
@@ -418,6 +456,8 @@ def main():
         log(f"[bench] top-k: {extra['topk']}")
     if not args.no_cfg3:
         extra["cfg3"] = cfg3_figures(torch, N, datagen, 3)
+        if not args.no_cpu:
+            extra["cfg3"]["cpu_baseline"] = cfg3_cpu_baseline(datagen)
         log(f"[bench] floating/record keys: {extra['cfg3']}")
     if not args.no_serving:
         log(f"[bench] serving: {args.serving_requests} requests")
@@ -430,7 +470,7 @@ def main():
         v, threads, times = cpu_baseline(keys, args.cpu_reps)
         cpu = {"value": round(v, 4), "unit": "Gkeys/s", "cores": threads, "kind": "port",
                "sample": f"full 2^{n.bit_length() - 1} uniform u32 array, 1 warm-up + median of {args.cpu_reps} "
-                         f"(s: {', '.join(f'{t:.2f}' for t in times)})"}
+                         f"(s: {', '.join(f'{t:.2f}' for t in times)}) on {cpu_model()}"}
         log(f"[bench] CPU: {v:.3f} Gkeys/s on {threads} threads")
 
     line = {
