@@ -225,6 +225,25 @@ def histogram_figures(torch, N, datagen, keys_host: np.ndarray, reps: int):
                 sys.exit(3)
             r["sort_us"] = round(sms * 1e3, 2)
             r["sort_gkeys_per_s"] = round(n / sms / 1e6, 2)
+            # the same sort through a context's graph cache (HPCS_OPT_GRAPHS):
+            # repeated calls launch one captured graph of the sequence
+            gctx = N.new_context(0)
+            N.load_tuning(N.TUNE_PATH, gctx)
+            N.set_option(N.HPCS_OPT_GRAPHS, 1, gctx)
+            gs = torch.cuda.Stream()
+            with torch.cuda.stream(gs):
+                gsort = lambda: N.check(lib.hpcs_dev_sort_radix_u32(gctx, P(work.data_ptr()), n, P(gs.cuda_stream)))
+                work.copy_(x)
+                gsort()                                   # first call: direct
+                gms = event_ms(torch, lambda: work.copy_(x), gsort, reps)
+            gs.synchronize()
+            if not torch.equal(work.to(torch.int64) & 0xFFFFFFFF, ref):
+                print("bench: graph-replayed 2^20 sort differs from torch.sort", file=sys.stderr)
+                sys.exit(3)
+            _, caps, replays = N.graph_cache_stats(gctx)
+            N.destroy_context(gctx)
+            r["sort_graph_us"] = round(gms * 1e3, 2)
+            r["sort_graph_replays"] = replays
             del work
         out[tag] = r
         del x, counts, xl, want

@@ -29,6 +29,7 @@ HPCS_OPT_RANK, HPCS_OPT_TILE_SHAPE, HPCS_OPT_FALLBACK_ROUNDS, HPCS_OPT_LOOKBACK_
 HPCS_OPT_PERSISTENT = 6
 HPCS_OPT_DIGIT_BYTES = 7
 HPCS_OPT_DIGIT_BITS = 8
+HPCS_OPT_GRAPHS = 9
 HPCS_OP_SORT, HPCS_OP_TOPK = 0, 1
 HPCS_SERVE_REALTIME, HPCS_SERVE_ALONE = 0, 1
 
@@ -62,6 +63,7 @@ SIGNATURES = {
     "hpcs_ctx_set_option": ([_p, _i32, _i64], _i32),
     "hpcs_profile_enable": ([_p, _i32], _i32),
     "hpcs_profile_read": ([_p, ctypes.POINTER(KernelStat), _i32, ctypes.POINTER(_i32)], _i32),
+    "hpcs_graph_cache_stats": ([_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)], _i32),
     "hpcs_sort_radix_u32": ([_p, _p, _i64], _i32),
     "hpcs_sort_radix_range_u32": ([_p, _p, _i64, _i64], _i32),
     "hpcs_sort_radix_u64": ([_p, _p, _i64], _i32),
@@ -201,6 +203,14 @@ def profile_read(ctx=None) -> dict[str, tuple[int, float]]:
     cnt = ctypes.c_int32()
     check(lib().hpcs_profile_read(ctx if ctx is not None else context(), buf, 64, ctypes.byref(cnt)))
     return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(cnt.value)}
+
+
+def graph_cache_stats(ctx=None) -> tuple[int, int, int]:
+    """(graphs held, captures, graph launches) of the context's graph cache."""
+    e, c, r = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    check(lib().hpcs_graph_cache_stats(ctx if ctx is not None else context(), ctypes.byref(e), ctypes.byref(c),
+                                       ctypes.byref(r)))
+    return e.value, c.value, r.value
 
 
 def set_option(option: int, value: int, ctx=None) -> None:

@@ -125,6 +125,23 @@ struct hpcs_ctx {
     std::vector<Pending> pending;
     std::vector<hipEvent_t> ev_pool;
     std::map<std::string, std::pair<int64_t, double>> stats;
+    // graph cache (HPCS_OPT_GRAPHS): a device-resident call that repeats
+    // (same entry point, pointers, size and stream) is replayed from a HIP
+    // graph of its launch sequence
+    struct GraphEntry {
+        int entry;
+        const void *p0, *p1;
+        int64_t n, aux;
+        hipStream_t st;
+        void *scratch, *io; // the context buffers when the call last ran directly
+        size_t scratch_cap, io_cap;
+        hipGraphExec_t exec; // null until the call repeats
+        uint64_t last_use;
+    };
+    int graphs = 0;
+    std::vector<GraphEntry> graph_cache;
+    uint64_t graph_clock = 0;
+    int64_t graph_captures = 0, graph_replays = 0;
 };
 
 namespace {
@@ -1034,6 +1051,109 @@ void host_histogram_nbit(hpcs_ctx *c, const K *a, int64_t l, int64_t r, int bits
 // Device-resident calls run on the caller's stream; NULL is HIP's null stream.
 hipStream_t pick_stream(hpcs_ctx *, void *stream) { return static_cast<hipStream_t>(stream); }
 
+// ---- graph cache ----
+constexpr size_t GRAPH_CACHE_MAX = 64;
+
+// Destroy every cached graph; the device is drained first, since a graph
+// launched earlier may still run.
+void graph_cache_clear(hpcs_ctx *c)
+{
+    bool any = false;
+    for (auto &e : c->graph_cache) any = any || e.exec != nullptr;
+    if (any) hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    for (auto &e : c->graph_cache)
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    c->graph_cache.clear();
+}
+
+// Run the launch sequence `f` on stream st, through the graph cache when it is
+// on (and the context is not profiling, st is not the null stream and is not
+// being captured by the caller).  The first call with a key runs `f`
+// directly, which sizes the context's buffers; the next call with the same
+// key, while those buffers are unchanged, captures `f` into a graph (so no
+// allocation happens under capture) and launches it; later calls launch the
+// graph.  Every input of the launch sequence is in the key or in the
+// context's options, and changing an option clears the cache.
+template <typename F>
+void run_cached(hpcs_ctx *c, const hpcs_ctx::GraphEntry &key, hipStream_t st, F &&f)
+{
+    if (!c->graphs || c->profile || st == nullptr) return f();
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
+    if (cs != hipStreamCaptureStatusNone) return f();
+    auto &cache = c->graph_cache;
+    size_t i = 0;
+    while (i < cache.size() && !(cache[i].entry == key.entry && cache[i].p0 == key.p0 && cache[i].p1 == key.p1 &&
+                                 cache[i].n == key.n && cache[i].aux == key.aux && cache[i].st == key.st))
+        ++i;
+    const bool same_buffers = i < cache.size() && cache[i].scratch == c->scratch && cache[i].io == c->io &&
+                              cache[i].scratch_cap == c->scratch_cap && cache[i].io_cap == c->io_cap;
+    if (i == cache.size() || !same_buffers) {
+        // first sighting (or the buffers moved): run directly, then remember
+        // the key with the buffers it used
+        f();
+        if (i < cache.size() && cache[i].exec) {
+            hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            (void)hipGraphExecDestroy(cache[i].exec);
+            cache[i].exec = nullptr;
+        }
+        if (i == cache.size()) {
+            if (cache.size() >= GRAPH_CACHE_MAX) {
+                // evict the least recently used entry, preferring one with no graph
+                size_t v = 0;
+                for (size_t j = 1; j < cache.size(); ++j) {
+                    const bool jg = cache[j].exec != nullptr, vg = cache[v].exec != nullptr;
+                    if ((!jg && vg) || (jg == vg && cache[j].last_use < cache[v].last_use)) v = j;
+                }
+                if (cache[v].exec) {
+                    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+                    (void)hipGraphExecDestroy(cache[v].exec);
+                }
+                cache.erase(cache.begin() + std::ptrdiff_t(v));
+            }
+            cache.push_back(key);
+            i = cache.size() - 1;
+            cache[i].exec = nullptr;
+        }
+        cache[i].scratch = c->scratch;
+        cache[i].io = c->io;
+        cache[i].scratch_cap = c->scratch_cap;
+        cache[i].io_cap = c->io_cap;
+        cache[i].last_use = ++c->graph_clock;
+        return;
+    }
+    auto &e = cache[i];
+    e.last_use = ++c->graph_clock;
+    if (!e.exec) {
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipGraph_t g = nullptr;
+        try {
+            f();
+        } catch (...) {
+            if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(st, &g), "hipStreamEndCapture");
+        hipGraphExec_t ex = nullptr;
+        const hipError_t err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(err, "hipGraphInstantiate");
+        e.exec = ex;
+        ++c->graph_captures;
+    }
+    hip_check(hipGraphLaunch(e.exec, st), "hipGraphLaunch");
+    ++c->graph_replays;
+}
+
+// A device-resident sort through the graph cache.
+template <typename K, bool HAS_VALS, int KIND>
+void dev_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
+{
+    if (n <= 1) return run_sort<K, HAS_VALS, KIND>(c, d_keys, d_vals, n, st);
+    const hpcs_ctx::GraphEntry key{int(sizeof(K)) * 16 + (HAS_VALS ? 8 : 0) + KIND, d_keys, d_vals, n, 0, st};
+    run_cached(c, key, st, [&] { run_sort<K, HAS_VALS, KIND>(c, d_keys, d_vals, n, st); });
+}
+
 // Byte histogram of the key inside each record (the key extracted into the
 // context's scratch, then the plain histogram).
 void dev_histogram_records(hpcs_ctx *c, const void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset,
@@ -1195,6 +1315,7 @@ HPCS_API int32_t hpcs_ctx_trim(hpcs_ctx *ctx)
 {
     return guarded(ctx, [&] {
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        graph_cache_clear(ctx);
         if (ctx->scratch) hip_check(hipFree(ctx->scratch), "hipFree");
         if (ctx->io) hip_check(hipFree(ctx->io), "hipFree");
         ctx->scratch = ctx->io = nullptr;
@@ -1223,7 +1344,12 @@ HPCS_API int32_t hpcs_ctx_destroy(hpcs_ctx *ctx)
 HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t value)
 {
     return guarded(ctx, [&] {
+        graph_cache_clear(ctx); // cached launch sequences follow the options
         switch (option) {
+        case HPCS_OPT_GRAPHS:
+            if (value != 0 && value != 1) fail(HPCS_E_INVALID_ARG, "graphs must be 0 or 1");
+            ctx->graphs = int(value);
+            break;
         case HPCS_OPT_RANK:
             if (value != RANK_BALLOT && value != RANK_LDSOR) fail(HPCS_E_INVALID_ARG, "rank mode must be 0 or 1");
             ctx->rank_mode = int(value);
@@ -1290,6 +1416,18 @@ HPCS_API int32_t hpcs_profile_read(hpcs_ctx *ctx, hpcs_kernel_stat *out, int32_t
             ++i;
         }
         *count = i;
+    });
+}
+
+HPCS_API int32_t hpcs_graph_cache_stats(hpcs_ctx *ctx, int64_t *entries, int64_t *captures, int64_t *replays)
+{
+    return guarded(ctx, [&] {
+        if (!entries || !captures || !replays) fail(HPCS_E_INVALID_ARG, "null output");
+        int64_t g = 0;
+        for (auto &e : ctx->graph_cache) g += e.exec != nullptr;
+        *entries = g;
+        *captures = ctx->graph_captures;
+        *replays = ctx->graph_replays;
     });
 }
 
@@ -1464,38 +1602,43 @@ HPCS_API int32_t hpcs_sort_segments_u32(hpcs_ctx *ctx, uint32_t *keys, const int
 HPCS_API int32_t hpcs_dev_sort_segments_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, const int64_t *d_seg_begin,
                                             int32_t nseg, void *stream)
 {
-    return guarded(ctx, [&] { run_segments(ctx, d_keys, n, d_seg_begin, nseg, pick_stream(ctx, stream)); });
+    return guarded(ctx, [&] {
+        hipStream_t st = pick_stream(ctx, stream);
+        if (n <= 1 || nseg <= 0) return run_segments(ctx, d_keys, n, d_seg_begin, nseg, st);
+        const hpcs_ctx::GraphEntry key{1, d_keys, d_seg_begin, n, nseg, st};
+        run_cached(ctx, key, st, [&] { run_segments(ctx, d_keys, n, d_seg_begin, nseg, st); });
+    });
 }
 
 HPCS_API int32_t hpcs_dev_sort_radix_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream)
 {
-    return guarded(ctx, [&] { run_sort<uint32_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });
+    return guarded(ctx, [&] { dev_sort<uint32_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });
 }
 HPCS_API int32_t hpcs_dev_sort_radix_u64(hpcs_ctx *ctx, uint64_t *d_keys, int64_t n, void *stream)
 {
-    return guarded(ctx, [&] { run_sort<uint64_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });
+    return guarded(ctx, [&] { dev_sort<uint64_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });
 }
 HPCS_API int32_t hpcs_dev_sort_radix_f32(hpcs_ctx *ctx, float *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] {
-        run_sort<uint32_t, false, true>(ctx, reinterpret_cast<uint32_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
+        dev_sort<uint32_t, false, true>(ctx, reinterpret_cast<uint32_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
     });
 }
 HPCS_API int32_t hpcs_dev_sort_radix_f64(hpcs_ctx *ctx, double *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] {
-        run_sort<uint64_t, false, true>(ctx, reinterpret_cast<uint64_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
+        dev_sort<uint64_t, false, true>(ctx, reinterpret_cast<uint64_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
     });
 }
 HPCS_API int32_t hpcs_dev_sort_radix_pairs_u64_u32(hpcs_ctx *ctx, uint64_t *d_keys, uint32_t *d_vals, int64_t n,
                                                    void *stream)
 {
-    return guarded(ctx, [&] { run_sort<uint64_t, true, false>(ctx, d_keys, d_vals, n, pick_stream(ctx, stream)); });
+    return guarded(ctx, [&] { dev_sort<uint64_t, true, false>(ctx, d_keys, d_vals, n, pick_stream(ctx, stream)); });
 }
 HPCS_API int32_t hpcs_dev_sort_radix_pairs_u32_u32(hpcs_ctx *ctx, uint32_t *d_keys, uint32_t *d_vals, int64_t n,
                                                    void *stream)
 {
-    return guarded(ctx, [&] { run_sort<uint32_t, true, false>(ctx, d_keys, d_vals, n, pick_stream(ctx, stream)); });
+    return guarded(ctx, [&] { dev_sort<uint32_t, true, false>(ctx, d_keys, d_vals, n, pick_stream(ctx, stream)); });
 }
 HPCS_API int32_t hpcs_dev_sort_radix_records(hpcs_ctx *ctx, void *d_recs, int64_t n, int32_t rec_size,
                                              int32_t key_offset, int32_t key_bits, int32_t key_kind, void *stream)
@@ -1597,14 +1740,14 @@ HPCS_API int32_t hpcs_sort_radix_i64(hpcs_ctx *ctx, int64_t *keys, int64_t n)
 HPCS_API int32_t hpcs_dev_sort_radix_i32(hpcs_ctx *ctx, int32_t *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] {
-        run_sort<uint32_t, false, KIND_INT>(ctx, reinterpret_cast<uint32_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
+        dev_sort<uint32_t, false, KIND_INT>(ctx, reinterpret_cast<uint32_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
     });
 }
 
 HPCS_API int32_t hpcs_dev_sort_radix_i64(hpcs_ctx *ctx, int64_t *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] {
-        run_sort<uint64_t, false, KIND_INT>(ctx, reinterpret_cast<uint64_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
+        dev_sort<uint64_t, false, KIND_INT>(ctx, reinterpret_cast<uint64_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
     });
 }
 
@@ -1724,6 +1867,7 @@ HPCS_API int32_t hpcs_dev_histogram_byte_components_records(hpcs_ctx *ctx, const
 HPCS_API int32_t hpcs_ctx_load_tuning(hpcs_ctx *ctx, const char *path)
 {
     return guarded(ctx, [&] {
+        graph_cache_clear(ctx); // cached launch sequences follow the table
         if (!path || !*path) {
             ctx->tune.clear();
             return;

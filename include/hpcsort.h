@@ -71,6 +71,13 @@ extern "C" {
                                       byte and the next pass counts its tiles from those n bytes; 0 the next
                                       pass re-reads the keys; 2 (default) 1 for 8-byte keys with values, else 0
                                       (scratch holds the n bytes either way) */
+#define HPCS_OPT_GRAPHS 9          /* 1: graph cache for the device-resident sorts (hpcs_dev_sort_radix_*,
+                                      hpcs_dev_sort_segments_u32) on a non-null stream: the first call with
+                                      given pointers, size and stream runs directly, the second captures its
+                                      launch sequence in a HIP graph, later ones launch that graph (one
+                                      launch instead of up to ~30); 0 (default) plain launches.  Setting any
+                                      option clears the cache; a call on a stream the caller is capturing
+                                      launches plainly */
 
 #define HPCS_DTYPE_U32 0
 #define HPCS_DTYPE_U64 1
@@ -139,6 +146,11 @@ int32_t hpcs_tuning_check(const char *path, int32_t *rows);
  * hpcs_profile_read waits for those events and returns up to max entries. */
 int32_t hpcs_profile_enable(hpcs_ctx *ctx, int32_t on);
 int32_t hpcs_profile_read(hpcs_ctx *ctx, hpcs_kernel_stat *out, int32_t max, int32_t *count);
+/* Graph cache of HPCS_OPT_GRAPHS (no counterpart in the reference, whose
+ * calls are synchronous CPU code; for callers that repeat the same
+ * device-resident sort, such as a fixed-shape pipeline): graphs held now,
+ * captures and graph launches since the context was created. */
+int32_t hpcs_graph_cache_stats(hpcs_ctx *ctx, int64_t *entries, int64_t *captures, int64_t *replays);
 
 /* ---- host-array drop-ins (synchronous, in place) ------------------------ */
 
