@@ -88,6 +88,14 @@ constexpr int SORT_BLOCK = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
 constexpr int HIST_BLOCK = 256;
+#ifndef HPCS_HIST_COPIES4
+#define HPCS_HIST_COPIES4 4
+#endif
+#ifndef HPCS_HIST_COPIES8
+#define HPCS_HIST_COPIES8 2
+#endif
+// interleaved copies of k_histogram's LDS table (16 KiB per block either way)
+template <typename K> constexpr int hist_copies() { return sizeof(K) == 4 ? HPCS_HIST_COPIES4 : HPCS_HIST_COPIES8; }
 constexpr size_t ALIGN = 256;
 
 size_t align_up(size_t x, size_t a = ALIGN) { return (x + a - 1) / a * a; }
@@ -279,10 +287,19 @@ void launch_histogram(hpcs_ctx *c, const K *d_keys, uint64_t n, uint32_t *d_hist
     const uint32_t head = uint32_t(((16 - (addr & 15)) & 15) / sizeof(K));
     const uint64_t chunks = (n > head ? n - head : 0) / (WAVE * VEC);
     const int waves_per_block = HIST_BLOCK / WAVE;
-    const int grid = grid_for(chunks, waves_per_block * HIST_UNROLL, c->num_cus * 8);
+    // at most the resident workgroups (a grid-stride loop: a second round of
+    // workgroups would run on a partly idle chip)
+    static const int occ = [] {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_histogram<K, FLOATK, HIST_BLOCK, hist_copies<K>()>,
+                                                         HIST_BLOCK, 0) != hipSuccess || o < 1)
+            o = 1;
+        return std::min(o, 8);
+    }();
+    const int grid = grid_for(chunks, waves_per_block * HIST_UNROLL, c->num_cus * occ);
     timed(c, st, "histogram", [&] {
-        hipLaunchKernelGGL((k_histogram<K, FLOATK, HIST_BLOCK>), dim3(grid), dim3(HIST_BLOCK), 0, st, d_keys, n,
-                           head, d_hist);
+        hipLaunchKernelGGL((k_histogram<K, FLOATK, HIST_BLOCK, hist_copies<K>()>), dim3(grid), dim3(HIST_BLOCK), 0,
+                           st, d_keys, n, head, d_hist);
     });
 }
 

@@ -172,16 +172,31 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t words, uint32_t va
 // ---------------------------------------------------------------------------
 // k_histogram: every digit's 256-bin count in one read.
 // Waves walk 1 KiB chunks (64 lanes x one 16-byte vector) in a grid-stride
-// loop; each digit update is an LDS atomic on the block's [P][256] table,
-// except that a wave whose 64 lanes share a digit adds 64 once (sorted and
-// constant inputs).  The tail below a whole chunk is counted lane by lane.
-// Block tables are added into the global [P][256] result with one atomic per
-// non-zero bin.
+// loop; each digit update is an LDS atomic on the block's table, except that
+// a chunk whose keys all share a digit adds its key count once (sorted and
+// constant inputs).  That test runs once per digit and chunk, on the OR over
+// a lane's keys of key XOR lane 0's first key; a test per key and digit cost
+// more instructions than the atomics it saved.  The table holds COPIES
+// interleaved copies of the [P][256] counts, word (d * 256 + bin) * COPIES +
+// lane % COPIES, so that a wave's atomics on random digits collide less on
+// LDS banks.  The tail below a whole chunk is counted lane by lane.  Block
+// tables (copies summed) are added into the global [P][256] result with one
+// atomic per non-zero bin.
 // ---------------------------------------------------------------------------
 // `head` (< VEC) keys precede the first 16-byte aligned key; they are counted
 // lane by lane together with the tail.
-constexpr int HIST_UNROLL = 4; // 1 KiB chunks per wave in flight
-template <typename K, int FLOATK, int BLOCK>
+#ifndef HPCS_HIST_UNROLL
+#define HPCS_HIST_UNROLL 4
+#endif
+constexpr int HIST_UNROLL = HPCS_HIST_UNROLL; // 1 KiB chunks per wave in flight
+
+// lane 0's value of x, for every lane
+__device__ inline uint32_t wave_first(uint32_t x) { return uint32_t(__builtin_amdgcn_readfirstlane(int(x))); }
+__device__ inline uint64_t wave_first(uint64_t x)
+{
+    return uint64_t(wave_first(uint32_t(x))) | (uint64_t(wave_first(uint32_t(x >> 32))) << 32);
+}
+template <typename K, int FLOATK, int BLOCK, int COPIES>
 __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys, uint64_t n, uint32_t head,
                                                      uint32_t *__restrict__ hist)
 {
@@ -190,12 +205,13 @@ __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys,
     constexpr int CHUNK = WAVE * VEC;
     constexpr int WAVES = BLOCK / WAVE;
     using Vec = typename Vec16<K>::type;
-    __shared__ uint32_t sh[P * RADIX];
+    __shared__ uint32_t sh[P * RADIX * COPIES];
 
-    for (int i = threadIdx.x; i < P * RADIX; i += BLOCK) sh[i] = 0;
+    for (int i = threadIdx.x; i < P * RADIX * COPIES; i += BLOCK) sh[i] = 0;
     __syncthreads();
 
     const int lane = threadIdx.x & (WAVE - 1);
+    const int copy = lane % COPIES;
     const uint64_t body = n > head ? n - head : 0;
     const uint64_t nchunks = body / CHUNK;
     const uint64_t wave_id = uint64_t(blockIdx.x) * WAVES + (threadIdx.x >> 6);
@@ -212,19 +228,24 @@ __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys,
 #pragma unroll
         for (int u = 0; u < HIST_UNROLL; ++u) {
             if (c0 + u >= nchunks) break; // wave-uniform
+            K kk[VEC];
 #pragma unroll
-            for (int j = 0; j < VEC; ++j) {
-                const K raw = v[u][j];
-                K key = FLOATK ? key_enc<K, FLOATK>(raw) : raw;
+            for (int j = 0; j < VEC; ++j) kk[j] = FLOATK ? key_enc<K, FLOATK>(v[u][j]) : v[u][j];
+            // the bits in which some key of the chunk differs from lane 0's
+            // first key: digit d is the same for the whole chunk iff its
+            // byte of `m` is zero in every lane (one test per digit and chunk)
+            const K ref = wave_first(kk[0]);
+            K m = 0;
 #pragma unroll
-                for (int d = 0; d < P; ++d) {
-                    const uint32_t dg = KeyOps<K>::digit(key, 8 * d);
-                    const uint32_t d0 = __builtin_amdgcn_readfirstlane(dg);
-                    if (__all(dg == d0)) {
-                        if (lane == 0) atomicAdd(&sh[d * RADIX + d0], uint32_t(WAVE));
-                    } else {
-                        atomicAdd(&sh[d * RADIX + dg], 1u);
-                    }
+            for (int j = 0; j < VEC; ++j) m |= kk[j] ^ ref;
+#pragma unroll
+            for (int d = 0; d < P; ++d) {
+                if (__all(KeyOps<K>::digit(m, 8 * d) == 0u)) {
+                    if (lane == 0) atomicAdd(&sh[(d * RADIX + KeyOps<K>::digit(ref, 8 * d)) * COPIES], uint32_t(CHUNK));
+                } else {
+#pragma unroll
+                    for (int j = 0; j < VEC; ++j)
+                        atomicAdd(&sh[(d * RADIX + KeyOps<K>::digit(kk[j], 8 * d)) * COPIES + copy], 1u);
                 }
             }
         }
@@ -237,12 +258,14 @@ __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys,
             const uint64_t i = t < head ? t : tail0 + (t - head);
             K key = FLOATK ? key_enc<K, FLOATK>(keys[i]) : keys[i];
 #pragma unroll
-            for (int d = 0; d < P; ++d) atomicAdd(&sh[d * RADIX + KeyOps<K>::digit(key, 8 * d)], 1u);
+            for (int d = 0; d < P; ++d) atomicAdd(&sh[(d * RADIX + KeyOps<K>::digit(key, 8 * d)) * COPIES + copy], 1u);
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < P * RADIX; i += BLOCK) {
-        const uint32_t c = sh[i];
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < COPIES; ++k) c += sh[i * COPIES + k];
         if (c) atomicAdd(&hist[i], c);
     }
 }
