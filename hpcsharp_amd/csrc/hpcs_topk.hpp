@@ -209,6 +209,15 @@ template <typename K> struct WaveStage {
         __builtin_amdgcn_wave_barrier();
         cnt = 0u;
     }
+    // Write the staged keys at dst[base...] (the slot already reserved).
+    __device__ inline void write_at(uint32_t base)
+    {
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = threadIdx.x & (WAVE - 1); i < cnt; i += WAVE)
+            if (base + i < cap) dst[base + i] = buf[i];
+        __builtin_amdgcn_wave_barrier();
+        cnt = 0u;
+    }
     // Append v from the lanes where `take` holds.
     __device__ inline void append(bool take, K v)
     {
@@ -332,8 +341,27 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_topk_compact(const K *__restrict
             }
         }
     }
-    so.flush();
-    sc.flush();
+    // the waves' remaining stages go out with one counter atomic per list and
+    // workgroup (one per wave put thousands of atomics on two words)
+    __shared__ uint32_t s_cnt[2][WAVES], s_base[2];
+    if (lane == 0) {
+        s_cnt[0][wave] = so.cnt;
+        s_cnt[1][wave] = sc.cnt;
+    }
+    __syncthreads();
+    if (tid < 2) {
+        uint32_t tot = 0;
+        for (int w = 0; w < WAVES; ++w) tot += s_cnt[tid][w];
+        s_base[tid] = tot ? atomicAdd(tid == 0 ? so.ctr : sc.ctr, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t b0 = s_base[0], b1 = s_base[1];
+    for (int w = 0; w < wave; ++w) {
+        b0 += s_cnt[0][w];
+        b1 += s_cnt[1][w];
+    }
+    so.write_at(b0);
+    sc.write_at(b1);
     if (keep_cand) {
         __syncthreads();
         uint32_t *hn = st->hist[(level + 1) % 3];
