@@ -187,3 +187,31 @@ def test_segments_replays(gctx):
         for i in range(nseg):
             assert np.array_equal(got[begin[i]:begin[i + 1]], np.sort(a[begin[i]:begin[i + 1]])), (rep, i)
     assert N.graph_cache_stats(gctx)[1:] == (1, 3)
+
+
+def test_segments_replay_follows_new_boundaries(gctx):
+    """The key holds the offsets pointer, not its contents: the segment
+    boundaries are read on the device, so a replay with new boundaries in the
+    same buffer (same total and count) must sort the new segments."""
+    import torch
+    s = torch.cuda.Stream()
+    nseg, total = 200, 1 << 20
+    buf = torch.empty(total, dtype=torch.int32, device="cuda")
+    d_begin = torch.empty(nseg + 1, dtype=torch.int64, device="cuda")
+    rng = np.random.default_rng(0x6D00)
+    for rep in range(4):
+        cuts = np.sort(rng.choice(np.arange(1, total), size=nseg - 1, replace=False))
+        begin = np.concatenate([[0], cuts, [total]]).astype(np.int64)
+        if rep == 3:  # one large segment and many empty ones
+            begin = np.concatenate([np.zeros(nseg // 2, np.int64), np.full(nseg - nseg // 2 + 1, total, np.int64)])
+        a = datagen.keys_u32(total, 0x6D10 + rep, "uniform")
+        buf.copy_(torch.from_numpy(a.view(np.int32)))
+        d_begin.copy_(torch.from_numpy(begin))
+        torch.cuda.synchronize()
+        N.check(N.lib().hpcs_dev_sort_segments_u32(gctx, P(buf.data_ptr()), total, P(d_begin.data_ptr()), nseg,
+                                                   P(s.cuda_stream)))
+        s.synchronize()
+        got = buf.cpu().numpy().view(np.uint32)
+        for i in range(nseg):
+            assert np.array_equal(got[begin[i]:begin[i + 1]], np.sort(a[begin[i]:begin[i + 1]])), (rep, i)
+    assert N.graph_cache_stats(gctx)[1:] == (1, 3)
