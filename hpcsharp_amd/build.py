@@ -1,6 +1,9 @@
 """Build libhpcsort.so (the gfx950 HIP library) in-tree with hipcc.
 
 Usage: python -m hpcsharp_amd.build  (or hpcsharp_amd.build.build()).
+       python -m hpcsharp_amd.build --comparator  also builds rocprim_compare,
+       which times libhpcsort against rocPRIM's radix sort (a yardstick only:
+       the library never uses rocPRIM).
 """
 from __future__ import annotations
 
@@ -66,5 +69,28 @@ def build_tools(verbose: bool = False) -> None:
         os.replace(out + ".tmp", out)
 
 
+COMPARATOR = os.path.join(HERE, "rocprim_compare")
+
+
+def build_comparator(verbose: bool = False) -> str:
+    """hpcsharp_amd/rocprim_compare: links libhpcsort.so (found next to the
+    binary) and rocPRIM's headers; not part of build()."""
+    src = os.path.join(HERE, "csrc", "rocprim_compare.hip")
+    cmd = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-I", os.path.join(REPO, "include"), src, "-L", HERE, "-lhpcsort", "-Wl,-rpath,$ORIGIN",
+           "-o", COMPARATOR + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building rocprim_compare")
+    os.replace(COMPARATOR + ".tmp", COMPARATOR)
+    return COMPARATOR
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    if "--comparator" in sys.argv[1:]:
+        print(build_comparator(verbose=True))
