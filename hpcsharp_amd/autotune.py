@@ -39,7 +39,9 @@ BLOCK_SORT_MAX = {4: 16384, 8: 8192}
 
 def candidates(key_bytes: int, vals: bool, n: int | None = None):
     """8-bit digits: every pass algorithm x tile shape x ranking, and the
-    one-workgroup sort where n fits it; 11-bit digits (keys without values):
+    one-workgroup sort where n fits it (not the MSD split, pass algorithm 4:
+    its speed depends on how the keys spread, which uniform keys do not
+    show); 11-bit digits (keys without values):
     their one form, onesweep passes with 256x16 tiles and ballot ranking."""
     shapes = (0, 1, 2, 3, 4) if key_bytes == 4 and not vals else (0, 4)
     out = [{"algo": a, "shape": s, "rank": r, "bits": 8} for a, s, r in itertools.product((0, 1), shapes, (1, 0))]

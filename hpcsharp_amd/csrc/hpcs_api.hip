@@ -9,6 +9,7 @@
 // device-resident call, so the sequence can be captured in a HIP graph.
 #include "hpcs_kernels.hpp"
 #include "hpcs_digit11.hpp"
+#include "hpcs_msd.hpp"
 #include "hpcs_segments.hpp"
 #include "hpcs_topk.hpp"
 #include "hpcs_tuning.hpp"
@@ -53,6 +54,9 @@ constexpr int PASS_ONESWEEP = 0;
 constexpr int PASS_RTS = 1;
 constexpr int PASS_AUTO = 2;
 constexpr int PASS_BLOCK = 3; // one workgroup sorts the whole array in LDS
+constexpr int PASS_MSD = 4;   // one MSD split, then one workgroup per bucket (keys without values)
+// largest arrays the MSD split serves (beyond, most buckets outgrow a tile)
+constexpr int64_t MSD_MAX4 = int64_t(1) << 22, MSD_MAX8 = int64_t(1) << 21;
 // largest arrays the one-workgroup sort takes (LDS: keys, values, counters)
 constexpr int64_t BLOCK_SORT_MAX4 = 1024 * 16; // 4-byte keys
 constexpr int64_t BLOCK_SORT_MAX8 = 1024 * 8;  // 8-byte keys
@@ -478,6 +482,7 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
     // scan's five kernels per pass (launch-bound), measured on MI355X.
     const int64_t block_max = key_bytes == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8;
     if (p.algo == PASS_BLOCK && n > block_max) p.algo = PASS_AUTO;
+    if (p.algo == PASS_MSD && (vals || n > (key_bytes == 4 ? MSD_MAX4 : MSD_MAX8))) p.algo = PASS_AUTO;
     // by size, measured on MI355X: one workgroup up to 4096 4-byte / 8192
     // 8-byte keys (beyond 4096 4-byte keys the 1024-thread workgroup on one
     // CU loses to onesweep passes over many CUs)
@@ -564,6 +569,68 @@ void run_sort11(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st)
     });
 }
 
+// Bucket tile for the MSD path: about 2.5x the mean bucket (n / 256) where a
+// workgroup's LDS holds it; larger buckets take the streaming passes.
+template <typename K, int FLOATK>
+void launch_sort_buckets(int64_t n, hipStream_t st, K *a, K *b, const DevPlan *plan)
+{
+    const int64_t want = n / RADIX * 5 / 2;
+    if (sizeof(K) == 4) {
+        if (want <= 4096)
+            hipLaunchKernelGGL((k_sort_buckets<K, FLOATK, 256, 16>), dim3(RADIX), dim3(256), 0, st, a, b, uint32_t(n),
+                               plan);
+        else if (want <= 8192)
+            hipLaunchKernelGGL((k_sort_buckets<K, FLOATK, 512, 16>), dim3(RADIX), dim3(512), 0, st, a, b, uint32_t(n),
+                               plan);
+        else
+            hipLaunchKernelGGL((k_sort_buckets<K, FLOATK, 1024, 16>), dim3(RADIX), dim3(1024), 0, st, a, b,
+                               uint32_t(n), plan);
+    } else {
+        if (want <= 2048)
+            hipLaunchKernelGGL((k_sort_buckets<K, FLOATK, 256, 8>), dim3(RADIX), dim3(256), 0, st, a, b, uint32_t(n),
+                               plan);
+        else if (want <= 4096)
+            hipLaunchKernelGGL((k_sort_buckets<K, FLOATK, 256, 16>), dim3(RADIX), dim3(256), 0, st, a, b, uint32_t(n),
+                               plan);
+        else
+            hipLaunchKernelGGL((k_sort_buckets<K, FLOATK, 512, 16>), dim3(RADIX), dim3(512), 0, st, a, b, uint32_t(n),
+                               plan);
+    }
+}
+
+// One MSD split by the top eight varying bits (onesweep pass -1), then one
+// workgroup per bucket (keys without values, n <= MSD_MAX4 / MSD_MAX8).
+template <typename K, int FLOATK>
+void run_sort_msd(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st, const SortParams &prm)
+{
+    const int shape = prm.shape;
+    const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
+    const Layout L = make_layout(n, sizeof(K), false, tile_keys);
+    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
+    DevPlan *plan = reinterpret_cast<DevPlan *>(base + L.off_plan);
+    uint32_t *lb0 = reinterpret_cast<uint32_t *>(base + L.off_lb0);
+    K *keysB = reinterpret_cast<K *>(base + L.off_keysB);
+    K *orslots = reinterpret_cast<K *>(base + L.off_hist); // the histogram area holds the OR slots
+    launch_fill(c, base, L.zero_bytes, st);                  // OR slots, plan, look-back words
+    const int grid = grid_for(uint64_t(n), MSD_BLOCK * 8, c->num_cus * 4);
+    timed(c, st, "msd_or_and", [&] {
+        hipLaunchKernelGGL((k_or_and<K, FLOATK>), dim3(grid), dim3(MSD_BLOCK), 0, st, d_keys, uint32_t(n), orslots);
+    });
+    timed(c, st, "msd_hist", [&] {
+        hipLaunchKernelGGL((k_msd_hist<K, FLOATK>), dim3(grid), dim3(MSD_BLOCK), 0, st, d_keys, uint32_t(n), orslots,
+                           plan);
+    });
+    timed(c, st, "plan", [&] {
+        hipLaunchKernelGGL((k_plan_msd<K>), dim3(1), dim3(RADIX), 0, st, orslots, plan, uint32_t(n), FLOATK ? 1 : 0);
+    });
+    timed(c, st, "msd_split", [&] {
+        launch_onesweep<K, false, FLOATK>(prm.rank, shape, c->lookback_window, c->fallback_rounds,
+                                          dim3(unsigned(L.tiles)), st, d_keys, keysB, nullptr, nullptr, uint32_t(n),
+                                          -1, plan, lb0, nullptr, nullptr);
+    });
+    timed(c, st, "msd_buckets", [&] { launch_sort_buckets<K, FLOATK>(n, st, d_keys, keysB, plan); });
+}
+
 template <typename K, bool HAS_VALS, int FLOATK>
 void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
 {
@@ -581,6 +648,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     }
     if constexpr (!HAS_VALS) {
         if (prm.bits == 11) return run_sort11<K, FLOATK>(c, d_keys, n, st);
+        if (prm.algo == PASS_MSD) return run_sort_msd<K, FLOATK>(c, d_keys, n, st, prm);
     }
     const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
@@ -1382,8 +1450,9 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             ctx->lookback_window = int(value);
             break;
         case HPCS_OPT_PASS_ALGO:
-            if (value != PASS_ONESWEEP && value != PASS_RTS && value != PASS_AUTO && value != PASS_BLOCK)
-                fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0, 1, 2 or 3");
+            if (value != PASS_ONESWEEP && value != PASS_RTS && value != PASS_AUTO && value != PASS_BLOCK &&
+                value != PASS_MSD)
+                fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0, 1, 2, 3 or 4");
             ctx->pass_algo = int(value);
             ctx->set_algo = value != PASS_AUTO;
             break;
@@ -1495,6 +1564,18 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
     const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
     const Layout L = make_layout(n, ks, with_values != 0, tile_keys);
+    if (prm.algo == PASS_MSD) {
+        // two reads (OR/AND, split histogram), the split (read + write), the
+        // bucket sorts (read + write; buckets within a tile)
+        info->digit_bits = RADIX_BITS;
+        info->passes = 2;
+        info->tile_keys = tile_keys;
+        info->block_threads = TILE_SHAPES[shape].block;
+        info->tiles = L.tiles;
+        info->ideal_bytes = n * int64_t(ks) * 6;
+        info->scratch_bytes = int64_t(L.total);
+        return HPCS_OK;
+    }
     const int64_t per_key = int64_t(ks) + passes * 2 * (int64_t(ks) + (with_values ? 4 : 0));
     info->digit_bits = RADIX_BITS;
     info->passes = passes;

@@ -65,7 +65,12 @@ struct DevPlan {
     uint32_t xout[MAX_PASSES];          // apply the inverse transform on store
     uint32_t copyback;                  // final data sits in buffer B
     uint32_t flags[MAX_PASSES];         // the four fields above packed, PF_*
-    uint32_t pad[47];
+    // MSD split (k_plan_msd): one scatter by the 8-bit digit at msd_shift,
+    // the top eight bits in which the keys differ
+    uint32_t msd_flags;                 // PF_* of that scatter (PF_TRIVIAL: every key equal)
+    uint32_t msd_shift;
+    uint32_t msd_start[RADIX];          // digit counts (k_msd_hist), then exclusive starts
+    uint32_t pad[45];
 };
 constexpr uint32_t PF_TRIVIAL = 1u, PF_SRC_B = 2u, PF_XIN = 4u, PF_XOUT = 8u;
 // Reduce-then-scan plans only: PF_FIRST marks the first pass that moves keys,
@@ -393,8 +398,9 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     // exactly one block per tile); that region's previous pass has finished.
     if (lb_next != nullptr && tid < RADIX) lb_next[size_t(blockIdx.x) * RADIX + tid] = 0u;
 
-    // one packed word of per-pass flags (one dependent load before the keys)
-    const uint32_t flags = plan->flags[pass];
+    // one packed word of per-pass flags (one dependent load before the keys);
+    // pass < 0: the MSD split, whose digit position the plan holds
+    const uint32_t flags = pass < 0 ? plan->msd_flags : plan->flags[pass];
     if (flags & PF_TRIVIAL) return;
 
     const uint32_t src = flags & PF_SRC_B;
@@ -404,7 +410,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     uint32_t *__restrict__ vout = src ? valsA : valsB;
     const bool xin = FLOATK && (flags & PF_XIN);
     const bool xout = FLOATK && (flags & PF_XOUT);
-    const int shift = pass * RADIX_BITS;
+    const int shift = pass < 0 ? int(plan->msd_shift) : pass * RADIX_BITS;
+    const uint32_t *__restrict__ gstart = pass < 0 ? plan->msd_start : plan->gstart[pass];
 
     // With precomputed offsets the tile order is free: give each XCD (work-
     // groups are dealt to the 8 XCDs round-robin) a contiguous range of tiles,
@@ -623,7 +630,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
         }
         if (tid < RADIX) {
             if (tile > 0) lb_store(&lb_cur[size_t(tile) * RADIX + tid], LB_INC | (excl + tile_cnt));
-            s_base[tid] = plan->gstart[pass][tid] + excl - tile_start;
+            s_base[tid] = gstart[tid] + excl - tile_start;
         }
     }
     __syncthreads();

@@ -28,7 +28,9 @@ struct TuneEntry {
 // dtype is u32, f32, i32 (4-byte keys), u64, f64, i64 (8-byte keys), pairs32
 // or pairs64 (keys with u32 values); algo 0 onesweep / 1 reduce-then-scan /
 // 3 one workgroup in LDS (arrays of at most 16384 4-byte or 8192 8-byte keys;
-// larger arrays fall back to the choice by size);
+// larger arrays fall back to the choice by size) / 4 one MSD split, then one
+// workgroup per bucket (keys without values, at most 2^22 4-byte or 2^21
+// 8-byte keys; larger arrays fall back to the choice by size);
 // shape 0-4 (1-3 for 4-byte keys without values only); rank 0 ballot / 1
 // LDS-OR; digit_bits 8 (default) or 11.  11-bit digits exist in one form
 // only: onesweep passes, 256x16 tiles, ballot ranking, keys without values,
@@ -67,10 +69,10 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
             err = "line " + std::to_string(line_no) + ": unknown dtype " + d;
             return false;
         }
-        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3) || rank < 0 || rank > 1 || shape < 0 ||
+        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3 && algo != 4) || rank < 0 || rank > 1 || shape < 0 ||
             shape >= TUNE_TILE_SHAPES ||
             (shape != 0 && shape != 4 && (e.key_bytes != 4 || e.vals)) || (bits != 8 && bits != 11) ||
-            (bits == 11 && (algo != 0 || shape != 0 || rank != 0 || e.vals))) {
+            (bits == 11 && (algo != 0 || shape != 0 || rank != 0 || e.vals)) || (algo == 4 && e.vals)) {
             err = "line " + std::to_string(line_no) + ": value out of range";
             return false;
         }

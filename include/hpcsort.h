@@ -61,7 +61,13 @@ extern "C" {
 #define HPCS_OPT_PASS_ALGO 5       /* 0: single-pass onesweep with look-back, 1: reduce-then-scan,
                                       2: by size (default: one workgroup up to 4096 4-byte / 8192 8-byte
                                       keys, onesweep for small arrays, reduce-then-scan beyond),
-                                      3: one workgroup sorts the array in LDS (larger arrays: by size) */
+                                      3: one workgroup sorts the array in LDS (larger arrays: by size),
+                                      4: one MSD split by the top eight varying bits, then one workgroup per
+                                      bucket (keys without values, up to 2^22 4-byte / 2^21 8-byte keys;
+                                      other sorts: by size).  Opt-in only: for keys spread over their range
+                                      it takes about half the time of 2: for 8-byte keys from 2^14 to 2^20,
+                                      but keys clustered in one bucket (a bucket larger than a workgroup's
+                                      tile is sorted by that one workgroup) cost 4-60x */
 #define HPCS_OPT_PERSISTENT 6      /* reduce-then-scan scatter: 0 one workgroup per tile (default), 1 persistent
                                       workgroups that prefetch the next tile */
 #define HPCS_OPT_DIGIT_BITS 8      /* 8 (default) or 11: 11-bit digits (3 passes for 4-byte keys, 6 for 8-byte
@@ -128,7 +134,8 @@ int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n);
  *   <dtype> <max_n> <algo> <shape> <rank> [<digit_bits>]
  * dtype u32|f32|i32|u64|f64|i64|pairs32|pairs64; a row applies to sorts of at
  * most max_n keys (the row with the smallest such max_n; beyond every row,
- * the largest); algo 0 onesweep, 1 reduce-then-scan, 3 one workgroup (see
+ * the largest); algo 0 onesweep, 1 reduce-then-scan, 3 one workgroup, 4 MSD
+ * split + one workgroup per bucket (not for pairs rows; see
  * HPCS_OPT_PASS_ALGO); shape 0-4
  * (HPCS_OPT_TILE_SHAPE; 1-3 for 4-byte keys without values only); rank 0
  * ballot, 1 LDS-OR; digit_bits 8 (default) or 11 (HPCS_OPT_DIGIT_BITS; an
