@@ -1,6 +1,7 @@
 """Parity over many option combinations: fixed-seed random cases, each a key
 type, a size (log-uniform, 2 .. 2^22), a distribution and explicitly set
-options (pass algorithm, tile shape, ranking, digit width).  The expected
+options (pass algorithm, including the MSD split from case 80, tile shape,
+ranking, digit width).  The expected
 result is numpy's sort (keys, in the order the reference's key transforms
 define) or its stable argsort (key-value pairs)."""
 from __future__ import annotations
@@ -54,12 +55,13 @@ def _float_order(x):
     return np.lexsort((sign == 0, x))
 
 
-@pytest.mark.parametrize("case", range(80))
+@pytest.mark.parametrize("case", range(120))
 def test_random_option_combinations(case):
     rng = np.random.default_rng(0xC0FFEE + case)
     kind = TYPES[case % len(TYPES)]
     n = int(np.exp(rng.uniform(np.log(2), np.log(1 << 22))))
-    algo = [2, 0, 1, 3][int(rng.integers(4))]
+    # cases from 80 also draw the MSD split (pass algorithm 4)
+    algo = [2, 0, 1, 3][int(rng.integers(4))] if case < 80 else [2, 0, 1, 3, 4][int(rng.integers(5))]
     rank = int(rng.integers(2))
     four = kind in ("u32", "f32", "i32")
     shape = int(rng.choice([0, 1, 2, 3, 4] if four else [0, 4]))
