@@ -937,6 +937,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
     for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
     __syncthreads();
     const uint32_t base = blockIdx.x * uint32_t(TILE);
+    if (base >= n) return; // a grid beyond the tiles (every launch sizes it to them): nothing to count
     const uint32_t cnt = min(uint32_t(TILE), n - base);
     if (dig != nullptr && !(flags & PF_FIRST)) {
         const uint32_t first = uint32_t(tid * ITEMS); // this thread's bytes [first, first + ITEMS)
@@ -1121,6 +1122,7 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
     for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
     __syncthreads();
     const uint32_t base = blockIdx.x * uint32_t(TILE);
+    if (base >= n) return; // a grid beyond the tiles (every launch sizes it to them): nothing to count
     const uint32_t cnt = min(uint32_t(TILE), n - base);
     K k[ITEMS];
     load_tile_keys<K, BLOCK, ITEMS>(keys + base, cnt, vec, k);
