@@ -144,6 +144,42 @@ def timed_calls(torch, fn, reps: int) -> float:
     return (time.perf_counter() - t0) / reps
 
 
+def msd_split_figures(torch, N, reps: int):
+    """The opt-in MSD split (HPCS_OPT_PASS_ALGO 4) beside the default (tuned)
+    sort on uniform keys, device-resident, event medians; outputs compared."""
+    lib = N.lib()
+    P = ctypes.c_void_p
+    cd = N.new_context(0)
+    cm = N.new_context(0)
+    out = {}
+    try:
+        for ctx in (cd, cm):
+            N.load_tuning(N.TUNE_PATH, ctx)
+        N.set_option(N.HPCS_OPT_PASS_ALGO, 4, cm)
+        s = torch.cuda.Stream()
+        for name, kb, fn in (("u32", 4, lib.hpcs_dev_sort_radix_u32), ("u64", 8, lib.hpcs_dev_sort_radix_u64)):
+            for lg in (16, 20):
+                n = 1 << lg
+                g = torch.Generator(device="cuda").manual_seed(lg + kb)
+                src = torch.randint(-2 ** 31, 2 ** 31 - 1, (n * kb // 4,), dtype=torch.int32, device="cuda",
+                                    generator=g)
+                res, outs = [], []
+                with torch.cuda.stream(s):
+                    for ctx in (cd, cm):
+                        work = src.clone()
+                        call = lambda: N.check(fn(ctx, P(work.data_ptr()), n, P(s.cuda_stream)))
+                        res.append(event_ms(torch, lambda: work.copy_(src), call, reps))
+                        outs.append(work)
+                s.synchronize()
+                out[f"{name}_2p{lg}"] = {"default_us": round(res[0] * 1e3, 1), "msd_us": round(res[1] * 1e3, 1),
+                                         "identical": bool(torch.equal(outs[0], outs[1]))}
+    finally:
+        N.destroy_context(cd)
+        N.destroy_context(cm)
+    out["note"] = "uniform keys; clustered keys make the MSD split several times slower, so it is opt-in"
+    return out
+
+
 def topk_figures(torch, N, datagen, reps: int):
     """Radix top-k, k = 10^4 of 2^26 keys (the serving workload's largest
     top-k requests), per key type; the u32 result is checked on the host."""
@@ -473,6 +509,9 @@ def main():
     if not args.no_topk:
         extra["topk"] = topk_figures(torch, N, datagen, 5)
         log(f"[bench] top-k: {extra['topk']}")
+    if not args.no_hist:
+        extra["msd_split"] = msd_split_figures(torch, N, 10)
+        log(f"[bench] MSD split (opt-in): {extra['msd_split']}")
     if not args.no_cfg3:
         extra["cfg3"] = cfg3_figures(torch, N, datagen, 3)
         if not args.no_cpu:
