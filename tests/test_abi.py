@@ -38,6 +38,18 @@ def test_library_exports_every_declared_symbol():
         assert getattr(lib, f) is not None
 
 
+def test_csharp_binding_imports_only_declared_entry_points():
+    """The C# binding given in INTEGRATION.md imports entry points by name:
+    each must be one that include/hpcsort.h declares."""
+    with open(os.path.join(REPO, "INTEGRATION.md")) as f:
+        integ = f.read()
+    with open(os.path.join(REPO, "include", "hpcsort.h")) as f:
+        declared = set(re.findall(r"\b(hpcs_\w+)\s*\(", f.read()))
+    imported = set(re.findall(r"partial (?:int|IntPtr) (hpcs_\w+)\(", integ))
+    assert imported, "no LibraryImport declarations found"
+    assert imported <= declared, sorted(imported - declared)
+
+
 def test_header_compiles_as_c():
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-x", "c", HEADER],
                        capture_output=True, text=True)
