@@ -575,7 +575,7 @@ template <typename K, int FLOATK>
 void launch_sort_buckets(int64_t n, hipStream_t st, K *a, K *b, const DevPlan *plan)
 {
     const int64_t want = n / RADIX * 5 / 2;
-    if (sizeof(K) == 4) {
+    if constexpr (sizeof(K) == 4) {
         if (want <= 4096)
             hipLaunchKernelGGL((k_sort_buckets<K, FLOATK, 256, 16>), dim3(RADIX), dim3(256), 0, st, a, b, uint32_t(n),
                                plan);

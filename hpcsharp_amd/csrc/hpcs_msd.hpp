@@ -151,6 +151,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_buckets(K *__restrict__ A, K *__
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int P = int(sizeof(K));
     __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint64_t s_match[WAVES][RADIX]; // LDS-OR match ranking
     __shared__ uint32_t s_scan[RADIX / WAVE];
     __shared__ uint32_t s_start[RADIX];
     __shared__ uint32_t s_off[RADIX];
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_buckets(K *__restrict__ A, K *__
                 }
                 __syncthreads();
             }
-            tile_rank_stage<K, BLOCK, ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_scan, s_start, s_keys);
+            tile_rank_stage_ldsor<K, BLOCK, ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_match, s_scan, s_start, s_keys);
             staged = true;
         }
         if (staged) {
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_buckets(K *__restrict__ A, K *__
                 const int o = j * WAVE + lane;
                 u[j] = o < wave_valid ? src[t0 + uint32_t(wave * WAVE_TILE + o)] : ~K(0);
             }
-            tile_rank_stage<K, BLOCK, ITEMS>(u, wave_valid, shift, s_cnt, s_scan, s_start, s_keys);
+            tile_rank_stage_ldsor<K, BLOCK, ITEMS>(u, wave_valid, shift, s_cnt, s_match, s_scan, s_start, s_keys);
             for (uint32_t j = tid; j < tl; j += BLOCK) {
                 const K k = s_keys[j];
                 const uint32_t d = KeyOps<K>::digit(k, shift);

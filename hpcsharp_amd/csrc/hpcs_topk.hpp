@@ -430,7 +430,11 @@ __global__ __launch_bounds__(BLOCK) void k_topk_radix_sort(const K *__restrict__
     constexpr int WAVE_TILE = WAVE * ITEMS;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int P = int(sizeof(K));
+    // LDS-OR match ranking where its masks fit beside the keys (u64 keys in
+    // 1024-thread tiles would need 176 KiB): faster than ballot ranking
+    constexpr bool LDSOR = size_t(TILE) * sizeof(K) + size_t(WAVES) * RADIX * 12 <= 144 * 1024;
     __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ uint64_t s_match[LDSOR ? WAVES : 1][LDSOR ? RADIX : 1];
     __shared__ uint32_t s_scan[RADIX / WAVE];
     __shared__ uint32_t s_start[RADIX];
     __shared__ K s_keys[TILE];
@@ -488,7 +492,10 @@ __global__ __launch_bounds__(BLOCK) void k_topk_radix_sort(const K *__restrict__
             }
             __syncthreads();
         }
-        tile_rank_stage<K, BLOCK, ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_scan, s_start, s_keys);
+        if constexpr (LDSOR)
+            tile_rank_stage_ldsor<K, BLOCK, ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_match, s_scan, s_start, s_keys);
+        else
+            tile_rank_stage<K, BLOCK, ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_scan, s_start, s_keys);
         staged = true;
     }
     if (!staged) { // every key equal: any order is sorted
