@@ -483,11 +483,11 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
     const int64_t block_max = key_bytes == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8;
     if (p.algo == PASS_BLOCK && n > block_max) p.algo = PASS_AUTO;
     if (p.algo == PASS_MSD && (vals || n > (key_bytes == 4 ? MSD_MAX4 : MSD_MAX8))) p.algo = PASS_AUTO;
-    // by size, measured on MI355X: one workgroup up to 4096 4-byte / 8192
-    // 8-byte keys (beyond 4096 4-byte keys the 1024-thread workgroup on one
-    // CU loses to onesweep passes over many CUs)
+    // by size, measured on MI355X: one workgroup up to 16384 4-byte / 8192
+    // 8-byte keys (with LDS-OR ranking it beats onesweep passes there: u32
+    // 16384 keys 42.2 against 48.4 us)
     if (p.algo == PASS_AUTO)
-        p.algo = n <= (key_bytes == 4 ? 4096 : BLOCK_SORT_MAX8) ? PASS_BLOCK
+        p.algo = n <= (key_bytes == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8) ? PASS_BLOCK
                  : n < (key_bytes == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21)) ? PASS_ONESWEEP
                                                                                    : PASS_RTS;
     // 11-bit digits exist for onesweep passes over keys without values (their

@@ -59,7 +59,7 @@ extern "C" {
                                       (default 24; 0 makes every wait use the local count, for tests) */
 #define HPCS_OPT_LOOKBACK_WINDOW 4 /* u32 keys: earlier tiles read per look-back round: 16, 32 (default), 64 */
 #define HPCS_OPT_PASS_ALGO 5       /* 0: single-pass onesweep with look-back, 1: reduce-then-scan,
-                                      2: by size (default: one workgroup up to 4096 4-byte / 8192 8-byte
+                                      2: by size (default: one workgroup up to 16384 4-byte / 8192 8-byte
                                       keys, onesweep for small arrays, reduce-then-scan beyond),
                                       3: one workgroup sorts the array in LDS (larger arrays: by size),
                                       4: one MSD split by the top eight varying bits, then one workgroup per

@@ -35,16 +35,15 @@ def _sort(ctx, a: np.ndarray) -> None:
     N.check(getattr(N.lib(), fn)(ctx, a.ctypes.data_as(P), a.size))
 
 
-def test_plan_of_small_arrays(ctxb, request):
+def test_plan_of_small_arrays(ctxb):
     info = N.PlanInfo()
-    explicit = request.node.callspec.params["ctxb"] == "explicit"
     N.check(N.lib().hpcs_plan_query(ctxb, N.HPCS_DTYPE_U32, 4096, 0, ctypes.byref(info)))
     assert (info.tiles, info.block_threads, info.tile_keys, info.scratch_bytes) == (1, 256, 4096, 0)
     N.check(N.lib().hpcs_plan_query(ctxb, N.HPCS_DTYPE_U32, 16384, 0, ctypes.byref(info)))
-    if explicit:   # the one-workgroup form up to 16384 4-byte keys when asked for
-        assert (info.tiles, info.block_threads, info.tile_keys, info.scratch_bytes) == (1, 1024, 16384, 0)
-    else:          # by size: onesweep tiles beyond 4096 4-byte keys
-        assert info.tiles > 1
+    # the one-workgroup form up to 16384 4-byte keys, asked for or by size
+    assert (info.tiles, info.block_threads, info.tile_keys, info.scratch_bytes) == (1, 1024, 16384, 0)
+    N.check(N.lib().hpcs_plan_query(ctxb, N.HPCS_DTYPE_U32, 16385, 0, ctypes.byref(info)))
+    assert info.tiles > 1
     N.check(N.lib().hpcs_plan_query(ctxb, N.HPCS_DTYPE_U64, 2048, 1, ctypes.byref(info)))
     assert (info.tiles, info.block_threads, info.tile_keys) == (1, 256, 2048)
     N.check(N.lib().hpcs_plan_query(ctxb, N.HPCS_DTYPE_U64, 8193, 0, ctypes.byref(info)))
