@@ -100,13 +100,14 @@ __device__ inline void tile_rank_stage(const K (&keys)[ITEMS], int wave_valid, i
     __syncthreads();
 }
 
-// As tile_rank_stage (keys only), ranking with LDS-OR match masks: per-wave
+// As tile_rank_stage, ranking with LDS-OR match masks: per-wave
 // counts by LDS atomics first, then each lane ORs its bit into its digit's
 // mask, reads the mask back and stages its key at base + lanes below it.
-template <typename K, int BLOCK, int ITEMS>
+template <typename K, int BLOCK, int ITEMS, bool HAS_VALS = false>
 __device__ inline void tile_rank_stage_ldsor(const K (&keys)[ITEMS], int wave_valid, int shift,
                                              uint32_t (*s_cnt)[RADIX], uint64_t (*s_match)[RADIX],
-                                             uint32_t *s_scan, uint32_t *s_start, K *s_keys)
+                                             uint32_t *s_scan, uint32_t *s_start, K *s_keys,
+                                             const uint32_t *vals = nullptr, uint32_t *s_vals = nullptr)
 {
     constexpr int WAVES = BLOCK / WAVE;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
@@ -161,7 +162,10 @@ __device__ inline void tile_rank_stage_ldsor(const K (&keys)[ITEMS], int wave_va
             cnt[d] = old + uint32_t(__popcll(peers));
             s_match[wave][d] = 0ull;
         }
-        if (valid) s_keys[old + below] = keys[j];
+        if (valid) {
+            s_keys[old + below] = keys[j];
+            if constexpr (HAS_VALS) s_vals[old + below] = vals[j];
+        }
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
@@ -482,10 +486,11 @@ __global__ __launch_bounds__(BLOCK) void k_sort_block(K *__restrict__ keys, uint
     constexpr int WAVE_TILE = WAVE * ITEMS;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int P = int(sizeof(K));
-    // keys only: LDS-OR match ranking (measured on MI355X against ballot
-    // ranking: u32 4096 keys 25.4 -> 22.4 us, 16384 48.8 -> 42.4 us, u64 8192
-    // 53.5 -> 48.7 us); with values the ballot ranking carries them
-    constexpr bool LDSOR = !HAS_VALS;
+    // LDS-OR match ranking where its masks fit in LDS beside the keys and
+    // values (measured on MI355X against ballot ranking: u32 4096 keys 25.4
+    // -> 22.4 us, 16384 48.8 -> 42.4 us, u64 8192 53.5 -> 48.7 us)
+    constexpr bool LDSOR = size_t(TILE) * (sizeof(K) + (HAS_VALS ? 4 : 0)) + size_t(WAVES) * RADIX * 12 <=
+                           size_t(150) * 1024;
     __shared__ uint32_t s_cnt[WAVES][RADIX];
     __shared__ uint64_t s_match[LDSOR ? WAVES : 1][LDSOR ? RADIX : 1];
     __shared__ uint32_t s_scan[RADIX / WAVE];
@@ -541,7 +546,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_block(K *__restrict__ keys, uint
             __syncthreads();
         }
         if constexpr (LDSOR)
-            tile_rank_stage_ldsor<K, BLOCK, ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_match, s_scan, s_start, s_keys);
+            tile_rank_stage_ldsor<K, BLOCK, ITEMS, HAS_VALS>(v, wave_valid, 8 * pass, s_cnt, s_match, s_scan, s_start,
+                                                             s_keys, pv, s_vals);
         else
             tile_rank_stage<K, BLOCK, ITEMS, HAS_VALS>(v, wave_valid, 8 * pass, s_cnt, s_scan, s_start, s_keys, pv,
                                                        s_vals);
