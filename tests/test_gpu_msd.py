@@ -127,3 +127,15 @@ def test_profile_names_msd_kernels(mctx):
         N.profile_enable(False, mctx)
     assert {"msd_or_and", "msd_hist", "msd_split", "msd_buckets"} <= set(prof)
     assert "onesweep" not in prof and "scatter" not in prof
+
+
+def test_plan_query_reports_the_split(mctx):
+    info = N.PlanInfo()
+    n = 1 << 20
+    N.check(N.lib().hpcs_plan_query(mctx, N.HPCS_DTYPE_U32, n, 0, ctypes.byref(info)))
+    assert (info.digit_bits, info.passes, info.ideal_bytes) == (8, 2, 6 * 4 * n)
+    # key-value sorts and arrays beyond the split's range take the by-size plan
+    N.check(N.lib().hpcs_plan_query(mctx, N.HPCS_DTYPE_U32, n, 1, ctypes.byref(info)))
+    assert info.passes == 4
+    N.check(N.lib().hpcs_plan_query(mctx, N.HPCS_DTYPE_U64, 1 << 22, 0, ctypes.byref(info)))
+    assert info.passes == 8
