@@ -143,8 +143,8 @@ int main(int argc, char **argv)
     hipStream_t st;
     CHECK(hipStreamCreate(&st));
     int rc = 0;
-    for (int lg : {16, 20, 24, 28}) rc |= run<uint32_t>(ctx, st, lg);
-    for (int lg : {16, 20, 24, 28}) rc |= run<uint64_t>(ctx, st, lg);
+    for (int lg : {12, 14, 16, 20, 24, 28}) rc |= run<uint32_t>(ctx, st, lg);
+    for (int lg : {12, 14, 16, 20, 24, 28}) rc |= run<uint64_t>(ctx, st, lg);
     CHECK(hipStreamDestroy(st));
     CHECK_LIB(hpcs_ctx_destroy(ctx));
     return rc;
