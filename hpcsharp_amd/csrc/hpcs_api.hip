@@ -8,6 +8,7 @@
 // pass; then a copy-back.  No host synchronisation happens inside a
 // device-resident call, so the sequence can be captured in a HIP graph.
 #include "hpcs_kernels.hpp"
+#include "hpcs_lines.hpp"
 #include "hpcs_digit11.hpp"
 #include "hpcs_msd.hpp"
 #include "hpcs_segments.hpp"
@@ -121,6 +122,7 @@ struct hpcs_ctx {
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
     int pass_algo = PASS_AUTO;    // PASS_ONESWEEP, PASS_RTS (reduce-then-scan) or PASS_AUTO (by size)
     int persistent = 0;           // reduce-then-scan scatter: persistent workgroups with prefetch
+    int line_chunk = 0;           // reduce-then-scan scatter: tiles per line-carrying workgroup (0: off)
     int digit_bytes = 2;          // reduce-then-scan: tile counts from next-digit bytes written by the
                                   // scatter: 0 off, 1 on, 2 by type (on for 8-byte keys with values)
     // tuning table (hpcs_ctx_load_tuning / HPCS_TUNE_FILE); options set
@@ -409,6 +411,19 @@ void launch_scatter_r(int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uin
                        vb, n, pass, plan, offsets, tiles);
 }
 
+// The line-carrying scatter (4-byte keys without values, 256x16 tiles):
+// one workgroup per `chunk` consecutive tiles.
+template <typename K, bool HAS_VALS, int FLOATK>
+void launch_scatter_lines(uint32_t chunk, hipStream_t st, K *a, K *b, uint32_t n, int pass, DevPlan *plan,
+                          const uint32_t *offsets, uint32_t tiles)
+{
+    if constexpr (sizeof(K) == 4 && !HAS_VALS)
+        hipLaunchKernelGGL((k_scatter_lines<K, FLOATK, 256, 16>), dim3((tiles + chunk - 1) / chunk), dim3(256), 0,
+                           st, a, b, n, pass, plan, offsets, tiles, chunk);
+    else
+        fail(HPCS_E_UNSUPPORTED, "the line-carrying scatter takes 4-byte keys without values");
+}
+
 template <typename K, bool HAS_VALS, int FLOATK>
 void launch_scatter(int rank_mode, int shape, int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb,
                     uint32_t n, int pass, DevPlan *plan, const uint32_t *offsets, uint32_t tiles)
@@ -675,6 +690,9 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     // beside u64 keys the sort gains 5%.
     const bool want_dig = c->digit_bytes == 1 || (c->digit_bytes == 2 && sizeof(K) == 8 && HAS_VALS);
     uint8_t *dig = (want_dig && !persistent) ? reinterpret_cast<uint8_t *>(base + L.off_dig) : nullptr;
+    // line-carrying scatter: 4-byte keys without values on 256x16 tiles (not
+    // with the persistent scatter or next-digit bytes)
+    const bool lines = sizeof(K) == 4 && !HAS_VALS && c->line_chunk > 0 && shape == 0 && !persistent && !dig;
     if (algo == PASS_RTS && tiles <= SCAN_DIGIT_MAX_TILES) {
         // few tiles: launches, not bytes, set the time.  One histogram of
         // every digit gives the plan (digit starts, trivial passes); per
@@ -694,9 +712,13 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
                 hipLaunchKernelGGL(k_scan_digit, dim3(RADIX), dim3(1024), 0, st, counts, tiles, p, plan);
             });
             timed(c, st, "scatter", [&] {
-                launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
-                                                     d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
-                                                     nullptr, nullptr);
+                if (lines)
+                    launch_scatter_lines<K, HAS_VALS, FLOATK>(uint32_t(c->line_chunk), st, d_keys, keysB,
+                                                              uint32_t(n), p, plan, counts, tiles);
+                else
+                    launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
+                                                         d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
+                                                         nullptr, nullptr);
             });
         }
         timed(c, st, "copyback", [&] {
@@ -742,7 +764,10 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
             });
             timed(c, st, "scatter", [&] {
                 // the persistent kernel keeps two tiles in registers: not for 512x32 tiles
-                if (persistent)
+                if (lines)
+                    launch_scatter_lines<K, HAS_VALS, FLOATK>(uint32_t(c->line_chunk), st, d_keys, keysB,
+                                                              uint32_t(n), p, plan, counts, tiles);
+                else if (persistent)
                     launch_scatter<K, HAS_VALS, FLOATK>(rank_mode, shape, c->num_cus, st, d_keys, keysB, d_vals,
                                                         valsB, uint32_t(n), p, plan, counts, tiles);
                 else
@@ -1464,6 +1489,10 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
         case HPCS_OPT_DIGIT_BYTES:
             if (value < 0 || value > 2) fail(HPCS_E_INVALID_ARG, "digit bytes must be 0, 1 or 2");
             ctx->digit_bytes = int(value);
+            break;
+        case HPCS_OPT_LINE_CHUNK:
+            if (value < 0 || value > 256) fail(HPCS_E_INVALID_ARG, "line chunk must be 0..256");
+            ctx->line_chunk = int(value);
             break;
         case HPCS_OPT_PERSISTENT:
             if (value != 0 && value != 1) fail(HPCS_E_INVALID_ARG, "persistent must be 0 or 1");

@@ -77,6 +77,12 @@ extern "C" {
                                       byte and the next pass counts its tiles from those n bytes; 0 the next
                                       pass re-reads the keys; 2 (default) 1 for 8-byte keys with values, else 0
                                       (scratch holds the n bytes either way) */
+#define HPCS_OPT_LINE_CHUNK 10     /* reduce-then-scan scatter of 4-byte keys without values, 256x16 tiles:
+                                      0 (default) one workgroup per tile; k in 1..256: a workgroup scatters k
+                                      consecutive tiles and writes each digit's output in whole 128-byte lines,
+                                      carrying in LDS the keys that complete no line to the next tile
+                                      (measured on MI355X: 2.5x slower than 0 at 2^28 keys, since its LDS
+                                      allows 2 workgroups per CU instead of 5) */
 #define HPCS_OPT_GRAPHS 9          /* 1: graph cache for the device-resident sorts (hpcs_dev_sort_radix_*,
                                       hpcs_dev_sort_segments_u32) on a non-null stream: the first call with
                                       given pointers, size and stream runs directly, the second captures its
