@@ -759,8 +759,10 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
                 hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals, dtot_p,
                                    p, plan);
                 hipLaunchKernelGGL(k_scan_columns, dim3(RADIX), dim3(1024), 0, st, totals, nranges, dtot_p, p, plan);
-                hipLaunchKernelGGL(k_range_offsets, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals, p,
-                                   plan);
+                // the default scatter adds the range bases itself
+                if (persistent || lines)
+                    hipLaunchKernelGGL(k_range_offsets, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals,
+                                       p, plan);
             });
             timed(c, st, "scatter", [&] {
                 // the persistent kernel keeps two tiles in registers: not for 512x32 tiles
@@ -773,7 +775,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
                 else
                     launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
                                                          d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
-                                                         nullptr, dig);
+                                                         totals, dig);
             });
             continue;
         }

@@ -358,6 +358,8 @@ __global__ __launch_bounds__(RADIX) void k_plan(const uint32_t *__restrict__ his
 //                commutative, so the mask does not depend on the order in
 //                which the LDS unit applies the lanes' updates.
 // ---------------------------------------------------------------------------
+// Tiles per range of the reduce-then-scan offset scan (k_range_totals).
+constexpr int SCAN_RANGE = 64;
 constexpr int RANK_BALLOT = 0;
 constexpr int RANK_LDSOR = 1;
 // Default look-back rounds without progress after which a block counts the
@@ -396,7 +398,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
 
     // Clear the next pass's look-back words for tile blockIdx.x (the grid has
     // exactly one block per tile); that region's previous pass has finished.
-    if (lb_next != nullptr && tid < RADIX) lb_next[size_t(blockIdx.x) * RADIX + tid] = 0u;
+    if (!PRESCANNED && lb_next != nullptr && tid < RADIX) lb_next[size_t(blockIdx.x) * RADIX + tid] = 0u;
 
     // one packed word of per-pass flags (one dependent load before the keys);
     // pass < 0: the MSD split, whose digit position the plan holds
@@ -441,8 +443,15 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
         }
     }
     uint32_t pre_off = 0;
-    if constexpr (PRESCANNED)
-        if (tid < RADIX) pre_off = lb_cur[size_t(tile) * RADIX + tid];
+    if constexpr (PRESCANNED) {
+        // lb_cur: global offsets, or (lb_next != nullptr) offsets within the
+        // tile's range of SCAN_RANGE tiles, lb_next then holding the ranges'
+        // global bases
+        if (tid < RADIX) {
+            pre_off = lb_cur[size_t(tile) * RADIX + tid];
+            if (lb_next != nullptr) pre_off += lb_next[size_t(tile / SCAN_RANGE) * RADIX + tid];
+        }
+    }
 
     for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
     if constexpr (RANK == RANK_LDSOR)
@@ -863,15 +872,16 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(K *__restrict__ keysA, K *__r
 // ---------------------------------------------------------------------------
 // Reduce-then-scan form of a pass (no waits between workgroups):
 //   k_tile_counts    per-tile digit counts, [tile][digit]
-//   k_range_totals   per range of SCAN_RANGE tiles, the sum of its counts
-//   k_scan_ranges    exclusive scan of the range sums over ranges, seeded with
-//                    the digit's global start
-//   k_range_offsets  per tile, the global offset of each digit's run
-// followed by k_onesweep<..., LOOKBACK_WINDOW = 0> reading those offsets.
+//   k_range_totals   per range of SCAN_RANGE tiles, the sum of its counts,
+//                    and each tile's offset within the range over its counts
+//   k_scan_columns   exclusive scan of the range sums over ranges, seeded with
+//                    the digit's global start: the ranges' global bases
+// followed by k_onesweep<..., LOOKBACK_WINDOW = 0> adding its range's base to
+// its in-range offsets (k_range_offsets adds the bases in place instead,
+// for the persistent and the line-carrying scatters).
 // It reads the keys twice per pass (12 B per u32 key instead of 8) and has
 // no chain of dependent loads between tiles.
 // ---------------------------------------------------------------------------
-constexpr int SCAN_RANGE = 64;
 
 // Keys of one tile for counting (order does not matter): with `vec` (the
 // buffer is 16-byte aligned) a full tile is read in 16-byte vectors, thread t
@@ -981,9 +991,10 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
 // one 128-byte line each, so the atomics do not queue behind each other.
 constexpr int LINE_WORDS = 32;
 
-// totals[r][d] = sum of counts[t][d] over the tiles of range r, and the
-// pass's digit totals dtot[d * LINE_WORDS] += totals[r][d].
-__global__ __launch_bounds__(RADIX) void k_range_totals(const uint32_t *__restrict__ counts, uint32_t tiles,
+// totals[r][d] = sum of counts[t][d] over the tiles of range r, each count
+// replaced by the sum of the range's earlier counts (the tile's offset within
+// the range), and the pass's digit totals dtot[d * LINE_WORDS] += totals[r][d].
+__global__ __launch_bounds__(RADIX) void k_range_totals(uint32_t *__restrict__ counts, uint32_t tiles,
                                                         uint32_t *__restrict__ totals, uint32_t *__restrict__ dtot,
                                                         int pass, const DevPlan *__restrict__ plan)
 {
@@ -993,9 +1004,13 @@ __global__ __launch_bounds__(RADIX) void k_range_totals(const uint32_t *__restri
     uint32_t c[SCAN_RANGE];
 #pragma unroll
     for (int k = 0; k < SCAN_RANGE; ++k) c[k] = t0 + k < t1 ? counts[size_t(t0 + k) * RADIX + d] : 0u;
+    // each tile's offset within the range, written over its count
     uint32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_RANGE; ++k) sum += c[k];
+    for (int k = 0; k < SCAN_RANGE; ++k) {
+        if (t0 + k < t1) counts[size_t(t0 + k) * RADIX + d] = sum;
+        sum += c[k];
+    }
     totals[size_t(r) * RADIX + d] = sum;
     if (sum) atomicAdd(&dtot[d * LINE_WORDS], sum);
 }
@@ -1043,8 +1058,8 @@ __global__ __launch_bounds__(1024) void k_scan_columns(uint32_t *__restrict__ to
     }
 }
 
-// Per range r (thread = digit): the tiles' global digit offsets, written over
-// their counts.
+// Per range r (thread = digit): the tiles' global digit offsets, the range's
+// base added to their in-range offsets in place.
 __global__ __launch_bounds__(RADIX) void k_range_offsets(uint32_t *__restrict__ counts, uint32_t tiles,
                                                          const uint32_t *__restrict__ totals, int pass,
                                                          const DevPlan *__restrict__ plan)
@@ -1052,15 +1067,10 @@ __global__ __launch_bounds__(RADIX) void k_range_offsets(uint32_t *__restrict__ 
     if (plan->flags[pass] & PF_TRIVIAL) return;
     const uint32_t r = blockIdx.x, d = threadIdx.x;
     const uint32_t t0 = r * SCAN_RANGE, t1 = min(tiles, t0 + SCAN_RANGE);
-    uint32_t c[SCAN_RANGE];
+    const uint32_t base = totals[size_t(r) * RADIX + d];
 #pragma unroll
-    for (int k = 0; k < SCAN_RANGE; ++k) c[k] = t0 + k < t1 ? counts[size_t(t0 + k) * RADIX + d] : 0u;
-    uint32_t run = totals[size_t(r) * RADIX + d];
-#pragma unroll
-    for (int k = 0; k < SCAN_RANGE; ++k) {
-        if (t0 + k < t1) counts[size_t(t0 + k) * RADIX + d] = run;
-        run += c[k];
-    }
+    for (int k = 0; k < SCAN_RANGE; ++k)
+        if (t0 + k < t1) counts[size_t(t0 + k) * RADIX + d] += base;
 }
 
 // The three scan kernels in one, for sorts of few tiles (the plan's digit
