@@ -31,6 +31,7 @@ HPCS_OPT_DIGIT_BYTES = 7
 HPCS_OPT_DIGIT_BITS = 8
 HPCS_OPT_GRAPHS = 9
 HPCS_OPT_LINE_CHUNK = 10
+HPCS_OPT_COUNT_TILES = 11
 HPCS_OP_SORT, HPCS_OP_TOPK = 0, 1
 HPCS_SERVE_REALTIME, HPCS_SERVE_ALONE = 0, 1
 

@@ -123,6 +123,7 @@ struct hpcs_ctx {
     int pass_algo = PASS_AUTO;    // PASS_ONESWEEP, PASS_RTS (reduce-then-scan) or PASS_AUTO (by size)
     int persistent = 0;           // reduce-then-scan scatter: persistent workgroups with prefetch
     int line_chunk = 0;           // reduce-then-scan scatter: tiles per line-carrying workgroup (0: off)
+    int count_tiles = 1;          // reduce-then-scan tile counts: tiles per workgroup (1, 2 or 4)
     int digit_bytes = 2;          // reduce-then-scan: tile counts from next-digit bytes written by the
                                   // scatter: 0 off, 1 on, 2 by type (on for 8-byte keys with values)
     // tuning table (hpcs_ctx_load_tuning / HPCS_TUNE_FILE); options set
@@ -369,11 +370,18 @@ bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 
 
 template <typename K, int FLOATK>
 void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const K *b, uint32_t n, int pass,
-                        const DevPlan *plan, uint32_t *counts, const uint8_t *dig)
+                        const DevPlan *plan, uint32_t *counts, const uint8_t *dig, int tpw = 1)
 {
     const int block = TILE_SHAPES[shape].block, items = TILE_SHAPES[shape].items;
     const int vec = aligned16(a) && aligned16(b);
-    if (block == 256 && items == 16)
+    const uint32_t tiles = grid.x;
+    if (block == 256 && items == 16 && dig == nullptr && tpw == 2)
+        hipLaunchKernelGGL((k_tile_counts_multi<K, FLOATK, 256, 16, 2>), dim3((tiles + 1) / 2), dim3(256), 0, st, a,
+                           b, n, pass, plan, counts, vec, tiles);
+    else if (sizeof(K) == 4 && block == 256 && items == 16 && dig == nullptr && tpw == 4)
+        hipLaunchKernelGGL((k_tile_counts_multi<K, FLOATK, 256, 16, 4>), dim3((tiles + 3) / 4), dim3(256), 0, st, a,
+                           b, n, pass, plan, counts, vec, tiles);
+    else if (block == 256 && items == 16)
         hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
                            vec, dig);
     else if (block == 512 && items == 16)
@@ -706,7 +714,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         for (int p = 0; p < P; ++p) {
             timed(c, st, "tile_counts", [&] {
                 launch_tile_counts<K, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan, counts,
-                                              nullptr);
+                                              nullptr, c->count_tiles);
             });
             timed(c, st, "scan", [&] {
                 hipLaunchKernelGGL(k_scan_digit, dim3(RADIX), dim3(1024), 0, st, counts, tiles, p, plan);
@@ -752,7 +760,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
             if (p > 0)
                 timed(c, st, "tile_counts", [&] {
                     launch_tile_counts<K, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan,
-                                                  counts, dig);
+                                                  counts, dig, c->count_tiles);
                 });
             timed(c, st, "scan", [&] {
                 uint32_t *dtot_p = dtot + size_t(p) * RADIX * LINE_WORDS;
@@ -1491,6 +1499,10 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
         case HPCS_OPT_DIGIT_BYTES:
             if (value < 0 || value > 2) fail(HPCS_E_INVALID_ARG, "digit bytes must be 0, 1 or 2");
             ctx->digit_bytes = int(value);
+            break;
+        case HPCS_OPT_COUNT_TILES:
+            if (value != 1 && value != 2 && value != 4) fail(HPCS_E_INVALID_ARG, "count tiles must be 1, 2 or 4");
+            ctx->count_tiles = int(value);
             break;
         case HPCS_OPT_LINE_CHUNK:
             if (value < 0 || value > 256) fail(HPCS_E_INVALID_ARG, "line chunk must be 0..256");

@@ -944,12 +944,28 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
     const bool xin = FLOATK && (flags & PF_XIN);
     const int shift = pass * RADIX_BITS;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
-    for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
-    __syncthreads();
     const uint32_t base = blockIdx.x * uint32_t(TILE);
     if (base >= n) return; // a grid beyond the tiles (every launch sizes it to them): nothing to count
     const uint32_t cnt = min(uint32_t(TILE), n - base);
-    if (dig != nullptr && !(flags & PF_FIRST)) {
+    if (dig == nullptr || (flags & PF_FIRST)) {
+        // keys: the loads go out first, so their latency covers the LDS set-up
+        K k[ITEMS];
+        load_tile_keys<K, BLOCK, ITEMS>(kin + base, cnt, vec, k);
+        for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint32_t j = uint32_t(i * BLOCK + tid);
+            const bool valid = j < cnt;
+            count_digit(sh, KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[i]) : k[i], shift), valid, lane);
+        }
+        __syncthreads();
+        for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
+        return;
+    }
+    for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
+    __syncthreads();
+    {
         const uint32_t first = uint32_t(tid * ITEMS); // this thread's bytes [first, first + ITEMS)
         uint32_t w[ITEMS / 4];
         if (cnt == uint32_t(TILE)) {
@@ -973,18 +989,51 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
             count_digit(sh, (w[i / 4] >> (8 * (i % 4))) & 0xFFu, first + uint32_t(i) < cnt, lane);
         __syncthreads();
         for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
-        return;
     }
-    K k[ITEMS];
-    load_tile_keys<K, BLOCK, ITEMS>(kin + base, cnt, vec, k);
+}
+
+// k_tile_counts for TPW adjacent tiles per workgroup (keys, not digit
+// bytes): the loads of all its tiles are issued before any is counted, so a
+// workgroup keeps TPW tiles' bytes in flight while it waits, and each tile
+// is counted into its own LDS row.  Same output, counts[tile][digit].
+template <typename K, int FLOATK, int BLOCK, int ITEMS, int TPW>
+__global__ __launch_bounds__(BLOCK) void k_tile_counts_multi(const K *__restrict__ keysA, const K *__restrict__ keysB,
+                                                             uint32_t n, int pass, const DevPlan *__restrict__ plan,
+                                                             uint32_t *__restrict__ counts, int vec, uint32_t tiles)
+{
+    constexpr int TILE = BLOCK * ITEMS;
+    __shared__ uint32_t sh[TPW][RADIX];
+    const uint32_t flags = plan->flags[pass];
+    if (flags & PF_TRIVIAL) return;
+    const K *__restrict__ kin = (flags & PF_SRC_B) ? keysB : keysA;
+    const bool xin = FLOATK && (flags & PF_XIN);
+    const int shift = pass * RADIX_BITS;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const uint32_t t0 = blockIdx.x * uint32_t(TPW);
+    if (t0 >= tiles) return; // the whole workgroup
+    K k[TPW][ITEMS];
+    uint32_t cnt[TPW];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const uint32_t j = uint32_t(i * BLOCK + tid);
-        const bool valid = j < cnt;
-        count_digit(sh, KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[i]) : k[i], shift), valid, lane);
+    for (int j = 0; j < TPW; ++j) {
+        const uint32_t base = (t0 + j) * uint32_t(TILE); // below n for every tile < tiles
+        cnt[j] = t0 + j < tiles ? min(uint32_t(TILE), n - base) : 0u;
+        if (cnt[j]) load_tile_keys<K, BLOCK, ITEMS>(kin + base, cnt[j], vec, k[j]);
+    }
+    for (int i = tid; i < TPW * RADIX; i += BLOCK) (&sh[0][0])[i] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const bool valid = uint32_t(i * BLOCK + tid) < cnt[j];
+            count_digit(sh[j], KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[j][i]) : k[j][i], shift), valid, lane);
+        }
     }
     __syncthreads();
-    for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j)
+        if (t0 + j < tiles)
+            for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(t0 + j) * RADIX + i] = sh[j][i];
 }
 
 // Spacing of per-digit accumulators that take atomics from many workgroups:
@@ -1129,13 +1178,14 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
     __shared__ uint32_t sh[RADIX];
     __shared__ K s_or[WAVES], s_nor[WAVES];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
-    for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
-    __syncthreads();
     const uint32_t base = blockIdx.x * uint32_t(TILE);
     if (base >= n) return; // a grid beyond the tiles (every launch sizes it to them): nothing to count
     const uint32_t cnt = min(uint32_t(TILE), n - base);
+    // the loads go out first, so their latency covers the LDS set-up
     K k[ITEMS];
     load_tile_keys<K, BLOCK, ITEMS>(keys + base, cnt, vec, k);
+    for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
+    __syncthreads();
     K vor = 0, vnor = 0;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {

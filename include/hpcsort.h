@@ -83,6 +83,9 @@ extern "C" {
                                       carrying in LDS the keys that complete no line to the next tile
                                       (measured on MI355X: 2.5x slower than 0 at 2^28 keys, since its LDS
                                       allows 2 workgroups per CU instead of 5) */
+#define HPCS_OPT_COUNT_TILES 11    /* reduce-then-scan tile counts on 256x16 tiles (keys, not digit bytes):
+                                      tiles per counting workgroup, 1 (default), 2 or 4 (4: 4-byte keys; 8-byte
+                                      keys count 1 tile per workgroup there) */
 #define HPCS_OPT_GRAPHS 9          /* 1: graph cache for the device-resident sorts (hpcs_dev_sort_radix_*,
                                       hpcs_dev_sort_segments_u32) on a non-null stream: the first call with
                                       given pointers, size and stream runs directly, the second captures its
