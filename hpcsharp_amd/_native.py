@@ -26,12 +26,9 @@ HPCS_KEY_FLOAT = 1
 HPCS_KEY_INT = 2
 HPCS_DTYPE_U32, HPCS_DTYPE_U64, HPCS_DTYPE_F32, HPCS_DTYPE_F64 = 0, 1, 2, 3
 HPCS_OPT_RANK, HPCS_OPT_TILE_SHAPE, HPCS_OPT_FALLBACK_ROUNDS, HPCS_OPT_LOOKBACK_WINDOW, HPCS_OPT_PASS_ALGO = 1, 2, 3, 4, 5
-HPCS_OPT_PERSISTENT = 6
 HPCS_OPT_DIGIT_BYTES = 7
 HPCS_OPT_DIGIT_BITS = 8
 HPCS_OPT_GRAPHS = 9
-HPCS_OPT_LINE_CHUNK = 10
-HPCS_OPT_COUNT_TILES = 11
 HPCS_OP_SORT, HPCS_OP_TOPK = 0, 1
 HPCS_SERVE_REALTIME, HPCS_SERVE_ALONE = 0, 1
 
@@ -62,6 +59,7 @@ SIGNATURES = {
     "hpcs_ctx_destroy": ([_p], _i32),
     "hpcs_ctx_trim": ([_p], _i32),
     "hpcs_plan_query": ([_p, _i32, _i64, _i32, ctypes.POINTER(PlanInfo)], _i32),
+    "hpcs_tile_shape_query": ([_i32, _i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)], _i32),
     "hpcs_ctx_set_option": ([_p, _i32, _i64], _i32),
     "hpcs_profile_enable": ([_p, _i32], _i32),
     "hpcs_profile_read": ([_p, ctypes.POINTER(KernelStat), _i32, ctypes.POINTER(_i32)], _i32),

@@ -16,7 +16,8 @@ REPO = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libhpcsort.so")
 SOURCES = [os.path.join(HERE, "csrc", s) for s in ("hpcs_api.hip", "hpcs_serve.hip")]
 HEADERS = [os.path.join(HERE, "csrc", h)
-           for h in ("hpcs_kernels.hpp", "hpcs_digit11.hpp", "hpcs_msd.hpp", "hpcs_segments.hpp", "hpcs_topk.hpp", "hpcs_arena.hpp", "hpcs_tuning.hpp")]
+           for h in ("hpcs_kernels.hpp", "hpcs_shapes.hpp", "hpcs_digit11.hpp", "hpcs_msd.hpp", "hpcs_segments.hpp",
+                     "hpcs_topk.hpp", "hpcs_arena.hpp", "hpcs_tuning.hpp")]
 DEPS = SOURCES + HEADERS + [os.path.join(REPO, "include", "hpcsort.h")]
 ARCH = os.environ.get("HPCS_OFFLOAD_ARCH", "gfx950")
 
@@ -47,26 +48,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libhpcsort.so")
     os.replace(tmp, LIB)
-    build_tools(verbose)
     return LIB
-
-
-# Stand-alone measurement programs (no library API): source -> binary in the package.
-TOOLS = {"scatter_ceiling": os.path.join(HERE, "csrc", "scatter_ceiling.hip")}
-
-
-def build_tools(verbose: bool = False) -> None:
-    for name, src in TOOLS.items():
-        out = os.path.join(HERE, name)
-        cmd = [hipcc(), "-O3", "-std=c++17", f"--offload-arch={ARCH}",
-               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", src, "-o", out + ".tmp"]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"hipcc failed building {name}")
-        os.replace(out + ".tmp", out)
 
 
 COMPARATOR = os.path.join(HERE, "rocprim_compare")

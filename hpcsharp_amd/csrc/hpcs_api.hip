@@ -8,7 +8,7 @@
 // pass; then a copy-back.  No host synchronisation happens inside a
 // device-resident call, so the sequence can be captured in a HIP graph.
 #include "hpcs_kernels.hpp"
-#include "hpcs_lines.hpp"
+#include "hpcs_shapes.hpp"
 #include "hpcs_digit11.hpp"
 #include "hpcs_msd.hpp"
 #include "hpcs_segments.hpp"
@@ -26,6 +26,8 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 using namespace hpcs;
@@ -63,20 +65,27 @@ constexpr int64_t BLOCK_SORT_MAX4 = 1024 * 16; // 4-byte keys
 constexpr int64_t BLOCK_SORT_MAX8 = 1024 * 8;  // 8-byte keys
 
 
-// Onesweep tile shapes (threads x keys per thread).  Shape 0 serves every key
-// type; the larger u32 shapes halve the tiles (and look-back work) per pass.
-struct TileShape {
-    int block, items;
-};
-constexpr TileShape TILE_SHAPES[] = {{256, 16}, {512, 16}, {256, 32}, {512, 32}, {256, 8}};
-constexpr int NUM_TILE_SHAPES = 5;
-
-// Shapes 1-3 serve 4-byte keys without values; 0 and 4 serve every sort.
-bool shape_allowed(int shape, int key_bytes, bool vals)
+// Tile shapes: TILE_SHAPES (hpcs_shapes.hpp).  with_shape runs f with the
+// row as a compile-time constant; every row that serves the key type has a
+// branch, and any other index fails (no default instantiation runs), so the
+// counting and scattering launchers of one sort always use one tile.
+template <int KB, bool V, int S = 0, typename F>
+void with_shape(int shape, F &&f)
 {
-    return shape == 0 || shape == 4 || (key_bytes == 4 && !vals);
+    if constexpr (S < NUM_TILE_SHAPES) {
+        if (shape == S) {
+            if constexpr (shape_serves(S, KB, V)) {
+                f(std::integral_constant<int, S>{});
+                return;
+            } else {
+                fail(HPCS_E_UNSUPPORTED, "tile shape " + std::to_string(shape) + " does not serve this key type");
+            }
+        }
+        with_shape<KB, V, S + 1>(shape, std::forward<F>(f));
+    } else {
+        fail(HPCS_E_UNSUPPORTED, "unknown tile shape " + std::to_string(shape));
+    }
 }
-static_assert(NUM_TILE_SHAPES == TUNE_TILE_SHAPES, "the tuning parser knows every tile shape");
 
 bool read_file(const char *path, std::string &text)
 {
@@ -89,9 +98,7 @@ bool read_file(const char *path, std::string &text)
     std::fclose(f);
     return true;
 }
-constexpr int SORT_BLOCK = 256;
-constexpr int SORT_ITEMS = 16;
-constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
+constexpr int SORT_TILE = shape_tile_keys(0);
 constexpr int HIST_BLOCK = 256;
 #ifndef HPCS_HIST_COPIES4
 #define HPCS_HIST_COPIES4 4
@@ -117,13 +124,10 @@ struct hpcs_ctx {
     void *io = nullptr;
     size_t io_cap = 0;
     int rank_mode = RANK_LDSOR;  // tuning knob, env HPCS_RANK=ballot|ldsor
-    int tile_shape = 0;          // tuning knob, env HPCS_TILE_SHAPE=0..4 (1-3: 4-byte keys without values)
+    int tile_shape = 0;          // tuning knob, env HPCS_TILE_SHAPE: a row of TILE_SHAPES
     int fallback_rounds = FALLBACK_ROUNDS;
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
     int pass_algo = PASS_AUTO;    // PASS_ONESWEEP, PASS_RTS (reduce-then-scan) or PASS_AUTO (by size)
-    int persistent = 0;           // reduce-then-scan scatter: persistent workgroups with prefetch
-    int line_chunk = 0;           // reduce-then-scan scatter: tiles per line-carrying workgroup (0: off)
-    int count_tiles = 1;          // reduce-then-scan tile counts: tiles per workgroup (1, 2 or 4)
     int digit_bytes = 2;          // reduce-then-scan: tile counts from next-digit bytes written by the
                                   // scatter: 0 off, 1 on, 2 by type (on for 8-byte keys with values)
     // tuning table (hpcs_ctx_load_tuning / HPCS_TUNE_FILE); options set
@@ -312,13 +316,13 @@ void launch_histogram(hpcs_ctx *c, const K *d_keys, uint64_t n, uint32_t *d_hist
 
 template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int W>
 void launch_onesweep_w(int rank_mode, int fb, dim3 grid, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb,
-                       uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next,
-                       uint8_t *dig)
+                       uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next, uint8_t *dig)
 {
-    if (rank_mode == RANK_BALLOT)
+    // 1024-thread tiles rank by ballot only (their LDS holds no match masks)
+    if (BLOCK >= 1024 || rank_mode == RANK_BALLOT)
         hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_BALLOT, W>), grid, dim3(BLOCK), 0, st,
                            a, b, va, vb, n, pass, plan, lb, next, fb, dig);
-    else
+    else if constexpr (BLOCK < 1024)
         hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_LDSOR, W>), grid, dim3(BLOCK), 0, st,
                            a, b, va, vb, n, pass, plan, lb, next, fb, dig);
 }
@@ -343,137 +347,45 @@ void launch_onesweep_shape(int rank_mode, int window, int fb, dim3 grid, hipStre
                                                              lb, next, dig);
 }
 
+// One pass: onesweep with look-back (window > 0) or the prescanned scatter
+// (window 0; lb holds the tile offsets).
 template <typename K, bool HAS_VALS, int FLOATK>
 void launch_onesweep(int rank_mode, int shape, int window, int fb, dim3 grid, hipStream_t st, K *a, K *b,
                      uint32_t *va, uint32_t *vb, uint32_t n, int pass, DevPlan *plan, uint32_t *lb, uint32_t *next,
                      uint8_t *dig)
 {
-    if constexpr (sizeof(K) == 4 && !HAS_VALS) {
-        if (shape == 1)
-            return launch_onesweep_shape<K, HAS_VALS, FLOATK, 512, 16>(rank_mode, window, fb, grid, st, a, b, va, vb,
-                                                                       n, pass, plan, lb, next, dig);
-        if (shape == 2)
-            return launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 32>(rank_mode, window, fb, grid, st, a, b, va, vb,
-                                                                       n, pass, plan, lb, next, dig);
-        if (shape == 3)
-            return launch_onesweep_shape<K, HAS_VALS, FLOATK, 512, 32>(rank_mode, window, fb, grid, st, a, b, va, vb,
-                                                                       n, pass, plan, lb, next, dig);
-    }
-    if (shape == 4)
-        return launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 8>(rank_mode, window, fb, grid, st, a, b, va, vb, n, pass,
-                                                                  plan, lb, next, dig);
-    launch_onesweep_shape<K, HAS_VALS, FLOATK, 256, 16>(rank_mode, window, fb, grid, st, a, b, va, vb, n, pass, plan,
-                                                        lb, next, dig);
+    with_shape<int(sizeof(K)), HAS_VALS>(shape, [&](auto sc) {
+        constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
+        launch_onesweep_shape<K, HAS_VALS, FLOATK, T.block, T.items>(rank_mode, window, fb, grid, st, a, b, va, vb,
+                                                                     n, pass, plan, lb, next, dig);
+    });
 }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
-template <typename K, int FLOATK>
+// Per-tile digit counts of one pass (grid = one workgroup per tile).
+template <typename K, bool HAS_VALS, int FLOATK>
 void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const K *b, uint32_t n, int pass,
-                        const DevPlan *plan, uint32_t *counts, const uint8_t *dig, int tpw = 1)
+                        const DevPlan *plan, uint32_t *counts, const uint8_t *dig)
 {
-    const int block = TILE_SHAPES[shape].block, items = TILE_SHAPES[shape].items;
     const int vec = aligned16(a) && aligned16(b);
-    const uint32_t tiles = grid.x;
-    if (block == 256 && items == 16 && dig == nullptr && tpw == 2)
-        hipLaunchKernelGGL((k_tile_counts_multi<K, FLOATK, 256, 16, 2>), dim3((tiles + 1) / 2), dim3(256), 0, st, a,
-                           b, n, pass, plan, counts, vec, tiles);
-    else if (sizeof(K) == 4 && block == 256 && items == 16 && dig == nullptr && tpw == 4)
-        hipLaunchKernelGGL((k_tile_counts_multi<K, FLOATK, 256, 16, 4>), dim3((tiles + 3) / 4), dim3(256), 0, st, a,
-                           b, n, pass, plan, counts, vec, tiles);
-    else if (block == 256 && items == 16)
-        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
-                           vec, dig);
-    else if (block == 512 && items == 16)
-        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts,
-                           vec, dig);
-    else if (block == 256 && items == 32)
-        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
-                           vec, dig);
-    else if (block == 256 && items == 8)
-        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 256, 8>), grid, dim3(256), 0, st, a, b, n, pass, plan, counts,
-                           vec, dig);
-    else
-        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, b, n, pass, plan, counts,
-                           vec, dig);
+    with_shape<int(sizeof(K)), HAS_VALS>(shape, [&](auto sc) {
+        constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
+        hipLaunchKernelGGL((k_tile_counts<K, FLOATK, T.block, T.items>), grid, dim3(T.block), 0, st, a, b, n, pass,
+                           plan, counts, vec, dig);
+    });
 }
 
-template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int RANK>
-void launch_scatter_r(int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb, uint32_t n, int pass,
-                      DevPlan *plan, const uint32_t *offsets, uint32_t tiles)
-{
-    // resident workgroups per CU for this instantiation (a function-local
-    // static: initialised once, safely, by the first of concurrent callers)
-    static const int occ = [] {
-        int o = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK>, BLOCK,
-                                                         0) != hipSuccess || o < 1)
-            o = 1;
-        return o;
-    }();
-    uint32_t grid = uint32_t(num_cus) * uint32_t(occ);
-    grid = grid / NUM_XCD * NUM_XCD;
-    if (grid < NUM_XCD) grid = NUM_XCD;
-    if (grid > tiles) grid = tiles;
-    hipLaunchKernelGGL((k_scatter<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK>), dim3(grid), dim3(BLOCK), 0, st, a, b, va,
-                       vb, n, pass, plan, offsets, tiles);
-}
-
-// The line-carrying scatter (4-byte keys without values, 256x16 tiles):
-// one workgroup per `chunk` consecutive tiles.
+// The first read of a reduce-then-scan sort (grid = one workgroup per tile).
 template <typename K, bool HAS_VALS, int FLOATK>
-void launch_scatter_lines(uint32_t chunk, hipStream_t st, K *a, K *b, uint32_t n, int pass, DevPlan *plan,
-                          const uint32_t *offsets, uint32_t tiles)
-{
-    if constexpr (sizeof(K) == 4 && !HAS_VALS)
-        hipLaunchKernelGGL((k_scatter_lines<K, FLOATK, 256, 16>), dim3((tiles + chunk - 1) / chunk), dim3(256), 0,
-                           st, a, b, n, pass, plan, offsets, tiles, chunk);
-    else
-        fail(HPCS_E_UNSUPPORTED, "the line-carrying scatter takes 4-byte keys without values");
-}
-
-template <typename K, bool HAS_VALS, int FLOATK>
-void launch_scatter(int rank_mode, int shape, int num_cus, hipStream_t st, K *a, K *b, uint32_t *va, uint32_t *vb,
-                    uint32_t n, int pass, DevPlan *plan, const uint32_t *offsets, uint32_t tiles)
-{
-    if constexpr (sizeof(K) == 4 && !HAS_VALS) {
-        if (shape == 1) {
-            if (rank_mode == RANK_BALLOT)
-                return launch_scatter_r<K, HAS_VALS, FLOATK, 512, 16, RANK_BALLOT>(num_cus, st, a, b, va, vb, n, pass,
-                                                                                    plan, offsets, tiles);
-            return launch_scatter_r<K, HAS_VALS, FLOATK, 512, 16, RANK_LDSOR>(num_cus, st, a, b, va, vb, n, pass,
-                                                                               plan, offsets, tiles);
-        }
-        if (shape == 2) {
-            if (rank_mode == RANK_BALLOT)
-                return launch_scatter_r<K, HAS_VALS, FLOATK, 256, 32, RANK_BALLOT>(num_cus, st, a, b, va, vb, n, pass,
-                                                                                    plan, offsets, tiles);
-            return launch_scatter_r<K, HAS_VALS, FLOATK, 256, 32, RANK_LDSOR>(num_cus, st, a, b, va, vb, n, pass,
-                                                                               plan, offsets, tiles);
-        }
-    }
-    if (rank_mode == RANK_BALLOT)
-        return launch_scatter_r<K, HAS_VALS, FLOATK, 256, 16, RANK_BALLOT>(num_cus, st, a, b, va, vb, n, pass, plan,
-                                                                            offsets, tiles);
-    launch_scatter_r<K, HAS_VALS, FLOATK, 256, 16, RANK_LDSOR>(num_cus, st, a, b, va, vb, n, pass, plan, offsets,
-                                                                tiles);
-}
-
-template <typename K, int FLOATK>
 void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n, uint32_t *counts, K *orslots)
 {
-    const int block = TILE_SHAPES[shape].block, items = TILE_SHAPES[shape].items;
     const int vec = aligned16(a);
-    if (block == 256 && items == 16)
-        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 16>), grid, dim3(256), 0, st, a, n, counts, orslots, vec);
-    else if (block == 512 && items == 16)
-        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 16>), grid, dim3(512), 0, st, a, n, counts, orslots, vec);
-    else if (block == 256 && items == 32)
-        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 32>), grid, dim3(256), 0, st, a, n, counts, orslots, vec);
-    else if (block == 256 && items == 8)
-        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 256, 8>), grid, dim3(256), 0, st, a, n, counts, orslots, vec);
-    else
-        hipLaunchKernelGGL((k_upsweep<K, FLOATK, 512, 32>), grid, dim3(512), 0, st, a, n, counts, orslots, vec);
+    with_shape<int(sizeof(K)), HAS_VALS>(shape, [&](auto sc) {
+        constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
+        hipLaunchKernelGGL((k_upsweep<K, FLOATK, T.block, T.items>), grid, dim3(T.block), 0, st, a, n, counts,
+                           orslots, vec);
+    });
 }
 
 // The device-resident sort: keys (and optional uint32 values) in place.
@@ -486,7 +398,7 @@ struct SortParams {
 // that holds n (the largest class beyond it), then the built-in defaults.
 SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n)
 {
-    SortParams p{c->pass_algo, shape_allowed(c->tile_shape, key_bytes, vals) ? c->tile_shape : 0, c->rank_mode,
+    SortParams p{c->pass_algo, shape_serves(c->tile_shape, key_bytes, vals) ? c->tile_shape : 0, c->rank_mode,
                  c->digit_bits};
     const TuneEntry *best = nullptr, *largest = nullptr;
     for (const TuneEntry &e : c->tune) {
@@ -497,7 +409,7 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
     if (!best) best = largest;
     if (best) {
         if (!c->set_algo) p.algo = best->algo;
-        if (!c->set_shape && shape_allowed(best->shape, key_bytes, vals)) p.shape = best->shape;
+        if (!c->set_shape && shape_serves(best->shape, key_bytes, vals)) p.shape = best->shape;
         if (!c->set_rank) p.rank = best->rank;
         if (!c->set_bits) p.bits = best->bits;
     }
@@ -520,6 +432,7 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
         p.shape = 0;
         p.rank = RANK_BALLOT;
     }
+    if (TILE_SHAPES[p.shape].ballot_only) p.rank = RANK_BALLOT;
     return p;
 }
 
@@ -691,16 +604,11 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     const int algo = (prm.algo == PASS_ONESWEEP && n >= LOOKBACK_MAX_KEYS) ? PASS_RTS : prm.algo;
     const int rank_mode = prm.rank;
     // reduce-then-scan: the scatter writes the next pass's digit bytes for
-    // its tile counts (not the persistent scatter)
-    const bool persistent = c->persistent && shape != 3 && shape != 4;
-    // Measured at 2^28 on MI355X: the byte stores cost the scatter about what
-    // the byte counts save for u64 keys, more for u32 keys; with u32 values
-    // beside u64 keys the sort gains 5%.
+    // its tile counts.  Measured at 2^28 on MI355X: the byte stores cost the
+    // scatter about what the byte counts save for u64 keys, more for u32
+    // keys; with u32 values beside u64 keys the sort gains 5%.
     const bool want_dig = c->digit_bytes == 1 || (c->digit_bytes == 2 && sizeof(K) == 8 && HAS_VALS);
-    uint8_t *dig = (want_dig && !persistent) ? reinterpret_cast<uint8_t *>(base + L.off_dig) : nullptr;
-    // line-carrying scatter: 4-byte keys without values on 256x16 tiles (not
-    // with the persistent scatter or next-digit bytes)
-    const bool lines = sizeof(K) == 4 && !HAS_VALS && c->line_chunk > 0 && shape == 0 && !persistent && !dig;
+    uint8_t *dig = want_dig ? reinterpret_cast<uint8_t *>(base + L.off_dig) : nullptr;
     if (algo == PASS_RTS && tiles <= SCAN_DIGIT_MAX_TILES) {
         // few tiles: launches, not bytes, set the time.  One histogram of
         // every digit gives the plan (digit starts, trivial passes); per
@@ -713,20 +621,16 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         uint32_t *counts = lb[0];
         for (int p = 0; p < P; ++p) {
             timed(c, st, "tile_counts", [&] {
-                launch_tile_counts<K, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan, counts,
-                                              nullptr, c->count_tiles);
+                launch_tile_counts<K, HAS_VALS, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan,
+                                                        counts, nullptr);
             });
             timed(c, st, "scan", [&] {
                 hipLaunchKernelGGL(k_scan_digit, dim3(RADIX), dim3(1024), 0, st, counts, tiles, p, plan);
             });
             timed(c, st, "scatter", [&] {
-                if (lines)
-                    launch_scatter_lines<K, HAS_VALS, FLOATK>(uint32_t(c->line_chunk), st, d_keys, keysB,
-                                                              uint32_t(n), p, plan, counts, tiles);
-                else
-                    launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
-                                                         d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
-                                                         nullptr, nullptr);
+                launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st, d_keys,
+                                                     keysB, d_vals, valsB, uint32_t(n), p, plan, counts, nullptr,
+                                                     nullptr);
             });
         }
         timed(c, st, "copyback", [&] {
@@ -740,7 +644,7 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         launch_fill(c, base, L.off_lb0, st);
         K *orslots = reinterpret_cast<K *>(hist);
         timed(c, st, "upsweep", [&] {
-            launch_upsweep<K, FLOATK>(shape, dim3(tiles), st, d_keys, uint32_t(n), lb[0], orslots);
+            launch_upsweep<K, HAS_VALS, FLOATK>(shape, dim3(tiles), st, d_keys, uint32_t(n), lb[0], orslots);
         });
         timed(c, st, "plan", [&] {
             hipLaunchKernelGGL((k_plan_rts<K>), dim3(1), dim3(64), 0, st, orslots, plan, uint64_t(n),
@@ -759,31 +663,19 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
             uint32_t *counts = lb[0], *totals = lb[1];
             if (p > 0)
                 timed(c, st, "tile_counts", [&] {
-                    launch_tile_counts<K, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan,
-                                                  counts, dig, c->count_tiles);
+                    launch_tile_counts<K, HAS_VALS, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p,
+                                                            plan, counts, dig);
                 });
             timed(c, st, "scan", [&] {
                 uint32_t *dtot_p = dtot + size_t(p) * RADIX * LINE_WORDS;
                 hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals, dtot_p,
                                    p, plan);
                 hipLaunchKernelGGL(k_scan_columns, dim3(RADIX), dim3(1024), 0, st, totals, nranges, dtot_p, p, plan);
-                // the default scatter adds the range bases itself
-                if (persistent || lines)
-                    hipLaunchKernelGGL(k_range_offsets, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals,
-                                       p, plan);
             });
             timed(c, st, "scatter", [&] {
-                // the persistent kernel keeps two tiles in registers: not for 512x32 tiles
-                if (lines)
-                    launch_scatter_lines<K, HAS_VALS, FLOATK>(uint32_t(c->line_chunk), st, d_keys, keysB,
-                                                              uint32_t(n), p, plan, counts, tiles);
-                else if (persistent)
-                    launch_scatter<K, HAS_VALS, FLOATK>(rank_mode, shape, c->num_cus, st, d_keys, keysB, d_vals,
-                                                        valsB, uint32_t(n), p, plan, counts, tiles);
-                else
-                    launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st,
-                                                         d_keys, keysB, d_vals, valsB, uint32_t(n), p, plan, counts,
-                                                         totals, dig);
+                // the scatter adds its range's base to its in-range offsets
+                launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st, d_keys,
+                                                     keysB, d_vals, valsB, uint32_t(n), p, plan, counts, totals, dig);
             });
             continue;
         }
@@ -1352,6 +1244,25 @@ void host_histogram(hpcs_ctx *c, const K *a, int64_t l, int64_t r, int32_t *coun
 }
 
 template <typename F>
+int32_t guarded_host(F &&f)
+{
+    try {
+        f();
+        return HPCS_OK;
+    } catch (const Err &e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host out of memory";
+        return HPCS_E_OOM;
+    } catch (...) {
+        g_last_error = "unexpected C++ exception";
+        return HPCS_E_HIP;
+    }
+}
+
+// The same for calls on a context: the context's lock and device are held.
+template <typename F>
 int32_t guarded(hpcs_ctx *c, F &&f)
 {
     if (!c) {
@@ -1500,18 +1411,6 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             if (value < 0 || value > 2) fail(HPCS_E_INVALID_ARG, "digit bytes must be 0, 1 or 2");
             ctx->digit_bytes = int(value);
             break;
-        case HPCS_OPT_COUNT_TILES:
-            if (value != 1 && value != 2 && value != 4) fail(HPCS_E_INVALID_ARG, "count tiles must be 1, 2 or 4");
-            ctx->count_tiles = int(value);
-            break;
-        case HPCS_OPT_LINE_CHUNK:
-            if (value < 0 || value > 256) fail(HPCS_E_INVALID_ARG, "line chunk must be 0..256");
-            ctx->line_chunk = int(value);
-            break;
-        case HPCS_OPT_PERSISTENT:
-            if (value != 0 && value != 1) fail(HPCS_E_INVALID_ARG, "persistent must be 0 or 1");
-            ctx->persistent = int(value);
-            break;
         case HPCS_OPT_FALLBACK_ROUNDS:
             if (value < 0 || value > (1 << 20)) fail(HPCS_E_INVALID_ARG, "fallback rounds out of range");
             ctx->fallback_rounds = int(value);
@@ -1557,6 +1456,25 @@ HPCS_API int32_t hpcs_graph_cache_stats(hpcs_ctx *ctx, int64_t *entries, int64_t
         *entries = g;
         *captures = ctx->graph_captures;
         *replays = ctx->graph_replays;
+    });
+}
+
+HPCS_API int32_t hpcs_tile_shape_query(int32_t shape, int32_t key_bytes, int32_t with_values, int32_t *block,
+                                       int32_t *items)
+{
+    return guarded_host([&] {
+        if (!block || !items) fail(HPCS_E_INVALID_ARG, "block/items is null");
+        if (key_bytes != 4 && key_bytes != 8) fail(HPCS_E_INVALID_ARG, "key_bytes must be 4 or 8");
+        // the dispatch every pass launcher goes through
+        auto report = [&](auto sc) {
+            constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
+            *block = T.block;
+            *items = T.items;
+        };
+        if (key_bytes == 4 && with_values) with_shape<4, true>(shape, report);
+        else if (key_bytes == 4) with_shape<4, false>(shape, report);
+        else if (with_values) with_shape<8, true>(shape, report);
+        else with_shape<8, false>(shape, report);
     });
 }
 
@@ -1645,8 +1563,8 @@ HPCS_API int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n)
         if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
         size_t need = 0;
         for (int shape = 0; shape < NUM_TILE_SHAPES; ++shape) {
-            if (!shape_allowed(shape, int(ks), false)) continue;
-            const int tk = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
+            if (!shape_serves(shape, int(ks), false)) continue;
+            const int tk = shape_tile_keys(shape);
             need = std::max(need, make_layout(n, ks, false, tk).total);
         }
         if (n < LOOKBACK_MAX_KEYS) need = std::max(need, make_layout11(n, ks).total);

@@ -5,7 +5,7 @@
 // Reduce-then-scan (large arrays): k_upsweep (pass-0 tile counts and the
 // OR / AND of all keys in one read), k_plan_rts (which passes move data),
 // then per pass k_tile_counts (passes >= 1), k_range_totals /
-// k_scan_columns / k_range_offsets (every tile's digit offsets) and
+// k_scan_columns (every tile's digit offsets) and
 // k_onesweep in its prescanned form as the scatter (tiles XCD-contiguous).
 //
 // Onesweep (small arrays):
@@ -29,6 +29,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 namespace hpcs {
 
@@ -367,14 +369,20 @@ constexpr int RANK_LDSOR = 1;
 // kernel takes the value as an argument so tests can force the fallback.
 constexpr int FALLBACK_ROUNDS = 24;
 
+// 1024-thread tiles (64 KiB of staged keys) rank by ballot and keep 16-bit
+// per-wave counters (a tile holds at most 16384 keys), so that two workgroups
+// fit in a CU's 160 KiB of LDS; the compiler is asked for the matching
+// register budget (8 waves per SIMD).
+template <int BLOCK> constexpr int onesweep_waves_per_eu() { return BLOCK >= 1024 ? 8 : 1; }
+
 template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int RANK, int LOOKBACK_WINDOW>
-__global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__restrict__ keysB,
-                                                    uint32_t *__restrict__ valsA, uint32_t *__restrict__ valsB,
-                                                    uint32_t n, int pass, DevPlan *__restrict__ plan,
-                                                    uint32_t *__restrict__ lb_cur, uint32_t *__restrict__ lb_next,
-                                                    int fb_rounds, uint8_t *__restrict__ dig)
+__global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesweep(
+    K *__restrict__ keysA, K *__restrict__ keysB, uint32_t *__restrict__ valsA, uint32_t *__restrict__ valsB,
+    uint32_t n, int pass, DevPlan *__restrict__ plan, uint32_t *__restrict__ lb_cur, uint32_t *__restrict__ lb_next,
+    int fb_rounds, uint8_t *__restrict__ dig)
 {
     static_assert(BLOCK >= RADIX && BLOCK % WAVE == 0, "one thread per digit is required");
+    static_assert(BLOCK < 1024 || RANK == RANK_BALLOT, "1024-thread tiles rank by ballot");
     constexpr int WAVES = BLOCK / WAVE;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int WAVE_TILE = WAVE * ITEMS;
@@ -382,8 +390,13 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     // LOOKBACK_WINDOW == 0: the tile's global digit offsets were computed
     // beforehand (k_tile_counts + scans) and lb_cur holds them, [tile][digit].
     constexpr bool PRESCANNED = LOOKBACK_WINDOW == 0;
+    // per-wave digit counters: 16-bit where ballot ranking writes them with
+    // plain stores (no LDS atomics) and the tile's counts fit
+    constexpr bool CNT16 = RANK == RANK_BALLOT && BLOCK >= 1024;
+    static_assert(!CNT16 || TILE < 65536, "16-bit counters hold a tile's counts");
+    using Cnt = typename std::conditional<CNT16, uint16_t, uint32_t>::type;
 
-    __shared__ uint32_t s_cnt[WAVES][RADIX];
+    __shared__ Cnt s_cnt[WAVES][RADIX];
     __shared__ uint64_t s_match[RANK == RANK_LDSOR ? WAVES : 1][RANK == RANK_LDSOR ? RADIX : 1];
     __shared__ uint32_t s_base[RADIX];
     __shared__ uint32_t s_scan[SCAN_WAVES];
@@ -427,19 +440,32 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     const int wave_valid = n > wave_base ? int(min(uint32_t(WAVE_TILE), n - wave_base)) : 0;
 
     // ---- load (issued before the LDS set-up so their latency overlaps it) ----
+    // A full wave (every wave but the last tile's) loads without per-key
+    // bounds tests, from one base address and immediate offsets.
     K keys[ITEMS];
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const int off = i * WAVE + lane;
-        keys[i] = off < wave_valid ? kin[wave_base + off] : ~K(0);
-    }
-    // values load with the keys, so their latency hides behind the ranking
     uint32_t vals[HAS_VALS ? ITEMS : 1];
-    if constexpr (HAS_VALS) {
+    if (wave_valid == WAVE_TILE) {
+        const K *__restrict__ kp = kin + wave_base + lane;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) keys[i] = kp[i * WAVE];
+        // values load with the keys, so their latency hides behind the ranking
+        if constexpr (HAS_VALS) {
+            const uint32_t *__restrict__ vp = vin + wave_base + lane;
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) vals[i] = vp[i * WAVE];
+        }
+    } else {
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
             const int off = i * WAVE + lane;
-            vals[i] = off < wave_valid ? vin[wave_base + off] : 0u;
+            keys[i] = off < wave_valid ? kin[wave_base + off] : ~K(0);
+        }
+        if constexpr (HAS_VALS) {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) {
+                const int off = i * WAVE + lane;
+                vals[i] = off < wave_valid ? vin[wave_base + off] : 0u;
+            }
         }
     }
     uint32_t pre_off = 0;
@@ -468,8 +494,10 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     // first and counts with LDS atomics here.  Ballot ranking ranks first
     // with wave-local counters (their final values are the counts) and stages
     // afterwards, so it issues no counting atomics.
-    uint32_t *cnt = s_cnt[wave];
-    uint32_t rank[RANK == RANK_BALLOT ? ITEMS : 1];
+    Cnt *cnt = s_cnt[wave];
+    // in-wave ranks (below 64 * ITEMS), two 16-bit ranks per register
+    static_assert(WAVE * ITEMS < 65536, "ranks fit 16 bits");
+    uint32_t rank2[RANK == RANK_BALLOT ? (ITEMS + 1) / 2 : 1];
     if constexpr (RANK == RANK_BALLOT) {
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
@@ -478,9 +506,17 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
             const uint64_t peers = match_digit8(d) & __ballot(valid);
             const uint32_t below = count_below(peers);
             const uint32_t old = cnt[d];
-            if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
-            rank[i] = old + below;
+            if (valid && below == 0) cnt[d] = Cnt(old + uint32_t(__popcll(peers)));
+            const uint32_t r = old + below;
+            rank2[i / 2] = (i & 1) ? (rank2[i / 2] | (r << 16)) : r;
             __builtin_amdgcn_wave_barrier();
+        }
+        // The staging step recomputes each key's counter address instead of
+        // keeping ITEMS addresses live across the block scan (an empty asm
+        // that the compiler must assume rewrites the keys; no instruction).
+        if constexpr (BLOCK >= 1024) {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) asm volatile("" : "+v"(keys[i]));
         }
     } else {
 #pragma unroll
@@ -495,7 +531,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
 #pragma unroll
         for (int w = 0; w < WAVES; ++w) {
             const uint32_t c = s_cnt[w][tid];
-            s_cnt[w][tid] = tile_cnt;
+            s_cnt[w][tid] = Cnt(tile_cnt);
             tile_cnt += c;
         }
         if constexpr (!PRESCANNED)
@@ -514,7 +550,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
     if (tid < RADIX) {
         for (int w = 0; w < wave; ++w) tile_start += s_scan[w];
 #pragma unroll
-        for (int w = 0; w < WAVES; ++w) s_cnt[w][tid] += tile_start;
+        for (int w = 0; w < WAVES; ++w) s_cnt[w][tid] = Cnt(s_cnt[w][tid] + tile_start);
     }
     __syncthreads();
 
@@ -525,7 +561,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
             if (i * WAVE + lane < wave_valid) {
-                const uint32_t at = cnt[KeyOps<K>::digit(keys[i], shift)] + rank[i];
+                const uint32_t at = cnt[KeyOps<K>::digit(keys[i], shift)] + ((rank2[i / 2] >> (16 * (i & 1))) & 0xFFFFu);
                 s_keys[at] = keys[i];
                 if constexpr (HAS_VALS) pos[i] = at;
             }
@@ -544,7 +580,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
             const uint32_t below = count_below(peers);
             const uint32_t old = cnt[d];
             if (valid && below == 0) {
-                cnt[d] = old + uint32_t(__popcll(peers));
+                cnt[d] = Cnt(old + uint32_t(__popcll(peers)));
                 s_match[wave][d] = 0ull;
             }
             if (valid) s_keys[old + below] = keys[i];
@@ -666,210 +702,6 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(K *__restrict__ keysA, K *__
 }
 
 // ---------------------------------------------------------------------------
-// k_scatter: the scatter step of the reduce-then-scan pass, persistent.
-//
-// Offsets were computed beforehand (offsets[tile][digit] = global position of
-// the tile's first key of each digit), so tiles are independent and the tile
-// order is free.  The grid is sized to the resident workgroups; workgroup b
-// serves the tiles of XCD b % 8's contiguous range (xcd_contiguous), taking
-// every (grid / 8)-th one, so the tiles an XCD has in flight are neighbours
-// and the partial cache lines at the ends of their digit runs meet in its L2.
-// While a tile is ranked and written, the next tile's keys and offsets are
-// already loading into registers, so each CU keeps reads in flight through
-// the compute phases (one tile per workgroup at a time left HBM half idle).
-//
-// Per tile: per-wave ranks (RANK, as in k_onesweep), per-wave bases from the
-// per-wave counts, tile digit starts by a block scan, keys staged in LDS in
-// sorted order, then written in order so each digit's run is contiguous.
-// ---------------------------------------------------------------------------
-template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS, int RANK>
-__global__ __launch_bounds__(BLOCK) void k_scatter(K *__restrict__ keysA, K *__restrict__ keysB,
-                                                   uint32_t *__restrict__ valsA, uint32_t *__restrict__ valsB,
-                                                   uint32_t n, int pass, const DevPlan *__restrict__ plan,
-                                                   const uint32_t *__restrict__ offsets, uint32_t tiles)
-{
-    static_assert(BLOCK >= RADIX && BLOCK % WAVE == 0, "one thread per digit is required");
-    constexpr int WAVES = BLOCK / WAVE;
-    constexpr int TILE = BLOCK * ITEMS;
-    constexpr int WAVE_TILE = WAVE * ITEMS;
-    constexpr int SCAN_WAVES = RADIX / WAVE;
-
-    __shared__ uint32_t s_cnt[WAVES][RADIX];
-    __shared__ uint64_t s_match[RANK == RANK_LDSOR ? WAVES : 1][RANK == RANK_LDSOR ? RADIX : 1];
-    __shared__ uint32_t s_base[RADIX];
-    __shared__ uint32_t s_scan[SCAN_WAVES];
-    __shared__ K s_keys[TILE];
-    __shared__ uint32_t s_vals[HAS_VALS ? TILE : 1];
-
-    if (plan->trivial[pass]) return;
-    const int tid = threadIdx.x;
-    const int lane = tid & (WAVE - 1);
-    const int wave = tid >> 6;
-    const uint32_t src = plan->src[pass];
-    const K *__restrict__ kin = src ? keysB : keysA;
-    K *__restrict__ kout = src ? keysA : keysB;
-    const uint32_t *__restrict__ vin = src ? valsB : valsA;
-    uint32_t *__restrict__ vout = src ? valsA : valsB;
-    const bool xin = FLOATK && plan->xin[pass];
-    const bool xout = FLOATK && plan->xout[pass];
-    const int shift = pass * RADIX_BITS;
-
-    // this workgroup's tiles: XCD x's range, every `stride`-th from i
-    const uint32_t x = blockIdx.x % NUM_XCD;
-    const uint32_t per = tiles / NUM_XCD, rem = tiles % NUM_XCD;
-    const uint32_t x_start = x * per + min(x, rem), x_count = per + (x < rem ? 1u : 0u);
-    const uint32_t stride = (gridDim.x - x + NUM_XCD - 1) / NUM_XCD; // workgroups on XCD x
-    uint32_t i = blockIdx.x / NUM_XCD;
-    if (i >= x_count) return;
-
-    auto load_tile = [&](uint32_t tile, K (&k)[ITEMS], uint32_t &off) {
-        const uint32_t wbase = tile * uint32_t(TILE) + uint32_t(wave * WAVE_TILE);
-        const int wv = n > wbase ? int(min(uint32_t(WAVE_TILE), n - wbase)) : 0;
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const int o = j * WAVE + lane;
-            k[j] = o < wv ? kin[wbase + o] : ~K(0);
-        }
-        off = tid < RADIX ? offsets[size_t(tile) * RADIX + tid] : 0u;
-    };
-
-    K cur[ITEMS];
-    uint32_t cur_off;
-    load_tile(x_start + i, cur, cur_off);
-    while (true) {
-        const uint32_t tile = x_start + i;
-        const uint32_t inext = i + stride;
-        const bool has_next = inext < x_count;
-        K nxt[ITEMS];
-        uint32_t nxt_off = 0;
-        if (has_next) load_tile(x_start + inext, nxt, nxt_off);
-
-        const uint32_t tile_base = tile * uint32_t(TILE);
-        const uint32_t wave_base = tile_base + uint32_t(wave * WAVE_TILE);
-        const int wave_valid = n > wave_base ? int(min(uint32_t(WAVE_TILE), n - wave_base)) : 0;
-        K keys[ITEMS];
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j)
-            keys[j] = (xin && j * WAVE + lane < wave_valid) ? key_enc<K, FLOATK>(cur[j]) : cur[j];
-
-        for (int q = tid; q < WAVES * RADIX; q += BLOCK) (&s_cnt[0][0])[q] = 0u;
-        if constexpr (RANK == RANK_LDSOR)
-            for (int q = tid; q < WAVES * RADIX; q += BLOCK) (&s_match[0][0])[q] = 0ull;
-        __syncthreads();
-
-        // ---- per-wave counts (ballot ranks first; LDS-OR counts first) ----
-        uint32_t *cnt = s_cnt[wave];
-        uint32_t rank[RANK == RANK_BALLOT ? ITEMS : 1];
-        if constexpr (RANK == RANK_BALLOT) {
-#pragma unroll
-            for (int j = 0; j < ITEMS; ++j) {
-                const uint32_t d = KeyOps<K>::digit(keys[j], shift);
-                const bool valid = j * WAVE + lane < wave_valid;
-                const uint64_t peers = match_digit8(d) & __ballot(valid);
-                const uint32_t below = count_below(peers);
-                const uint32_t old = cnt[d];
-                if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
-                rank[j] = old + below;
-                __builtin_amdgcn_wave_barrier();
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < ITEMS; ++j)
-                if (j * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[j], shift)], 1u);
-        }
-        __syncthreads();
-
-        // ---- wave bases, tile digit starts, global bases ----
-        uint32_t tile_cnt = 0, tile_start = 0;
-        if (tid < RADIX) {
-#pragma unroll
-            for (int w = 0; w < WAVES; ++w) {
-                const uint32_t c = s_cnt[w][tid];
-                s_cnt[w][tid] = tile_cnt;
-                tile_cnt += c;
-            }
-            uint32_t xs = tile_cnt;
-#pragma unroll
-            for (int off = 1; off < WAVE; off <<= 1) {
-                const uint32_t y = __shfl_up(xs, off);
-                if (lane >= off) xs += y;
-            }
-            if (lane == WAVE - 1) s_scan[wave] = xs;
-            tile_start = xs - tile_cnt;
-        }
-        __syncthreads();
-        if (tid < RADIX) {
-            for (int w = 0; w < wave; ++w) tile_start += s_scan[w];
-#pragma unroll
-            for (int w = 0; w < WAVES; ++w) s_cnt[w][tid] += tile_start;
-            s_base[tid] = cur_off - tile_start;
-        }
-        __syncthreads();
-
-        // ---- stage the tile in LDS in sorted order ----
-        uint32_t pos[HAS_VALS ? ITEMS : 1];
-        if constexpr (RANK == RANK_BALLOT) {
-#pragma unroll
-            for (int j = 0; j < ITEMS; ++j) {
-                if (j * WAVE + lane < wave_valid) {
-                    const uint32_t at = cnt[KeyOps<K>::digit(keys[j], shift)] + rank[j];
-                    s_keys[at] = keys[j];
-                    if constexpr (HAS_VALS) pos[j] = at;
-                }
-            }
-        } else {
-            uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
-            const uint32_t lanebit = 1u << (lane & 31);
-            const int half = lane >> 5;
-#pragma unroll
-            for (int j = 0; j < ITEMS; ++j) {
-                const uint32_t d = KeyOps<K>::digit(keys[j], shift);
-                const bool valid = j * WAVE + lane < wave_valid;
-                if (valid) atomicOr(&match32[2 * d + half], lanebit);
-                __builtin_amdgcn_wave_barrier();
-                const uint64_t peers = s_match[wave][d];
-                const uint32_t below = count_below(peers);
-                const uint32_t old = cnt[d];
-                if (valid && below == 0) {
-                    cnt[d] = old + uint32_t(__popcll(peers));
-                    s_match[wave][d] = 0ull;
-                }
-                if (valid) s_keys[old + below] = keys[j];
-                if constexpr (HAS_VALS) pos[j] = old + below;
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
-        if constexpr (HAS_VALS) {
-#pragma unroll
-            for (int j = 0; j < ITEMS; ++j) {
-                const int o = j * WAVE + lane;
-                if (o < wave_valid) s_vals[pos[j]] = vin[wave_base + o];
-            }
-        }
-        __syncthreads();
-
-        // ---- write the tile out, each digit's run contiguous ----
-        const int tile_valid = n > tile_base ? int(min(uint32_t(TILE), n - tile_base)) : 0;
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const int jj = j * BLOCK + tid;
-            if (jj < tile_valid) {
-                const K k = s_keys[jj];
-                const uint32_t dst = s_base[KeyOps<K>::digit(k, shift)] + uint32_t(jj);
-                kout[dst] = xout ? key_dec<K, FLOATK>(k) : k;
-                if constexpr (HAS_VALS) vout[dst] = s_vals[jj];
-            }
-        }
-        if (!has_next) break;
-        __syncthreads(); // LDS is reused by the next tile
-        i = inext;
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) cur[j] = nxt[j];
-        cur_off = nxt_off;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Reduce-then-scan form of a pass (no waits between workgroups):
 //   k_tile_counts    per-tile digit counts, [tile][digit]
 //   k_range_totals   per range of SCAN_RANGE tiles, the sum of its counts,
@@ -877,8 +709,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(K *__restrict__ keysA, K *__r
 //   k_scan_columns   exclusive scan of the range sums over ranges, seeded with
 //                    the digit's global start: the ranges' global bases
 // followed by k_onesweep<..., LOOKBACK_WINDOW = 0> adding its range's base to
-// its in-range offsets (k_range_offsets adds the bases in place instead,
-// for the persistent and the line-carrying scatters).
+// its in-range offsets.
 // It reads the keys twice per pass (12 B per u32 key instead of 8) and has
 // no chain of dependent loads between tiles.
 // ---------------------------------------------------------------------------
@@ -992,50 +823,6 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
     }
 }
 
-// k_tile_counts for TPW adjacent tiles per workgroup (keys, not digit
-// bytes): the loads of all its tiles are issued before any is counted, so a
-// workgroup keeps TPW tiles' bytes in flight while it waits, and each tile
-// is counted into its own LDS row.  Same output, counts[tile][digit].
-template <typename K, int FLOATK, int BLOCK, int ITEMS, int TPW>
-__global__ __launch_bounds__(BLOCK) void k_tile_counts_multi(const K *__restrict__ keysA, const K *__restrict__ keysB,
-                                                             uint32_t n, int pass, const DevPlan *__restrict__ plan,
-                                                             uint32_t *__restrict__ counts, int vec, uint32_t tiles)
-{
-    constexpr int TILE = BLOCK * ITEMS;
-    __shared__ uint32_t sh[TPW][RADIX];
-    const uint32_t flags = plan->flags[pass];
-    if (flags & PF_TRIVIAL) return;
-    const K *__restrict__ kin = (flags & PF_SRC_B) ? keysB : keysA;
-    const bool xin = FLOATK && (flags & PF_XIN);
-    const int shift = pass * RADIX_BITS;
-    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
-    const uint32_t t0 = blockIdx.x * uint32_t(TPW);
-    if (t0 >= tiles) return; // the whole workgroup
-    K k[TPW][ITEMS];
-    uint32_t cnt[TPW];
-#pragma unroll
-    for (int j = 0; j < TPW; ++j) {
-        const uint32_t base = (t0 + j) * uint32_t(TILE); // below n for every tile < tiles
-        cnt[j] = t0 + j < tiles ? min(uint32_t(TILE), n - base) : 0u;
-        if (cnt[j]) load_tile_keys<K, BLOCK, ITEMS>(kin + base, cnt[j], vec, k[j]);
-    }
-    for (int i = tid; i < TPW * RADIX; i += BLOCK) (&sh[0][0])[i] = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < TPW; ++j) {
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const bool valid = uint32_t(i * BLOCK + tid) < cnt[j];
-            count_digit(sh[j], KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[j][i]) : k[j][i], shift), valid, lane);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < TPW; ++j)
-        if (t0 + j < tiles)
-            for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(t0 + j) * RADIX + i] = sh[j][i];
-}
-
 // Spacing of per-digit accumulators that take atomics from many workgroups:
 // one 128-byte line each, so the atomics do not queue behind each other.
 constexpr int LINE_WORDS = 32;
@@ -1105,21 +892,6 @@ __global__ __launch_bounds__(1024) void k_scan_columns(uint32_t *__restrict__ to
         totals[at] = run;
         run += c;
     }
-}
-
-// Per range r (thread = digit): the tiles' global digit offsets, the range's
-// base added to their in-range offsets in place.
-__global__ __launch_bounds__(RADIX) void k_range_offsets(uint32_t *__restrict__ counts, uint32_t tiles,
-                                                         const uint32_t *__restrict__ totals, int pass,
-                                                         const DevPlan *__restrict__ plan)
-{
-    if (plan->flags[pass] & PF_TRIVIAL) return;
-    const uint32_t r = blockIdx.x, d = threadIdx.x;
-    const uint32_t t0 = r * SCAN_RANGE, t1 = min(tiles, t0 + SCAN_RANGE);
-    const uint32_t base = totals[size_t(r) * RADIX + d];
-#pragma unroll
-    for (int k = 0; k < SCAN_RANGE; ++k)
-        if (t0 + k < t1) counts[size_t(t0 + k) * RADIX + d] += base;
 }
 
 // The three scan kernels in one, for sorts of few tiles (the plan's digit

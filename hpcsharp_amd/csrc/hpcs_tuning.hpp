@@ -8,9 +8,9 @@
 #include <string>
 #include <vector>
 
-namespace hpcs {
+#include "hpcs_shapes.hpp"
 
-constexpr int TUNE_TILE_SHAPES = 5; // tile shapes 0-4 (threads x keys per thread)
+namespace hpcs {
 
 // One row of a tuning table: for sorts of up to max_n keys of key_bytes
 // (with or without u32 values), the pass algorithm, tile shape, ranking and
@@ -31,8 +31,9 @@ struct TuneEntry {
 // larger arrays fall back to the choice by size) / 4 one MSD split, then one
 // workgroup per bucket (keys without values, at most 2^22 4-byte or 2^21
 // 8-byte keys; larger arrays fall back to the choice by size);
-// shape 0-4 (1-3 for 4-byte keys without values only); rank 0 ballot / 1
-// LDS-OR; digit_bits 8 (default) or 11.  11-bit digits exist in one form
+// shape: a row of TILE_SHAPES (hpcs_shapes.hpp) that serves the dtype; rank
+// 0 ballot / 1 LDS-OR (0 only for ballot-only shapes); digit_bits 8
+// (default) or 11.  11-bit digits exist in one form
 // only: onesweep passes, 256x16 tiles, ballot ranking, keys without values,
 // so a row with digit_bits 11 reads `<dtype> <max_n> 0 0 0 11`.
 inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, std::string &err)
@@ -69,9 +70,9 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
             err = "line " + std::to_string(line_no) + ": unknown dtype " + d;
             return false;
         }
-        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3 && algo != 4) || rank < 0 || rank > 1 || shape < 0 ||
-            shape >= TUNE_TILE_SHAPES ||
-            (shape != 0 && shape != 4 && (e.key_bytes != 4 || e.vals)) || (bits != 8 && bits != 11) ||
+        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3 && algo != 4) || rank < 0 || rank > 1 ||
+            !shape_serves(shape, e.key_bytes, e.vals != 0) || (rank == 1 && TILE_SHAPES[shape].ballot_only) ||
+            (bits != 8 && bits != 11) ||
             (bits == 11 && (algo != 0 || shape != 0 || rank != 0 || e.vals)) || (algo == 4 && e.vals)) {
             err = "line " + std::to_string(line_no) + ": value out of range";
             return false;

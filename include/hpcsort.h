@@ -53,8 +53,12 @@ extern "C" {
 
 /* Tuning options for hpcs_ctx_set_option. */
 #define HPCS_OPT_RANK 1       /* 0: ballot match, 1: LDS-OR match (default) */
-#define HPCS_OPT_TILE_SHAPE 2 /* 0: 256x16 (default), 4: 256x8 (threads x keys per thread), any sort;
-                                 4-byte keys without values also 1: 512x16, 2: 256x32, 3: 512x32 */
+#define HPCS_OPT_TILE_SHAPE 2 /* threads x keys per thread of a pass's tile: 0: 256x16 (default) and
+                                 4: 256x8, any sort; 1: 512x16, 2: 256x32, 3: 512x32 and 5: 1024x16,
+                                 4-byte keys without values; 6: 1024x8, 8-byte keys without values or
+                                 4-byte keys with values.  Shapes 5 and 6 rank by ballot (HPCS_OPT_RANK
+                                 is ignored for them).  A shape that does not serve a sort is ignored
+                                 for it (the table's or the default shape is used) */
 #define HPCS_OPT_FALLBACK_ROUNDS 3 /* look-back rounds on an unpublished tile before counting it locally
                                       (default 24; 0 makes every wait use the local count, for tests) */
 #define HPCS_OPT_LOOKBACK_WINDOW 4 /* u32 keys: earlier tiles read per look-back round: 16, 32 (default), 64 */
@@ -68,8 +72,6 @@ extern "C" {
                                       it takes about half the time of 2: for 8-byte keys from 2^14 to 2^20,
                                       but keys clustered in one bucket (a bucket larger than a workgroup's
                                       tile is sorted by that one workgroup) cost 4-60x */
-#define HPCS_OPT_PERSISTENT 6      /* reduce-then-scan scatter: 0 one workgroup per tile (default), 1 persistent
-                                      workgroups that prefetch the next tile */
 #define HPCS_OPT_DIGIT_BITS 8      /* 8 (default) or 11: 11-bit digits (3 passes for 4-byte keys, 6 for 8-byte
                                       keys) in the onesweep form, for sorts of keys without values below 2^30
                                       keys; other sorts keep 8-bit digits */
@@ -77,15 +79,6 @@ extern "C" {
                                       byte and the next pass counts its tiles from those n bytes; 0 the next
                                       pass re-reads the keys; 2 (default) 1 for 8-byte keys with values, else 0
                                       (scratch holds the n bytes either way) */
-#define HPCS_OPT_LINE_CHUNK 10     /* reduce-then-scan scatter of 4-byte keys without values, 256x16 tiles:
-                                      0 (default) one workgroup per tile; k in 1..256: a workgroup scatters k
-                                      consecutive tiles and writes each digit's output in whole 128-byte lines,
-                                      carrying in LDS the keys that complete no line to the next tile
-                                      (measured on MI355X: 2.5x slower than 0 at 2^28 keys, since its LDS
-                                      allows 2 workgroups per CU instead of 5) */
-#define HPCS_OPT_COUNT_TILES 11    /* reduce-then-scan tile counts on 256x16 tiles (keys, not digit bytes):
-                                      tiles per counting workgroup, 1 (default), 2 or 4 (4: 4-byte keys; 8-byte
-                                      keys count 1 tile per workgroup there) */
 #define HPCS_OPT_GRAPHS 9          /* 1: graph cache for the device-resident sorts (hpcs_dev_sort_radix_*,
                                       hpcs_dev_sort_segments_u32) on a non-null stream: the first call with
                                       given pointers, size and stream runs directly, the second captures its
@@ -129,6 +122,12 @@ int32_t hpcs_ctx_destroy(hpcs_ctx *ctx);
 /* Release the context's cached scratch memory. */
 int32_t hpcs_ctx_trim(hpcs_ctx *ctx);
 int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_t with_values, hpcs_plan_info *info);
+/* The tile (threads x keys per thread) that every pass kernel of a sort of
+ * key_bytes (4 or 8) keys, with or without u32 values, runs with tile shape
+ * `shape` (HPCS_OPT_TILE_SHAPE); HPCS_E_UNSUPPORTED for a shape that does not
+ * serve that sort.  No device is needed.  No counterpart in the reference
+ * (a diagnostic of this library's kernel dispatch). */
+int32_t hpcs_tile_shape_query(int32_t shape, int32_t key_bytes, int32_t with_values, int32_t *block, int32_t *items);
 /* Set a kernel tuning option (HPCS_OPT_*) for later calls on this context;
  * no counterpart in the reference (whose knobs are static properties such as
  * SortRadixParallelWorkQuanta, RadixSortLsdParallel.cs:763). */

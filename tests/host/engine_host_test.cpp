@@ -81,7 +81,12 @@ static int tuning_test()
     CHECK(rows[0].bits == 8 && rows[1].bits == 8);
     CHECK(hpcs::parse_tuning("u32 65536 0 0 0 11  # eleven\nf64 9 1 0 1 8", rows, err) && rows.size() == 2);
     CHECK(rows[0].bits == 11 && rows[0].algo == 0 && rows[1].bits == 8 && rows[1].algo == 1);
-    const char *bad[] = {"u8 1 0 0 0", "u32 0 0 0 0", "u32 1 2 0 0", "u32 1 0 5 0", "u32 1 0 0 2", "u64 1 0 1 0",
+    // rows of the shape table: 5 (1024x16) serves 4-byte keys, 6 (1024x8)
+    // 8-byte keys and 4-byte keys with values, both with ballot ranking only
+    CHECK(hpcs::parse_tuning("u32 1 1 5 0\npairs32 2 1 6 0\nf64 3 0 6 0", rows, err) && rows.size() == 3);
+    CHECK(rows[0].shape == 5 && rows[1].shape == 6 && rows[2].shape == 6);
+    const char *bad[] = {"u8 1 0 0 0", "u32 0 0 0 0", "u32 1 2 0 0", "u32 1 0 7 0", "u32 1 0 0 2", "u64 1 0 1 0",
+                         "u32 1 1 5 1", "pairs32 1 1 5 0", "u32 1 1 6 0", "pairs64 1 1 6 0", "u32 1 0 -1 0",
                          "pairs32 1 0 3 1", "u32 1 0 0 1 9", "u32 1 0", "u32 -5 0 0 1",
                          "u32 1 1 0 0 11", "u32 1 0 4 0 11", "u32 1 0 0 1 11", "pairs64 1 0 0 0 11",
                          "u32 1 0 0 0 11 8", "u32 1 0 0 0 x",

@@ -140,3 +140,38 @@ def test_partition_radix_argument_checks():
     assert a.tolist() == [3, 1, 2]
     assert issubclass(ArgumentNullException, ArgumentException)
     assert issubclass(ArgumentOutOfRangeException, ArgumentException)
+
+
+def test_tile_shapes_dispatch_agrees_with_the_tuning_parser(tmp_path):
+    """Every tile shape the kernel dispatch accepts for a sort runs all of
+    that sort's pass kernels (upsweep, tile counts, scatter / onesweep) with
+    one (threads, keys per thread) tile; a shape the dispatch rejects is also
+    rejected in a tuning row for that sort, and the other way round.  No
+    device is used."""
+    if not os.path.exists(N.LIB_PATH):
+        pytest.skip("libhpcsort.so not built")
+    lib = N.lib()
+    kinds = {(4, 0): "u32", (4, 1): "pairs32", (8, 0): "u64", (8, 1): "pairs64"}
+    served = {k: [] for k in kinds}
+    for shape in range(-1, 12):
+        for (kb, v), dtype in kinds.items():
+            b, i = ctypes.c_int32(0), ctypes.c_int32(0)
+            rc = lib.hpcs_tile_shape_query(shape, kb, v, ctypes.byref(b), ctypes.byref(i))
+            row = tmp_path / f"row_{shape}_{dtype}.txt"
+            row.write_text(f"{dtype} 1024 1 {shape} 0\n")
+            rows = ctypes.c_int32(0)
+            parsed = lib.hpcs_tuning_check(str(row).encode(), ctypes.byref(rows)) == 0
+            if rc == N.HPCS_OK:
+                served[(kb, v)].append(shape)
+                assert b.value in (256, 512, 1024) and i.value in (8, 16, 32), (shape, dtype)
+                # the staging area of a tile fits one workgroup's LDS budget
+                assert b.value * i.value * (kb + 4 * v) <= 64 * 1024, (shape, dtype)
+                assert parsed, (shape, dtype)
+            else:
+                assert rc == N.HPCS_E_UNSUPPORTED, (shape, dtype, rc)
+                assert not parsed, (shape, dtype)
+    # shape 0 serves every sort; the 1024-thread rows exist
+    assert all(0 in s for s in served.values())
+    assert 5 in served[(4, 0)] and 6 in served[(8, 0)] and 6 in served[(4, 1)]
+    b, i = ctypes.c_int32(0), ctypes.c_int32(0)
+    assert lib.hpcs_tile_shape_query(0, 3, 0, ctypes.byref(b), ctypes.byref(i)) == N.HPCS_E_INVALID_ARG

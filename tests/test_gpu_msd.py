@@ -139,3 +139,25 @@ def test_plan_query_reports_the_split(mctx):
     assert info.passes == 4
     N.check(N.lib().hpcs_plan_query(mctx, N.HPCS_DTYPE_U64, 1 << 22, 0, ctypes.byref(info)))
     assert info.passes == 8
+
+
+@pytest.mark.parametrize("kind", ["u32", "u64"])
+@pytest.mark.parametrize("dist", ["uniform", "skewed", "partial"])
+def test_split_forced_lookback_fallback(oracle, kind, dist):
+    """The MSD split's look-back pass with every wait resolved by counting
+    the earlier tile locally (fallback threshold 0): still the oracle's
+    output, on spread and on clustered keys."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    ctx = N.new_context(0)
+    try:
+        N.set_option(N.HPCS_OPT_PASS_ALGO, 4, ctx)
+        N.set_option(N.HPCS_OPT_FALLBACK_ROUNDS, 0, ctx)
+        n = (1 << 20) + 4099 if kind == "u32" else (1 << 19) + 77
+        a = _keys(kind, n, 31 + n, dist)
+        ref = a.copy()
+        oracle.sort_radix_lsd(ref)
+        assert np.array_equal(_sort(ctx, a), ref), (kind, dist)
+    finally:
+        N.destroy_context(ctx)

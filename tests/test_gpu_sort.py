@@ -273,10 +273,11 @@ def test_full_size_2pow28_properties():
     assert int(h_in.view(4, 256).sum(dim=1).min()) == n
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6])
 def test_forced_lookback_fallback(oracle, shape):
     """Every look-back wait resolved by counting the earlier tile locally
-    (fallback threshold 0) still gives the oracle's output."""
+    (fallback threshold 0) still gives the oracle's output, on every tile
+    shape (a shape that does not serve a key type leaves it on its default)."""
     ctx = N.new_context(0)
     try:
         N.set_option(N.HPCS_OPT_PASS_ALGO, 0, ctx)          # single-pass look-back
@@ -304,7 +305,7 @@ def test_forced_lookback_fallback(oracle, shape):
 
 
 @pytest.mark.parametrize("algo", [0, 1])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
 def test_tile_shapes_u32(oracle, shape, algo):
     ctx = N.new_context(0)
     try:
@@ -321,12 +322,14 @@ def test_tile_shapes_u32(oracle, shape, algo):
 
 
 @pytest.mark.parametrize("algo", [0, 1])
-def test_small_tile_shape_eight_byte_keys(oracle, algo):
-    """The 256x8 tile shape is accepted for 8-byte keys and for pairs, and
-    gives the oracle's output in both pass algorithms."""
+@pytest.mark.parametrize("shape", [4, 6])
+def test_small_tile_shape_eight_byte_keys(oracle, algo, shape):
+    """The 256x8 tile shape (any sort) and the 1024x8 shape (8-byte keys,
+    4-byte keys with values) give the oracle's output in both pass
+    algorithms, for 8-byte keys and for pairs."""
     ctx = N.new_context(0)
     try:
-        N.set_option(N.HPCS_OPT_TILE_SHAPE, 4, ctx)
+        N.set_option(N.HPCS_OPT_TILE_SHAPE, shape, ctx)
         N.set_option(N.HPCS_OPT_PASS_ALGO, algo, ctx)
         for n in (2047, 2048 * 3 + 1, (1 << 21) + 5):
             a = datagen.keys_u64(n, 7 + n)
@@ -340,11 +343,17 @@ def test_small_tile_shape_eight_byte_keys(oracle, algo):
             N.check(N.lib().hpcs_sort_radix_pairs_u64_u32(ctx, k.ctypes.data_as(ctypes.c_void_p),
                                                           v.ctypes.data_as(ctypes.c_void_p), n))
             assert np.array_equal(v, order.astype(np.uint32)), n
+            k32 = datagen.keys_u32(n, 13 + n) & np.uint32(0xF0F0F)
+            v32 = np.arange(n, dtype=np.uint32)
+            order32 = np.argsort(k32, kind="stable")
+            N.check(N.lib().hpcs_sort_radix_pairs_u32_u32(ctx, k32.ctypes.data_as(ctypes.c_void_p),
+                                                          v32.ctypes.data_as(ctypes.c_void_p), n))
+            assert np.array_equal(v32, order32.astype(np.uint32)), n
     finally:
         N.destroy_context(ctx)
 
 
-@pytest.mark.parametrize("mode", ["bytes", "keys", "persistent"])
+@pytest.mark.parametrize("mode", ["bytes", "keys"])
 def test_rts_next_digit_bytes(oracle, mode):
     """Reduce-then-scan passes count their tiles from the digit bytes the
     previous moving pass wrote (or from the keys): the same output as the
@@ -356,7 +365,6 @@ def test_rts_next_digit_bytes(oracle, mode):
         with pytest.raises(N.HpcsError):
             N.set_option(N.HPCS_OPT_DIGIT_BYTES, 3, ctx)
         N.set_option(N.HPCS_OPT_DIGIT_BYTES, 0 if mode == "keys" else 1, ctx)
-        N.set_option(N.HPCS_OPT_PERSISTENT, 1 if mode == "persistent" else 0, ctx)
         n = 3 * (1 << 20) + 4097                    # a ragged last tile
         base = datagen.keys_u32(n, 808)
         for mask in (0xFFFFFFFF, 0xFFFFFF00, 0x00FF00FF, 0xFF000000, 0x0000FF00):
