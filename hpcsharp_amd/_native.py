@@ -77,6 +77,7 @@ SIGNATURES = {
     "hpcs_sort_segments_u32": ([_p, _p, _p, _i32], _i32),
     "hpcs_dev_sort_segments_u32": ([_p, _p, _i64, _p, _i32, _p], _i32),
     "hpcs_dev_sort_radix_u32": ([_p, _p, _i64, _p], _i32),
+    "hpcs_dev_sort_radix_u32_noalloc": ([_p, _p, _i64, _p], _i32),
     "hpcs_dev_sort_radix_u64": ([_p, _p, _i64, _p], _i32),
     "hpcs_dev_sort_radix_f32": ([_p, _p, _i64, _p], _i32),
     "hpcs_dev_sort_radix_f64": ([_p, _p, _i64, _p], _i32),

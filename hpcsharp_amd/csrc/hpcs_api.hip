@@ -114,8 +114,25 @@ size_t align_up(size_t x, size_t a = ALIGN) { return (x + a - 1) / a * a; }
 
 } // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of hpcs_dev_sort_radix_u32_noalloc: a device array in the code
+// object, one copy per GPU, held until the process ends.  Host code never
+// passes `scratch` itself to a kernel or a copy (on the host the name is a
+// stand-in, not a device address): each context resolves the array's device
+// address on its GPU once, at creation, with hipGetSymbolAddress, and the
+// entry point passes that address.  Every call on one GPU shares the array,
+// so overlapping calls on one GPU race; the sort clears what must start at
+// zero itself.  Largest sort: 2^22 keys (the layout of a 2^22-key sort with
+// 2048-key tiles, the smallest tile, is 24.3 MiB).
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_MAX_KEYS = int64_t(1) << 22;
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(26) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 struct hpcs_ctx {
     int device = 0;
+    void *noalloc_scratch = nullptr; // device address of `scratch` on this context's GPU
+    std::string noalloc_error;       // why it is null
     int num_cus = 256;
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -567,8 +584,15 @@ void run_sort_msd(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st, const SortP
     timed(c, st, "msd_buckets", [&] { launch_sort_buckets<K, FLOATK>(n, st, d_keys, keysB, plan); });
 }
 
+// Scratch memory a caller hands to run_sort instead of the context's pool
+// (the no-allocation entry point): its device address and size.
+struct ExtScratch {
+    void *ptr;
+    size_t bytes;
+};
+
 template <typename K, bool HAS_VALS, int FLOATK>
-void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
+void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st, const ExtScratch *ext = nullptr)
 {
     if (n < 0) fail(HPCS_E_INVALID_ARG, "n must be non-negative");
     if (n > 0 && !d_keys) fail(HPCS_E_INVALID_ARG, "keys is null");
@@ -577,7 +601,16 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     if (n >= MAX_KEYS) fail(HPCS_E_UNSUPPORTED, "n must be below 2^31");
     if (reinterpret_cast<uintptr_t>(d_keys) % sizeof(K)) fail(HPCS_E_INVALID_ARG, "keys pointer is misaligned");
     constexpr int P = int(sizeof(K));
-    const SortParams prm = resolve_params(c, int(sizeof(K)), HAS_VALS, n);
+    SortParams prm = resolve_params(c, int(sizeof(K)), HAS_VALS, n);
+    if (ext) {
+        // caller-provided scratch: the 8-bit passes (whose layout is
+        // make_layout's), never the 11-bit or MSD-split layouts
+        prm.bits = 8;
+        if (prm.algo == PASS_MSD)
+            prm.algo = n <= (sizeof(K) == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8) ? PASS_BLOCK
+                       : n < (sizeof(K) == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21)) ? PASS_ONESWEEP
+                                                                                         : PASS_RTS;
+    }
     if (prm.algo == PASS_BLOCK) {
         static_assert(BLOCK_SORT_MAX4 == 1024 * 16 && BLOCK_SORT_MAX8 == 1024 * 8, "k_sort_block sizes");
         return run_sort_block<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
@@ -589,7 +622,8 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
     const Layout L = make_layout(n, sizeof(K), HAS_VALS, tile_keys);
-    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
+    if (ext && L.total > ext->bytes) fail(HPCS_E_UNSUPPORTED, "the sort's scratch exceeds the static scratch");
+    char *base = ext ? static_cast<char *>(ext->ptr) : static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
     uint32_t *hist = reinterpret_cast<uint32_t *>(base + L.off_hist);
     DevPlan *plan = reinterpret_cast<DevPlan *>(base + L.off_plan);
     uint32_t *lb[2] = {reinterpret_cast<uint32_t *>(base + L.off_lb0), reinterpret_cast<uint32_t *>(base + L.off_lb1)};
@@ -1243,6 +1277,27 @@ void host_histogram(hpcs_ctx *c, const K *a, int64_t l, int64_t r, int32_t *coun
     for (int i = 0; i < P * RADIX; ++i) counts[i] = int32_t(tmp[i]);
 }
 
+// The device address of `scratch` on `device` (current), resolved once per
+// device and kept; null (with the reason in `why`) if the runtime gives none,
+// which fails only the no-allocation entry point, not the context.
+void *noalloc_scratch_address(int device, std::string &why)
+{
+    static std::mutex mu;
+    static std::map<int, void *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError(); // leave no sticky error for later launches
+        why = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return nullptr;
+    }
+    addr[device] = p;
+    return p;
+}
+
 template <typename F>
 int32_t guarded_host(F &&f)
 {
@@ -1329,6 +1384,7 @@ HPCS_API int32_t hpcs_ctx_create(int32_t device, hpcs_ctx **out)
             if (!parse_tuning(text, c->tune, err)) fail(HPCS_E_INVALID_ARG, "HPCS_TUNE_FILE: " + err);
         }
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        c->noalloc_scratch = noalloc_scratch_address(device, c->noalloc_error);
         *out = c;
         return HPCS_OK;
     } catch (const Err &e) {
@@ -1672,6 +1728,26 @@ HPCS_API int32_t hpcs_dev_sort_segments_u32(hpcs_ctx *ctx, uint32_t *d_keys, int
 HPCS_API int32_t hpcs_dev_sort_radix_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] { dev_sort<uint32_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });
+}
+HPCS_API int32_t hpcs_dev_sort_radix_u32_noalloc(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream)
+{
+    return guarded(ctx, [&] {
+        if (n > NOALLOC_MAX_KEYS)
+            fail(HPCS_E_UNSUPPORTED, "hpcs_dev_sort_radix_u32_noalloc sorts at most 2^22 keys (n = " +
+                                         std::to_string(n) + ")");
+        if (!ctx->noalloc_scratch) fail(HPCS_E_HIP, "the static scratch has no device address: " + ctx->noalloc_error);
+        // no event pool, graph cache or context scratch: nothing allocates
+        const bool profile = ctx->profile;
+        ctx->profile = false;
+        const ExtScratch ext{ctx->noalloc_scratch, NOALLOC_SCRATCH_BYTES};
+        try {
+            run_sort<uint32_t, false, KIND_UINT>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream), &ext);
+        } catch (...) {
+            ctx->profile = profile;
+            throw;
+        }
+        ctx->profile = profile;
+    });
 }
 HPCS_API int32_t hpcs_dev_sort_radix_u64(hpcs_ctx *ctx, uint64_t *d_keys, int64_t n, void *stream)
 {

@@ -212,6 +212,17 @@ int32_t hpcs_sort_segments_u32(hpcs_ctx *ctx, uint32_t *keys, const int64_t *seg
 
 /* The same sorts on device buffers (d_* are device pointers). */
 int32_t hpcs_dev_sort_radix_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream);
+/* hpcs_dev_sort_radix_u32 that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch is a static device array of 26 MiB in the library (one per GPU,
+ * held until the process ends), so n is at most 2^22 keys
+ * (HPCS_E_UNSUPPORTED beyond).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other contexts or streams) race, so the
+ * caller orders them.  Options and the tuning table apply as for
+ * hpcs_dev_sort_radix_u32, with 8-bit digits and no MSD split; profiling and
+ * the graph cache do not record this call.  Same output, bit for bit. */
+int32_t hpcs_dev_sort_radix_u32_noalloc(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream);
 int32_t hpcs_dev_sort_radix_u64(hpcs_ctx *ctx, uint64_t *d_keys, int64_t n, void *stream);
 int32_t hpcs_dev_sort_radix_f32(hpcs_ctx *ctx, float *d_keys, int64_t n, void *stream);
 int32_t hpcs_dev_sort_radix_f64(hpcs_ctx *ctx, double *d_keys, int64_t n, void *stream);
