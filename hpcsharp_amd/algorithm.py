@@ -107,15 +107,77 @@ def _sort_range(a: np.ndarray, startIndex: int, length: int, ctx=None) -> None:
         _call(getattr(N.lib(), _SORT[a.dtype]), _ptr(sub), sub.size, ctx=ctx)
 
 
+def _extract_keys(items, getKey, start: int, length: int, keyBits: int | None = None) -> np.ndarray:
+    """getKey (the reference's Func<T, UInt32 / UInt64>) applied on the host
+    to items[start, start + length): uint32 keys for keyBits 32 (a UInt32
+    extractor), uint64 for 64, and without keyBits uint32 when every key fits
+    32 bits (the sorted order is the same either way).  A key the width does
+    not hold raises ArgumentOutOfRangeException."""
+    if keyBits not in (None, 32, 64):
+        raise ArgumentOutOfRangeException("keyBits must be 32 or 64")
+    vals = [int(getKey(items[start + i])) for i in range(length)]
+    limit = 1 << (keyBits or 64)
+    if any(v < 0 or v >= limit for v in vals):
+        raise ArgumentOutOfRangeException(f"getKey returned a value outside UInt{keyBits or 64}")
+    wide = keyBits == 64 or (keyBits is None and any(v >= 1 << 32 for v in vals))
+    return np.array(vals, dtype=np.uint64 if wide else np.uint32)
+
+
+def _order_by_keys(keys: np.ndarray) -> np.ndarray:
+    """The stable order of keys: (key, index) pairs sorted on the GPU, the
+    indices returned (keys is sorted in place too)."""
+    idx = np.arange(keys.size, dtype=np.uint32)
+    fn = N.lib().hpcs_sort_radix_pairs_u64_u32 if keys.dtype == np.uint64 else N.lib().hpcs_sort_radix_pairs_u32_u32
+    _call(fn, _ptr(keys), _ptr(idx), keys.size)
+    return idx
+
+
+def _sort_by_extractor(items, getKey, start: int = 0, length: int | None = None):
+    """Stable sort of items[start, start + length) by getKey, in place: the
+    keys are extracted on the host (a delegate cannot run on the GPU), the
+    (key, index) pairs sorted on the GPU, the items gathered on the host."""
+    if items is None:
+        raise ArgumentNullException("inputArray")
+    if getKey is None:
+        raise ArgumentNullException("getKey")
+    if length is None:
+        length = len(items) - start
+    if start < 0 or length < 0 or start + length > len(items):
+        raise ArgumentOutOfRangeException("start/length outside the array")
+    idx = _order_by_keys(_extract_keys(items, getKey, start, length)).astype(np.int64) + start
+    if isinstance(items, np.ndarray):
+        items[start:start + length] = items[idx]
+    else:
+        items[start:start + length] = [items[i] for i in idx]
+    return items
+
+
 class Algorithm:
-    """Serial-surface methods (namespace HPCsharp, class Algorithm)."""
+    """Serial-surface methods (namespace HPCsharp, class Algorithm).
+
+    Key-extractor overloads take getKey as a Python callable (the reference's
+    Func<T, UInt32 / UInt64>) over a numpy array (the reference's T[]: sorted
+    in place and returned) or, where the reference has a List<T> overload, a
+    Python list (a new sorted list is returned, as the reference returns a new
+    List<T>)."""
 
     # -- uint[] / ulong[] LSD --------------------------------------------------
     @staticmethod
-    def SortRadixLsd(inOutArray: np.ndarray, startIndex: int | None = None, length: int | None = None) -> None:
+    def SortRadixLsd(inOutArray, startIndex=None, length: int | None = None):
         """SortRadixLsd(this uint[]) RadixSortLsd.cs:255-260; (start, length) :242-247
         with the Experimental overload's sub-range semantics (:1409-1457);
-        ulong[] :424-442; int[] / long[] :267-357, 444-532 (signed order)."""
+        ulong[] :424-442; int[] / long[] :267-357, 444-532 (signed order).
+        SortRadixLsd(List<uint>) :615-623: a Python list of keys gives a new
+        sorted list.  SortRadixLsd<T>(this T[], Func<T, UInt32 / UInt64>)
+        :714-750, :783-820: a callable second argument sorts the items stably
+        by it (all 64 key bits; the reference's UInt64 form stops after four
+        bytes, :795, :814) and returns them."""
+        if callable(startIndex):
+            return _sort_by_extractor(inOutArray, startIndex)
+        if isinstance(inOutArray, list):
+            a = np.array(inOutArray, dtype=np.uint32)
+            _sort_whole(a)
+            return [int(x) for x in a]
         _require(inOutArray, "inOutArray", (np.uint32, np.uint64, np.int32, np.int64))
         if startIndex is None:
             _sort_whole(inOutArray)
@@ -161,13 +223,42 @@ class Algorithm:
         return inOutArray
 
     @staticmethod
-    def SortRadix(inputArray: np.ndarray, keys: np.ndarray | None = None):
-        """Two reference shapes share this name:
+    def SortRadix(inputArray, keys=None, length: int | None = None, outputArray=None, getKey=None):
+        """The reference shapes that share this name:
         - SortRadix(this uint[]) -- NuGet 3.17.0's name for SortRadixLsdFunc
           (HPCsharpExamples Program.cs:26): sorts in place, returns the array;
         - SortRadix<T>(T[] items, UInt32[]/UInt64[] keys) RadixSortLsd.cs:
           1183-1299: items (any fixed-size dtype, e.g. a structured array)
-          sorted stably by a parallel key array, returned as (items, keys)."""
+          sorted stably by a parallel key array, returned as (items, keys);
+          with a list of items, SortRadix<T>(List<T>, keys) :1307-1338: a new
+          list and the sorted keys;
+        - SortRadix<T>(List<T>, Func<T, UInt32 / UInt64>) :1151-1182: a new
+          list sorted stably by the callable;
+        - SortRadix<T>(T[] in, start, length, T[] out, Func<T, UInt32>)
+          :1091-1135, called SortRadix(a, start, length, out, getKey): the
+          range sorted in `a` and copied to `out` (the reference leaves it in
+          both arrays)."""
+        if getKey is not None or outputArray is not None:
+            start = keys
+            if inputArray is None or outputArray is None:
+                raise ArgumentNullException("inputArray" if inputArray is None else "outputArray")
+            if start is None or length is None:
+                raise ArgumentOutOfRangeException("start/length")
+            if start < 0 or length < 0 or start + length > min(len(inputArray), len(outputArray)):
+                raise ArgumentOutOfRangeException("start/length outside the arrays")
+            _sort_by_extractor(inputArray, getKey, start, length)
+            outputArray[start:start + length] = inputArray[start:start + length]
+            return None
+        if callable(keys):
+            if inputArray is None:
+                raise ArgumentNullException("inputList")
+            return _sort_by_extractor(list(inputArray), keys)
+        if isinstance(inputArray, list) and keys is not None:
+            _require(keys, "inKeys", (np.uint32, np.uint64))
+            if keys.size != len(inputArray):
+                raise ArgumentOutOfRangeException("inKeys must have one key per item")
+            idx = _order_by_keys(keys)
+            return [inputArray[i] for i in idx], keys
         if keys is None:
             _require(inputArray, "inputArray", tuple(_SORT))
             _sort_whole(inputArray)
@@ -245,16 +336,20 @@ class Algorithm:
 
     # -- key-extractor records -------------------------------------------------
     @staticmethod
-    def SortRadixLsd2(inputArray: np.ndarray, getKey: str) -> np.ndarray:
+    def SortRadixLsd2(inputArray, getKey):
         """SortRadixLsd2<T>(this T[], Func<T,UInt32/UInt64>) RadixSortLsd.cs:751-775, 821-846.
 
-        ``inputArray`` is a numpy structured array (the unmanaged struct) and
-        ``getKey`` names its key field (uint32/uint64/float32/float64); the
-        records are sorted stably in place and the array is returned."""
+        With a callable getKey (the Func), the items (a numpy array of any
+        dtype, or a list) are sorted stably in place by it and returned.  With
+        a field name of a numpy structured array (the unmanaged struct form;
+        uint32/uint64/float32/float64 key), the GPU reads the key from each
+        record and moves the records."""
         if inputArray is None:
             raise ArgumentNullException("inputArray")
         if getKey is None:
             raise ArgumentNullException("getKey")
+        if callable(getKey):
+            return _sort_by_extractor(inputArray, getKey)
         if not isinstance(inputArray, np.ndarray) or inputArray.dtype.fields is None:
             raise TypeError("inputArray must be a numpy structured array")
         if not inputArray.flags["C_CONTIGUOUS"]:
@@ -270,11 +365,23 @@ class Algorithm:
         return inputArray
 
     SortRadixLsd_T = SortRadixLsd2
-    SortRadixFaster = SortRadixLsd2
+    SortRadixFaster2 = SortRadixLsd2
+
+    @staticmethod
+    def SortRadixFaster(inputArray, getKey):
+        """SortRadixFaster<T>(this T[], Func<T, UInt32 / UInt64>) RadixSortLsd.cs:
+        861-915, 961-1014 (all 64 key bits; the reference's UInt64 form stops
+        after four bytes, :975, :1006): as SortRadixLsd2.  With a list,
+        SortRadixFaster<T>(List<T>, Func) :1059-1090: a new sorted list."""
+        if isinstance(inputArray, list):
+            if getKey is None:
+                raise ArgumentNullException("getKey")
+            return _sort_by_extractor(list(inputArray), getKey)
+        return Algorithm.SortRadixLsd2(inputArray, getKey)
 
     # -- histograms ------------------------------------------------------------
     @staticmethod
-    def HistogramByteComponents(inArray: np.ndarray, l: int, r: int, getKey: str | None = None) -> np.ndarray:
+    def HistogramByteComponents(inArray, l: int, r: int, getKey=None, keyBits: int = 32) -> np.ndarray:
         """HistogramByteComponents(uint[]/ulong[], int l, int r) Histogram.cs:261-296, 957-990:
         int32[digits][256] of [l, r] inclusive; int[] / long[] :992-1052 count
         the top byte ^ 128.  With getKey (a field name of a structured array,
@@ -282,6 +389,8 @@ class Algorithm:
         bytes of each record's key are counted."""
         if inArray is None:
             raise ArgumentNullException("inArray")
+        if callable(getKey):
+            return Algorithm.HistogramByteComponentsAndKeyArray(inArray, l, r, getKey, keyBits)[0]
         if getKey is not None:
             if not isinstance(inArray, np.ndarray) or inArray.dtype.names is None or getKey not in inArray.dtype.names:
                 raise TypeError("getKey must name a field of a structured array")
@@ -302,6 +411,28 @@ class Algorithm:
         fn = getattr(N.lib(), "hpcs_histogram_byte_components_" + ("i" if inArray.dtype.kind == "i" else "u") + str(8 * P))
         _call(fn, _ptr(inArray), l, r, _ptr(out))
         return out
+
+    @staticmethod
+    def HistogramByteComponentsAndKeyArray(inArray, l: int, r: int, getKey, keyBits: int = 32):
+        """HistogramByteComponentsAndKeyArray<T>(T[], l, r, Func<T, UInt32 /
+        UInt64>) Histogram.cs:846-934 (and HistogramByteComponents<T> with a
+        Func, :760-844): the callable's keys of [l, r] extracted on the host,
+        their byte components counted on the GPU.  Returns (int32[digits][256],
+        keys), keys holding each key at its index (0 outside [l, r]); all 8
+        bytes of 64-bit keys are counted (the reference's UInt64 form sizes 4
+        rows and throws, :894, :904).  keyBits (32 or 64) is the extractor's
+        result type, UInt32 or UInt64."""
+        if inArray is None:
+            raise ArgumentNullException("inArray")
+        if getKey is None:
+            raise ArgumentNullException("getKey")
+        if l < 0 or (r >= l and r >= len(inArray)):
+            raise IndexError("l..r outside the array")
+        span = max(0, r - l + 1)
+        k = _extract_keys(inArray, getKey, l, span, keyBits)
+        keys = np.zeros(len(inArray), dtype=k.dtype)
+        keys[l:l + span] = k
+        return Algorithm.HistogramByteComponents(keys, l, r), keys
 
     @staticmethod
     def Histogram11bitComponents(inArray: np.ndarray, l: int, r: int) -> np.ndarray:

@@ -175,3 +175,162 @@ def test_tile_shapes_dispatch_agrees_with_the_tuning_parser(tmp_path):
     assert 5 in served[(4, 0)] and 6 in served[(8, 0)] and 6 in served[(4, 1)]
     b, i = ctypes.c_int32(0), ctypes.c_int32(0)
     assert lib.hpcs_tile_shape_query(0, 3, 0, ctypes.byref(b), ctypes.byref(i)) == N.HPCS_E_INVALID_ARG
+
+
+# ---- the C# binding in INTEGRATION.md ----------------------------------------
+
+_CS_WIDTH = {"int": ("int", 4), "uint": ("int", 4), "long": ("int", 8), "ulong": ("int", 8)}
+_C_WIDTH = {"int32_t": ("int", 4), "int64_t": ("int", 8), "uint32_t": ("int", 4), "uint64_t": ("int", 8),
+            "int": ("int", 4)}
+
+
+def _cs_param_kind(p: str):
+    p = p.strip()
+    toks = p.replace("*", " * ").split()
+    if toks[0] == "out" or "*" in toks or toks[0] == "IntPtr":
+        return ("ptr", 8)
+    return _CS_WIDTH[toks[0]]
+
+
+def _c_param_kind(p: str):
+    p = p.strip()
+    if "*" in p:
+        return ("ptr", 8)
+    toks = [t for t in p.split() if t != "const"]
+    return _C_WIDTH[toks[0]]
+
+
+def _c_prototypes() -> dict[str, list]:
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    protos = {}
+    for m in re.finditer(r"\b(?:int32_t|const char\s*\*)\s*(hpcs_\w+)\s*\(([^)]*)\)\s*;", text):
+        args = [a for a in m.group(2).split(",") if a.strip() and a.strip() != "void"]
+        protos[m.group(1)] = [_c_param_kind(a) for a in args]
+    return protos
+
+
+def _csharp_source() -> str:
+    with open(os.path.join(REPO, "INTEGRATION.md")) as f:
+        text = f.read()
+    blocks = re.findall(r"```csharp\n(.*?)```", text, flags=re.S)
+    assert blocks, "no C# block in INTEGRATION.md"
+    return "\n".join(blocks)
+
+
+def test_csharp_library_imports_match_the_header():
+    """Every [LibraryImport] prototype of the C# binding has the header's
+    parameter count, and at each position a pointer for a pointer and an
+    integer of the same width for an integer."""
+    protos = _c_prototypes()
+    src = _csharp_source()
+    imports = re.findall(r"\[LibraryImport\(Lib\)\]\s*internal static (?:unsafe )?partial (\w+) (hpcs_\w+)\(([^)]*)\);",
+                         src)
+    assert len(imports) >= 20, len(imports)
+    for ret, name, params in imports:
+        assert name in protos, name
+        cs = [_cs_param_kind(p) for p in params.split(",") if p.strip()]
+        assert cs == protos[name], (name, cs, protos[name])
+        assert ret in ("int", "IntPtr"), (name, ret)
+
+
+def _normalize_cs_type(t: str) -> str:
+    t = re.sub(r"\s+", "", t)
+    for a, b in (("UInt32", "uint"), ("UInt64", "ulong"), ("Int32", "int"), ("Int64", "long")):
+        t = re.sub(rf"\b{a}\b", b, t)
+    return t
+
+
+def _cs_methods(src: str) -> set[tuple]:
+    out = set()
+    for m in re.finditer(r"public static [^(=]*?\b(\w+)(?:<T>)?\(([^)]*)\)", src):
+        params = []
+        depth, cur = 0, ""
+        for ch in m.group(2):
+            if ch == "<":
+                depth += 1
+            elif ch == ">":
+                depth -= 1
+            if ch == "," and depth == 0:
+                params.append(cur)
+                cur = ""
+            else:
+                cur += ch
+        if cur.strip():
+            params.append(cur)
+        types = []
+        for p in params:
+            p = p.split("=")[0].strip()
+            if p.startswith("this "):
+                p = p[5:]
+            types.append(_normalize_cs_type(p.rsplit(" ", 1)[0]))
+        out.add((m.group(1), tuple(types)))
+    return out
+
+
+# The reference's drop-in surface for this path (HPCsharp/RadixSortLsd.cs,
+# RadixSortLsdParallel.cs, Histogram.cs, HistogramParallel.cs; line of each):
+# name and parameter types, which a caller's code binds to.
+REFERENCE_OVERLOADS = [
+    ("SortRadixLsd", ("uint[]",)),                                   # RadixSortLsd.cs:255
+    ("SortRadixLsd", ("uint[]", "int", "int")),                      # :242
+    ("SortRadixLsdFunc", ("uint[]",)),                               # :227
+    ("SortRadixLsdFunc", ("uint[]", "int", "int")),                  # :213
+    ("SortRadixLsd", ("int[]",)), ("SortRadixLsd", ("int[]", "int", "int")),          # :352, :339
+    ("SortRadixLsdFunc", ("int[]",)), ("SortRadixLsdFunc", ("int[]", "int", "int")),  # :324, :310
+    ("SortRadixLsd", ("ulong[]",)), ("SortRadixLsd", ("ulong[]", "int", "int")),      # :437, :424
+    ("SortRadixLsdFunc", ("ulong[]",)), ("SortRadixLsdFunc", ("ulong[]", "int", "int")),  # :409, :395
+    ("SortRadixLsd", ("long[]",)), ("SortRadixLsd", ("long[]", "int", "int")),        # :527, :514
+    ("SortRadixLsdFunc", ("long[]",)), ("SortRadixLsdFunc", ("long[]", "int", "int")),    # :500, :486
+    ("SortRadixLsd", ("List<uint>",)),                               # :615
+    ("SortRadixLsd", ("T[]", "Func<T,uint>")),                       # :714
+    ("SortRadixLsd2", ("T[]", "Func<T,uint>")),                      # :751
+    ("SortRadixLsd", ("T[]", "Func<T,ulong>")),                      # :783
+    ("SortRadixLsd2", ("T[]", "Func<T,ulong>")),                     # :821
+    ("SortRadixFaster", ("T[]", "Func<T,uint>")),                    # :861
+    ("SortRadixFaster2", ("T[]", "Func<T,uint>")),                   # :917
+    ("SortRadixFaster", ("T[]", "Func<T,ulong>")),                   # :961
+    ("SortRadixFaster2", ("T[]", "Func<T,ulong>")),                  # :1016
+    ("SortRadixFaster", ("List<T>", "Func<T,uint>")),                # :1059
+    ("SortRadixFaster", ("List<T>", "Func<T,ulong>")),               # :1075
+    ("SortRadix", ("T[]", "int", "int", "T[]", "Func<T,uint>")),     # :1091
+    ("SortRadix", ("List<T>", "Func<T,uint>")),                      # :1151
+    ("SortRadix", ("List<T>", "Func<T,ulong>")),                     # :1167
+    ("SortRadix", ("T[]", "uint[]")),                                # :1183
+    ("SortRadix", ("T[]", "ulong[]")),                               # :1245
+    ("SortRadix", ("List<T>", "uint[]")),                            # :1307
+    ("SortRadix", ("List<T>", "ulong[]")),                           # :1323
+    ("SortRadixLsd11bit", ("uint[]",)),                              # :1540
+    ("SortRadixLsdNbit", ("uint[]", "int", "int", "int")),           # :1616
+    ("SortRadixLsdNbit", ("ulong[]", "int", "int", "int")),          # :1839
+    ("SortRadixPar", ("uint[]", "int")),                             # RadixSortLsdParallel.cs:145
+    ("SortRadixPar", ("int[]",)),                                    # :446
+    ("SortRadixPar", ("long[]",)),                                   # :560
+    ("SortRadixPar", ("ulong[]",)),                                  # :690
+    ("HistogramByteComponents", ("uint[]", "int", "int")),           # Histogram.cs:261
+    ("Histogram11bitComponents", ("uint[]", "int", "int")),          # :298
+    ("HistogramNbitComponents", ("uint[]", "int", "int", "int")),    # :381
+    ("HistogramNbitComponents", ("ulong[]", "int", "int", "int")),   # :405
+    ("HistogramByteComponents", ("T[]", "int", "int", "Func<T,uint>")),    # :760
+    ("HistogramByteComponents", ("T[]", "int", "int", "Func<T,ulong>")),   # :799
+    ("HistogramByteComponentsAndKeyArray", ("T[]", "int", "int", "Func<T,uint>")),   # :846
+    ("HistogramByteComponentsAndKeyArray", ("T[]", "int", "int", "Func<T,ulong>")),  # :887
+    ("HistogramByteComponents", ("ulong[]", "int", "int")),          # :957
+    ("HistogramByteComponents", ("int[]", "int", "int")),            # :992
+    ("HistogramByteComponents", ("long[]", "int", "int")),           # :1019
+    ("HistogramByteComponentsPar", ("uint[]", "int", "int", "int")),   # HistogramParallel.cs:214
+    ("HistogramByteComponentsPar", ("int[]", "int", "int", "int")),    # :416
+    ("HistogramByteComponentsPar", ("ulong[]", "int", "int", "int")),  # :547
+    ("HistogramByteComponentsPar", ("long[]", "int", "int", "int")),   # :617
+    ("SelectTopK", ("uint[]", "int", "int", "int")),                 # SelectionTopKRadix.cs:142
+    ("SelectTopK", ("uint[]", "int")),                               # :160
+    ("SortRadixMsd", ("float[]",)), ("SortRadixMsd", ("double[]",)),   # RadixSortMsd.cs:1563, :1587
+]
+
+
+def test_csharp_binding_covers_the_reference_overloads():
+    """Every overload of the reference's surface for this path has a binding
+    method of the same name and parameter types, so a caller's code (e.g.
+    `a.SortRadixLsd2(e => e.Key)`) compiles against HPCsharp.Gpu unchanged."""
+    have = _cs_methods(_csharp_source())
+    missing = [m for m in REFERENCE_OVERLOADS if m not in have]
+    assert not missing, missing
