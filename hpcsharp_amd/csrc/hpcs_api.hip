@@ -764,7 +764,8 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     uint32_t *counts = reinterpret_cast<uint32_t *>(base + L.off_counts);
     uint32_t *keysB = reinterpret_cast<uint32_t *>(base + L.off_keysB);
     timed(c, st, "seg_plan", [&] {
-        hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, d_begin, nseg, n, sp, tile0, small, large);
+        hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, d_begin, nseg, n, sp, tile0, small, large,
+                           L.max_tiles);
     });
     const int list_grid = std::min(nseg, c->num_cus * 8);
     timed(c, st, "seg_tiles", [&] {

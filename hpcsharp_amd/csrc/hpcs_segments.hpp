@@ -32,7 +32,8 @@ struct SegPlan {
     uint32_t num_tiles; // tiles of the large segments
     uint32_t num_small; // small segments
     uint32_t num_large; // large segments
-    uint32_t bad;       // segments with end < begin or outside [0, n): skipped
+    uint32_t bad;       // segments skipped: end < begin, outside [0, n), or starting before
+                        // the end of an earlier segment (overlap)
     uint32_t pad[60];
 };
 
@@ -173,24 +174,51 @@ __device__ inline void tile_rank_stage_ldsor(const K (&keys)[ITEMS], int wave_va
 
 // One block of 1024 threads.  begin has nseg + 1 entries.  Per segment: its
 // first tile (seg_tile0) and its place in the small or large list.
+// A segment is sorted only if 0 <= begin <= end <= n and it starts at or
+// after the end of every earlier segment with valid bounds (a block-wide
+// running maximum of the ends), so the sorted segments never overlap, their
+// keys total at most n and their tiles at most n / TILE + nseg, the bound the
+// host sized the tile records and counts by (max_tiles; a plan beyond it
+// sorts no large segment).
 __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ begin, int nseg, int64_t n,
                                                    SegPlan *__restrict__ sp, uint32_t *__restrict__ seg_tile0,
                                                    uint32_t *__restrict__ small_list,
-                                                   uint32_t *__restrict__ large_list)
+                                                   uint32_t *__restrict__ large_list, uint64_t max_tiles)
 {
     __shared__ uint32_t s_wsum[16];
+    __shared__ int64_t s_wmax[16];
     __shared__ uint32_t s_carry[3];
+    __shared__ int64_t s_end_carry; // largest valid end of the earlier chunks
     __shared__ uint32_t s_bad;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
     if (tid < 3) s_carry[tid] = 0u;
-    if (tid == 0) s_bad = 0u;
+    if (tid == 0) {
+        s_bad = 0u;
+        s_end_carry = 0;
+    }
     __syncthreads();
     for (int base = 0; base < nseg; base += 1024) {
         const int s = base + tid;
         uint32_t tiles = 0, small = 0;
+        const int64_t b = s < nseg ? begin[s] : 0, e = s < nseg ? begin[s + 1] : 0;
+        const bool bounds_ok = s < nseg && b >= 0 && e >= b && e <= n;
+        // exclusive running maximum of the valid ends before segment s
+        int64_t m = bounds_ok ? e : 0;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const int64_t y = __shfl_up(m, off);
+            if (lane >= off) m = max(m, y);
+        }
+        if (lane == WAVE - 1) s_wmax[wave] = m;
+        __syncthreads();
+        int64_t before = s_end_carry;
+        for (int w = 0; w < wave; ++w) before = max(before, s_wmax[w]);
+        int64_t prev_lane = __shfl_up(m, 1);
+        if (lane > 0) before = max(before, prev_lane);
+        __syncthreads();
+        if (tid == 1023) s_end_carry = max(before, bounds_ok ? e : int64_t(0));
         if (s < nseg) {
-            const int64_t b = begin[s], e = begin[s + 1];
-            if (b < 0 || e < b || e > n) {
+            if (!bounds_ok || b < before) {
                 atomicAdd(&s_bad, 1u);
             } else {
                 const int64_t len = e - b;
@@ -225,10 +253,11 @@ __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ b
     }
     __syncthreads();
     if (tid == 0) {
-        sp->num_tiles = s_carry[0];
+        const bool fits = uint64_t(s_carry[0]) <= max_tiles; // holds for non-overlapping segments
+        sp->num_tiles = fits ? s_carry[0] : 0u;
         sp->num_small = s_carry[1];
-        sp->num_large = s_carry[2];
-        sp->bad = s_bad;
+        sp->num_large = fits ? s_carry[2] : 0u;
+        sp->bad = s_bad + (fits ? 0u : s_carry[2]);
     }
 }
 

@@ -231,9 +231,12 @@ int32_t hpcs_dev_sort_radix_pairs_u32_u32(hpcs_ctx *ctx, uint32_t *d_keys, uint3
 int32_t hpcs_dev_sort_radix_records(hpcs_ctx *ctx, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset,
                                     int32_t key_bits, int32_t key_kind, void *stream);
 /* Batched segments on the device: d_seg_begin is a device array of nseg + 1
- * offsets into d_keys[0, n).  The launch sequence does not depend on the
- * segment lengths (planning runs on the device), so after one warm-up call
- * (which sizes the scratch) the call can be captured in a HIP graph. */
+ * non-decreasing offsets into d_keys[0, n).  The offsets are checked on the
+ * device: a segment outside [0, n), with end < begin, or starting before the
+ * end of an earlier segment is skipped (its keys untouched).  The launch
+ * sequence does not depend on the segment lengths (planning runs on the
+ * device), so after one warm-up call (which sizes the scratch) the call can
+ * be captured in a HIP graph. */
 int32_t hpcs_dev_sort_segments_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, const int64_t *d_seg_begin,
                                    int32_t nseg, void *stream);
 /* Byte-component histogram of d_keys[0, n): d_counts is uint32[4][256]

@@ -470,6 +470,51 @@ def test_segments_device_with_gaps_and_distributions(oracle):
     assert np.array_equal(t.cpu().numpy().view(np.uint32), ref)
 
 
+@pytest.mark.parametrize("m", [150, 200000])
+def test_segments_device_rejects_overlapping_offsets(m):
+    """Offsets that are not non-decreasing, [0, m, 0, m, ...] and [0, m, m/2,
+    3m/2, 2m, 2m + 500]: only segments within [0, n) that start at or after
+    the end of every earlier valid segment are sorted; every other key,
+    and a canary past the keys, is untouched, and the same context sorts a
+    normal batch correctly afterwards."""
+    import torch
+    n = 2 * m + 1000
+    canary = 64
+    a = datagen.keys_u32(n + canary, 0x5EED0400 + m)
+    for begin in (np.array([0, m, 0, m, 0, m, 0, m], dtype=np.int64),
+                  np.array([0, m, m // 2, m + m // 2, 2 * m, 2 * m + 500], dtype=np.int64)):
+        nseg = begin.size - 1
+        want = a.copy()
+        want[:m] = np.sort(a[:m])
+        if begin.size == 6:
+            # [m/2, 3m/2) overlaps [0, m) and is skipped; [3m/2, 2m) and
+            # [2m, 2m + 500) start at the end of an earlier segment: sorted
+            want[m + m // 2:2 * m] = np.sort(a[m + m // 2:2 * m])
+            want[2 * m:2 * m + 500] = np.sort(a[2 * m:2 * m + 500])
+        t = torch.from_numpy(a.view(np.int32).copy()).cuda()
+        b = torch.from_numpy(begin).cuda()
+        st = torch.cuda.current_stream().cuda_stream
+        N.check(N.lib().hpcs_dev_sort_segments_u32(N.context(), ctypes.c_void_p(t.data_ptr()), n,
+                                                   ctypes.c_void_p(b.data_ptr()), nseg, ctypes.c_void_p(st)))
+        torch.cuda.synchronize()
+        got = t.cpu().numpy().view(np.uint32)
+        assert np.array_equal(got[:n], want[:n]), begin.tolist()
+        assert np.array_equal(got[n:], a[n:]), "canary past the keys"
+    lens = datagen.segment_lengths(64, 3, 16, 1 << 16)
+    begin = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    k = datagen.keys_u32(int(begin[-1]), 4)
+    want = k.copy()
+    for s0, s1 in zip(begin[:-1], begin[1:]):
+        want[s0:s1] = np.sort(want[s0:s1])
+    t = torch.from_numpy(k.view(np.int32).copy()).cuda()
+    b = torch.from_numpy(begin).cuda()
+    N.check(N.lib().hpcs_dev_sort_segments_u32(N.context(), ctypes.c_void_p(t.data_ptr()), int(begin[-1]),
+                                               ctypes.c_void_p(b.data_ptr()), len(lens),
+                                               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy().view(np.uint32), want)
+
+
 def test_segments_graph_replays_match_eager(oracle):
     """The batched call captured once in a HIP graph: every replay (with eager
     work between replays) equals the eager result."""
