@@ -303,9 +303,20 @@ def cfg3_figures(torch, N, datagen, reps: int):
         N.check(lib.hpcs_dev_sort_radix_f64(ctx, ctypes.c_void_p(work.data_ptr()), n, st))
     t_copy = timed_calls(torch, lambda: work.copy_(src), reps)
     t = timed_calls(torch, f64, reps) - t_copy
-    w = work.cpu().numpy().view(np.float64)
-    ok = bool(np.all(w[1:] >= w[:-1])) and bool(np.signbit(w[np.nonzero(w == 0)[0][0]]))
-    out["f64_sort"] = {"ms": round(t * 1e3, 3), "gkeys_per_s": round(n / t / 1e9, 2), "sorted": ok}
+    # bit-exact against an independent sort: torch.sort of the radix keys
+    # (negative: all bits flipped; others: sign bit flipped) as order-
+    # preserving int64, mapped back to double bits; -0.0 sorts before +0.0
+    sign = -(1 << 63)
+    neg = src < 0
+    key = torch.where(neg, ~src, src ^ sign) ^ sign          # radix key, in signed order
+    want_key = torch.sort(key)[0] ^ sign                      # radix keys ascending (bits)
+    want = torch.where(want_key < 0, want_key ^ sign, ~want_key)   # inverse transform
+    ok = bool(torch.equal(work, want))
+    del neg, key, want_key, want
+    out["f64_sort"] = {"ms": round(t * 1e3, 3), "gkeys_per_s": round(n / t / 1e9, 2), "bit_exact": ok}
+    if not ok:
+        print("bench: f64 output differs from the independent sort", file=sys.stderr)
+        sys.exit(3)
     del src, work
     rec = np.zeros(n, dtype=np.dtype({"names": ["key", "payload"], "formats": [np.uint64, np.uint32],
                                       "offsets": [0, 8], "itemsize": 16}))
@@ -321,10 +332,16 @@ def cfg3_figures(torch, N, datagen, reps: int):
     t_copy = timed_calls(torch, lambda: rwork.copy_(rsrc), reps)
     t = timed_calls(torch, recs, reps) - t_copy
     r = rwork.cpu().numpy().view(rec.dtype)
+    ties = r["key"][1:] == r["key"][:-1]
     ok = bool(np.all(r["key"][1:] >= r["key"][:-1])) and bool(
-        np.array_equal(rec["key"][r["payload"]], r["key"]))
+        np.array_equal(np.sort(r["payload"]), rec["payload"])) and bool(
+        np.array_equal(rec["key"][r["payload"]], r["key"])) and bool(
+        np.all(r["payload"][1:][ties] > r["payload"][:-1][ties]))
     out["records_u64_key_u32_payload"] = {"ms": round(t * 1e3, 3), "gkeys_per_s": round(n / t / 1e9, 2),
-                                          "sorted_and_permutation": ok}
+                                          "sorted_stable_permutation": ok}
+    if not ok:
+        print("bench: record sort is not a stable sort of the input", file=sys.stderr)
+        sys.exit(3)
     return {"n": n, **out}
 
 

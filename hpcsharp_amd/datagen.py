@@ -96,3 +96,52 @@ def segment_lengths(nseg: int, seed: int, lo: int = 16, hi: int = 1 << 20) -> np
     """floor(exp(U(ln lo, ln hi))) segment lengths (log-uniform)."""
     u = uniform01(nseg, seed)
     return np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
+
+
+# ---- the same generators on a torch device (GPU parity tests at full size) ----
+# torch has no uint64 arithmetic: int64 holds the same bits, multiplication
+# wraps modulo 2^64, and a logical right shift is the arithmetic shift masked
+# to the low 64 - k bits.  The integer outputs equal splitmix64's bit for bit.
+
+def _as_i64(x: int) -> int:
+    x &= 0xFFFFFFFFFFFFFFFF
+    return x - (1 << 64) if x >= 1 << 63 else x
+
+
+def splitmix64_torch(seed: int, n: int, device="cuda", chunk: int = 1 << 26):
+    """int64 tensor holding the bits of splitmix64(seed) outputs [0, n)."""
+    import torch
+
+    def srl(z, k):
+        return (z >> k) & ((1 << (64 - k)) - 1)
+
+    out = torch.empty(n, dtype=torch.int64, device=device)
+    golden, m1, m2 = _as_i64(int(_GOLDEN)), _as_i64(int(_M1)), _as_i64(int(_M2))
+    s = _as_i64(seed)
+    for lo in range(0, n, chunk):
+        hi = min(n, lo + chunk)
+        z = torch.arange(lo + 1, hi + 1, dtype=torch.int64, device=device)
+        z.mul_(golden).add_(s)
+        z ^= srl(z, 30)
+        z.mul_(m1)
+        z ^= srl(z, 27)
+        z.mul_(m2)
+        z ^= srl(z, 31)
+        out[lo:hi] = z
+    return out
+
+
+def keys_u32_torch(n: int, seed: int, dist: str = "uniform", device="cuda"):
+    """keys_u32 as an int32 tensor of the same bits."""
+    import torch
+    if dist == "sorted":
+        return torch.arange(n, dtype=torch.int64, device=device).to(torch.int32)
+    if dist == "constant":
+        return torch.zeros(n, dtype=torch.int32, device=device)
+    low = splitmix64_torch(seed, n, device) & 0xFFFFFFFF
+    if dist == "low8":
+        c = int(splitmix64(seed ^ 0xC0FFEE, 1)[0]) & 0xFFFFFF00
+        low = (low & 0xFF) | c
+    elif dist != "uniform":
+        raise ValueError(f"unknown distribution {dist}")
+    return (low - ((low >> 31) << 32)).to(torch.int32)   # the low 32 bits as a signed int32
