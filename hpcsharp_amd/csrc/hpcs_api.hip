@@ -140,7 +140,7 @@ struct hpcs_ctx {
     size_t scratch_cap = 0;
     void *io = nullptr;
     size_t io_cap = 0;
-    int rank_mode = RANK_LDSOR;  // tuning knob, env HPCS_RANK=ballot|ldsor
+    int rank_mode = RANK_LDSOR;  // tuning knob, env HPCS_RANK=ballot|ldsor|mix2|mix4
     int tile_shape = 0;          // tuning knob, env HPCS_TILE_SHAPE: a row of TILE_SHAPES
     int fallback_rounds = FALLBACK_ROUNDS;
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
@@ -339,9 +339,17 @@ void launch_onesweep_w(int rank_mode, int fb, dim3 grid, hipStream_t st, K *a, K
     if (BLOCK >= 1024 || rank_mode == RANK_BALLOT)
         hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_BALLOT, W>), grid, dim3(BLOCK), 0, st,
                            a, b, va, vb, n, pass, plan, lb, next, fb, dig);
-    else if constexpr (BLOCK < 1024)
-        hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_LDSOR, W>), grid, dim3(BLOCK), 0, st,
-                           a, b, va, vb, n, pass, plan, lb, next, fb, dig);
+    else if constexpr (BLOCK < 1024) {
+        if (rank_mode == RANK_MIX2)
+            hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_MIX2, W>), grid, dim3(BLOCK), 0,
+                               st, a, b, va, vb, n, pass, plan, lb, next, fb, dig);
+        else if (rank_mode == RANK_MIX4)
+            hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_MIX4, W>), grid, dim3(BLOCK), 0,
+                               st, a, b, va, vb, n, pass, plan, lb, next, fb, dig);
+        else
+            hipLaunchKernelGGL((k_onesweep<K, HAS_VALS, FLOATK, BLOCK, ITEMS, RANK_LDSOR, W>), grid, dim3(BLOCK), 0,
+                               st, a, b, va, vb, n, pass, plan, lb, next, fb, dig);
+    }
 }
 
 template <typename K, bool HAS_VALS, int FLOATK, int BLOCK, int ITEMS>
@@ -1371,7 +1379,10 @@ HPCS_API int32_t hpcs_ctx_create(int32_t device, hpcs_ctx **out)
         hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         if (const char *r = std::getenv("HPCS_RANK")) {
-            c->rank_mode = std::strcmp(r, "ballot") == 0 ? RANK_BALLOT : RANK_LDSOR;
+            c->rank_mode = std::strcmp(r, "ballot") == 0 ? RANK_BALLOT
+                           : std::strcmp(r, "mix2") == 0  ? RANK_MIX2
+                           : std::strcmp(r, "mix4") == 0  ? RANK_MIX4
+                                                          : RANK_LDSOR;
             c->set_rank = true;
         }
         if (const char *t = std::getenv("HPCS_TILE_SHAPE")) {
@@ -1439,7 +1450,7 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             ctx->graphs = int(value);
             break;
         case HPCS_OPT_RANK:
-            if (value != RANK_BALLOT && value != RANK_LDSOR) fail(HPCS_E_INVALID_ARG, "rank mode must be 0 or 1");
+            if (value < RANK_BALLOT || value > RANK_MIX4) fail(HPCS_E_INVALID_ARG, "rank mode must be 0, 1, 2 or 3");
             ctx->rank_mode = int(value);
             ctx->set_rank = true;
             break;

@@ -359,11 +359,22 @@ __global__ __launch_bounds__(RADIX) void k_plan(const uint32_t *__restrict__ his
 //                digit and reads the mask back (LDS work, few VALU ops); OR is
 //                commutative, so the mask does not depend on the order in
 //                which the LDS unit applies the lanes' updates.
+//   RANK_MIX2 / RANK_MIX4  LDS-OR masks, except that every second / fourth
+//                item is matched by ballots: the two methods load different
+//                pipes (LDS vs VALU), and the ranking, not the writes, bounds
+//                the scatter (measured on MI355X: a 256x16 scatter of 2^28
+//                keys ranks and stages its tiles in 482 us with LDS-OR masks
+//                alone, 395 us with every second item by ballot, against 386 us
+//                for a plain copy of the same bytes).
 // ---------------------------------------------------------------------------
 // Tiles per range of the reduce-then-scan offset scan (k_range_totals).
 constexpr int SCAN_RANGE = 64;
 constexpr int RANK_BALLOT = 0;
 constexpr int RANK_LDSOR = 1;
+constexpr int RANK_MIX2 = 2;
+constexpr int RANK_MIX4 = 3;
+// every MIX-th item of a wave is matched by ballots (0: none)
+template <int RANK> constexpr int rank_mix_period() { return RANK == RANK_MIX2 ? 2 : RANK == RANK_MIX4 ? 4 : 0; }
 // Default look-back rounds without progress after which a block counts the
 // digits of the unpublished earlier tile itself (decoupled fallback); the
 // kernel takes the value as an argument so tests can force the fallback.
@@ -397,7 +408,7 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
     using Cnt = typename std::conditional<CNT16, uint16_t, uint32_t>::type;
 
     __shared__ Cnt s_cnt[WAVES][RADIX];
-    __shared__ uint64_t s_match[RANK == RANK_LDSOR ? WAVES : 1][RANK == RANK_LDSOR ? RADIX : 1];
+    __shared__ uint64_t s_match[RANK != RANK_BALLOT ? WAVES : 1][RANK != RANK_BALLOT ? RADIX : 1];
     __shared__ uint32_t s_base[RADIX];
     __shared__ uint32_t s_scan[SCAN_WAVES];
     __shared__ uint32_t s_fb[RADIX];
@@ -480,7 +491,7 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
     }
 
     for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
-    if constexpr (RANK == RANK_LDSOR)
+    if constexpr (RANK != RANK_BALLOT)
         for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_match[0][0])[i] = 0ull;
     __syncthreads();
     if (xin) {
@@ -495,22 +506,30 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
     // with wave-local counters (their final values are the counts) and stages
     // afterwards, so it issues no counting atomics.
     Cnt *cnt = s_cnt[wave];
+    // Every tile but the last is full: its loops below run without per-key
+    // bounds tests (a tile-uniform branch; no barrier inside either copy).
+    const bool full = n - tile_base >= uint32_t(TILE);
     // in-wave ranks (below 64 * ITEMS), two 16-bit ranks per register
     static_assert(WAVE * ITEMS < 65536, "ranks fit 16 bits");
     uint32_t rank2[RANK == RANK_BALLOT ? (ITEMS + 1) / 2 : 1];
     if constexpr (RANK == RANK_BALLOT) {
+        auto rank_loop = [&](auto full_c) {
+            constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const uint32_t d = KeyOps<K>::digit(keys[i], shift);
-            const bool valid = i * WAVE + lane < wave_valid;
-            const uint64_t peers = match_digit8(d) & __ballot(valid);
-            const uint32_t below = count_below(peers);
-            const uint32_t old = cnt[d];
-            if (valid && below == 0) cnt[d] = Cnt(old + uint32_t(__popcll(peers)));
-            const uint32_t r = old + below;
-            rank2[i / 2] = (i & 1) ? (rank2[i / 2] | (r << 16)) : r;
-            __builtin_amdgcn_wave_barrier();
-        }
+            for (int i = 0; i < ITEMS; ++i) {
+                const uint32_t d = KeyOps<K>::digit(keys[i], shift);
+                const bool valid = FULL || i * WAVE + lane < wave_valid;
+                const uint64_t peers = FULL ? match_digit8(d) : match_digit8(d) & __ballot(valid);
+                const uint32_t below = count_below(peers);
+                const uint32_t old = cnt[d];
+                if (valid && below == 0) cnt[d] = Cnt(old + uint32_t(__popcll(peers)));
+                const uint32_t r = old + below;
+                rank2[i / 2] = (i & 1) ? (rank2[i / 2] | (r << 16)) : r;
+                __builtin_amdgcn_wave_barrier();
+            }
+        };
+        if (full) rank_loop(std::true_type{});
+        else rank_loop(std::false_type{});
         // The staging step recomputes each key's counter address instead of
         // keeping ITEMS addresses live across the block scan (an empty asm
         // that the compiler must assume rewrites the keys; no instruction).
@@ -519,9 +538,14 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
             for (int i = 0; i < ITEMS; ++i) asm volatile("" : "+v"(keys[i]));
         }
     } else {
+        if (full) {
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i)
-            if (i * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[i], shift)], 1u);
+            for (int i = 0; i < ITEMS; ++i) atomicAdd(&cnt[KeyOps<K>::digit(keys[i], shift)], 1u);
+        } else {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i)
+                if (i * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[i], shift)], 1u);
+        }
     }
     __syncthreads();
 
@@ -556,45 +580,61 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
 
     // ---- 3. rank and stage the tile in LDS in sorted order ----
     uint32_t pos[HAS_VALS ? ITEMS : 1];
-    if constexpr (RANK == RANK_BALLOT) {
-        // cnt[d] now holds the wave's base for digit d
+    auto stage_loop = [&](auto full_c) {
+        constexpr bool FULL = decltype(full_c)::value;
+        if constexpr (RANK == RANK_BALLOT) {
+            // cnt[d] now holds the wave's base for digit d
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            if (i * WAVE + lane < wave_valid) {
-                const uint32_t at = cnt[KeyOps<K>::digit(keys[i], shift)] + ((rank2[i / 2] >> (16 * (i & 1))) & 0xFFFFu);
-                s_keys[at] = keys[i];
-                if constexpr (HAS_VALS) pos[i] = at;
+            for (int i = 0; i < ITEMS; ++i) {
+                if (FULL || i * WAVE + lane < wave_valid) {
+                    const uint32_t at =
+                        cnt[KeyOps<K>::digit(keys[i], shift)] + ((rank2[i / 2] >> (16 * (i & 1))) & 0xFFFFu);
+                    s_keys[at] = keys[i];
+                    if constexpr (HAS_VALS) pos[i] = at;
+                }
+            }
+        } else {
+            uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
+            const uint32_t lanebit = 1u << (lane & 31);
+            const int half = lane >> 5;
+            constexpr int MIX = rank_mix_period<RANK>();
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) {
+                const uint32_t d = KeyOps<K>::digit(keys[i], shift);
+                const bool valid = FULL || i * WAVE + lane < wave_valid;
+                if (MIX && i % MIX == MIX - 1) {
+                    // this item: the peers by ballots (no LDS traffic)
+                    const uint64_t peers = FULL ? match_digit8(d) : match_digit8(d) & __ballot(valid);
+                    const uint32_t below = count_below(peers);
+                    const uint32_t old = cnt[d];
+                    if (valid && below == 0) cnt[d] = Cnt(old + uint32_t(__popcll(peers)));
+                    if (valid) s_keys[old + below] = keys[i];
+                    if constexpr (HAS_VALS) pos[i] = old + below;
+                    __builtin_amdgcn_wave_barrier();
+                } else {
+                    if (valid) atomicOr(&match32[2 * d + half], lanebit);
+                    __builtin_amdgcn_wave_barrier();
+                    const uint64_t peers = s_match[wave][d];
+                    const uint32_t below = count_below(peers);
+                    const uint32_t old = cnt[d];
+                    if (valid && below == 0) {
+                        cnt[d] = Cnt(old + uint32_t(__popcll(peers)));
+                        s_match[wave][d] = 0ull;
+                    }
+                    if (valid) s_keys[old + below] = keys[i];
+                    if constexpr (HAS_VALS) pos[i] = old + below;
+                    __builtin_amdgcn_wave_barrier();
+                }
             }
         }
-    } else {
-        uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
-        const uint32_t lanebit = 1u << (lane & 31);
-        const int half = lane >> 5;
+        if constexpr (HAS_VALS) {
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const uint32_t d = KeyOps<K>::digit(keys[i], shift);
-            const bool valid = i * WAVE + lane < wave_valid;
-            if (valid) atomicOr(&match32[2 * d + half], lanebit);
-            __builtin_amdgcn_wave_barrier();
-            const uint64_t peers = s_match[wave][d];
-            const uint32_t below = count_below(peers);
-            const uint32_t old = cnt[d];
-            if (valid && below == 0) {
-                cnt[d] = Cnt(old + uint32_t(__popcll(peers)));
-                s_match[wave][d] = 0ull;
-            }
-            if (valid) s_keys[old + below] = keys[i];
-            if constexpr (HAS_VALS) pos[i] = old + below;
-            __builtin_amdgcn_wave_barrier();
+            for (int i = 0; i < ITEMS; ++i)
+                if (FULL || i * WAVE + lane < wave_valid) s_vals[pos[i]] = vals[i];
         }
-    }
-    if constexpr (HAS_VALS) {
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const int off = i * WAVE + lane;
-            if (off < wave_valid) s_vals[pos[i]] = vals[i];
-        }
-    }
+    };
+    if (full) stage_loop(std::true_type{});
+    else stage_loop(std::false_type{});
 
     // ---- 4. decoupled look-back over a window of earlier tiles ----
     // A round reads LOOKBACK_WINDOW words at once; after meeting an
@@ -688,17 +728,22 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
     const bool wdig = PRESCANNED && dig != nullptr && next_pass != PF_NEXT_NONE;
     const int next_shift = int(next_pass) * RADIX_BITS;
     const int tile_valid = n > tile_base ? int(min(uint32_t(TILE), n - tile_base)) : 0;
+    auto write_loop = [&](auto full_c) {
+        constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const int j = i * BLOCK + tid;
-        if (j < tile_valid) {
-            const K k = s_keys[j];
-            const uint32_t dst = s_base[KeyOps<K>::digit(k, shift)] + uint32_t(j);
-            kout[dst] = xout ? key_dec<K, FLOATK>(k) : k;
-            if constexpr (HAS_VALS) vout[dst] = s_vals[j];
-            if (wdig) dig[dst] = uint8_t(KeyOps<K>::digit(k, next_shift));
+        for (int i = 0; i < ITEMS; ++i) {
+            const int j = i * BLOCK + tid;
+            if (FULL || j < tile_valid) {
+                const K k = s_keys[j];
+                const uint32_t dst = s_base[KeyOps<K>::digit(k, shift)] + uint32_t(j);
+                kout[dst] = xout ? key_dec<K, FLOATK>(k) : k;
+                if constexpr (HAS_VALS) vout[dst] = s_vals[j];
+                if (wdig) dig[dst] = uint8_t(KeyOps<K>::digit(k, next_shift));
+            }
         }
-    }
+    };
+    if (full) write_loop(std::true_type{});
+    else write_loop(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -784,11 +829,16 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
         load_tile_keys<K, BLOCK, ITEMS>(kin + base, cnt, vec, k);
         for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
         __syncthreads();
+        if (cnt == uint32_t(TILE)) { // a full tile: no per-key bounds test
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const uint32_t j = uint32_t(i * BLOCK + tid);
-            const bool valid = j < cnt;
-            count_digit(sh, KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[i]) : k[i], shift), valid, lane);
+            for (int i = 0; i < ITEMS; ++i)
+                count_digit(sh, KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[i]) : k[i], shift), true, lane);
+        } else {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) {
+                const uint32_t j = uint32_t(i * BLOCK + tid);
+                count_digit(sh, KeyOps<K>::digit(xin ? key_enc<K, FLOATK>(k[i]) : k[i], shift), j < cnt, lane);
+            }
         }
         __syncthreads();
         for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
@@ -826,6 +876,10 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
 // Spacing of per-digit accumulators that take atomics from many workgroups:
 // one 128-byte line each, so the atomics do not queue behind each other.
 constexpr int LINE_WORDS = 32;
+// Slots for OR(keys) / OR(~keys) of the first read (k_upsweep): atomics
+// spread over ORSLOTS lines.
+constexpr int ORSLOTS = 64;
+template <typename K> constexpr int OR_STRIDE = 128 / int(sizeof(K)); // one 128-byte line per slot
 
 // totals[r][d] = sum of counts[t][d] over the tiles of range r, each count
 // replaced by the sum of the range's earlier counts (the tile's offset within
@@ -938,8 +992,6 @@ __global__ __launch_bounds__(1024) void k_scan_digit(uint32_t *__restrict__ coun
 // identity pass) iff no bit of its byte varies.  The two ORs go to one of
 // ORSLOTS word pairs (by block) to keep atomics on one address few.
 // ---------------------------------------------------------------------------
-constexpr int ORSLOTS = 64;
-template <typename K> constexpr int OR_STRIDE = 128 / int(sizeof(K)); // one 128-byte line per slot
 
 template <typename K, int FLOATK, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, uint32_t n,
@@ -959,23 +1011,22 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
     for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
     __syncthreads();
     K vor = 0, vnor = 0;
+    auto count_loop = [&](auto full_c) {
+        constexpr bool FULL = decltype(full_c)::value; // a full tile: no per-key bounds test
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const uint32_t j = uint32_t(i * BLOCK + tid);
-        const bool valid = j < cnt;
-        const K key = FLOATK ? key_enc<K, FLOATK>(k[i]) : k[i];
-        if (valid) {
-            vor |= key;
-            vnor |= ~key;
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint32_t j = uint32_t(i * BLOCK + tid);
+            const bool valid = FULL || j < cnt;
+            const K key = FLOATK ? key_enc<K, FLOATK>(k[i]) : k[i];
+            if (valid) {
+                vor |= key;
+                vnor |= ~key;
+            }
+            count_digit(sh, KeyOps<K>::digit(key, 0), valid, lane);
         }
-        const uint32_t d = KeyOps<K>::digit(key, 0);
-        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        if (__all(valid && d == d0)) {
-            if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
-        } else if (valid) {
-            atomicAdd(&sh[d], 1u);
-        }
-    }
+    };
+    if (cnt == uint32_t(TILE)) count_loop(std::true_type{});
+    else count_loop(std::false_type{});
 #pragma unroll
     for (int off = 1; off < WAVE; off <<= 1) {
         vor |= __shfl_xor(vor, off);

@@ -32,8 +32,8 @@ struct TuneEntry {
 // workgroup per bucket (keys without values, at most 2^22 4-byte or 2^21
 // 8-byte keys; larger arrays fall back to the choice by size);
 // shape: a row of TILE_SHAPES (hpcs_shapes.hpp) that serves the dtype; rank
-// 0 ballot / 1 LDS-OR (0 only for ballot-only shapes); digit_bits 8
-// (default) or 11.  11-bit digits exist in one form
+// 0 ballot / 1 LDS-OR / 2 and 3 LDS-OR with every second / fourth item by
+// ballot (0 only for ballot-only shapes); digit_bits 8 (default) or 11.  11-bit digits exist in one form
 // only: onesweep passes, 256x16 tiles, ballot ranking, keys without values,
 // so a row with digit_bits 11 reads `<dtype> <max_n> 0 0 0 11`.
 inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, std::string &err)
@@ -70,8 +70,8 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
             err = "line " + std::to_string(line_no) + ": unknown dtype " + d;
             return false;
         }
-        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3 && algo != 4) || rank < 0 || rank > 1 ||
-            !shape_serves(shape, e.key_bytes, e.vals != 0) || (rank == 1 && TILE_SHAPES[shape].ballot_only) ||
+        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3 && algo != 4) || rank < 0 || rank > 3 ||
+            !shape_serves(shape, e.key_bytes, e.vals != 0) || (rank != 0 && TILE_SHAPES[shape].ballot_only) ||
             (bits != 8 && bits != 11) ||
             (bits == 11 && (algo != 0 || shape != 0 || rank != 0 || e.vals)) || (algo == 4 && e.vals)) {
             err = "line " + std::to_string(line_no) + ": value out of range";

@@ -52,7 +52,9 @@ extern "C" {
 #define HPCS_KEY_INT 2   /* two's-complement signed key (int / long) */
 
 /* Tuning options for hpcs_ctx_set_option. */
-#define HPCS_OPT_RANK 1       /* 0: ballot match, 1: LDS-OR match (default) */
+#define HPCS_OPT_RANK 1       /* how a wave finds the keys that share its digit: 0: ballots, 1: LDS-OR match
+                                 masks (default), 2 / 3: LDS-OR masks with every second / fourth item matched
+                                 by ballots (the two methods load different pipes) */
 #define HPCS_OPT_TILE_SHAPE 2 /* threads x keys per thread of a pass's tile: 0: 256x16 (default) and
                                  4: 256x8, any sort; 1: 512x16, 2: 256x32, 3: 512x32 and 5: 1024x16,
                                  4-byte keys without values; 6: 1024x8, 8-byte keys without values or
@@ -145,8 +147,9 @@ int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n);
  * the largest); algo 0 onesweep, 1 reduce-then-scan, 3 one workgroup, 4 MSD
  * split + one workgroup per bucket (not for pairs rows; see
  * HPCS_OPT_PASS_ALGO); shape 0-4
- * (HPCS_OPT_TILE_SHAPE; 1-3 for 4-byte keys without values only); rank 0
- * ballot, 1 LDS-OR; digit_bits 8 (default) or 11 (HPCS_OPT_DIGIT_BITS; an
+ * (HPCS_OPT_TILE_SHAPE, a shape that serves the dtype; see
+ * hpcs_tile_shape_query); rank 0-3 (HPCS_OPT_RANK; 0 for shapes 5 and 6);
+ * digit_bits 8 (default) or 11 (HPCS_OPT_DIGIT_BITS; an
  * 11-bit row reads `<dtype> <max_n> 0 0 0 11`, keys without values only).
  * Options set with
  * hpcs_ctx_set_option (or HPCS_RANK / HPCS_TILE_SHAPE) take precedence.
