@@ -3,9 +3,10 @@ GPU and write the tuning table libhpcsort.so reads (hpcs_ctx_load_tuning).
 
 Variants: digit width (8 bits; 11 bits for keys without values), pass
 algorithm (onesweep with look-back, reduce-then-scan, one workgroup in LDS
-for small arrays), tile shape (256x16 and
-256x8 threads x keys for every sort; 512x16, 256x32 and 512x32 for 4-byte
-keys without values) and ranking (ballot, LDS-OR).
+for small arrays), tile shape (every row of the library's shape table that
+serves the key type, hpcs_tile_shape_query: 256, 512 and 1024 threads, 8 to
+32 keys per thread) and ranking (ballot, LDS-OR, LDS-OR with every second or
+fourth item by ballot; ballot only on the 1024-thread shapes).
 Size classes are powers of two; a class's row applies to every n up to it.
 Each measured cell is appended to a JSON log as it finishes, and a rerun
 skips the cells the log already holds (the sweep resumes).
@@ -29,8 +30,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 KINDS = {
     "u32": (4, False, "hpcs_dev_sort_radix_u32"),
     "u64": (8, False, "hpcs_dev_sort_radix_u64"),
+    "pairs32": (4, True, "hpcs_dev_sort_radix_pairs_u32_u32"),
     "pairs64": (8, True, "hpcs_dev_sort_radix_pairs_u64_u32"),
 }
+RANKS = (1, 2, 3, 0)  # LDS-OR, every 2nd / 4th item by ballot, ballot
 
 
 # largest arrays the one-workgroup sort (pass algorithm 3) takes
@@ -43,12 +46,27 @@ def candidates(key_bytes: int, vals: bool, n: int | None = None):
     its speed depends on how the keys spread, which uniform keys do not
     show); 11-bit digits (keys without values):
     their one form, onesweep passes with 256x16 tiles and ballot ranking."""
-    shapes = (0, 1, 2, 3, 4) if key_bytes == 4 and not vals else (0, 4)
-    out = [{"algo": a, "shape": s, "rank": r, "bits": 8} for a, s, r in itertools.product((0, 1), shapes, (1, 0))]
+    out = [{"algo": a, "shape": s, "rank": r, "bits": 8}
+           for a, (s, ballot_only), r in itertools.product((0, 1), shapes_for(key_bytes, vals), RANKS)
+           if r == 0 or not ballot_only]
     if not vals:
         out.append({"algo": 0, "shape": 0, "rank": 0, "bits": 11})
     if n is not None and n <= BLOCK_SORT_MAX[key_bytes]:
         out.append({"algo": 3, "shape": 0, "rank": 1, "bits": 8})
+    return out
+
+
+def shapes_for(key_bytes: int, vals: bool):
+    """(shape, ballot only) for every row of the library's tile-shape table
+    that serves the key type (the kernel dispatch answers, no device)."""
+    out = []
+    for shape in range(64):
+        b, i = ctypes.c_int32(0), ctypes.c_int32(0)
+        rc = N.lib().hpcs_tile_shape_query(shape, key_bytes, int(vals), ctypes.byref(b), ctypes.byref(i))
+        if rc == N.HPCS_OK:
+            out.append((shape, b.value >= 1024))
+        elif rc != N.HPCS_E_UNSUPPORTED:
+            N.check(rc)
     return out
 
 
@@ -97,7 +115,8 @@ def table_text(results: dict) -> str:
     lines = ["# libhpcsort tuning table (hpcs_ctx_load_tuning), measured on "
              + results.get("device", "this GPU") + " by hpcsharp_amd.autotune",
              "# dtype max_n algo shape rank digit_bits   # median ms per sort of the best variant"]
-    alias = {"u32": ("u32", "f32", "i32"), "u64": ("u64", "f64", "i64"), "pairs64": ("pairs64",)}
+    alias = {"u32": ("u32", "f32", "i32"), "u64": ("u64", "f64", "i64"), "pairs32": ("pairs32",),
+             "pairs64": ("pairs64",)}
     for kind in KINDS:
         for lg in sorted({int(k.split(":")[1]) for k in results["cells"] if k.startswith(kind + ":")}):
             cells = [c for k, c in results["cells"].items() if k.startswith(f"{kind}:{lg}:")]
@@ -111,7 +130,7 @@ def table_text(results: dict) -> str:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--classes", default="12,13,14,16,18,20,22,24,26,28")
-    ap.add_argument("--kinds", default="u32,u64,pairs64")
+    ap.add_argument("--kinds", default="u32,u64,pairs32,pairs64")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--log", default=os.path.join(HERE, "tune_gfx950.json"))
     ap.add_argument("--out", default=N.TUNE_PATH)

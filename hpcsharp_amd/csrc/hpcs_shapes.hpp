@@ -27,7 +27,8 @@ struct TileShape {
 
 // 0-4: 4096 / 8192 / 16384 / 2048-key tiles; 5 and 6: 1024-thread
 // workgroups whose 64 KiB staging area and 16-bit per-wave counters leave
-// room for two workgroups per CU (ballot ranking only).
+// room for two workgroups per CU (ballot ranking only); 7-9: 256 threads x
+// 12 / 20 / 24 keys.
 constexpr TileShape TILE_SHAPES[] = {
     {256, 16, SERVE_ALL, false},              // 0
     {512, 16, SERVE_K4, false},               // 1
@@ -36,6 +37,9 @@ constexpr TileShape TILE_SHAPES[] = {
     {256, 8, SERVE_ALL, false},               // 4
     {1024, 16, SERVE_K4, true},               // 5: 16384 4-byte keys
     {1024, 8, SERVE_K4V | SERVE_K8, true},    // 6: 8192 keys (4-byte with values, or 8-byte)
+    {256, 12, SERVE_K4, false},               // 7
+    {256, 20, SERVE_K4, false},               // 8
+    {256, 24, SERVE_K4, false},               // 9
 };
 constexpr int NUM_TILE_SHAPES = int(sizeof(TILE_SHAPES) / sizeof(TILE_SHAPES[0]));
 

@@ -87,7 +87,7 @@ static int tuning_test()
     CHECK(rows[0].shape == 5 && rows[1].shape == 6 && rows[2].shape == 6);
     // ranks 2 and 3: LDS-OR masks with every second / fourth item by ballot
     CHECK(hpcs::parse_tuning("u32 1 1 0 2\nu64 2 1 4 3", rows, err) && rows.size() == 2 && rows[1].rank == 3);
-    const char *bad[] = {"u8 1 0 0 0", "u32 0 0 0 0", "u32 1 2 0 0", "u32 1 0 7 0", "u32 1 0 0 4", "u64 1 0 1 0",
+    const char *bad[] = {"u8 1 0 0 0", "u32 0 0 0 0", "u32 1 2 0 0", "u32 1 0 10 0", "u32 1 0 0 4", "u64 1 0 7 0", "u64 1 0 1 0",
                          "u32 1 1 5 1", "pairs32 1 1 5 0", "u32 1 1 6 0", "pairs64 1 1 6 0", "u32 1 0 -1 0",
                          "pairs32 1 0 3 1", "u32 1 0 0 1 9", "u32 1 0", "u32 -5 0 0 1",
                          "u32 1 1 0 0 11", "u32 1 0 4 0 11", "u32 1 0 0 1 11", "pairs64 1 0 0 0 11",

@@ -163,7 +163,7 @@ def test_tile_shapes_dispatch_agrees_with_the_tuning_parser(tmp_path):
             parsed = lib.hpcs_tuning_check(str(row).encode(), ctypes.byref(rows)) == 0
             if rc == N.HPCS_OK:
                 served[(kb, v)].append(shape)
-                assert b.value in (256, 512, 1024) and i.value in (8, 16, 32), (shape, dtype)
+                assert b.value in (256, 512, 1024) and i.value in (8, 12, 16, 20, 24, 32), (shape, dtype)
                 # the staging area of a tile fits one workgroup's LDS budget
                 assert b.value * i.value * (kb + 4 * v) <= 64 * 1024, (shape, dtype)
                 assert parsed, (shape, dtype)

@@ -62,10 +62,10 @@ def test_random_option_combinations(case):
     n = int(np.exp(rng.uniform(np.log(2), np.log(1 << 22))))
     # cases from 80 also draw the MSD split (pass algorithm 4)
     algo = [2, 0, 1, 3][int(rng.integers(4))] if case < 80 else [2, 0, 1, 3, 4][int(rng.integers(5))]
-    rank = int(rng.integers(2))
+    rank = int(rng.integers(4))      # ballot, LDS-OR, and the two mixes
     four = kind in ("u32", "f32", "i32")
     # the rows of the tile-shape table that serve the key type
-    shape = int(rng.choice([0, 1, 2, 3, 4, 5] if four else [0, 4, 6] if kind != "pairs64" else [0, 4]))
+    shape = int(rng.choice([0, 1, 2, 3, 4, 5, 7, 8, 9] if four else [0, 4, 6] if kind != "pairs64" else [0, 4]))
     bits = 11 if (not kind.startswith("pairs") and rng.random() < 0.25) else 8
     ctx = N.new_context(0)
     try:

@@ -273,7 +273,7 @@ def test_full_size_2pow28_properties():
     assert int(h_in.view(4, 256).sum(dim=1).min()) == n
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
 def test_forced_lookback_fallback(oracle, shape):
     """Every look-back wait resolved by counting the earlier tile locally
     (fallback threshold 0) still gives the oracle's output, on every tile
@@ -305,7 +305,7 @@ def test_forced_lookback_fallback(oracle, shape):
 
 
 @pytest.mark.parametrize("algo", [0, 1])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 7, 8, 9])
 def test_tile_shapes_u32(oracle, shape, algo):
     ctx = N.new_context(0)
     try:
