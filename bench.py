@@ -6,11 +6,13 @@ digit pass: tile counts, scan, scatter) of a distinct copy of the same
 2^28-key array (uniform 32-bit keys from splitmix64, seed 0x5EED0002),
 already resident in HBM.  W untimed
 warm-up sorts, then exactly K timed sorts bracketed by device
-synchronisation; the output of the last step is checked against an
-independent sort before anything is printed.
+synchronisation; every timed output is checked against an independent sort
+(torch.sort) before anything is printed.
 
-Prints one JSON line: Gkeys/s, the pass scatter kernel's roofline (algorithmic
-bytes = 8 B per key per pass, from HIP events on the launch stream) and the
+Prints one JSON line: Gkeys/s (the K timed sorts, no events among them), the
+pass scatter kernel's roofline (algorithmic bytes = 8 B per key per pass,
+divided by its launch duration from HIP events on the launch stream, recorded
+over a second region of K sorts) and the
 CPU baseline (the C restatement of ParallelAlgorithm.SortRadixPar, timed on
 this host's cores).  Per-distribution figures go to stderr and to the JSON
 line's "extra" field.
@@ -46,31 +48,43 @@ def run_sort(N, lib, ctx, t, n, stream):
 
 
 def measure(torch, N, host_keys: np.ndarray, steps: int, warmup: int, profile: bool):
+    """Seconds per sort over `steps` timed sorts of distinct device-resident
+    copies (no events inside that region); with `profile`, a second region of
+    `steps` sorts brackets every kernel with HIP events on the launch stream
+    (the per-kernel times; the events themselves add about 0.1 ms per sort,
+    so they stay out of the first region).  Returns (s per sort, per-kernel
+    stats, s per sort of the evented region, output correct)."""
     lib, ctx = N.lib(), N.context(0)
     n = host_keys.size
     src = torch.from_numpy(host_keys.view(np.int32)).cuda()
-    bufs = gpu_sorts(torch, N, src, warmup + steps)
+    regions = 2 if profile else 1
+    bufs = gpu_sorts(torch, N, src, warmup + regions * steps)
     stream = torch.cuda.current_stream().cuda_stream
     for i in range(warmup):
         run_sort(N, lib, ctx, bufs[i], n, stream)
     torch.cuda.synchronize()
-    if profile:
-        N.profile_enable(True)
     t0 = time.perf_counter()
     for i in range(steps):
         run_sort(N, lib, ctx, bufs[warmup + i], n, stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    stats = N.profile_read() if profile else {}
+    stats, t_ev = {}, None
     if profile:
+        N.profile_enable(True)
+        t2 = time.perf_counter()
+        for i in range(steps):
+            run_sort(N, lib, ctx, bufs[warmup + steps + i], n, stream)
+        torch.cuda.synchronize()
+        t_ev = (time.perf_counter() - t2) / steps
+        stats = N.profile_read()
         N.profile_enable(False)
-    out = bufs[-1]
-    # correctness before reporting: equal to an independent library sort
+    # correctness before reporting: every timed output equals an independent
+    # library sort
     ref = torch.sort(src.to(torch.int64) & 0xFFFFFFFF)[0]
-    ok = bool(torch.equal(out.to(torch.int64) & 0xFFFFFFFF, ref))
-    del bufs, ref, src, out
+    ok = all(bool(torch.equal(b.to(torch.int64) & 0xFFFFFFFF, ref)) for b in bufs[warmup:])
+    del bufs, ref, src
     torch.cuda.empty_cache()
-    return (t1 - t0) / steps, stats, ok
+    return (t1 - t0) / steps, stats, t_ev, ok
 
 
 def batched_vs_plain(torch, N, datagen, nseg: int, seed: int, reps: int):
@@ -430,6 +444,46 @@ def cpu_baseline(host_keys: np.ndarray, reps: int):
     return host_keys.size / t / 1e9, threads, times
 
 
+def segments_only(torch, N, datagen, nseg: int, seed: int, reps: int):
+    """The batched call alone (4096 segments, lengths log-uniform in [16,
+    2^20]), event medians over `reps` calls on freshly copied keys; the output
+    is checked against torch.sort of (segment, key) pairs."""
+    lens = datagen.segment_lengths(nseg, seed, 16, 1 << 20)
+    begin = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(begin[-1])
+    src = datagen.keys_u32_torch(total, seed + 1)
+    d_begin = torch.from_numpy(begin).cuda()
+    work = src.clone()
+    lib, ctx = N.lib(), N.context(0)
+    P = ctypes.c_void_p
+    st = P(torch.cuda.current_stream().cuda_stream)
+    call = lambda: N.check(lib.hpcs_dev_sort_segments_u32(ctx, P(work.data_ptr()), total, P(d_begin.data_ptr()),
+                                                          nseg, st))
+    ms = event_ms(torch, lambda: work.copy_(src), call, reps)
+    seg = torch.repeat_interleave(torch.arange(nseg, dtype=torch.int64, device="cuda"), torch.from_numpy(lens).cuda())
+    want = torch.sort((seg << 32) | (src.to(torch.int64) & 0xFFFFFFFF))[0] & 0xFFFFFFFF
+    ok = bool(torch.equal(work.to(torch.int64) & 0xFFFFFFFF, want))
+    if not ok:
+        print("bench: batched segments differ from torch.sort", file=sys.stderr)
+        sys.exit(3)
+    return {"keys": total, "segments": nseg, "ms": round(ms, 4), "gkeys_per_s": round(total / ms / 1e6, 2)}
+
+
+def only_section(torch, N, datagen, args):
+    """One workload alone (bench.py --only X), for profiling runs."""
+    if args.only == "histogram":
+        return histogram_figures(torch, N, datagen, datagen.keys_u32(args.n, SEED, "uniform"), 10)
+    if args.only == "topk":
+        return topk_figures(torch, N, datagen, 5)
+    if args.only == "segments":
+        return segments_only(torch, N, datagen, 4096, 0x5EED0008, 5)
+    if args.only == "serving":
+        return serving_figures(args.serving_requests, 4)
+    if args.only == "cfg3":
+        return cfg3_figures(torch, N, datagen, 3)
+    return msd_split_figures(torch, N, 10)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -446,7 +500,13 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-hist", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=20000)
+    ap.add_argument("--only", choices=("sort", "histogram", "topk", "segments", "serving", "cfg3", "msd"),
+                    help="run one workload alone (for profiling runs): 'sort' is the headline sort's line "
+                         "without the other workloads; the others print that workload's figures as JSON")
     args = ap.parse_args()
+    if args.only == "sort":
+        args.no_cpu = args.no_dists = args.no_batched = args.no_topk = args.no_host = True
+        args.no_cfg3 = args.no_serving = args.no_hist = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)
         sys.exit(2)
@@ -459,12 +519,15 @@ def main():
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
     n = args.n
+    if args.only not in (None, "sort"):
+        print(json.dumps({"workload": args.only, **only_section(torch, N, datagen, args)}), flush=True)
+        return
     log(f"[bench] generating {n} uniform u32 keys (seed {SEED:#x})")
     keys = datagen.keys_u32(n, SEED, "uniform")
     info = N.plan_query(N.HPCS_DTYPE_U32, n)
     log(f"[bench] plan: {info.passes} passes x {info.digit_bits} bits, tile {info.tile_keys}, "
         f"{info.tiles} tiles, ideal {info.ideal_bytes / 1e9:.3f} GB")
-    t_step, stats, ok = measure(torch, N, keys, args.steps, args.warmup, profile=True)
+    t_step, stats, t_events, ok = measure(torch, N, keys, args.steps, args.warmup, profile=True)
     if not ok:
         print("bench: sorted output differs from the reference sort", file=sys.stderr)
         sys.exit(3)
@@ -492,6 +555,7 @@ def main():
             traffic = int(json.load(f)["traffic_bytes_per_launch"])
 
     extra = {"sort_ms": round(t_step * 1e3, 4),
+             "sort_ms_with_kernel_events": round(t_events * 1e3, 4),
              "sort_ideal_bytes": int(info.ideal_bytes),
              "sort_roofline_frac": round(sort_gbs / HBM_PEAK_GBS, 4),
              "kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in stats.items()},
@@ -501,7 +565,7 @@ def main():
         dists = {}
         for dist in ("low8", "sorted", "constant"):
             k2 = datagen.keys_u32(n, SEED + 1, dist)
-            t2, _, ok2 = measure(torch, N, k2, max(2, args.steps // 2), 1, profile=False)
+            t2, _, _, ok2 = measure(torch, N, k2, max(2, args.steps // 2), 1, profile=False)
             if not ok2:
                 print(f"bench: {dist} output differs from the reference sort", file=sys.stderr)
                 sys.exit(3)
@@ -566,7 +630,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(d_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(d_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": d_bytes,
-                     "kernel": f"k_onesweep as {dom} (one 8-bit pass: read + write 4 B per key)"},
+                     "kernel": f"k_onesweep as {dom} (one 8-bit pass: read + write 4 B per key); launch "
+                               f"durations from HIP events on the launch stream over a second timed region of "
+                               f"{args.steps} sorts"},
         "cpu_baseline": cpu,
         "extra": extra,
     }
