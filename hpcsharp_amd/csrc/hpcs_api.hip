@@ -10,6 +10,7 @@
 #include "hpcs_kernels.hpp"
 #include "hpcs_shapes.hpp"
 #include "hpcs_digit11.hpp"
+#include "hpcs_merge.hpp"
 #include "hpcs_msd.hpp"
 #include "hpcs_segments.hpp"
 #include "hpcs_topk.hpp"
@@ -145,6 +146,8 @@ struct hpcs_ctx {
     int fallback_rounds = FALLBACK_ROUNDS;
     int lookback_window = 32;    // u32 keys: 16, 32 or 64 earlier tiles per look-back round
     int pass_algo = PASS_AUTO;    // PASS_ONESWEEP, PASS_RTS (reduce-then-scan) or PASS_AUTO (by size)
+    size_t scratch_limit = 0;     // bytes of scratch a sort may use (0: no limit); keys-only sorts that
+                                  // need more sort each half and merge (HPCS_OPT_SCRATCH_LIMIT)
     int digit_bytes = 2;          // reduce-then-scan: tile counts from next-digit bytes written by the
                                   // scatter: 0 off, 1 on, 2 by type (on for 8-byte keys with values)
     // tuning table (hpcs_ctx_load_tuning / HPCS_TUNE_FILE); options set
@@ -223,7 +226,7 @@ struct Layout {
 
 static_assert(ORSLOTS * 128 <= MAX_PASSES * RADIX * sizeof(uint32_t), "OR slots live in the histogram area");
 
-Layout make_layout(int64_t n, size_t key_size, bool vals, int tile_keys = SORT_TILE)
+Layout make_layout(int64_t n, size_t key_size, bool vals, int tile_keys = SORT_TILE, bool dig = true)
 {
     Layout L{};
     L.tiles = (n + tile_keys - 1) / tile_keys;
@@ -237,8 +240,18 @@ Layout make_layout(int64_t n, size_t key_size, bool vals, int tile_keys = SORT_T
     L.off_keysB = align_up(L.off_lb1 + lb_bytes);
     L.off_valsB = align_up(L.off_keysB + size_t(n) * key_size);
     L.off_dig = align_up(L.off_valsB + (vals ? size_t(n) * sizeof(uint32_t) : 0));
-    L.total = align_up(L.off_dig + size_t(n));
+    L.total = align_up(L.off_dig + (dig ? size_t(n) : 0));
     return L;
+}
+
+// The reduce-then-scan scatter writes the next pass's digit bytes for its
+// tile counts (HPCS_OPT_DIGIT_BYTES).  Measured at 2^28 on MI355X: the byte
+// stores cost the scatter about what the byte counts save for u64 keys, more
+// for u32 keys; with u32 values beside u64 keys the sort gains 5%.  Sorts
+// without them leave the layout's n-byte digit region out.
+bool digit_bytes_on(const hpcs_ctx *c, size_t key_size, bool vals)
+{
+    return c->digit_bytes == 1 || (c->digit_bytes == 2 && key_size == 8 && vals);
 }
 
 hipEvent_t take_event(hpcs_ctx *c)
@@ -566,7 +579,7 @@ void run_sort_msd(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st, const SortP
 {
     const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
-    const Layout L = make_layout(n, sizeof(K), false, tile_keys);
+    const Layout L = make_layout(n, sizeof(K), false, tile_keys, false);
     char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
     DevPlan *plan = reinterpret_cast<DevPlan *>(base + L.off_plan);
     uint32_t *lb0 = reinterpret_cast<uint32_t *>(base + L.off_lb0);
@@ -629,7 +642,8 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     }
     const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
-    const Layout L = make_layout(n, sizeof(K), HAS_VALS, tile_keys);
+    const bool want_dig = digit_bytes_on(c, sizeof(K), HAS_VALS);
+    const Layout L = make_layout(n, sizeof(K), HAS_VALS, tile_keys, want_dig);
     if (ext && L.total > ext->bytes) fail(HPCS_E_UNSUPPORTED, "the sort's scratch exceeds the static scratch");
     char *base = ext ? static_cast<char *>(ext->ptr) : static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
     uint32_t *hist = reinterpret_cast<uint32_t *>(base + L.off_hist);
@@ -645,11 +659,6 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     // reduce-then-scan passes whatever the option or table says
     const int algo = (prm.algo == PASS_ONESWEEP && n >= LOOKBACK_MAX_KEYS) ? PASS_RTS : prm.algo;
     const int rank_mode = prm.rank;
-    // reduce-then-scan: the scatter writes the next pass's digit bytes for
-    // its tile counts.  Measured at 2^28 on MI355X: the byte stores cost the
-    // scatter about what the byte counts save for u64 keys, more for u32
-    // keys; with u32 values beside u64 keys the sort gains 5%.
-    const bool want_dig = c->digit_bytes == 1 || (c->digit_bytes == 2 && sizeof(K) == 8 && HAS_VALS);
     uint8_t *dig = want_dig ? reinterpret_cast<uint8_t *>(base + L.off_dig) : nullptr;
     if (algo == PASS_RTS && tiles <= SCAN_DIGIT_MAX_TILES) {
         // few tiles: launches, not bytes, set the time.  One histogram of
@@ -732,6 +741,99 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         hipLaunchKernelGGL((k_copyback<K>), dim3(grid_for(uint64_t(n), 256, c->num_cus * 8)), dim3(256), 0, st,
                            d_keys, keysB, HAS_VALS ? d_vals : nullptr, valsB, uint64_t(n), plan);
     });
+}
+
+// Scratch bytes run_sort takes for n keys with the context's options and
+// table (ext: the plan a caller-provided scratch gets: 8-bit passes, no MSD
+// split); the one-workgroup sort takes none.
+template <typename K, bool HAS_VALS>
+size_t sort_scratch_bytes(const hpcs_ctx *c, int64_t n, bool ext)
+{
+    if (n <= 1) return 0;
+    SortParams prm = resolve_params(c, int(sizeof(K)), HAS_VALS, n);
+    if (ext) {
+        prm.bits = 8;
+        if (prm.algo == PASS_MSD)
+            prm.algo = n <= (sizeof(K) == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8) ? PASS_BLOCK : PASS_RTS;
+    }
+    if (prm.algo == PASS_BLOCK) return 0;
+    if (!HAS_VALS && prm.bits == 11) return make_layout11(n, sizeof(K)).total;
+    return make_layout(n, sizeof(K), HAS_VALS, shape_tile_keys(prm.shape),
+                       prm.algo != PASS_MSD && digit_bytes_on(c, sizeof(K), HAS_VALS))
+        .total;
+}
+
+// The memory-tight keys-only sort (hpcs_merge.hpp): each half sorted in place
+// by the LSD passes, then the halves merged with half the keys of scratch.
+template <typename K, int FLOATK>
+void run_sort_tight(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st)
+{
+    const int64_t h = (n + 1) / 2, nb = n - h;
+    const size_t merge_bytes = align_up(size_t(h) * sizeof(K)) + align_up(sizeof(MergeSplit));
+    const size_t need = std::max({merge_bytes, sort_scratch_bytes<K, false>(c, h, true),
+                                  sort_scratch_bytes<K, false>(c, nb, true)});
+    if (c->scratch_limit && need > c->scratch_limit)
+        fail(HPCS_E_BUDGET, "the sort needs " + std::to_string(need) + " bytes of scratch even with halves merged (" +
+                                "limit " + std::to_string(c->scratch_limit) + ")");
+    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, need));
+    const ExtScratch ext{base, std::max(need, c->scratch_cap)};
+    run_sort<K, false, FLOATK>(c, d_keys, nullptr, h, st, &ext);
+    run_sort<K, false, FLOATK>(c, d_keys + h, nullptr, nb, st, &ext);
+    K *S = reinterpret_cast<K *>(base);
+    MergeSplit *sp = reinterpret_cast<MergeSplit *>(base + align_up(size_t(h) * sizeof(K)));
+    K *R = d_keys + h;
+    timed(c, st, "merge", [&] {
+        hipLaunchKernelGGL((k_copy_keys<K>), dim3(grid_for(uint64_t(h), 256, c->num_cus * 8)), dim3(256), 0, st, S,
+                           d_keys, uint64_t(h));
+    });
+    const dim3 grid(unsigned((h + MERGE_TILE - 1) / MERGE_TILE));
+    timed(c, st, "merge", [&] {
+        hipLaunchKernelGGL((k_merge_split<K, FLOATK>), dim3(1), dim3(64), 0, st, S, uint64_t(h), R, uint64_t(nb), sp);
+    });
+    timed(c, st, "merge", [&] {
+        hipLaunchKernelGGL((k_merge<K, FLOATK>), grid, dim3(MERGE_BLOCK), 0, st, S, uint64_t(h), R, uint64_t(nb), sp, 0,
+                           d_keys);
+    });
+    timed(c, st, "merge", [&] {
+        hipLaunchKernelGGL((k_merge_tail<K>), dim3(grid_for(uint64_t(nb), 256, c->num_cus * 8)), dim3(256), 0, st, R,
+                           uint64_t(nb), sp, S);
+    });
+    timed(c, st, "merge", [&] {
+        hipLaunchKernelGGL((k_merge<K, FLOATK>), grid, dim3(MERGE_BLOCK), 0, st, S, uint64_t(h), R, uint64_t(nb), sp, 1,
+                           R);
+    });
+}
+
+// A sort with the context's scratch limit: keys-only sorts whose full
+// scratch exceeds the limit, or whose scratch allocation fails, take the
+// memory-tight form (as SortRadixInPlaceAdaptive falls back to an in-place
+// sort on OutOfMemoryException, RadixSortMsd.cs:725-770); sorts with values
+// beyond the limit fail with HPCS_E_BUDGET before any work; others run
+// directly.
+template <typename K, bool HAS_VALS, int FLOATK>
+void run_sort_limited(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
+{
+    if constexpr (!HAS_VALS) {
+        if (n > 1 && n < MAX_KEYS && d_keys && reinterpret_cast<uintptr_t>(d_keys) % sizeof(K) == 0) {
+            if (c->scratch_limit && sort_scratch_bytes<K, false>(c, n, false) > c->scratch_limit)
+                return run_sort_tight<K, FLOATK>(c, d_keys, n, st);
+            try {
+                return run_sort<K, false, FLOATK>(c, d_keys, nullptr, n, st);
+            } catch (const Err &e) {
+                if (e.code != HPCS_E_OOM) throw;
+                (void)hipGetLastError();
+            }
+            return run_sort_tight<K, FLOATK>(c, d_keys, n, st);
+        }
+    } else {
+        if (c->scratch_limit && n > 1 && n < MAX_KEYS) {
+            const size_t need = sort_scratch_bytes<K, true>(c, n, false);
+            if (need > c->scratch_limit)
+                fail(HPCS_E_BUDGET, "the sort of keys with values needs " + std::to_string(need) +
+                                        " bytes of scratch (limit " + std::to_string(c->scratch_limit) + ")");
+        }
+    }
+    run_sort<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
 }
 
 // Batched segments: scratch [SegPlan | seg_tile0 | small | large | tile
@@ -1206,7 +1308,7 @@ void dev_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
 {
     if (n <= 1) return run_sort<K, HAS_VALS, KIND>(c, d_keys, d_vals, n, st);
     const hpcs_ctx::GraphEntry key{int(sizeof(K)) * 16 + (HAS_VALS ? 8 : 0) + KIND, d_keys, d_vals, n, 0, st};
-    run_cached(c, key, st, [&] { run_sort<K, HAS_VALS, KIND>(c, d_keys, d_vals, n, st); });
+    run_cached(c, key, st, [&] { run_sort_limited<K, HAS_VALS, KIND>(c, d_keys, d_vals, n, st); });
 }
 
 // Byte histogram of the key inside each record (the key extracted into the
@@ -1258,7 +1360,7 @@ void host_sort(hpcs_ctx *c, K *keys, uint32_t *vals, int64_t n)
     hipStream_t st = c->stream;
     hip_check(hipMemcpyAsync(d_keys, keys, kbytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
     if (HAS_VALS) hip_check(hipMemcpyAsync(d_vals, vals, size_t(n) * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
-    run_sort<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
+    run_sort_limited<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
     hip_check(hipMemcpyAsync(keys, d_keys, kbytes, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
     if (HAS_VALS) hip_check(hipMemcpyAsync(vals, d_vals, size_t(n) * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -1479,6 +1581,10 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             if (value < 0 || value > 2) fail(HPCS_E_INVALID_ARG, "digit bytes must be 0, 1 or 2");
             ctx->digit_bytes = int(value);
             break;
+        case HPCS_OPT_SCRATCH_LIMIT:
+            if (value < 0) fail(HPCS_E_INVALID_ARG, "scratch limit must be non-negative");
+            ctx->scratch_limit = size_t(value);
+            break;
         case HPCS_OPT_FALLBACK_ROUNDS:
             if (value < 0 || value > (1 << 20)) fail(HPCS_E_INVALID_ARG, "fallback rounds out of range");
             ctx->fallback_rounds = int(value);
@@ -1592,7 +1698,9 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
     }
     const int shape = prm.shape;
     const int tile_keys = TILE_SHAPES[shape].block * TILE_SHAPES[shape].items;
-    const Layout L = make_layout(n, ks, with_values != 0, tile_keys);
+    const Layout L =
+        make_layout(n, ks, with_values != 0, tile_keys,
+                    prm.algo != PASS_MSD && (ctx ? digit_bytes_on(ctx, ks, with_values != 0) : ks == 8 && with_values));
     if (prm.algo == PASS_MSD) {
         // two reads (OR/AND, split histogram), the split (read + write), the
         // bucket sorts (read + write; buckets within a tile)

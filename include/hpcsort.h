@@ -80,7 +80,15 @@ extern "C" {
 #define HPCS_OPT_DIGIT_BYTES 7     /* reduce-then-scan: 1 the scatter also writes each key's next digit as a
                                       byte and the next pass counts its tiles from those n bytes; 0 the next
                                       pass re-reads the keys; 2 (default) 1 for 8-byte keys with values, else 0
-                                      (scratch holds the n bytes either way) */
+                                      (the n bytes are scratch only where they are written) */
+#define HPCS_OPT_SCRATCH_LIMIT 12   /* bytes of device scratch one sort may use (0, the default: no limit).  A sort
+                                      of keys without values whose scratch would exceed it, or whose scratch
+                                      allocation fails (HPCS_E_OOM), sorts each half in place and merges the
+                                      halves with half the keys of scratch (the GPU form of the reference's
+                                      fallback to an in-place sort on OutOfMemoryException, RadixSortMsd.cs:
+                                      725-770); beyond even that, and for sorts with values beyond the limit:
+                                      HPCS_E_BUDGET, before any work.  Same output.  Applies to
+                                      hpcs_sort_radix_* and hpcs_dev_sort_radix_* */
 #define HPCS_OPT_GRAPHS 9          /* 1: graph cache for the device-resident sorts (hpcs_dev_sort_radix_*,
                                       hpcs_dev_sort_segments_u32) on a non-null stream: the first call with
                                       given pointers, size and stream runs directly, the second captures its
