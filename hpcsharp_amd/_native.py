@@ -88,9 +88,11 @@ SIGNATURES = {
     "hpcs_dev_histogram_byte_components_u32": ([_p, _p, _i64, _p, _p], _i32),
     "hpcs_dev_histogram_byte_components_u64": ([_p, _p, _i64, _p, _p], _i32),
     "hpcs_ctx_reserve": ([_p, _i32, _i64], _i32),
+    "hpcs_ctx_reserve_segments": ([_p, _i64, _i32], _i32),
     "hpcs_ctx_load_tuning": ([_p, ctypes.c_char_p], _i32),
     "hpcs_tuning_check": ([ctypes.c_char_p, _p], _i32),
     "hpcs_serve_run": ([_i32, _i32, _i64, _p, _i64, _i32, _p, _p], _i32),
+    "hpcs_serve_run_config": ([_i32, _p, _p, _i64, _p, _p], _i32),
     "hpcs_serve_last_error": ([], ctypes.c_char_p),
     "hpcs_sort_radix_keys_items": ([_p, _p, _i32, _p, _i32, _i64], _i32),
     "hpcs_dev_sort_radix_keys_items": ([_p, _p, _i32, _p, _i32, _i64, _p], _i32),

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libhpcsort.so")
 SOURCES = [os.path.join(HERE, "csrc", s) for s in ("hpcs_api.hip", "hpcs_serve.hip")]
 HEADERS = [os.path.join(HERE, "csrc", h)
            for h in ("hpcs_kernels.hpp", "hpcs_shapes.hpp", "hpcs_merge.hpp", "hpcs_digit11.hpp", "hpcs_msd.hpp", "hpcs_segments.hpp",
-                     "hpcs_topk.hpp", "hpcs_arena.hpp", "hpcs_tuning.hpp")]
+                     "hpcs_topk.hpp", "hpcs_arena.hpp", "hpcs_serve_core.hpp", "hpcs_tuning.hpp")]
 DEPS = SOURCES + HEADERS + [os.path.join(REPO, "include", "hpcsort.h")]
 ARCH = os.environ.get("HPCS_OFFLOAD_ARCH", "gfx950")
 

@@ -1750,6 +1750,15 @@ HPCS_API int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n)
     });
 }
 
+HPCS_API int32_t hpcs_ctx_reserve_segments(hpcs_ctx *ctx, int64_t n, int32_t nseg)
+{
+    return guarded(ctx, [&] {
+        if (n < 0 || nseg < 0) fail(HPCS_E_INVALID_ARG, "n and nseg must be non-negative");
+        if (n >= (int64_t(1) << 32)) fail(HPCS_E_UNSUPPORTED, "segments must lie below 2^32 keys");
+        if (nseg > 0 && n > 1) ensure(ctx->scratch, ctx->scratch_cap, seg_layout(n, nseg).total);
+    });
+}
+
 // host arrays
 HPCS_API int32_t hpcs_sort_radix_u32(hpcs_ctx *ctx, uint32_t *keys, int64_t n)
 {

@@ -1,38 +1,33 @@
-// hpcs_serve.hip -- the serving loop of libhpcsort.so.
+// hpcs_serve.hip -- the serving loop of libhpcsort.so: the HIP executor of
+// hpcs_serve_core.hpp.
 //
 // A trace of sort and radix top-k requests is replayed against one GPU.  The
 // payload of a request (n keys from a seeded generator) is produced on the
 // device when the request is admitted, so every buffer stays resident in HBM.
-// Admission is first come, first served, against an HBM budget: a request is
-// admitted when its slot has room and its buffers can be carved from the
-// arena, one device allocation made up front and managed on the host
-// (first fit, neighbours merged on release), so the hot path makes no
-// allocator calls.  Each slot owns a stream and a context (whose scratch is
-// reserved up front for the largest accepted request), a submitter thread
-// that admits and enqueues, and a reaper thread that retires completions in
-// order and returns their arena blocks; a slot keeps up to SLOT_DEPTH
-// requests enqueued, so the host prepares the next while the GPU runs the
-// previous ones, and the slots' streams overlap on the device.  A request
-// that cannot fit beside the slots' scratch on its own is rejected with
-// HPCS_E_BUDGET.
+// Admission (hpcs_serve_core.hpp) is first come, first served, against an
+// HBM budget: a request is admitted when its slot has room and its buffers
+// can be carved from the arena, one device allocation made up front and
+// managed on the host, so the hot path makes no allocator calls.  Each slot
+// owns a stream and a context (whose scratch is reserved up front for the
+// largest accepted request and for the coalesced batches), a submitter
+// thread that admits and enqueues, and a reaper thread that retires
+// completions in order, each within a deadline, and returns their arena
+// blocks; the slots' streams overlap on the device.  Small sorts of 4-byte
+// keys queued together may be coalesced: one payload launch, one segmented
+// sort and one checksum launch for the whole batch.  A request that cannot
+// fit beside the slots' scratch on its own is rejected with HPCS_E_BUDGET.
 //
 // The reference has no serving loop; this is the serving pattern built on its
 // sort and top-k surface (SortRadix, RadixSortLsd.cs; SelectTopK,
 // SelectionTopKRadix.cs:142-171).
 #include "../../include/hpcsort.h"
-#include "hpcs_arena.hpp"
+#include "hpcs_serve_core.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
-#include <condition_variable>
-#include <cstdio>
 #include <cstring>
-#include <deque>
-#include <map>
-#include <memory>
-#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -92,6 +87,52 @@ __global__ __launch_bounds__(256) void k_checksum_partial(const K *__restrict__ 
     if (threadIdx.x == 0) partial[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
+// A coalesced batch (BatchLayout): workgroup row y generates segment y's keys
+// (request ids[y]'s payload); float payloads are written as their radix keys
+// (sign flipped, negative keys inverted), so the unsigned segmented sort
+// gives their radix order.
+__global__ __launch_bounds__(256) void k_gen_batch(uint32_t *__restrict__ keys, const int64_t *__restrict__ begin,
+                                                  const uint64_t *__restrict__ seeds,
+                                                  const uint32_t *__restrict__ flt)
+{
+    const int y = blockIdx.y;
+    const uint64_t b = uint64_t(begin[y]), n = uint64_t(begin[y + 1]) - b, seed = seeds[y];
+    const bool f = flt[y] != 0;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t v = uint32_t(mix64(seed + (i + 1) * GOLDEN));
+        if (f) {
+            if (((v >> 23) & 0xFFu) == 0xFFu) v &= 0xFF7FFFFFu;
+            v ^= (v >> 31) ? 0xFFFFFFFFu : 0x80000000u;
+        }
+        keys[b + i] = v;
+    }
+}
+
+// One workgroup per segment of a sorted batch: the segment's checksum (its
+// float keys mapped back from radix keys to their bits) into sums[ids[y]].
+__global__ __launch_bounds__(256) void k_checksum_batch(const uint32_t *__restrict__ keys,
+                                                       const int64_t *__restrict__ begin,
+                                                       const int64_t *__restrict__ ids,
+                                                       const uint32_t *__restrict__ flt,
+                                                       unsigned long long *__restrict__ sums)
+{
+    __shared__ uint64_t s_w[4];
+    const int y = blockIdx.x;
+    const uint64_t b = uint64_t(begin[y]), n = uint64_t(begin[y + 1]) - b;
+    const bool f = flt[y] != 0;
+    uint64_t s = 0;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        uint32_t v = keys[b + i];
+        if (f) v ^= (v >> 31) ? 0x80000000u : 0xFFFFFFFFu;
+        s += mix64(uint64_t(v) + (i + 1) * GOLDEN);
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) sums[ids[y]] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
 __global__ __launch_bounds__(1024) void k_checksum_final(const uint64_t *__restrict__ partial, int count,
                                                          unsigned long long *__restrict__ out)
 {
@@ -109,116 +150,80 @@ __global__ __launch_bounds__(1024) void k_checksum_final(const uint64_t *__restr
 
 thread_local std::string g_serve_error;
 
-struct HipFail {
-    int32_t code;
-    std::string msg;
-};
+using hpcs_serve::Fail;
+using hpcs_serve::key_size;
+using hpcs_serve::round_up;
+using hpcs_serve::SLOT_DEPTH;
 
 void hchk(hipError_t e, const char *what)
 {
-    if (e != hipSuccess) throw HipFail{HPCS_E_HIP, std::string(what) + ": " + hipGetErrorString(e)};
-}
-
-size_t key_size(int32_t dtype) { return (dtype == HPCS_DTYPE_U32 || dtype == HPCS_DTYPE_F32) ? 4 : 8; }
-
-size_t round_up(size_t x) { return (x + 255) / 256 * 256; }
-
-// Device bytes a request holds while in flight: its payload and, for top-k,
-// the output.
-size_t request_bytes(const hpcs_request &r)
-{
-    const size_t ks = key_size(r.dtype);
-    return round_up(size_t(r.n) * ks) + (r.op == HPCS_OP_TOPK ? round_up(size_t(r.k) * ks) : 0);
-}
-
-// Scratch a slot's context holds for sorts and top-k of up to n keys.
-int64_t slot_scratch(int32_t dtype, int64_t n)
-{
-    if (n <= 0) return 0;
-    hpcs_plan_info pi{};
-    int64_t s = 0;
-    if (hpcs_plan_query(nullptr, dtype, n, 0, &pi) == HPCS_OK) s = pi.scratch_bytes;
-    return std::max<int64_t>(s, int64_t(2 * size_t(n) * key_size(dtype)) + (int64_t(1) << 20));
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) throw Fail{HPCS_E_OOM, std::string(what) + ": out of device memory"};
+        throw Fail{HPCS_E_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+    }
 }
 
 int grid_for(uint64_t n) { return int(std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), 2048)); }
 
-using Clock = std::chrono::steady_clock;
-
-// Requests a slot may have enqueued on its stream at once: the host enqueues
-// the next while the GPU runs the previous ones.
-constexpr int SLOT_DEPTH = 4;
-
-struct InFlight {
-    int64_t i;
-    size_t off, bytes; // arena block
-    hipEvent_t ev;
-};
-
-// One worker slot: a stream, a context, checksum partials, and the requests
-// enqueued on it in order.  Its submitter thread admits and enqueues; its
-// reaper thread waits for completions in order and releases the memory.
-struct Slot {
-    hipStream_t st = nullptr;
-    hpcs_ctx *ctx = nullptr;
-    uint64_t *partial = nullptr;
-    std::vector<hipEvent_t> events; // SLOT_DEPTH, free ones in `spare`
-    std::vector<hipEvent_t> spare;
-    std::deque<InFlight> inflight;
-    std::mutex m;
-    std::condition_variable cv;
-    bool submit_done = false;
-    int depth = 0; // guarded by Server::mu
-};
-
-struct Server {
+// The executor on one HIP device: per slot a stream, a context, checksum
+// partials, SLOT_DEPTH completion events and pinned staging for the batch
+// tables; the arena and the per-request checksums.
+struct HipExec {
+    struct Slot {
+        hipStream_t st = nullptr;
+        hpcs_ctx *ctx = nullptr;
+        uint64_t *partial = nullptr;
+        hipEvent_t ev[SLOT_DEPTH] = {};
+        char *stage = nullptr; // pinned: SLOT_DEPTH tables of stage_bytes
+    };
     int device = 0;
-    int64_t budget = 0;  // bytes for request buffers (after slot scratch)
-    int64_t scratch = 0; // bytes reserved per slot
+    std::vector<Slot> slots;
     char *arena_mem = nullptr;
-    hpcs::Arena arena; // guarded by mu
-    std::vector<std::unique_ptr<Slot>> slots;
     unsigned long long *d_sums = nullptr;
+    size_t stage_bytes = 0;
 
-    const hpcs_request *reqs = nullptr;
-    hpcs_request_result *res = nullptr;
-    int64_t nreq = 0;
-
-    std::mutex mu; // admission state below
-    std::condition_variable cv;
-    std::deque<int64_t> queue;
-    bool arrivals_done = false;
-    int64_t reserved = 0;
-    int64_t peak = 0;
-    Clock::time_point t0;
-    std::string first_error;
-    std::vector<char> rejected; // requests that exceed the budget on their own
-
-    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
-
-    void fail_all(const std::string &msg)
+    // Scratch a slot's context holds for sorts and top-k of up to n keys.
+    int64_t slot_scratch(int32_t dtype, int64_t n)
     {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            if (first_error.empty()) first_error = msg;
-        }
-        cv.notify_all();
+        if (n <= 0) return 0;
+        hpcs_plan_info pi{};
+        int64_t s = 0;
+        if (hpcs_plan_query(nullptr, dtype, n, 0, &pi) == HPCS_OK) s = pi.scratch_bytes;
+        return std::max<int64_t>(s, int64_t(2 * size_t(n) * key_size(dtype)) + (int64_t(1) << 20));
+    }
+
+    void setup(int nslots, const int64_t nmax[2], int64_t batch_keys, int batch_requests, size_t arena_bytes,
+               int64_t nreq)
+    {
+        hchk(hipSetDevice(device), "hipSetDevice");
+        stage_bytes = batch_requests ? round_up(size_t(batch_requests + 1) * 8) + 2 * round_up(size_t(batch_requests) * 8) +
+                                           round_up(size_t(batch_requests) * 4)
+                                     : 0;
+        slots.resize(size_t(nslots));
         for (auto &sl : slots) {
-            std::lock_guard<std::mutex> lk(sl->m);
-            sl->cv.notify_all();
+            int32_t rc = hpcs_ctx_create(device, &sl.ctx);
+            if (rc != HPCS_OK) throw Fail{rc, std::string("context: ") + hpcs_last_error()};
+            if (nmax[0] && (rc = hpcs_ctx_reserve(sl.ctx, HPCS_DTYPE_U32, nmax[0])) != HPCS_OK)
+                throw Fail{rc, std::string("reserve: ") + hpcs_last_error()};
+            if (nmax[1] && (rc = hpcs_ctx_reserve(sl.ctx, HPCS_DTYPE_U64, nmax[1])) != HPCS_OK)
+                throw Fail{rc, std::string("reserve: ") + hpcs_last_error()};
+            if (batch_keys && (rc = hpcs_ctx_reserve_segments(sl.ctx, batch_keys, batch_requests)) != HPCS_OK)
+                throw Fail{rc, std::string("reserve: ") + hpcs_last_error()};
+            hchk(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking), "hipStreamCreate");
+            hchk(hipMalloc(&sl.partial, sizeof(uint64_t) * CHECK_BLOCKS), "hipMalloc partials");
+            for (auto &e : sl.ev) hchk(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            if (stage_bytes) hchk(hipHostMalloc(&sl.stage, stage_bytes * SLOT_DEPTH, hipHostMallocDefault), "hipHostMalloc");
         }
+        if (arena_bytes) hchk(hipMalloc(&arena_mem, arena_bytes), "hipMalloc arena");
+        hchk(hipMalloc(&d_sums, sizeof(unsigned long long) * size_t(std::max<int64_t>(nreq, 1))), "hipMalloc sums");
+        hchk(hipDeviceSynchronize(), "hipDeviceSynchronize");
     }
 
-    bool failed()
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        return !first_error.empty();
-    }
+    char *arena_base() { return arena_mem; }
 
-    // Enqueue request i on slot s (no waiting); returns the call's status.
-    int32_t enqueue(Slot &sl, int64_t i, char *block, hipEvent_t ev)
+    // One request: payload, the sort or top-k call, its checksum.
+    int32_t submit_one(Slot &sl, const hpcs_request &r, int64_t i, char *block)
     {
-        const hpcs_request &r = reqs[i];
         hipStream_t st = sl.st;
         const size_t ks = key_size(r.dtype);
         void *in = block, *out = nullptr;
@@ -267,273 +272,145 @@ struct Server {
                                static_cast<const uint64_t *>(result), rn, sl.partial);
         hipLaunchKernelGGL(k_checksum_final, dim3(1), dim3(1024), 0, st, sl.partial, g, d_sums + i);
         hchk(hipGetLastError(), "checksum");
-        hchk(hipEventRecord(ev, st), "hipEventRecord");
         return rc;
     }
 
-    void submitter(int s)
+    // A coalesced batch: its tables through the token's pinned staging, one
+    // payload launch, one segmented sort, one checksum launch.
+    int32_t submit_batch(Slot &sl, int token, const hpcs_request *reqs, const int64_t *idx, int count, char *block)
     {
-        Slot &sl = *slots[size_t(s)];
-        try {
-            hchk(hipSetDevice(device), "hipSetDevice");
-            for (;;) {
-                int64_t i = -1;
-                size_t bytes = 0, off = SIZE_MAX;
-                {
-                    std::unique_lock<std::mutex> lk(mu);
-                    cv.wait(lk, [&] {
-                        if (!first_error.empty()) return true;
-                        if (queue.empty()) return arrivals_done;
-                        if (rejected[size_t(queue.front())]) return true;
-                        if (sl.depth >= SLOT_DEPTH) return false;
-                        off = arena.alloc(request_bytes(reqs[queue.front()]));
-                        return off != SIZE_MAX;
-                    });
-                    if (!first_error.empty() || queue.empty()) break;
-                    i = queue.front();
-                    queue.pop_front();
-                    res[i].admit_s = now();
-                    res[i].slot = s;
-                    if (rejected[size_t(i)]) {
-                        res[i].enqueued_s = res[i].done_s = res[i].admit_s;
-                        res[i].status = HPCS_E_BUDGET;
-                        continue;
-                    }
-                    bytes = request_bytes(reqs[i]);
-                    reserved += int64_t(bytes);
-                    peak = std::max(peak, reserved);
-                    ++sl.depth;
-                }
-                hipEvent_t ev;
-                {
-                    std::lock_guard<std::mutex> lk(sl.m);
-                    ev = sl.spare.back(); // depth <= SLOT_DEPTH keeps one free
-                    sl.spare.pop_back();
-                }
-                res[i].status = enqueue(sl, i, arena.at(off), ev);
-                res[i].enqueued_s = now();
-                {
-                    std::lock_guard<std::mutex> lk(sl.m);
-                    sl.inflight.push_back({i, off, bytes, ev});
-                }
-                sl.cv.notify_one();
-            }
-        } catch (const HipFail &e) {
-            fail_all(e.msg);
+        hipStream_t st = sl.st;
+        const hpcs_serve::BatchLayout L = hpcs_serve::batch_layout(reqs, idx, count);
+        char *stage = sl.stage + size_t(token) * stage_bytes;
+        const size_t tb = L.total - L.off_begin;            // the tables' bytes in the block
+        if (tb > stage_bytes) throw Fail{HPCS_E_HIP, "batch tables exceed the staging area"};
+        int64_t *begin = reinterpret_cast<int64_t *>(stage);
+        uint64_t *seeds = reinterpret_cast<uint64_t *>(stage + (L.off_seed - L.off_begin));
+        int64_t *ids = reinterpret_cast<int64_t *>(stage + (L.off_ids - L.off_begin));
+        uint32_t *flt = reinterpret_cast<uint32_t *>(stage + (L.off_flt - L.off_begin));
+        int64_t at = 0, most = 0;
+        for (int k = 0; k < count; ++k) {
+            const hpcs_request &r = reqs[idx[k]];
+            begin[k] = at;
+            seeds[k] = r.seed;
+            ids[k] = idx[k];
+            flt[k] = r.dtype == HPCS_DTYPE_F32 ? 1u : 0u;
+            at += r.n;
+            most = std::max(most, r.n);
         }
-        {
-            std::lock_guard<std::mutex> lk(sl.m);
-            sl.submit_done = true;
+        begin[count] = at;
+        char *dt = block + L.off_begin;
+        hchk(hipMemcpyAsync(dt, stage, tb, hipMemcpyHostToDevice, st), "batch tables");
+        uint32_t *keys = reinterpret_cast<uint32_t *>(block);
+        const int64_t *d_begin = reinterpret_cast<const int64_t *>(dt);
+        const uint64_t *d_seeds = reinterpret_cast<const uint64_t *>(block + L.off_seed);
+        const int64_t *d_ids = reinterpret_cast<const int64_t *>(block + L.off_ids);
+        const uint32_t *d_flt = reinterpret_cast<const uint32_t *>(block + L.off_flt);
+        const unsigned gx = unsigned(std::min<int64_t>(std::max<int64_t>((most + 1023) / 1024, 1), 64));
+        hipLaunchKernelGGL(k_gen_batch, dim3(gx, unsigned(count)), dim3(256), 0, st, keys, d_begin, d_seeds, d_flt);
+        hchk(hipGetLastError(), "batch payload generator");
+        const int32_t rc = hpcs_dev_sort_segments_u32(sl.ctx, keys, L.keys, d_begin, count, st);
+        if (rc == HPCS_OK) {
+            hipLaunchKernelGGL(k_checksum_batch, dim3(unsigned(count)), dim3(256), 0, st, keys, d_begin, d_ids, d_flt,
+                               d_sums);
+            hchk(hipGetLastError(), "batch checksum");
         }
-        sl.cv.notify_one();
+        return rc;
     }
 
-    void reaper(int s)
+    void submit(int s, int token, const hpcs_request *reqs, const int64_t *idx, int count, char *block,
+                int32_t *status)
     {
-        Slot &sl = *slots[size_t(s)];
-        try {
-            hchk(hipSetDevice(device), "hipSetDevice");
-            for (;;) {
-                InFlight f;
-                {
-                    std::unique_lock<std::mutex> lk(sl.m);
-                    sl.cv.wait(lk, [&] { return !sl.inflight.empty() || sl.submit_done; });
-                    if (sl.inflight.empty()) return;
-                    f = sl.inflight.front();
-                }
-                hipError_t q;
-                while ((q = hipEventQuery(f.ev)) == hipErrorNotReady) std::this_thread::yield();
-                hchk(q, "request completion");
-                res[f.i].done_s = now();
-                {
-                    std::lock_guard<std::mutex> lk(sl.m);
-                    sl.inflight.pop_front();
-                    sl.spare.push_back(f.ev);
-                }
-                {
-                    std::lock_guard<std::mutex> lk(mu);
-                    reserved -= int64_t(f.bytes);
-                    arena.release(f.off, f.bytes);
-                    --sl.depth;
-                }
-                cv.notify_all();
-            }
-        } catch (const HipFail &e) {
-            fail_all(e.msg);
-            // drain: wait for what this slot enqueued so that nothing is freed under it
-            std::unique_lock<std::mutex> lk(sl.m);
-            sl.cv.wait(lk, [&] { return sl.submit_done; });
-            (void)hipStreamSynchronize(sl.st);
+        Slot &sl = slots[size_t(s)];
+        if (count == 1) {
+            status[0] = submit_one(sl, reqs[idx[0]], idx[0], block);
+        } else {
+            const int32_t rc = submit_batch(sl, token, reqs, idx, count, block);
+            for (int k = 0; k < count; ++k) status[k] = rc;
         }
+        hchk(hipEventRecord(sl.ev[token], sl.st), "hipEventRecord");
+    }
+
+    bool poll(int s, int token)
+    {
+        const hipError_t q = hipEventQuery(slots[size_t(s)].ev[token]);
+        if (q == hipErrorNotReady) return false;
+        hchk(q, "request completion");
+        return true;
+    }
+
+    bool drain(int s, double seconds)
+    {
+        const auto end = hpcs_serve::Clock::now() +
+                         std::chrono::duration_cast<hpcs_serve::Clock::duration>(std::chrono::duration<double>(seconds));
+        for (;;) {
+            const hipError_t q = hipStreamQuery(slots[size_t(s)].st);
+            if (q != hipErrorNotReady) return true; // complete, or failed: nothing runs any more
+            if (hpcs_serve::Clock::now() > end) return false;
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+        }
+    }
+
+    void checksums(uint64_t *out, int64_t nreq)
+    {
+        hchk(hipMemcpy(out, d_sums, sizeof(unsigned long long) * size_t(std::max<int64_t>(nreq, 1)),
+                       hipMemcpyDeviceToHost),
+             "checksums");
+    }
+
+    void release(bool leak)
+    {
+        for (auto &sl : slots) {
+            if (!leak && sl.st) (void)hipStreamSynchronize(sl.st);
+            if (!leak && sl.ctx) hpcs_ctx_destroy(sl.ctx);
+            if (!leak && sl.st) (void)hipStreamDestroy(sl.st);
+            if (!leak)
+                for (auto e : sl.ev)
+                    if (e) (void)hipEventDestroy(e);
+            if (!leak && sl.partial) (void)hipFree(sl.partial);
+            if (!leak && sl.stage) (void)hipHostFree(sl.stage);
+        }
+        slots.clear();
+        if (!leak && arena_mem) (void)hipFree(arena_mem);
+        if (!leak && d_sums) (void)hipFree(d_sums);
+        arena_mem = nullptr;
+        d_sums = nullptr;
     }
 };
-
-void validate(const hpcs_request *reqs, int64_t nreq)
-{
-    double prev = 0.0;
-    for (int64_t i = 0; i < nreq; ++i) {
-        const hpcs_request &r = reqs[i];
-        if (r.op != HPCS_OP_SORT && r.op != HPCS_OP_TOPK) throw HipFail{HPCS_E_INVALID_ARG, "unknown op"};
-        if (r.dtype < HPCS_DTYPE_U32 || r.dtype > HPCS_DTYPE_F64) throw HipFail{HPCS_E_INVALID_ARG, "unknown dtype"};
-        if (r.n < 1 || r.n >= (int64_t(1) << 31)) throw HipFail{HPCS_E_INVALID_ARG, "n must lie in [1, 2^31)"};
-        if (r.op == HPCS_OP_TOPK && (r.k < 1 || r.k > r.n)) throw HipFail{HPCS_E_INVALID_ARG, "k must lie in [1, n]"};
-        if (r.arrival_s < prev) throw HipFail{HPCS_E_INVALID_ARG, "arrivals must be non-decreasing"};
-        prev = r.arrival_s;
-    }
-}
-
-void release(Server &S)
-{
-    for (auto &p : S.slots) {
-        Slot &sl = *p;
-        if (sl.st) (void)hipStreamSynchronize(sl.st);
-        if (sl.ctx) hpcs_ctx_destroy(sl.ctx);
-        if (sl.st) (void)hipStreamDestroy(sl.st);
-        for (auto ev : sl.events) (void)hipEventDestroy(ev);
-        if (sl.partial) (void)hipFree(sl.partial);
-    }
-    S.slots.clear();
-    if (S.arena_mem) (void)hipFree(S.arena_mem);
-    if (S.d_sums) (void)hipFree(S.d_sums);
-    S.arena_mem = nullptr;
-    S.d_sums = nullptr;
-}
 
 } // namespace
 
 HPCS_API const char *hpcs_serve_last_error(void) { return g_serve_error.c_str(); }
 
+HPCS_API int32_t hpcs_serve_run_config(int32_t device, const hpcs_serve_config *cfg, const hpcs_request *reqs,
+                                       int64_t nreq, hpcs_request_result *results, hpcs_serve_stats *stats)
+{
+    HipExec ex;
+    ex.device = device;
+    try {
+        if (!cfg) throw Fail{HPCS_E_INVALID_ARG, "config is null"};
+        if (!reqs || !results || nreq < 0) throw Fail{HPCS_E_INVALID_ARG, "requests or results are null"};
+        const hpcs_serve_config c = hpcs_serve::resolve_config(*cfg);
+        hpcs_serve::validate(reqs, nreq);
+        std::memset(results, 0, sizeof(hpcs_request_result) * size_t(nreq));
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        hpcs_serve::Server<HipExec> S(ex, c, reqs, nreq, results);
+        std::string err;
+        const int32_t rc = S.run(stats, &err);
+        if (rc != HPCS_OK) g_serve_error = err;
+        return rc;
+    } catch (const Fail &e) {
+        g_serve_error = e.msg;
+        return e.code;
+    }
+}
+
 HPCS_API int32_t hpcs_serve_run(int32_t device, int32_t slots, int64_t hbm_budget, const hpcs_request *reqs,
                                 int64_t nreq, int32_t mode, hpcs_request_result *results, hpcs_serve_stats *stats)
 {
-    Server S;
-    try {
-        if (!reqs || !results || nreq < 0) throw HipFail{HPCS_E_INVALID_ARG, "requests or results are null"};
-        if (slots < 1 || slots > 64) throw HipFail{HPCS_E_INVALID_ARG, "slots must lie in [1, 64]"};
-        if (mode != HPCS_SERVE_REALTIME && mode != HPCS_SERVE_ALONE) throw HipFail{HPCS_E_INVALID_ARG, "unknown mode"};
-        if (hbm_budget <= 0) throw HipFail{HPCS_E_INVALID_ARG, "budget must be positive"};
-        validate(reqs, nreq);
-        if (mode == HPCS_SERVE_ALONE) slots = 1;
-        std::memset(results, 0, sizeof(hpcs_request_result) * size_t(nreq));
-        if (stats) std::memset(stats, 0, sizeof(*stats));
-        hchk(hipSetDevice(device), "hipSetDevice");
-        S.device = device;
-        S.reqs = reqs;
-        S.res = results;
-        S.nreq = nreq;
-        // Slot scratch covers the largest accepted request of each key size; a
-        // request whose payload does not fit beside the slots' scratch is
-        // rejected (HPCS_E_BUDGET), largest first, until the rest fit.
-        S.rejected.assign(size_t(nreq), 0);
-        std::vector<int64_t> order(static_cast<size_t>(nreq));
-        for (int64_t i = 0; i < nreq; ++i) order[size_t(i)] = i;
-        std::sort(order.begin(), order.end(),
-                  [&](int64_t a, int64_t b) { return request_bytes(reqs[a]) > request_bytes(reqs[b]); });
-        int64_t nmax[2] = {0, 0};
-        for (size_t cut = 0;; ++cut) {
-            nmax[0] = nmax[1] = 0;
-            for (size_t j = cut; j < order.size(); ++j) {
-                const hpcs_request &r = reqs[order[j]];
-                const int w = key_size(r.dtype) == 4 ? 0 : 1;
-                nmax[w] = std::max(nmax[w], r.n);
-            }
-            const int64_t per_slot = std::max(slot_scratch(HPCS_DTYPE_U32, nmax[0]), slot_scratch(HPCS_DTYPE_U64, nmax[1]));
-            const int64_t left = hbm_budget - per_slot * slots;
-            if (cut == order.size()) {
-                if (left <= 0) throw HipFail{HPCS_E_BUDGET, "the budget does not cover the slots' scratch"};
-                S.scratch = per_slot;
-                S.budget = left;
-                break;
-            }
-            if (left > 0 && int64_t(request_bytes(reqs[order[cut]])) <= left) {
-                S.scratch = per_slot;
-                S.budget = left;
-                break;
-            }
-            S.rejected[size_t(order[cut])] = 1;
-        }
-        for (int s = 0; s < slots; ++s) {
-            S.slots.emplace_back(new Slot());
-            Slot &sl = *S.slots.back();
-            int32_t rc = hpcs_ctx_create(device, &sl.ctx);
-            if (rc != HPCS_OK) throw HipFail{rc, std::string("context: ") + hpcs_last_error()};
-            if (nmax[0] && (rc = hpcs_ctx_reserve(sl.ctx, HPCS_DTYPE_U32, nmax[0])) != HPCS_OK)
-                throw HipFail{rc, std::string("reserve: ") + hpcs_last_error()};
-            if (nmax[1] && (rc = hpcs_ctx_reserve(sl.ctx, HPCS_DTYPE_U64, nmax[1])) != HPCS_OK)
-                throw HipFail{rc, std::string("reserve: ") + hpcs_last_error()};
-            hchk(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking), "hipStreamCreate");
-            hchk(hipMalloc(&sl.partial, sizeof(uint64_t) * CHECK_BLOCKS), "hipMalloc partials");
-            for (int d = 0; d < SLOT_DEPTH; ++d) {
-                hipEvent_t ev;
-                hchk(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
-                sl.events.push_back(ev);
-                sl.spare.push_back(ev);
-            }
-        }
-        // the arena never needs more than what the slots can hold in flight
-        size_t most = 0;
-        for (int64_t i = 0; i < nreq; ++i)
-            if (!S.rejected[size_t(i)]) most = std::max(most, request_bytes(reqs[i]));
-        const size_t arena_bytes = std::min<size_t>(size_t(S.budget), most * size_t(slots) * SLOT_DEPTH);
-        if (arena_bytes) hchk(hipMalloc(&S.arena_mem, arena_bytes), "hipMalloc arena");
-        S.arena.reset(S.arena_mem, arena_bytes);
-        hchk(hipMalloc(&S.d_sums, sizeof(unsigned long long) * size_t(std::max<int64_t>(nreq, 1))), "hipMalloc sums");
-        hchk(hipDeviceSynchronize(), "hipDeviceSynchronize");
-
-        S.t0 = Clock::now();
-        std::vector<std::thread> th;
-        for (int s = 0; s < slots; ++s) {
-            th.emplace_back([&S, s] { S.submitter(s); });
-            th.emplace_back([&S, s] { S.reaper(s); });
-        }
-        if (mode == HPCS_SERVE_ALONE) {
-            std::lock_guard<std::mutex> lk(S.mu);
-            for (int64_t i = 0; i < nreq; ++i) S.queue.push_back(i);
-            S.arrivals_done = true;
-            S.cv.notify_all();
-        } else {
-            for (int64_t i = 0; i < nreq; ++i) {
-                const auto due = S.t0 + std::chrono::duration_cast<Clock::duration>(
-                                            std::chrono::duration<double>(reqs[i].arrival_s));
-                for (;;) {
-                    const auto left = due - Clock::now();
-                    if (left <= Clock::duration::zero()) break;
-                    if (left > std::chrono::microseconds(300)) std::this_thread::sleep_for(left - std::chrono::microseconds(200));
-                    else std::this_thread::yield();
-                }
-                std::lock_guard<std::mutex> lk(S.mu);
-                S.queue.push_back(i);
-                S.cv.notify_all();
-            }
-            std::lock_guard<std::mutex> lk(S.mu);
-            S.arrivals_done = true;
-            S.cv.notify_all();
-        }
-        for (auto &t : th) t.join();
-        if (!S.first_error.empty()) throw HipFail{HPCS_E_HIP, S.first_error};
-        std::vector<unsigned long long> sums(size_t(std::max<int64_t>(nreq, 1)));
-        hchk(hipMemcpy(sums.data(), S.d_sums, sizeof(unsigned long long) * sums.size(), hipMemcpyDeviceToHost),
-             "checksums");
-        double last = 0.0;
-        int64_t keys = 0;
-        for (int64_t i = 0; i < nreq; ++i) {
-            results[i].checksum = sums[size_t(i)];
-            last = std::max(last, results[i].done_s);
-            if (results[i].status == HPCS_OK) keys += reqs[i].n;
-        }
-        if (stats) {
-            stats->wall_s = last;
-            stats->keys = keys;
-            stats->peak_bytes = S.peak;
-            stats->request_budget_bytes = S.budget;
-            stats->scratch_bytes_per_slot = S.scratch;
-            stats->slots = slots;
-        }
-    } catch (const HipFail &e) {
-        g_serve_error = e.msg;
-        release(S);
-        return e.code;
-    }
-    release(S);
-    return HPCS_OK;
+    hpcs_serve_config c{};
+    c.slots = slots;
+    c.mode = mode;
+    c.hbm_budget = hbm_budget;
+    return hpcs_serve_run_config(device, &c, reqs, nreq, results, stats);
 }

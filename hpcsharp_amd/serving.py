@@ -27,7 +27,15 @@ RESULT = np.dtype([("admit_s", np.float64), ("enqueued_s", np.float64), ("done_s
 class ServeStats(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("keys", ctypes.c_int64), ("peak_bytes", ctypes.c_int64),
                 ("request_budget_bytes", ctypes.c_int64), ("scratch_bytes_per_slot", ctypes.c_int64),
-                ("slots", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("slots", ctypes.c_int32), ("pad", ctypes.c_int32), ("batches", ctypes.c_int64),
+                ("coalesced_requests", ctypes.c_int64)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int32), ("mode", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
+                ("request_timeout_s", ctypes.c_double), ("coalesce_max_keys", ctypes.c_int64),
+                ("coalesce_max_requests", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("coalesce_max_batch_keys", ctypes.c_int64)]
 
 
 DTYPES = {N.HPCS_DTYPE_U32: np.uint32, N.HPCS_DTYPE_U64: np.uint64,
@@ -55,15 +63,23 @@ def make_trace(nreq: int = 20000, rate: float = 2000.0, seed: int = 0x5EED0009, 
 
 
 def run(trace: np.ndarray, slots: int = 4, hbm_budget: int = 200 * 10 ** 9,
-        mode: int = N.HPCS_SERVE_REALTIME, device: int = 0):
-    """Replay the trace on one GPU; returns (results, stats)."""
+        mode: int = N.HPCS_SERVE_REALTIME, device: int = 0, timeout_s: float = 0.0,
+        coalesce_max_keys: int = 0, coalesce_max_requests: int = 0, coalesce_max_batch_keys: int = 0):
+    """Replay the trace on one GPU; returns (results, stats).  timeout_s: the
+    longest wait for one request's completion (0: 5 s + 20 ns per key);
+    coalesce_max_keys > 0: queued u32 / f32 sorts of at most that many keys
+    run together as one segmented sort (coalesce_max_requests per batch, 0:
+    256; coalesce_max_batch_keys keys per batch, 0: 2^22)."""
     trace = np.ascontiguousarray(trace, dtype=REQUEST)
     if not os.environ.get("HPCS_TUNE_FILE") and os.path.exists(N.TUNE_PATH):
         os.environ["HPCS_TUNE_FILE"] = N.TUNE_PATH   # the slots' contexts load the measured table
     res = np.zeros(trace.size, dtype=RESULT)
     st = ServeStats()
-    rc = N.lib().hpcs_serve_run(device, slots, int(hbm_budget), trace.ctypes.data_as(ctypes.c_void_p), trace.size,
-                                mode, res.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st))
+    cfg = ServeConfig(slots=slots, mode=mode, hbm_budget=int(hbm_budget), request_timeout_s=float(timeout_s),
+                      coalesce_max_keys=int(coalesce_max_keys), coalesce_max_requests=int(coalesce_max_requests),
+                      coalesce_max_batch_keys=int(coalesce_max_batch_keys))
+    rc = N.lib().hpcs_serve_run_config(device, ctypes.byref(cfg), trace.ctypes.data_as(ctypes.c_void_p), trace.size,
+                                       res.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st))
     if rc != N.HPCS_OK:
         raise N.HpcsError(rc, N.lib().hpcs_serve_last_error().decode())
     return res, st
@@ -131,5 +147,6 @@ def summarize(trace: np.ndarray, res: np.ndarray, stats: ServeStats | None = Non
     if stats is not None:
         out.update(slots=int(stats.slots), peak_request_bytes=int(stats.peak_bytes),
                    request_budget_bytes=int(stats.request_budget_bytes),
-                   scratch_bytes_per_slot=int(stats.scratch_bytes_per_slot))
+                   scratch_bytes_per_slot=int(stats.scratch_bytes_per_slot),
+                   batches=int(stats.batches), coalesced_requests=int(stats.coalesced_requests))
     return out

@@ -146,6 +146,10 @@ int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t value);
  * of up to n keys of `dtype` allocate nothing (no counterpart in the
  * reference; for latency-bound callers such as the serving loop). */
 int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n);
+/* Size the context's scratch for hpcs_dev_sort_segments_u32 of up to n keys
+ * in up to nseg segments, so those calls allocate nothing (no counterpart in
+ * the reference: for serving loops that reserve memory up front). */
+int32_t hpcs_ctx_reserve_segments(hpcs_ctx *ctx, int64_t n, int32_t nseg);
 /* Tuning tables (no counterpart in the reference, whose tuning knobs are
  * static properties, RadixSortLsdParallel.cs:763, SystemAttributes.cs:9).
  * A table is text, one row per line (`#` comments):
@@ -391,11 +395,35 @@ typedef struct hpcs_serve_stats {
     int64_t scratch_bytes_per_slot; /* scratch reserved per slot */
     int32_t slots;
     int32_t pad;
+    int64_t batches;                /* launch chains enqueued: single requests plus coalesced batches */
+    int64_t coalesced_requests;     /* requests that ran in a coalesced batch of two or more */
 } hpcs_serve_stats;
 
-/* Run the requests on `device` with `slots` worker slots (one stream and one
- * context each) within hbm_budget bytes of device memory (request buffers
- * plus the slots' scratch).  results[i] belongs to reqs[i]. */
+typedef struct hpcs_serve_config {
+    int32_t slots;             /* worker slots, 1-64 (one stream and one context each; HPCS_SERVE_ALONE: 1) */
+    int32_t mode;              /* HPCS_SERVE_REALTIME or HPCS_SERVE_ALONE */
+    int64_t hbm_budget;        /* bytes of device memory: request buffers plus the slots' scratch */
+    double request_timeout_s;  /* longest wait for a request's (or batch's) completion once its slot's
+                                  earlier requests are done; past it the run fails with HPCS_E_HIP and a
+                                  message naming the request.  0: 5 s + 20 ns per key */
+    int64_t coalesce_max_keys; /* sorts of u32 / f32 payloads of at most this many keys that are queued
+                                  together are admitted as one batch and run as one segmented sort
+                                  (hpcs_dev_sort_segments_u32, floats in their radix order); same results.
+                                  0: no coalescing */
+    int32_t coalesce_max_requests; /* requests per batch, 2-1024 (0: 256) */
+    int32_t pad;
+    int64_t coalesce_max_batch_keys; /* keys per batch (0: 2^22) */
+} hpcs_serve_config;
+
+/* Run the requests on `device` under `cfg`.  Admission is first come, first
+ * served; a request that cannot fit beside the slots' scratch on its own
+ * gets status HPCS_E_BUDGET.  results[i] belongs to reqs[i]; on a failed
+ * run (the return code, hpcs_serve_last_error naming the first error) the
+ * results are not meaningful. */
+int32_t hpcs_serve_run_config(int32_t device, const hpcs_serve_config *cfg, const hpcs_request *reqs, int64_t nreq,
+                              hpcs_request_result *results, hpcs_serve_stats *stats);
+/* hpcs_serve_run_config with `slots`, `mode` and hbm_budget, the derived
+ * request timeout and no coalescing. */
 int32_t hpcs_serve_run(int32_t device, int32_t slots, int64_t hbm_budget, const hpcs_request *reqs, int64_t nreq,
                        int32_t mode, hpcs_request_result *results, hpcs_serve_stats *stats);
 const char *hpcs_serve_last_error(void);

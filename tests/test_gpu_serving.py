@@ -82,3 +82,32 @@ def test_budget_admission_and_rejection():
     res2, st3 = S.run(t2[:20], slots=2, hbm_budget=2 * st.scratch_bytes_per_slot + 3 * big)
     assert res2["status"][5] == N.HPCS_E_BUDGET
     assert np.all(np.delete(res2["status"], 5) == N.HPCS_OK)
+
+
+def test_coalesced_small_sorts_match_the_oracle(oracle):
+    """Small u32 / f32 sorts queued together run as one segmented sort: the
+    same checksums as the oracle's results, alone or coalesced."""
+    t = S.make_trace(nreq=400, rate=200000.0, seed=0x5EED0077, n_lo=1 << 4, n_hi=1 << 14, sort_frac=0.85,
+                     dtypes=(N.HPCS_DTYPE_U32, N.HPCS_DTYPE_F32, N.HPCS_DTYPE_U64))
+    res, st = S.run(t, slots=2, coalesce_max_keys=1 << 14, coalesce_max_requests=64)
+    assert np.all(res["status"] == N.HPCS_OK)
+    assert st.coalesced_requests > 50 and st.batches < t.size
+    want = np.array([_expected(oracle, r) for r in t], dtype=np.uint64)
+    bad = np.nonzero(res["checksum"] != want)[0]
+    assert bad.size == 0, [(int(i), t[i]) for i in bad[:5]]
+    alone, st1 = S.run(t, slots=2)
+    assert st1.coalesced_requests == 0
+    assert np.array_equal(alone["checksum"], res["checksum"])
+
+
+def test_forced_deadline_fails_naming_a_request():
+    """A deadline shorter than any request (forced from the host; every
+    kernel still runs to its end) fails the run with HPCS_E_HIP and a
+    message naming the request; a later run on the same device succeeds."""
+    t = _trace()
+    with pytest.raises(N.HpcsError) as e:
+        S.run(t, slots=2, timeout_s=1e-7)
+    assert e.value.code == N.HPCS_E_HIP
+    assert "did not complete within" in str(e.value) and "request " in str(e.value)
+    res, _ = S.run(t[:40], slots=2)
+    assert np.all(res["status"] == N.HPCS_OK)

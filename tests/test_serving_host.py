@@ -5,6 +5,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -40,9 +41,10 @@ def test_layouts_match_the_header():
 #include <stddef.h>
 #include "hpcsort.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(hpcs_request), offsetof(hpcs_request, seed),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(hpcs_request), offsetof(hpcs_request, seed),
          offsetof(hpcs_request, arrival_s), sizeof(hpcs_request_result), offsetof(hpcs_request_result, status),
-         sizeof(hpcs_serve_stats));
+         sizeof(hpcs_serve_stats), offsetof(hpcs_serve_stats, coalesced_requests), sizeof(hpcs_serve_config),
+         offsetof(hpcs_serve_config, request_timeout_s), offsetof(hpcs_serve_config, coalesce_max_batch_keys));
   return 0; }
 """
     exe = os.path.join("/tmp", f"hpcs_layout_{os.getpid()}")
@@ -52,7 +54,9 @@ int main(void) {
     got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     os.remove(exe)
     assert got == [S.REQUEST.itemsize, S.REQUEST.fields["seed"][1], S.REQUEST.fields["arrival_s"][1],
-                   S.RESULT.itemsize, S.RESULT.fields["status"][1], ctypes.sizeof(S.ServeStats)]
+                   S.RESULT.itemsize, S.RESULT.fields["status"][1], ctypes.sizeof(S.ServeStats),
+                   S.ServeStats.coalesced_requests.offset, ctypes.sizeof(S.ServeConfig),
+                   S.ServeConfig.request_timeout_s.offset, S.ServeConfig.coalesce_max_batch_keys.offset]
 
 
 def test_checksum_known_answers():
@@ -93,13 +97,20 @@ def test_summary_percentiles():
     assert s["topk_p50_ms"] == pytest.approx(30.0)
 
 
-@pytest.mark.parametrize("bad", ["op", "n", "k", "order", "slots", "budget"])
+@pytest.mark.parametrize("bad", ["op", "n", "k", "order", "slots", "budget", "timeout", "batch_requests",
+                                 "batch_keys"])
 def test_engine_rejects_bad_requests_before_touching_a_device(bad):
     if not os.path.exists(N.LIB_PATH):
         pytest.skip("libhpcsort.so not built")
     t = S.make_trace(nreq=8)
-    slots, budget = 2, 10 ** 9
-    if bad == "op":
+    slots, budget, extra = 2, 10 ** 9, {}
+    if bad == "timeout":
+        extra = dict(timeout_s=-1.0)
+    elif bad == "batch_requests":
+        extra = dict(coalesce_max_keys=4096, coalesce_max_requests=1)
+    elif bad == "batch_keys":
+        extra = dict(coalesce_max_keys=-5)
+    elif bad == "op":
         t["op"][3] = 7
     elif bad == "n":
         t["n"][2] = 0
@@ -112,5 +123,26 @@ def test_engine_rejects_bad_requests_before_touching_a_device(bad):
     else:
         budget = 0
     with pytest.raises(N.HpcsError) as e:
-        S.run(t, slots=slots, hbm_budget=budget)
+        S.run(t, slots=slots, hbm_budget=budget, **extra)
     assert e.value.code == N.HPCS_E_INVALID_ARG
+
+
+@pytest.mark.parametrize("sanitize", ["thread", "address,undefined"])
+def test_serving_core_with_a_mock_executor(tmp_path, sanitize):
+    """The engine's admission, submitter and reaper threads (the same
+    hpcs_serve_core.hpp the library builds) driven by a mock executor:
+    results, budget, coalesced batches, injected allocation failures and
+    completion deadlines, under the thread sanitizer and, separately, the
+    address and undefined-behaviour sanitizers."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path / "serve_mock_test")
+    flags = ["-fsanitize=" + sanitize] + (["-fno-sanitize-recover=undefined"] if "undefined" in sanitize else [])
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-g", *flags, "-pthread", "-Wall", "-Wextra", "-Werror",
+                        "-I", os.path.join(REPO, "include"), os.path.join(REPO, "tests", "host", "serve_mock_test.cpp"),
+                        "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66", ASAN_OPTIONS="detect_leaks=1")
+    run = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=env)
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert "all checks passed" in run.stdout
