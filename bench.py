@@ -409,8 +409,11 @@ def serving_figures(nreq: int, slots: int, sweep: bool = True):
     """The serving workload: Poisson arrivals (2000/s), 70% sort / 30% top-k,
     n log-uniform in [2^10, 2^26), u32/f32/u64, 200 GB budget; once as is,
     once with small sorts coalesced, then (sweep) the coalescing engine at
-    2x, 4x, 8x the arrival rate until p99 exceeds 10x p50.  Every request's
-    checksum must be the same in every run (the same payloads and results)."""
+    1x, 2x, 4x, 8x the arrival rate.  Sizes span 2^16, so p99 is far above
+    p50 even on an idle GPU: saturation is judged against the 1x run, as the
+    highest rate whose p99 stays within 10x the 1x p99 while the achieved
+    throughput keeps within 5% of the offered.  Every request's checksum must
+    be the same in every run (the same payloads and results)."""
     from hpcsharp_amd import serving as S
 
     def fig(t, **kw):
@@ -423,23 +426,26 @@ def serving_figures(nreq: int, slots: int, sweep: bool = True):
     res1, out["coalesced"] = fig(t, coalesce_max_keys=COALESCE_MAX_KEYS)
     out["coalesced"]["same_checksums"] = bool(np.array_equal(res0["checksum"], res1["checksum"]))
     if sweep:
-        rows, best = [], None
+        rows, best, base_p99 = [], None, None
         for mult in (1, 2, 4, 8):
             tm = S.make_trace(nreq=nreq, rate=2000.0 * mult)
             resm, f = fig(tm, coalesce_max_keys=COALESCE_MAX_KEYS)
             row = {"rate_per_s": 2000 * mult, "p50_ms": f["p50_ms"], "p99_ms": f["p99_ms"],
                    "gkeys_per_s": f["gkeys_per_s"], "offered_gkeys_per_s": f["offered_gkeys_per_s"],
+                   "coalesced_requests": f["coalesced_requests"], "batches": f["batches"],
                    "completed": f["completed"], "same_checksums": bool(np.array_equal(resm["checksum"], res0["checksum"]))}
             rows.append(row)
-            if f["p99_ms"] > 10 * f["p50_ms"]:
-                break
-            best = row
+            base_p99 = base_p99 or f["p99_ms"]
+            if f["p99_ms"] <= 10 * base_p99 and f["gkeys_per_s"] >= 0.95 * f["offered_gkeys_per_s"]:
+                best = row
         out["saturation_sweep"] = {
             "rows": rows,
             "highest_sustained_gkeys_per_s": best["gkeys_per_s"] if best else None,
             "sustained_rate_per_s": best["rate_per_s"] if best else None,
-            "note": "coalescing engine; sustained: the highest arrival rate (x1, x2, x4, x8 of 2000/s) whose p99 "
-                    "latency stays within 10x its p50; the same seeds at every rate, so checksums must match"}
+            "peak_gkeys_per_s": max(r["gkeys_per_s"] for r in rows),
+            "note": "coalescing engine at 1x-8x of 2000 requests/s, the same seeds at every rate (checksums must "
+                    "match); sustained: the highest rate whose p99 stays within 10x the 1x p99 with the achieved "
+                    "throughput within 5% of the offered; peak: the most Gkeys/s achieved at any rate"}
     return out
 
 

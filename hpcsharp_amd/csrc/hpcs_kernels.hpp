@@ -184,18 +184,16 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t words, uint32_t va
 // constant inputs).  That test runs once per digit and chunk, on the OR over
 // a lane's keys of key XOR lane 0's first key; a test per key and digit cost
 // more instructions than the atomics it saved.  The table holds COPIES
-// interleaved copies of the [P][256] counts, word (d * 256 + bin) * COPIES +
-// lane % COPIES, so that a wave's atomics on random digits collide less on
-// LDS banks.  The tail below a whole chunk is counted lane by lane.  Block
+// interleaved copies of the [P][256] counts (32-bit), word (d * 256 + bin) *
+// COPIES + lane % COPIES, so that a wave's atomics on random digits collide
+// less on LDS banks: with 16 copies (4-byte keys) or 8 (8-byte keys) the
+// table fills 64 KiB, two workgroups per CU.  UNROLL chunks per wave are in
+// flight.  The tail below a whole chunk is counted lane by lane.  Block
 // tables (copies summed) are added into the global [P][256] result with one
 // atomic per non-zero bin.
 // ---------------------------------------------------------------------------
 // `head` (< VEC) keys precede the first 16-byte aligned key; they are counted
 // lane by lane together with the tail.
-#ifndef HPCS_HIST_UNROLL
-#define HPCS_HIST_UNROLL 4
-#endif
-constexpr int HIST_UNROLL = HPCS_HIST_UNROLL; // 1 KiB chunks per wave in flight
 
 // lane 0's value of x, for every lane
 __device__ inline uint32_t wave_first(uint32_t x) { return uint32_t(__builtin_amdgcn_readfirstlane(int(x))); }
@@ -203,7 +201,7 @@ __device__ inline uint64_t wave_first(uint64_t x)
 {
     return uint64_t(wave_first(uint32_t(x))) | (uint64_t(wave_first(uint32_t(x >> 32))) << 32);
 }
-template <typename K, int FLOATK, int BLOCK, int COPIES>
+template <typename K, int FLOATK, int BLOCK, int COPIES, int UNROLL>
 __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys, uint64_t n, uint32_t head,
                                                      uint32_t *__restrict__ hist)
 {
@@ -225,15 +223,15 @@ __global__ __launch_bounds__(BLOCK) void k_histogram(const K *__restrict__ keys,
     const uint64_t nwaves = uint64_t(gridDim.x) * WAVES;
     const Vec *vkeys = reinterpret_cast<const Vec *>(keys + head);
 
-    // HIST_UNROLL chunks per wave and round, all loads issued before any is
+    // UNROLL chunks per wave and round, all loads issued before any is
     // counted (one 1 KiB load per wave in flight left HBM mostly idle)
-    for (uint64_t c0 = wave_id * HIST_UNROLL; c0 < nchunks; c0 += nwaves * HIST_UNROLL) {
-        Vec v[HIST_UNROLL];
+    for (uint64_t c0 = wave_id * UNROLL; c0 < nchunks; c0 += nwaves * UNROLL) {
+        Vec v[UNROLL];
 #pragma unroll
-        for (int u = 0; u < HIST_UNROLL; ++u)
+        for (int u = 0; u < UNROLL; ++u)
             if (c0 + u < nchunks) v[u] = __builtin_nontemporal_load(&vkeys[(c0 + u) * WAVE + lane]);
 #pragma unroll
-        for (int u = 0; u < HIST_UNROLL; ++u) {
+        for (int u = 0; u < UNROLL; ++u) {
             if (c0 + u >= nchunks) break; // wave-uniform
             K kk[VEC];
 #pragma unroll
