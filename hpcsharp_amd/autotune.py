@@ -8,6 +8,10 @@ serves the key type, hpcs_tile_shape_query: 256, 512 and 1024 threads, 8 to
 32 keys per thread) and ranking (ballot, LDS-OR, LDS-OR with every second or
 fourth item by ballot; ballot only on the 1024-thread shapes).
 Size classes are powers of two; a class's row applies to every n up to it.
+The batched segmented sort (kind "segs") is tuned per class of total keys:
+the largest segment one workgroup sorts in LDS (4096 or 16384) and the
+ranking of the large segments' scatter, on a batch of segments with
+log-uniform lengths in [16, min(2^20, total / 16)].
 Each measured cell is appended to a JSON log as it finishes, and a rerun
 skips the cells the log already holds (the sweep resumes).
 
@@ -34,6 +38,7 @@ KINDS = {
     "pairs64": (8, True, "hpcs_dev_sort_radix_pairs_u64_u32"),
 }
 RANKS = (1, 2, 3, 0)  # LDS-OR, every 2nd / 4th item by ballot, ballot
+SEG_SMALL = (4096, 16384)  # largest segment sorted in LDS (HPCS_OPT_SEG_SMALL)
 
 
 # largest arrays the one-workgroup sort (pass algorithm 3) takes
@@ -111,6 +116,45 @@ def time_variant(torch, ctx, kind: str, lg: int, cand: dict, reps: int) -> float
     return times[len(times) // 2]
 
 
+def seg_batch(lg: int):
+    """Segment lengths (log-uniform in [16, min(2^20, 2^lg / 16)]) whose
+    expected total is 2^lg keys."""
+    import numpy as np
+    from hpcsharp_amd import datagen
+    lo, hi = 16, min(1 << 20, (1 << lg) // 16)
+    mean = (hi - lo) / np.log(hi / lo)
+    nseg = max(1, int(round((1 << lg) / mean)))
+    return datagen.segment_lengths(nseg, 0x5E6 + lg, lo, hi)
+
+
+def time_segments(torch, ctx, lg: int, small: int, rank: int, reps: int) -> float:
+    """Median seconds per batched call (device-resident keys)."""
+    import numpy as np
+    N.set_option(N.HPCS_OPT_SEG_SMALL, small, ctx)
+    N.set_option(N.HPCS_OPT_SEG_RANK, rank, ctx)
+    lens = seg_batch(lg)
+    begin = torch.from_numpy(np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)).cuda()
+    n = int(lens.sum())
+    g = torch.Generator(device="cuda").manual_seed(lg)
+    src = torch.randint(-2 ** 31, 2 ** 31 - 1, (n,), dtype=torch.int32, device="cuda", generator=g)
+    keys = src.clone()
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    times = []
+    for r in range(reps + 1):
+        keys.copy_(src)
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(st)
+        N.check(N.lib().hpcs_dev_sort_segments_u32(ctx, ctypes.c_void_p(keys.data_ptr()), n,
+                                                   ctypes.c_void_p(begin.data_ptr()), int(lens.size), sp))
+        end.record(st)
+        end.synchronize()
+        if r:
+            times.append(start.elapsed_time(end) * 1e-3)
+    times.sort()
+    return times[len(times) // 2]
+
+
 def table_text(results: dict) -> str:
     lines = ["# libhpcsort tuning table (hpcs_ctx_load_tuning), measured on "
              + results.get("device", "this GPU") + " by hpcsharp_amd.autotune",
@@ -124,13 +168,19 @@ def table_text(results: dict) -> str:
             for name in alias[kind]:
                 lines.append(f"{name} {1 << lg} {best['algo']} {best['shape']} {best['rank']} {best.get('bits', 8)}"
                              f"   # {best['s'] * 1e3:.4f} ms")
+    lines.append("# segs max_n small_max rank   # median ms per batched call of the best variant")
+    for lg in sorted({int(k.split(":")[1]) for k in results["cells"] if k.startswith("segs:")}):
+        cells = [c for k, c in results["cells"].items() if k.startswith(f"segs:{lg}:")]
+        best = min(cells, key=lambda c: c["s"])
+        lines.append(f"segs {1 << lg} {best['small']} {best['rank']}   # {best['s'] * 1e3:.4f} ms")
     return "\n".join(lines) + "\n"
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--classes", default="12,13,14,16,18,20,22,24,26,28")
-    ap.add_argument("--kinds", default="u32,u64,pairs32,pairs64")
+    ap.add_argument("--kinds", default="u32,u64,pairs32,pairs64,segs")
+    ap.add_argument("--seg-classes", default="16,20,24,28")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--log", default=os.path.join(HERE, "tune_gfx950.json"))
     ap.add_argument("--out", default=N.TUNE_PATH)
@@ -147,6 +197,18 @@ def main(argv=None):
     ctx = N.new_context(0)
     try:
         for kind in args.kinds.split(","):
+            if kind == "segs":
+                for lg in (int(x) for x in args.seg_classes.split(",")):
+                    for small, rank in itertools.product(SEG_SMALL, RANKS):
+                        key = f"segs:{lg}:{small}:{rank}"
+                        if key in results["cells"]:
+                            continue
+                        s = time_segments(torch, ctx, lg, small, rank, args.reps)
+                        results["cells"][key] = {"small": small, "rank": rank, "s": s}
+                        print(f"[autotune] {key}: {s * 1e3:.4f} ms", flush=True)
+                        with open(args.log, "w") as f:
+                            json.dump(results, f, indent=1, sort_keys=True)
+                continue
             key_bytes, vals, _ = KINDS[kind]
             for lg in (int(x) for x in args.classes.split(",")):
                 for cand in candidates(key_bytes, vals, 1 << lg):

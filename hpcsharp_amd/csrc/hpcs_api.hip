@@ -152,8 +152,13 @@ struct hpcs_ctx {
     // tuning table (hpcs_ctx_load_tuning / HPCS_TUNE_FILE); options set
     // explicitly with hpcs_ctx_set_option or the environment take precedence
     int digit_bits = 8;           // 8, or 11 for onesweep sorts of keys without values
+    // batched segments: ranking of the large segments' scatter, largest
+    // segment sorted in LDS (HPCS_OPT_SEG_RANK, HPCS_OPT_SEG_SMALL)
+    int seg_rank = RANK_MIX2;
+    int seg_small = SEG_SMALL;
     std::vector<TuneEntry> tune;
     bool set_algo = false, set_shape = false, set_rank = false, set_bits = false;
+    bool set_seg_rank = false, set_seg_small = false;
     // kernel timing
     struct Pending {
         const char *name;
@@ -837,10 +842,12 @@ void run_sort_limited(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipSt
     run_sort<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
 }
 
-// Batched segments: scratch [SegPlan | seg_tile0 | small | large | tile
-// records | counts | keys B].  The plan's word `bad` counts rejected segments.
+// Batched segments: scratch [SegPlan | seg_tile0 | small | medium | large |
+// tile records | 16-bit counts | offsets | keys B].  The plan's word `bad`
+// counts rejected segments.
 struct SegLayout {
-    size_t off_plan, off_tile0, off_small, off_large, off_tinfo, off_counts, off_keysB, total;
+    size_t off_plan, off_tile0, off_small, off_medium, off_large, off_tinfo, off_counts, off_offsets, off_keysB,
+        total;
     uint64_t max_tiles;
 };
 
@@ -851,12 +858,53 @@ SegLayout seg_layout(int64_t n, int nseg)
     L.off_plan = 0;
     L.off_tile0 = align_up(sizeof(SegPlan));
     L.off_small = align_up(L.off_tile0 + size_t(nseg) * 4);
-    L.off_large = align_up(L.off_small + size_t(nseg) * 4);
+    L.off_medium = align_up(L.off_small + size_t(nseg) * 4);
+    L.off_large = align_up(L.off_medium + size_t(nseg) * 4);
     L.off_tinfo = align_up(L.off_large + size_t(nseg) * 4);
     L.off_counts = align_up(L.off_tinfo + L.max_tiles * sizeof(uint4));
-    L.off_keysB = align_up(L.off_counts + L.max_tiles * RADIX * 4);
+    L.off_offsets = align_up(L.off_counts + L.max_tiles * RADIX * sizeof(uint16_t));
+    L.off_keysB = align_up(L.off_offsets + L.max_tiles * RADIX * sizeof(uint32_t));
     L.total = align_up(L.off_keysB + size_t(n) * 4);
     return L;
+}
+
+// The batched sort's knobs for n keys in all: options set explicitly win,
+// then the tuning table's `segs` row for the smallest size class that holds
+// n (the largest class beyond it), then the defaults.
+struct SegParams {
+    int small_max, rank;
+};
+
+SegParams resolve_seg_params(const hpcs_ctx *c, int64_t n)
+{
+    SegParams p{c->seg_small, c->seg_rank};
+    const TuneEntry *best = nullptr, *largest = nullptr;
+    for (const TuneEntry &e : c->tune) {
+        if (e.key_bytes != 0) continue;
+        if (!largest || e.max_n > largest->max_n) largest = &e;
+        if (e.max_n >= n && (!best || e.max_n < best->max_n)) best = &e;
+    }
+    if (!best) best = largest;
+    if (best) {
+        if (!c->set_seg_small) p.small_max = best->seg_small;
+        if (!c->set_seg_rank) p.rank = best->rank;
+    }
+    return p;
+}
+
+void launch_seg_scatter(int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass, const SegPlan *sp,
+                        const uint4 *tinfo, const uint32_t *offsets)
+{
+    if (rank == RANK_BALLOT)
+        hipLaunchKernelGGL(k_seg_scatter<RANK_BALLOT>, grid, dim3(SEG_BLOCK), 0, st, a, b, pass, sp, tinfo, offsets);
+    else if (rank == RANK_MIX2)
+        hipLaunchKernelGGL(k_seg_scatter<RANK_MIX2>, grid, dim3(SEG_BLOCK), 0, st, a, b, pass, sp, tinfo, offsets);
+    else if (rank == RANK_MIX4)
+        hipLaunchKernelGGL(k_seg_scatter<RANK_MIX4>, grid, dim3(SEG_BLOCK), 0, st, a, b, pass, sp, tinfo, offsets);
+    else if (rank == RANK_LDSOR)
+        hipLaunchKernelGGL(k_seg_scatter<RANK_LDSOR>, grid, dim3(SEG_BLOCK), 0, st, a, b, pass, sp, tinfo, offsets);
+    else
+        fail(HPCS_E_UNSUPPORTED, "unknown ranking mode " + std::to_string(rank));
 }
 
 void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_begin, int nseg, hipStream_t st)
@@ -870,20 +918,30 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     SegPlan *sp = reinterpret_cast<SegPlan *>(base + L.off_plan);
     uint32_t *tile0 = reinterpret_cast<uint32_t *>(base + L.off_tile0);
     uint32_t *small = reinterpret_cast<uint32_t *>(base + L.off_small);
+    uint32_t *medium = reinterpret_cast<uint32_t *>(base + L.off_medium);
     uint32_t *large = reinterpret_cast<uint32_t *>(base + L.off_large);
+    const SegParams prm = resolve_seg_params(c, n);
     uint4 *tinfo = reinterpret_cast<uint4 *>(base + L.off_tinfo);
-    uint32_t *counts = reinterpret_cast<uint32_t *>(base + L.off_counts);
+    uint16_t *counts = reinterpret_cast<uint16_t *>(base + L.off_counts);
+    uint32_t *offsets = reinterpret_cast<uint32_t *>(base + L.off_offsets);
     uint32_t *keysB = reinterpret_cast<uint32_t *>(base + L.off_keysB);
     timed(c, st, "seg_plan", [&] {
-        hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, d_begin, nseg, n, sp, tile0, small, large,
-                           L.max_tiles);
+        hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, d_begin, nseg, n, sp, tile0, small, medium,
+                           large, L.max_tiles, prm.small_max);
     });
     const int list_grid = std::min(nseg, c->num_cus * 8);
     timed(c, st, "seg_tiles", [&] {
         hipLaunchKernelGGL(k_seg_tiles, dim3(list_grid), dim3(256), 0, st, d_begin, sp, tile0, large, tinfo);
     });
     timed(c, st, "seg_small", [&] {
-        hipLaunchKernelGGL(k_seg_small, dim3(list_grid), dim3(SEG_BLOCK), 0, st, d_keys, d_begin, sp, small);
+        hipLaunchKernelGGL((k_seg_small<SEG_BLOCK>), dim3(list_grid), dim3(SEG_BLOCK), 0, st, d_keys, d_begin, sp,
+                           small);
+    });
+    // the medium list is empty when small_max is SEG_SMALL (its workgroups
+    // return at once; the launch sequence stays the same)
+    timed(c, st, "seg_medium", [&] {
+        hipLaunchKernelGGL((k_seg_small<SEG_MEDIUM_BLOCK>), dim3(std::min(nseg, c->num_cus)), dim3(SEG_MEDIUM_BLOCK),
+                           0, st, d_keys, d_begin, sp, medium);
     });
     // one workgroup per tile of the large segments (the device knows how
     // many; the grid is the bound and spare workgroups return)
@@ -893,11 +951,11 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
             hipLaunchKernelGGL(k_seg_counts, tile_grid, dim3(SEG_BLOCK), 0, st, d_keys, keysB, p, sp, tinfo, counts);
         });
         timed(c, st, "seg_scan", [&] {
-            hipLaunchKernelGGL(k_seg_scan, dim3(list_grid), dim3(SEG_SCAN_GROUPS * RADIX), 0, st, d_begin, sp, tile0,
-                               large, counts);
+            hipLaunchKernelGGL(k_seg_scan, dim3(list_grid), dim3(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS), 0, st, d_begin, sp,
+                               tile0, large, counts, offsets);
         });
         timed(c, st, "seg_scatter", [&] {
-            hipLaunchKernelGGL(k_seg_scatter, tile_grid, dim3(SEG_BLOCK), 0, st, d_keys, keysB, p, sp, tinfo, counts);
+            launch_seg_scatter(prm.rank, tile_grid, st, d_keys, keysB, p, sp, tinfo, offsets);
         });
     }
 }
@@ -1585,6 +1643,16 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
         case HPCS_OPT_SCRATCH_LIMIT:
             if (value < 0) fail(HPCS_E_INVALID_ARG, "scratch limit must be non-negative");
             ctx->scratch_limit = size_t(value);
+            break;
+        case HPCS_OPT_SEG_RANK:
+            if (value < RANK_BALLOT || value > RANK_MIX4) fail(HPCS_E_INVALID_ARG, "rank mode must be 0, 1, 2 or 3");
+            ctx->seg_rank = int(value);
+            ctx->set_seg_rank = true;
+            break;
+        case HPCS_OPT_SEG_SMALL:
+            if (!seg_small_valid(value)) fail(HPCS_E_INVALID_ARG, "the LDS segment limit must be 4096 or 16384");
+            ctx->seg_small = int(value);
+            ctx->set_seg_small = true;
             break;
         case HPCS_OPT_FALLBACK_ROUNDS:
             if (value < 0 || value > (1 << 20)) fail(HPCS_E_INVALID_ARG, "fallback rounds out of range");

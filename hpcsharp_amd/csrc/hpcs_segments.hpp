@@ -6,17 +6,23 @@
 // Segment s is keys[begin[s], begin[s+1]).  Everything is planned on the
 // device, so the launch sequence does not depend on the segment lengths and
 // can be captured once in a HIP graph and replayed:
-//   k_seg_plan     classify: segments of at most SEG_SMALL keys go to a list
+//   k_seg_plan     classify: segments of at most SEG_SMALL keys go to the
+//                  small list, those of at most small_max (SEG_SMALL or
+//                  SEG_MEDIUM, a launch argument) to the medium list, both
 //                  sorted by k_seg_small; larger ones get a range of tiles of
 //                  TILE keys (a tile never crosses a segment)
 //   k_seg_tiles    one 16-byte record per tile (its keys, its segment)
-//   k_seg_small    one workgroup per small segment: four LSD passes entirely
-//                  in LDS (one HBM read and one write per key)
+//   k_seg_small    one workgroup per small (256 threads) or medium (1024
+//                  threads) segment: the LSD passes entirely in LDS (one HBM
+//                  read and one write per key)
 //   per pass p of the large segments:
-//     k_seg_counts   per tile, the digit counts
+//     k_seg_counts   per tile, the digit counts (16-bit: a tile holds at most
+//                    SEG_TILE keys)
 //     k_seg_scan     per segment, digit starts (segment base + exclusive scan
 //                    of the segment's digit totals) and per-tile offsets
-//     k_seg_scatter  rank, stage in LDS, write each digit run contiguously
+//                    (32-bit, positions within the segment)
+//     k_seg_scatter  rank (RANK as the plain sort's scatter), stage in LDS,
+//                    write each digit run contiguously
 // Four passes leave every large segment back in the caller's buffer.
 #pragma once
 #include "hpcs_kernels.hpp"
@@ -26,15 +32,18 @@ namespace hpcs {
 constexpr int SEG_BLOCK = 256;
 constexpr int SEG_ITEMS = 16;
 constexpr int SEG_TILE = SEG_BLOCK * SEG_ITEMS; // 4096 keys
-constexpr int SEG_SMALL = SEG_TILE;             // largest segment sorted in LDS
+constexpr int SEG_SMALL = SEG_TILE;             // largest segment of the small list (256 threads)
+constexpr int SEG_MEDIUM_BLOCK = 1024;
+constexpr int SEG_MEDIUM = SEG_MEDIUM_BLOCK * SEG_ITEMS; // largest segment one workgroup sorts in LDS
 
 struct SegPlan {
-    uint32_t num_tiles; // tiles of the large segments
-    uint32_t num_small; // small segments
-    uint32_t num_large; // large segments
-    uint32_t bad;       // segments skipped: end < begin, outside [0, n), or starting before
-                        // the end of an earlier segment (overlap)
-    uint32_t pad[60];
+    uint32_t num_tiles;  // tiles of the large segments
+    uint32_t num_small;  // small segments
+    uint32_t num_large;  // large segments
+    uint32_t bad;        // segments skipped: end < begin, outside [0, n), or starting before
+                         // the end of an earlier segment (overlap)
+    uint32_t num_medium; // medium segments (SEG_SMALL < keys <= small_max)
+    uint32_t pad[59];
 };
 
 // Ballot ranking of one tile held lane-striped in registers (wave w, item i,
@@ -183,15 +192,17 @@ __device__ inline void tile_rank_stage_ldsor(const K (&keys)[ITEMS], int wave_va
 __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ begin, int nseg, int64_t n,
                                                    SegPlan *__restrict__ sp, uint32_t *__restrict__ seg_tile0,
                                                    uint32_t *__restrict__ small_list,
-                                                   uint32_t *__restrict__ large_list, uint64_t max_tiles)
+                                                   uint32_t *__restrict__ medium_list,
+                                                   uint32_t *__restrict__ large_list, uint64_t max_tiles,
+                                                   int small_max)
 {
-    __shared__ uint32_t s_wsum[16];
+    __shared__ uint64_t s_wsum[16];
     __shared__ int64_t s_wmax[16];
-    __shared__ uint32_t s_carry[3];
+    __shared__ uint32_t s_carry[4];
     __shared__ int64_t s_end_carry; // largest valid end of the earlier chunks
     __shared__ uint32_t s_bad;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
-    if (tid < 3) s_carry[tid] = 0u;
+    if (tid < 4) s_carry[tid] = 0u;
     if (tid == 0) {
         s_bad = 0u;
         s_end_carry = 0;
@@ -199,7 +210,7 @@ __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ b
     __syncthreads();
     for (int base = 0; base < nseg; base += 1024) {
         const int s = base + tid;
-        uint32_t tiles = 0, small = 0;
+        uint32_t tiles = 0, small = 0, medium = 0;
         const int64_t b = s < nseg ? begin[s] : 0, e = s < nseg ? begin[s + 1] : 0;
         const bool bounds_ok = s < nseg && b >= 0 && e >= b && e <= n;
         // exclusive running maximum of the valid ends before segment s
@@ -222,33 +233,44 @@ __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ b
                 atomicAdd(&s_bad, 1u);
             } else {
                 const int64_t len = e - b;
-                if (len > SEG_SMALL) tiles = uint32_t((len + SEG_TILE - 1) / SEG_TILE);
+                if (len > small_max) tiles = uint32_t((len + SEG_TILE - 1) / SEG_TILE);
+                else if (len > SEG_SMALL) medium = 1;
                 else if (len > 1) small = 1;
             }
         }
         const uint32_t large = tiles ? 1u : 0u;
-        // block-wide exclusive scans of (tiles, small, large), one at a time
-        uint32_t vals[3] = {tiles, small, large}, excl[3];
-        for (int k = 0; k < 3; ++k) {
-            uint32_t x = vals[k];
+        // one block-wide exclusive scan of (tiles, small, large, medium)
+        // packed in 64 bits: within a chunk of 1024 segments the tiles sum to
+        // at most n / SEG_TILE + 1024 < 2^21 (the sorted segments do not
+        // overlap and n < 2^32) and each list count to at most 1024 < 2^11
+        constexpr int SH[4] = {0, 22, 33, 44};
+        constexpr uint64_t MASK[4] = {(1ull << 22) - 1, (1ull << 11) - 1, (1ull << 11) - 1, (1ull << 11) - 1};
+        const uint64_t v = uint64_t(tiles) | (uint64_t(small) << SH[1]) | (uint64_t(large) << SH[2]) |
+                           (uint64_t(medium) << SH[3]);
+        uint64_t x = v;
 #pragma unroll
-            for (int off = 1; off < WAVE; off <<= 1) {
-                const uint32_t y = __shfl_up(x, off);
-                if (lane >= off) x += y;
-            }
-            if (lane == WAVE - 1) s_wsum[wave] = x;
-            __syncthreads();
-            uint32_t add = s_carry[k];
-            for (int w = 0; w < wave; ++w) add += s_wsum[w];
-            excl[k] = add + x - vals[k];
-            __syncthreads();
-            if (tid == 1023) s_carry[k] = add + x;
-            __syncthreads();
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint64_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
         }
+        if (lane == WAVE - 1) s_wsum[wave] = x;
+        __syncthreads();
+        uint64_t add = 0;
+        for (int w = 0; w < wave; ++w) add += s_wsum[w];
+        const uint64_t ex = add + x - v, inc = add + x;
+        uint32_t excl[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) excl[k] = s_carry[k] + uint32_t((ex >> SH[k]) & MASK[k]);
+        __syncthreads();
+        if (tid == 1023)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s_carry[k] += uint32_t((inc >> SH[k]) & MASK[k]);
+        __syncthreads();
         if (s < nseg) {
             seg_tile0[s] = excl[0];
             if (small) small_list[excl[1]] = uint32_t(s);
             if (large) large_list[excl[2]] = uint32_t(s);
+            if (medium) medium_list[excl[3]] = uint32_t(s);
         }
     }
     __syncthreads();
@@ -257,6 +279,7 @@ __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ b
         sp->num_tiles = fits ? s_carry[0] : 0u;
         sp->num_small = s_carry[1];
         sp->num_large = fits ? s_carry[2] : 0u;
+        sp->num_medium = s_carry[3];
         sp->bad = s_bad + (fits ? 0u : s_carry[2]);
     }
 }
@@ -283,27 +306,32 @@ __global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restrict__ b
     }
 }
 
-// One workgroup per small segment (grid-stride): four LSD passes in LDS.
-// Digit positions where every key of the segment agrees are skipped.
-__global__ __launch_bounds__(SEG_BLOCK) void k_seg_small(uint32_t *__restrict__ keys,
-                                                         const int64_t *__restrict__ begin,
-                                                         const SegPlan *__restrict__ sp,
-                                                         const uint32_t *__restrict__ small_list)
+// One workgroup per segment of a list (grid-stride): the LSD passes in LDS.
+// BLOCK = SEG_BLOCK: the small list (at most SEG_SMALL keys); BLOCK =
+// SEG_MEDIUM_BLOCK: the medium list (at most SEG_MEDIUM keys, one workgroup
+// per CU by its LDS).  Digit positions where every key of the segment agrees
+// are skipped.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_seg_small(uint32_t *__restrict__ keys, const int64_t *__restrict__ begin,
+                                                     const SegPlan *__restrict__ sp,
+                                                     const uint32_t *__restrict__ list)
 {
-    constexpr int WAVES = SEG_BLOCK / WAVE;
+    static_assert(BLOCK == SEG_BLOCK || BLOCK == SEG_MEDIUM_BLOCK, "the small or the medium list");
+    constexpr int WAVES = BLOCK / WAVE;
     constexpr int WAVE_TILE = WAVE * SEG_ITEMS;
+    constexpr int TILE = BLOCK * SEG_ITEMS;
     __shared__ uint32_t s_cnt[WAVES][RADIX];
     __shared__ uint64_t s_match[WAVES][RADIX];
     __shared__ uint32_t s_scan[RADIX / WAVE];
     __shared__ uint32_t s_start[RADIX];
-    __shared__ uint32_t s_keys[SEG_TILE];
+    __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_or[WAVES], s_nor[WAVES];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
-    const uint32_t nsmall = sp->num_small;
-    for (uint32_t k = blockIdx.x; k < nsmall; k += gridDim.x) {
-        const uint32_t s = small_list[k];
+    const uint32_t nlist = BLOCK == SEG_BLOCK ? sp->num_small : sp->num_medium;
+    for (uint32_t k = blockIdx.x; k < nlist; k += gridDim.x) {
+        const uint32_t s = list[k];
         const int64_t b0 = begin[s];
-        const int len = int(begin[s + 1] - b0);
+        const int len = int(begin[s + 1] - b0); // at most TILE (k_seg_plan)
         uint32_t *seg = keys + b0;
         const int wave_valid = len > wave * WAVE_TILE ? min(WAVE_TILE, len - wave * WAVE_TILE) : 0;
         uint32_t v[SEG_ITEMS];
@@ -344,12 +372,12 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_small(uint32_t *__restrict__ 
                 }
                 __syncthreads();
             }
-            tile_rank_stage_ldsor<uint32_t, SEG_BLOCK, SEG_ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_match, s_scan,
-                                                                  s_start, s_keys);
+            tile_rank_stage_ldsor<uint32_t, BLOCK, SEG_ITEMS>(v, wave_valid, 8 * pass, s_cnt, s_match, s_scan,
+                                                              s_start, s_keys);
             staged = true;
         }
         if (staged) {
-            for (int j = tid; j < len; j += SEG_BLOCK) seg[j] = s_keys[j];
+            for (int j = tid; j < len; j += BLOCK) seg[j] = s_keys[j];
         }
         __syncthreads();
     }
@@ -361,8 +389,9 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t *__rest
                                                           const uint32_t *__restrict__ keysB, int pass,
                                                           const SegPlan *__restrict__ sp,
                                                           const uint4 *__restrict__ tinfo,
-                                                          uint32_t *__restrict__ counts)
+                                                          uint16_t *__restrict__ counts)
 {
+    static_assert(SEG_TILE < 65536, "a tile's digit counts fit 16 bits");
     __shared__ uint32_t sh[RADIX];
     const uint32_t t = blockIdx.x;
     if (t >= sp->num_tiles) return;
@@ -392,24 +421,33 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t *__rest
         }
     }
     __syncthreads();
-    counts[size_t(t) * RADIX + tid] = sh[tid];
+    counts[size_t(t) * RADIX + tid] = uint16_t(sh[tid]);
 }
 
 // Grid-stride over the large segments, one workgroup each: digit starts of
-// the segment and the per-tile offsets, written in place of the counts.  The
-// SEG_SCAN_GROUPS x RADIX threads split the segment's tiles into as many
-// consecutive parts; thread (g, d) owns digit d of part g.
-constexpr int SEG_SCAN_GROUPS = 4;
-__global__ __launch_bounds__(SEG_SCAN_GROUPS * RADIX) void k_seg_scan(const int64_t *__restrict__ begin,
-                                                                     const SegPlan *__restrict__ sp,
-                                                                     const uint32_t *__restrict__ seg_tile0,
-                                                                     const uint32_t *__restrict__ large_list,
-                                                                     uint32_t *__restrict__ counts)
+// the segment and the per-tile offsets (from the 16-bit counts).  The
+// SEG_SCAN_GROUPS x SEG_SCAN_PAIRS threads split the segment's tiles into
+// SEG_SCAN_GROUPS consecutive parts; thread (g, q) owns digits 2q and 2q + 1
+// of part g (one 32-bit load of two counts per tile, one 8-byte store of two
+// offsets), so the serial walk over a part is an eighth of the segment.
+constexpr int SEG_SCAN_GROUPS = 8;
+constexpr int SEG_SCAN_PAIRS = RADIX / 2;
+__global__ __launch_bounds__(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS) void k_seg_scan(const int64_t *__restrict__ begin,
+                                                                              const SegPlan *__restrict__ sp,
+                                                                              const uint32_t *__restrict__ seg_tile0,
+                                                                              const uint32_t *__restrict__ large_list,
+                                                                              const uint16_t *__restrict__ counts,
+                                                                              uint32_t *__restrict__ offsets)
 {
+    static_assert(SEG_SCAN_PAIRS % WAVE == 0 && SEG_SCAN_PAIRS / WAVE == 2, "two waves of digit pairs");
     __shared__ uint32_t s_part[SEG_SCAN_GROUPS][RADIX];
-    __shared__ uint32_t s_wsum[RADIX / WAVE];
-    const int d = threadIdx.x & (RADIX - 1), g = threadIdx.x / RADIX;
-    const int lane = d & (WAVE - 1), wave = d >> 6;
+    __shared__ uint32_t s_wsum[SEG_SCAN_PAIRS / WAVE];
+    const int q = threadIdx.x % SEG_SCAN_PAIRS, g = threadIdx.x / SEG_SCAN_PAIRS;
+    const int lane = q & (WAVE - 1), wave = q / WAVE;
+    // two 16-bit counts per word, two 32-bit offsets per 8 bytes: row t holds
+    // SEG_SCAN_PAIRS of each
+    const uint32_t *__restrict__ c2 = reinterpret_cast<const uint32_t *>(counts);
+    uint2 *__restrict__ o2 = reinterpret_cast<uint2 *>(offsets);
     const uint32_t nlarge = sp->num_large;
     for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
         const uint32_t s = large_list[k];
@@ -417,20 +455,31 @@ __global__ __launch_bounds__(SEG_SCAN_GROUPS * RADIX) void k_seg_scan(const int6
         const uint32_t nt = uint32_t((begin[s + 1] - begin[s] + SEG_TILE - 1) / SEG_TILE);
         const uint32_t per = (nt + SEG_SCAN_GROUPS - 1) / SEG_SCAN_GROUPS;
         const uint32_t ta = min(nt, uint32_t(g) * per), tz = min(nt, ta + per); // this part's tiles
-        uint32_t part = 0;
+        uint32_t p0 = 0, p1 = 0;
 #pragma unroll 8
-        for (uint32_t t = ta; t < tz; ++t) part += counts[size_t(t0 + t) * RADIX + d];
-        s_part[g][d] = part;
+        for (uint32_t t = ta; t < tz; ++t) {
+            const uint32_t w = c2[size_t(t0 + t) * SEG_SCAN_PAIRS + q];
+            p0 += w & 0xFFFFu;
+            p1 += w >> 16;
+        }
+        s_part[g][2 * q] = p0;
+        s_part[g][2 * q + 1] = p1;
         __syncthreads();
-        // digit d's total over the segment, exclusive scan over digits (group 0)
-        uint32_t tot = 0, before = 0;
+        // the two digits' totals over the segment and over the earlier parts
+        uint32_t tot0 = 0, tot1 = 0, before0 = 0, before1 = 0;
 #pragma unroll
         for (int h = 0; h < SEG_SCAN_GROUPS; ++h) {
-            const uint32_t v = s_part[h][d];
-            tot += v;
-            if (h < g) before += v;
+            const uint32_t v0 = s_part[h][2 * q], v1 = s_part[h][2 * q + 1];
+            tot0 += v0;
+            tot1 += v1;
+            if (h < g) {
+                before0 += v0;
+                before1 += v1;
+            }
         }
-        uint32_t x = tot;
+        // exclusive scan over the digits: inclusive scan of the pair sums
+        const uint32_t pair = tot0 + tot1;
+        uint32_t x = pair;
 #pragma unroll
         for (int off = 1; off < WAVE; off <<= 1) {
             const uint32_t y = __shfl_up(x, off);
@@ -438,34 +487,44 @@ __global__ __launch_bounds__(SEG_SCAN_GROUPS * RADIX) void k_seg_scan(const int6
         }
         if (g == 0 && lane == WAVE - 1) s_wsum[wave] = x;
         __syncthreads();
-        uint32_t run = x - tot + before;
-        for (int w = 0; w < wave; ++w) run += s_wsum[w];
+        const uint32_t below = x - pair + (wave ? s_wsum[0] : 0u); // digits before 2q
+        uint32_t run0 = below + before0, run1 = below + tot0 + before1;
         // offsets are positions within the segment (added to its first key at use)
         for (uint32_t t = ta; t < tz; ++t) {
-            const size_t at = size_t(t0 + t) * RADIX + d;
-            const uint32_t c = counts[at];
-            counts[at] = run;
-            run += c;
+            const size_t row = size_t(t0 + t) * SEG_SCAN_PAIRS + q;
+            const uint32_t w = c2[row];
+            o2[row] = make_uint2(run0, run1);
+            run0 += w & 0xFFFFu;
+            run1 += w >> 16;
         }
         __syncthreads();
     }
 }
 
 // Scatter of pass `pass` for the large segments' tiles, one workgroup per
-// tile (spare workgroups of the grid return).  Workgroup b < num_tiles takes
-// tile xcd_contiguous(b, num_tiles): XCD b % 8 walks a contiguous share of the
-// tiles, so neighbouring tiles (adjacent digit runs) meet in one L2.
+// tile (spare workgroups of the grid return): the plain sort's prescanned
+// scatter (k_onesweep with precomputed offsets) on a tile record.  Workgroup
+// b < num_tiles takes tile xcd_contiguous(b, num_tiles): XCD b % 8 walks a
+// contiguous share of the tiles, so neighbouring tiles (adjacent digit runs)
+// meet in one L2.  The keys are loaded before the LDS set-up; a full wave
+// (every wave of a full tile) loads and a full tile ranks and writes without
+// per-key bounds tests; RANK ranks as in k_onesweep (RANK_LDSOR counts with
+// LDS atomics first and stages while it ranks, RANK_MIX2 / RANK_MIX4 match
+// every second / fourth item by ballots, RANK_BALLOT ranks by ballots with
+// wave-local counters and stages after the scan).
+template <int RANK>
 __global__ __launch_bounds__(SEG_BLOCK) void k_seg_scatter(uint32_t *__restrict__ keysA, uint32_t *__restrict__ keysB,
                                                            int pass, const SegPlan *__restrict__ sp,
                                                            const uint4 *__restrict__ tinfo,
                                                            const uint32_t *__restrict__ offsets)
 {
-    constexpr int WAVES = SEG_BLOCK / WAVE;
-    constexpr int WAVE_TILE = WAVE * SEG_ITEMS;
+    constexpr int BLOCK = SEG_BLOCK, ITEMS = SEG_ITEMS;
+    constexpr int WAVES = BLOCK / WAVE;
+    constexpr int WAVE_TILE = WAVE * ITEMS;
+    constexpr int SCAN_WAVES = RADIX / WAVE;
     __shared__ uint32_t s_cnt[WAVES][RADIX];
-    __shared__ uint64_t s_match[WAVES][RADIX];
-    __shared__ uint32_t s_scan[RADIX / WAVE];
-    __shared__ uint32_t s_start[RADIX];
+    __shared__ uint64_t s_match[RANK != RANK_BALLOT ? WAVES : 1][RANK != RANK_BALLOT ? RADIX : 1];
+    __shared__ uint32_t s_scan[SCAN_WAVES];
     __shared__ uint32_t s_base[RADIX];
     __shared__ uint32_t s_keys[SEG_TILE];
     const uint32_t ntiles = sp->num_tiles;
@@ -477,26 +536,152 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_scatter(uint32_t *__restrict_
     const uint32_t t = xcd_contiguous(blockIdx.x, ntiles);
     const uint4 ti = tinfo[t];
     const uint32_t tb = ti.x, sb = ti.y;
-    const int cnt = int(ti.z);
-    const int wave_valid = cnt > wave * WAVE_TILE ? min(WAVE_TILE, cnt - wave * WAVE_TILE) : 0;
-    uint32_t v[SEG_ITEMS];
+    const int tile_valid = int(ti.z);
+    const int wave_valid = tile_valid > wave * WAVE_TILE ? min(WAVE_TILE, tile_valid - wave * WAVE_TILE) : 0;
+
+    uint32_t keys[ITEMS];
+    if (wave_valid == WAVE_TILE) {
+        const uint32_t *__restrict__ kp = kin + tb + uint32_t(wave * WAVE_TILE + lane);
 #pragma unroll
-    for (int j = 0; j < SEG_ITEMS; ++j) {
-        const int o = j * WAVE + lane;
-        v[j] = o < wave_valid ? kin[size_t(tb) + wave * WAVE_TILE + o] : ~0u;
+        for (int i = 0; i < ITEMS; ++i) keys[i] = kp[i * WAVE];
+    } else {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int off = i * WAVE + lane;
+            keys[i] = off < wave_valid ? kin[tb + uint32_t(wave * WAVE_TILE + off)] : ~0u;
+        }
     }
-    const uint32_t off = offsets[size_t(t) * RADIX + (tid & (RADIX - 1))];
-    tile_rank_stage_ldsor<uint32_t, SEG_BLOCK, SEG_ITEMS>(v, wave_valid, shift, s_cnt, s_match, s_scan, s_start,
-                                                          s_keys);
-    if (tid < RADIX) s_base[tid] = off - s_start[tid];
+    // the tile's digit offsets within its segment
+    const uint32_t pre_off = tid < RADIX ? offsets[size_t(t) * RADIX + tid] : 0u;
+
+    for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
+    if constexpr (RANK != RANK_BALLOT)
+        for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_match[0][0])[i] = 0ull;
     __syncthreads();
-    uint32_t *out = kout + sb;
+
+    // ---- per-wave digit counts (ballot ranking: the ranks themselves) ----
+    uint32_t *cnt = s_cnt[wave];
+    const bool full = tile_valid == SEG_TILE;
+    uint32_t rank2[RANK == RANK_BALLOT ? (ITEMS + 1) / 2 : 1];
+    if constexpr (RANK == RANK_BALLOT) {
+        auto rank_loop = [&](auto full_c) {
+            constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
-    for (int j = 0; j < SEG_ITEMS; ++j) {
-        const int jj = j * SEG_BLOCK + tid;
-        if (jj < cnt) {
-            const uint32_t k = s_keys[jj];
-            out[s_base[(k >> shift) & 0xFFu] + uint32_t(jj)] = k;
+            for (int i = 0; i < ITEMS; ++i) {
+                const uint32_t d = (keys[i] >> shift) & (RADIX - 1);
+                const bool valid = FULL || i * WAVE + lane < wave_valid;
+                const uint64_t peers = FULL ? match_digit8(d) : match_digit8(d) & __ballot(valid);
+                const uint32_t below = count_below(peers);
+                const uint32_t old = cnt[d];
+                if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
+                const uint32_t r = old + below;
+                rank2[i / 2] = (i & 1) ? (rank2[i / 2] | (r << 16)) : r;
+                __builtin_amdgcn_wave_barrier();
+            }
+        };
+        if (full) rank_loop(std::true_type{});
+        else rank_loop(std::false_type{});
+    } else {
+        if (full) {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) atomicAdd(&cnt[(keys[i] >> shift) & (RADIX - 1)], 1u);
+        } else {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i)
+                if (i * WAVE + lane < wave_valid) atomicAdd(&cnt[(keys[i] >> shift) & (RADIX - 1)], 1u);
+        }
+    }
+    __syncthreads();
+
+    // ---- per-digit wave bases and the tile's digit starts ----
+    uint32_t tile_cnt = 0, tile_start = 0;
+    if (tid < RADIX) {
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = s_cnt[w][tid];
+            s_cnt[w][tid] = tile_cnt;
+            tile_cnt += c;
+        }
+        uint32_t x = tile_cnt;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == WAVE - 1) s_scan[wave] = x;
+        tile_start = x - tile_cnt;
+    }
+    __syncthreads();
+    if (tid < RADIX) {
+        for (int w = 0; w < wave; ++w) tile_start += s_scan[w];
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) s_cnt[w][tid] += tile_start;
+        s_base[tid] = pre_off - tile_start;
+    }
+    __syncthreads();
+
+    // ---- rank and stage the tile in LDS in sorted order ----
+    auto stage_loop = [&](auto full_c) {
+        constexpr bool FULL = decltype(full_c)::value;
+        if constexpr (RANK == RANK_BALLOT) {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i)
+                if (FULL || i * WAVE + lane < wave_valid)
+                    s_keys[cnt[(keys[i] >> shift) & (RADIX - 1)] + ((rank2[i / 2] >> (16 * (i & 1))) & 0xFFFFu)] =
+                        keys[i];
+        } else {
+            uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
+            const uint32_t lanebit = 1u << (lane & 31);
+            const int half = lane >> 5;
+            constexpr int MIX = rank_mix_period<RANK>();
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) {
+                const uint32_t d = (keys[i] >> shift) & (RADIX - 1);
+                const bool valid = FULL || i * WAVE + lane < wave_valid;
+                if (MIX && i % MIX == MIX - 1) {
+                    const uint64_t peers = FULL ? match_digit8(d) : match_digit8(d) & __ballot(valid);
+                    const uint32_t below = count_below(peers);
+                    const uint32_t old = cnt[d];
+                    if (valid && below == 0) cnt[d] = old + uint32_t(__popcll(peers));
+                    if (valid) s_keys[old + below] = keys[i];
+                    __builtin_amdgcn_wave_barrier();
+                } else {
+                    if (valid) atomicOr(&match32[2 * d + half], lanebit);
+                    __builtin_amdgcn_wave_barrier();
+                    const uint64_t peers = s_match[wave][d];
+                    const uint32_t below = count_below(peers);
+                    const uint32_t old = cnt[d];
+                    if (valid && below == 0) {
+                        cnt[d] = old + uint32_t(__popcll(peers));
+                        s_match[wave][d] = 0ull;
+                    }
+                    if (valid) s_keys[old + below] = keys[i];
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        }
+    };
+    if (full) stage_loop(std::true_type{});
+    else stage_loop(std::false_type{});
+    __syncthreads();
+
+    // ---- write the tile out, each digit's run contiguous ----
+    uint32_t *__restrict__ out = kout + sb;
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int j = i * BLOCK + tid;
+            const uint32_t k = s_keys[j];
+            out[s_base[(k >> shift) & (RADIX - 1)] + uint32_t(j)] = k;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int j = i * BLOCK + tid;
+            if (j < tile_valid) {
+                const uint32_t k = s_keys[j];
+                out[s_base[(k >> shift) & (RADIX - 1)] + uint32_t(j)] = k;
+            }
         }
     }
 }

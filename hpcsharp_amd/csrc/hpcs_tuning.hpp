@@ -1,5 +1,7 @@
 // hpcs_tuning.hpp -- tuning tables of libhpcsort.so: one row per line,
 //   <dtype> <max_n> <algo> <shape> <rank> [<digit_bits>]
+// or, for the batched segmented sort,
+//   segs <max_n> <small_max> <rank>
 // Plain C++ (no device code), so the parser is tested on the host with
 // sanitizers.
 #pragma once
@@ -14,14 +16,21 @@ namespace hpcs {
 
 // One row of a tuning table: for sorts of up to max_n keys of key_bytes
 // (with or without u32 values), the pass algorithm, tile shape, ranking and
-// digit width.
+// digit width.  A `segs` row (key_bytes 0): for batched segmented sorts of
+// up to max_n keys in all, the largest segment sorted in LDS (seg_small) and
+// the ranking of the large segments' scatter (rank).
 struct TuneEntry {
     int key_bytes;
     int vals;
     int64_t max_n;
     int algo, shape, rank;
     int bits;
+    int seg_small;
 };
+
+// The sizes of segment that one workgroup may sort in LDS (hpcs_segments.hpp:
+// SEG_SMALL, SEG_MEDIUM).
+inline bool seg_small_valid(long long v) { return v == 4096 || v == 16384; }
 
 // Parse a tuning table: one row per line, `#` starts a comment,
 //   <dtype> <max_n> <algo> <shape> <rank> [<digit_bits>]
@@ -53,6 +62,26 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
         long long max_n = 0;
         int algo = 0, shape = 0, rank = 0, bits = 8, consumed = 0, consumed6 = 0;
         if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        if (std::sscanf(line.c_str(), " %15s", dt) == 1 && std::string(dt) == "segs") {
+            long long small_max = 0;
+            const int got = std::sscanf(line.c_str(), " %15s %lld %lld %d %n", dt, &max_n, &small_max, &rank, &consumed);
+            if (got < 4 || line.find_first_not_of(" \t\r", size_t(consumed)) != std::string::npos) {
+                err = "line " + std::to_string(line_no) + ": expected segs <max_n> <small_max> <rank>";
+                return false;
+            }
+            if (max_n < 1 || !seg_small_valid(small_max) || rank < 0 || rank > 3) {
+                err = "line " + std::to_string(line_no) + ": value out of range";
+                return false;
+            }
+            TuneEntry e{};
+            e.key_bytes = 0;
+            e.max_n = max_n;
+            e.rank = rank;
+            e.bits = 8;
+            e.seg_small = int(small_max);
+            out.push_back(e);
+            continue;
+        }
         const int got = std::sscanf(line.c_str(), " %15s %lld %d %d %d %n%d %n", dt, &max_n, &algo, &shape, &rank,
                                     &consumed, &bits, &consumed6);
         if (got == 6) consumed = consumed6;

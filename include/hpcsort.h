@@ -96,6 +96,12 @@ extern "C" {
                                       launch instead of up to ~30); 0 (default) plain launches.  Setting any
                                       option clears the cache; a call on a stream the caller is capturing
                                       launches plainly */
+#define HPCS_OPT_SEG_RANK 13       /* hpcs_[dev_]sort_segments_u32: ranking of the large segments' scatter
+                                      (as HPCS_OPT_RANK: 0 ballots, 1 LDS-OR, 2 / 3 LDS-OR with every second /
+                                      fourth item by ballots); default from the tuning table, else 2 */
+#define HPCS_OPT_SEG_SMALL 14      /* hpcs_[dev_]sort_segments_u32: segments of at most this many keys are
+                                      sorted by one workgroup in LDS (one read and one write per key): 4096 or
+                                      16384; default from the tuning table, else 4096 */
 
 #define HPCS_DTYPE_U32 0
 #define HPCS_DTYPE_U64 1

@@ -98,6 +98,17 @@ static int tuning_test()
         CHECK(err.rfind("line 1", 0) == 0);
     }
     CHECK(!hpcs::parse_tuning("u32 1 0 0 1\nbogus", rows, err) && err.rfind("line 2", 0) == 0);
+    // batched-segment rows: segs <max_n> <small_max> <rank>
+    CHECK(hpcs::parse_tuning("segs 1000 4096 1 # lds\nu32 9 1 0 3\n segs 268435456 16384 3\r", rows, err));
+    CHECK(rows.size() == 3 && rows[0].key_bytes == 0 && rows[0].seg_small == 4096 && rows[0].rank == 1);
+    CHECK(rows[1].key_bytes == 4 && rows[2].key_bytes == 0 && rows[2].max_n == 268435456 &&
+          rows[2].seg_small == 16384 && rows[2].rank == 3);
+    const char *bad_segs[] = {"segs 1 8192 1", "segs 0 4096 1", "segs 1 4096 4", "segs 1 4096 -1", "segs 1 4096",
+                              "segs 1 4096 1 0", "segs 1 16384 x", "segs -3 4096 0"};
+    for (const char *b : bad_segs) {
+        CHECK(!hpcs::parse_tuning(b, rows, err));
+        CHECK(err.rfind("line 1", 0) == 0);
+    }
     return 0;
 }
 
