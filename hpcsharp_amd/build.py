@@ -22,13 +22,18 @@ DEPS = SOURCES + HEADERS + [os.path.join(REPO, "include", "hpcsort.h")]
 ARCH = os.environ.get("HPCS_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "-fPIC",
     f"--offload-arch={ARCH}",
     # packed FP32 instructions stay off in every HIP build of this library
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-fvisibility=hidden",
-    "-Wall", "-Wno-unused-result",
+    # a part leaves the helpers of the other parts' entry points unused
+    "-Wall", "-Wno-unused-result", "-Wno-unused-function",
 ]
+# hpcs_api.hip is compiled once per part (-DHPCS_PART=k, in parallel); each
+# part holds one group of entry points and the kernels they instantiate
+API_PARTS = 7
+OBJ_DIR = os.path.join(HERE, "_build")
 
 
 def hipcc() -> str:
@@ -39,14 +44,33 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    inc = ["-I", os.path.join(REPO, "include")]
+    api, serve = SOURCES
+    jobs = [([hipcc(), *FLAGS, *inc, f"-DHPCS_PART={k}", "-c", api, "-o", os.path.join(OBJ_DIR, f"hpcs_api_{k}.o")])
+            for k in range(API_PARTS)]
+    jobs.append([hipcc(), *FLAGS, *inc, "-c", serve, "-o", os.path.join(OBJ_DIR, "hpcs_serve.o")])
+    procs = []
+    for cmd in jobs:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+    failed = False
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed compiling libhpcsort.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *FLAGS, "-I", os.path.join(REPO, "include"), *SOURCES, "-o", tmp]
+    link = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *[j[-1] for j in jobs], "-o", tmp]
     if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
+        print(" ".join(link), flush=True)
+    r = subprocess.run(link, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libhpcsort.so")
+        raise RuntimeError("hipcc failed linking libhpcsort.so")
     os.replace(tmp, LIB)
     return LIB
 

@@ -35,9 +35,23 @@ using namespace hpcs;
 
 #define HPCS_API extern "C" __attribute__((visibility("default")))
 
+// Split compilation: hpcsharp_amd/build.py compiles this file once per part
+// (-DHPCS_PART=k, in parallel), each object holding one group of entry points
+// and the kernels they instantiate; without HPCS_PART one object holds all.
+#ifdef HPCS_PART
+#define HPCS_IN_PART(k) (HPCS_PART == (k))
+#else
+#define HPCS_IN_PART(k) 1
+#endif
+
+namespace hpcs_state {
+// the calling thread's last error message: one object for all parts
+inline thread_local std::string g_last_error;
+} // namespace hpcs_state
+
 namespace {
 
-thread_local std::string g_last_error;
+using hpcs_state::g_last_error;
 
 struct Err {
     int32_t code;
@@ -123,11 +137,13 @@ size_t align_up(size_t x, size_t a = ALIGN) { return (x + a - 1) / a * a; }
 // entry point passes that address.  Every call on one GPU shares the array,
 // so overlapping calls on one GPU race; the sort clears what must start at
 // zero itself.  Largest sort: 2^22 keys (the layout of a 2^22-key sort with
-// 2048-key tiles, the smallest tile, is 24.3 MiB).
+// 2048-key tiles, the smallest tile, is 25.3 MiB).
 // ---------------------------------------------------------------------------
 constexpr int64_t NOALLOC_MAX_KEYS = int64_t(1) << 22;
 constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(26) << 20;
+#if HPCS_IN_PART(0) // the array and its address lookup live in the part that creates contexts
 static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+#endif
 
 struct hpcs_ctx {
     int device = 0;
@@ -222,9 +238,10 @@ void *ensure(void *&buf, size_t &cap, size_t need)
 
 // Scratch layout of one sort:
 //   [hist P x 256 u32 | DevPlan | look-back region 0]  zero-filled first
-//   [look-back region 1] [keys B] [values B] [next-pass digit bytes, n]
+//   [look-back region 1] [16-bit tile counts] [keys B] [values B]
+//   [next-pass digit bytes, n]
 struct Layout {
-    size_t off_hist, off_plan, off_dtot, off_lb0, zero_bytes, off_lb1, off_keysB, off_valsB, off_dig, total;
+    size_t off_hist, off_plan, off_dtot, off_lb0, zero_bytes, off_lb1, off_cnt, off_keysB, off_valsB, off_dig, total;
     int64_t tiles;
 };
 
@@ -241,7 +258,8 @@ Layout make_layout(int64_t n, size_t key_size, bool vals, int tile_keys = SORT_T
     L.off_lb0 = align_up(L.off_dtot + size_t(MAX_PASSES) * RADIX * LINE_WORDS * sizeof(uint32_t));
     L.zero_bytes = L.off_lb0 + lb_bytes;
     L.off_lb1 = align_up(L.zero_bytes);
-    L.off_keysB = align_up(L.off_lb1 + lb_bytes);
+    L.off_cnt = align_up(L.off_lb1 + lb_bytes); // 16-bit tile counts of the reduce-then-scan passes
+    L.off_keysB = align_up(L.off_cnt + lb_bytes / 2);
     L.off_valsB = align_up(L.off_keysB + size_t(n) * key_size);
     L.off_dig = align_up(L.off_valsB + (vals ? size_t(n) * sizeof(uint32_t) : 0));
     L.total = align_up(L.off_dig + (dig ? size_t(n) : 0));
@@ -379,7 +397,8 @@ void launch_onesweep_shape(int rank_mode, int window, int fb, dim3 grid, hipStre
     if (window == 0)
         return launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 0>(rank_mode, fb, grid, st, a, b, va, vb, n,
                                                                        pass, plan, lb, next, dig);
-    if constexpr (sizeof(K) == 4 && !HAS_VALS) {
+    // windows 16 and 64 serve the default 256x16 tiles (others take 32)
+    if constexpr (sizeof(K) == 4 && !HAS_VALS && BLOCK == 256 && ITEMS == 16) {
         if (window <= 16)
             return launch_onesweep_w<K, HAS_VALS, FLOATK, BLOCK, ITEMS, 16>(rank_mode, fb, grid, st, a, b, va, vb, n,
                                                                             pass, plan, lb, next, dig);
@@ -410,7 +429,7 @@ bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 
 // Per-tile digit counts of one pass (grid = one workgroup per tile).
 template <typename K, bool HAS_VALS, int FLOATK>
 void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const K *b, uint32_t n, int pass,
-                        const DevPlan *plan, uint32_t *counts, const uint8_t *dig)
+                        const DevPlan *plan, uint16_t *counts, const uint8_t *dig)
 {
     const int vec = aligned16(a) && aligned16(b);
     with_shape<int(sizeof(K)), HAS_VALS>(shape, [&](auto sc) {
@@ -422,7 +441,7 @@ void launch_tile_counts(int shape, dim3 grid, hipStream_t st, const K *a, const 
 
 // The first read of a reduce-then-scan sort (grid = one workgroup per tile).
 template <typename K, bool HAS_VALS, int FLOATK>
-void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n, uint32_t *counts, K *orslots)
+void launch_upsweep(int shape, dim3 grid, hipStream_t st, const K *a, uint32_t n, uint16_t *counts, K *orslots)
 {
     const int vec = aligned16(a);
     with_shape<int(sizeof(K)), HAS_VALS>(shape, [&](auto sc) {
@@ -675,18 +694,20 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         timed(c, st, "plan", [&] {
             hipLaunchKernelGGL((k_plan<P>), dim3(1), dim3(RADIX), 0, st, hist, plan, uint64_t(n), FLOATK ? 1 : 0);
         });
-        uint32_t *counts = lb[0];
+        // 16-bit tile counts, then 32-bit global offsets in look-back region 0
+        uint16_t *counts = reinterpret_cast<uint16_t *>(base + L.off_cnt);
+        uint32_t *offs = lb[0];
         for (int p = 0; p < P; ++p) {
             timed(c, st, "tile_counts", [&] {
                 launch_tile_counts<K, HAS_VALS, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan,
                                                         counts, nullptr);
             });
             timed(c, st, "scan", [&] {
-                hipLaunchKernelGGL(k_scan_digit, dim3(RADIX), dim3(1024), 0, st, counts, tiles, p, plan);
+                hipLaunchKernelGGL(k_scan_digit, dim3(RADIX), dim3(1024), 0, st, counts, offs, tiles, p, plan);
             });
             timed(c, st, "scatter", [&] {
                 launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st, d_keys,
-                                                     keysB, d_vals, valsB, uint32_t(n), p, plan, counts, nullptr,
+                                                     keysB, d_vals, valsB, uint32_t(n), p, plan, offs, nullptr,
                                                      nullptr);
             });
         }
@@ -701,7 +722,8 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         launch_fill(c, base, L.off_lb0, st);
         K *orslots = reinterpret_cast<K *>(hist);
         timed(c, st, "upsweep", [&] {
-            launch_upsweep<K, HAS_VALS, FLOATK>(shape, dim3(tiles), st, d_keys, uint32_t(n), lb[0], orslots);
+            launch_upsweep<K, HAS_VALS, FLOATK>(shape, dim3(tiles), st, d_keys, uint32_t(n),
+                                                reinterpret_cast<uint16_t *>(base + L.off_cnt), orslots);
         });
         timed(c, st, "plan", [&] {
             hipLaunchKernelGGL((k_plan_rts<K>), dim3(1), dim3(64), 0, st, orslots, plan, uint64_t(n),
@@ -716,8 +738,10 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
     }
     for (int p = 0; p < P; ++p) {
         if (algo == PASS_RTS) {
-            // counts in look-back region 0, range sums in region 1
-            uint32_t *counts = lb[0], *totals = lb[1];
+            // 16-bit tile counts; in-range offsets in look-back region 0,
+            // range sums in region 1
+            uint16_t *counts = reinterpret_cast<uint16_t *>(base + L.off_cnt);
+            uint32_t *offs = lb[0], *totals = lb[1];
             if (p > 0)
                 timed(c, st, "tile_counts", [&] {
                     launch_tile_counts<K, HAS_VALS, FLOATK>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p,
@@ -725,14 +749,14 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
                 });
             timed(c, st, "scan", [&] {
                 uint32_t *dtot_p = dtot + size_t(p) * RADIX * LINE_WORDS;
-                hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, tiles, totals, dtot_p,
-                                   p, plan);
+                hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, offs, tiles, totals,
+                                   dtot_p, p, plan);
                 hipLaunchKernelGGL(k_scan_columns, dim3(RADIX), dim3(1024), 0, st, totals, nranges, dtot_p, p, plan);
             });
             timed(c, st, "scatter", [&] {
                 // the scatter adds its range's base to its in-range offsets
                 launch_onesweep<K, HAS_VALS, FLOATK>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st, d_keys,
-                                                     keysB, d_vals, valsB, uint32_t(n), p, plan, counts, totals, dig);
+                                                     keysB, d_vals, valsB, uint32_t(n), p, plan, offs, totals, dig);
             });
             continue;
         }
@@ -1447,6 +1471,7 @@ void host_histogram(hpcs_ctx *c, const K *a, int64_t l, int64_t r, int32_t *coun
     for (int i = 0; i < P * RADIX; ++i) counts[i] = int32_t(tmp[i]);
 }
 
+#if HPCS_IN_PART(0)
 // The device address of `scratch` on `device` (current), resolved once per
 // device and kept; null (with the reason in `why`) if the runtime gives none,
 // which fails only the no-allocation entry point, not the context.
@@ -1467,6 +1492,7 @@ void *noalloc_scratch_address(int device, std::string &why)
     addr[device] = p;
     return p;
 }
+#endif
 
 template <typename F>
 int32_t guarded_host(F &&f)
@@ -1517,6 +1543,7 @@ int32_t guarded(hpcs_ctx *c, F &&f)
 // C ABI
 // ---------------------------------------------------------------------------
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_version(void) { return 100; }
 
 HPCS_API const char *hpcs_last_error(void) { return g_last_error.c_str(); }
@@ -1570,7 +1597,9 @@ HPCS_API int32_t hpcs_ctx_create(int32_t device, hpcs_ctx **out)
         return HPCS_E_OOM;
     }
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_ctx_trim(hpcs_ctx *ctx)
 {
     return guarded(ctx, [&] {
@@ -1582,7 +1611,9 @@ HPCS_API int32_t hpcs_ctx_trim(hpcs_ctx *ctx)
         ctx->scratch_cap = ctx->io_cap = 0;
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_ctx_destroy(hpcs_ctx *ctx)
 {
     if (!ctx) return HPCS_OK;
@@ -1600,7 +1631,9 @@ HPCS_API int32_t hpcs_ctx_destroy(hpcs_ctx *ctx)
     delete ctx;
     return rc;
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t value)
 {
     return guarded(ctx, [&] {
@@ -1662,7 +1695,9 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
         }
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_profile_enable(hpcs_ctx *ctx, int32_t on)
 {
     return guarded(ctx, [&] {
@@ -1671,7 +1706,9 @@ HPCS_API int32_t hpcs_profile_enable(hpcs_ctx *ctx, int32_t on)
         ctx->profile = on != 0;
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_profile_read(hpcs_ctx *ctx, hpcs_kernel_stat *out, int32_t max, int32_t *count)
 {
     return guarded(ctx, [&] {
@@ -1689,7 +1726,9 @@ HPCS_API int32_t hpcs_profile_read(hpcs_ctx *ctx, hpcs_kernel_stat *out, int32_t
         *count = i;
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_graph_cache_stats(hpcs_ctx *ctx, int64_t *entries, int64_t *captures, int64_t *replays)
 {
     return guarded(ctx, [&] {
@@ -1701,7 +1740,9 @@ HPCS_API int32_t hpcs_graph_cache_stats(hpcs_ctx *ctx, int64_t *entries, int64_t
         *replays = ctx->graph_replays;
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_tile_shape_query(int32_t shape, int32_t key_bytes, int32_t with_values, int32_t *block,
                                        int32_t *items)
 {
@@ -1720,7 +1761,9 @@ HPCS_API int32_t hpcs_tile_shape_query(int32_t shape, int32_t key_bytes, int32_t
         else with_shape<8, false>(shape, report);
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_t with_values, hpcs_plan_info *info)
 {
     if (!info) {
@@ -1792,7 +1835,9 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
     info->scratch_bytes = int64_t(L.total);
     return HPCS_OK;
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n)
 {
     return guarded(ctx, [&] {
@@ -1818,7 +1863,9 @@ HPCS_API int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n)
         ensure(ctx->scratch, ctx->scratch_cap, need);
     });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_ctx_reserve_segments(hpcs_ctx *ctx, int64_t n, int32_t nseg)
 {
     return guarded(ctx, [&] {
@@ -1827,12 +1874,16 @@ HPCS_API int32_t hpcs_ctx_reserve_segments(hpcs_ctx *ctx, int64_t n, int32_t nse
         if (nseg > 0 && n > 1) ensure(ctx->scratch, ctx->scratch_cap, seg_layout(n, nseg).total);
     });
 }
+#endif
 
 // host arrays
+#if HPCS_IN_PART(1)
 HPCS_API int32_t hpcs_sort_radix_u32(hpcs_ctx *ctx, uint32_t *keys, int64_t n)
 {
     return guarded(ctx, [&] { host_sort<uint32_t, false, false>(ctx, keys, nullptr, n); });
 }
+#endif
+#if HPCS_IN_PART(1)
 HPCS_API int32_t hpcs_sort_radix_range_u32(hpcs_ctx *ctx, uint32_t *keys, int64_t start, int64_t length)
 {
     return guarded(ctx, [&] {
@@ -1841,26 +1892,38 @@ HPCS_API int32_t hpcs_sort_radix_range_u32(hpcs_ctx *ctx, uint32_t *keys, int64_
         host_sort<uint32_t, false, false>(ctx, keys ? keys + start : keys, nullptr, length);
     });
 }
+#endif
+#if HPCS_IN_PART(2)
 HPCS_API int32_t hpcs_sort_radix_u64(hpcs_ctx *ctx, uint64_t *keys, int64_t n)
 {
     return guarded(ctx, [&] { host_sort<uint64_t, false, false>(ctx, keys, nullptr, n); });
 }
+#endif
+#if HPCS_IN_PART(5)
 HPCS_API int32_t hpcs_sort_radix_f32(hpcs_ctx *ctx, float *keys, int64_t n)
 {
     return guarded(ctx, [&] { host_sort<uint32_t, false, true>(ctx, reinterpret_cast<uint32_t *>(keys), nullptr, n); });
 }
+#endif
+#if HPCS_IN_PART(2)
 HPCS_API int32_t hpcs_sort_radix_f64(hpcs_ctx *ctx, double *keys, int64_t n)
 {
     return guarded(ctx, [&] { host_sort<uint64_t, false, true>(ctx, reinterpret_cast<uint64_t *>(keys), nullptr, n); });
 }
+#endif
+#if HPCS_IN_PART(3)
 HPCS_API int32_t hpcs_sort_radix_pairs_u64_u32(hpcs_ctx *ctx, uint64_t *keys, uint32_t *vals, int64_t n)
 {
     return guarded(ctx, [&] { host_sort<uint64_t, true, false>(ctx, keys, vals, n); });
 }
+#endif
+#if HPCS_IN_PART(3)
 HPCS_API int32_t hpcs_sort_radix_pairs_u32_u32(hpcs_ctx *ctx, uint32_t *keys, uint32_t *vals, int64_t n)
 {
     return guarded(ctx, [&] { host_sort<uint32_t, true, false>(ctx, keys, vals, n); });
 }
+#endif
+#if HPCS_IN_PART(3)
 HPCS_API int32_t hpcs_sort_radix_records(hpcs_ctx *ctx, void *recs, int64_t n, int32_t rec_size, int32_t key_offset,
                                          int32_t key_bits, int32_t key_kind)
 {
@@ -1876,17 +1939,23 @@ HPCS_API int32_t hpcs_sort_radix_records(hpcs_ctx *ctx, void *recs, int64_t n, i
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
 }
+#endif
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *a, int64_t l, int64_t r,
                                                     int32_t *counts)
 {
     return guarded(ctx, [&] { host_histogram<uint32_t>(ctx, a, l, r, counts); });
 }
+#endif
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *a, int64_t l, int64_t r,
                                                     int32_t *counts)
 {
     return guarded(ctx, [&] { host_histogram<uint64_t>(ctx, a, l, r, counts); });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_sort_segments_u32(hpcs_ctx *ctx, uint32_t *keys, const int64_t *seg_begin, int32_t nseg)
 {
     return guarded(ctx, [&] {
@@ -1910,8 +1979,10 @@ HPCS_API int32_t hpcs_sort_segments_u32(hpcs_ctx *ctx, uint32_t *keys, const int
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
 }
+#endif
 
 // device-resident
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_dev_sort_segments_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, const int64_t *d_seg_begin,
                                             int32_t nseg, void *stream)
 {
@@ -1922,11 +1993,15 @@ HPCS_API int32_t hpcs_dev_sort_segments_u32(hpcs_ctx *ctx, uint32_t *d_keys, int
         run_cached(ctx, key, st, [&] { run_segments(ctx, d_keys, n, d_seg_begin, nseg, st); });
     });
 }
+#endif
 
+#if HPCS_IN_PART(1)
 HPCS_API int32_t hpcs_dev_sort_radix_u32(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] { dev_sort<uint32_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });
 }
+#endif
+#if HPCS_IN_PART(1)
 HPCS_API int32_t hpcs_dev_sort_radix_u32_noalloc(hpcs_ctx *ctx, uint32_t *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] {
@@ -1947,32 +2022,44 @@ HPCS_API int32_t hpcs_dev_sort_radix_u32_noalloc(hpcs_ctx *ctx, uint32_t *d_keys
         ctx->profile = profile;
     });
 }
+#endif
+#if HPCS_IN_PART(2)
 HPCS_API int32_t hpcs_dev_sort_radix_u64(hpcs_ctx *ctx, uint64_t *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] { dev_sort<uint64_t, false, false>(ctx, d_keys, nullptr, n, pick_stream(ctx, stream)); });
 }
+#endif
+#if HPCS_IN_PART(5)
 HPCS_API int32_t hpcs_dev_sort_radix_f32(hpcs_ctx *ctx, float *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] {
         dev_sort<uint32_t, false, true>(ctx, reinterpret_cast<uint32_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
     });
 }
+#endif
+#if HPCS_IN_PART(2)
 HPCS_API int32_t hpcs_dev_sort_radix_f64(hpcs_ctx *ctx, double *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] {
         dev_sort<uint64_t, false, true>(ctx, reinterpret_cast<uint64_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
     });
 }
+#endif
+#if HPCS_IN_PART(3)
 HPCS_API int32_t hpcs_dev_sort_radix_pairs_u64_u32(hpcs_ctx *ctx, uint64_t *d_keys, uint32_t *d_vals, int64_t n,
                                                    void *stream)
 {
     return guarded(ctx, [&] { dev_sort<uint64_t, true, false>(ctx, d_keys, d_vals, n, pick_stream(ctx, stream)); });
 }
+#endif
+#if HPCS_IN_PART(3)
 HPCS_API int32_t hpcs_dev_sort_radix_pairs_u32_u32(hpcs_ctx *ctx, uint32_t *d_keys, uint32_t *d_vals, int64_t n,
                                                    void *stream)
 {
     return guarded(ctx, [&] { dev_sort<uint32_t, true, false>(ctx, d_keys, d_vals, n, pick_stream(ctx, stream)); });
 }
+#endif
+#if HPCS_IN_PART(3)
 HPCS_API int32_t hpcs_dev_sort_radix_records(hpcs_ctx *ctx, void *d_recs, int64_t n, int32_t rec_size,
                                              int32_t key_offset, int32_t key_bits, int32_t key_kind, void *stream)
 {
@@ -1980,62 +2067,82 @@ HPCS_API int32_t hpcs_dev_sort_radix_records(hpcs_ctx *ctx, void *d_recs, int64_
         dev_records(ctx, d_recs, n, rec_size, key_offset, key_bits, key_kind, pick_stream(ctx, stream));
     });
 }
+#endif
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_dev_histogram_byte_components_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n,
                                                         uint32_t *d_counts, void *stream)
 {
     return guarded(ctx, [&] { dev_histogram<uint32_t, false>(ctx, d_keys, n, d_counts, pick_stream(ctx, stream)); });
 }
+#endif
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_dev_histogram_byte_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n,
                                                         uint32_t *d_counts, void *stream)
 {
     return guarded(ctx, [&] { dev_histogram<uint64_t, false>(ctx, d_keys, n, d_counts, pick_stream(ctx, stream)); });
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Radix top-k
 // ---------------------------------------------------------------------------
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_select_topk_u32(hpcs_ctx *ctx, uint32_t *a, int64_t start, int64_t length, int64_t k)
 {
     return guarded(ctx, [&] { host_select_topk<uint32_t, false>(ctx, a, start, length, k); });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_topk_u32(hpcs_ctx *ctx, const uint32_t *keys, int64_t n, int64_t k, uint32_t *out)
 {
     return guarded(ctx, [&] { host_topk<uint32_t, false>(ctx, keys, n, k, out); });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_topk_u64(hpcs_ctx *ctx, const uint64_t *keys, int64_t n, int64_t k, uint64_t *out)
 {
     return guarded(ctx, [&] { host_topk<uint64_t, false>(ctx, keys, n, k, out); });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_topk_f32(hpcs_ctx *ctx, const float *keys, int64_t n, int64_t k, float *out)
 {
     return guarded(ctx, [&] {
         host_topk<uint32_t, true>(ctx, reinterpret_cast<const uint32_t *>(keys), n, k, reinterpret_cast<uint32_t *>(out));
     });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_topk_f64(hpcs_ctx *ctx, const double *keys, int64_t n, int64_t k, double *out)
 {
     return guarded(ctx, [&] {
         host_topk<uint64_t, true>(ctx, reinterpret_cast<const uint64_t *>(keys), n, k, reinterpret_cast<uint64_t *>(out));
     });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_dev_topk_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n, int64_t k, uint32_t *d_out,
                                    void *stream)
 {
     return guarded(ctx, [&] { run_topk<uint32_t, false>(ctx, d_keys, n, k, d_out, nullptr, true, pick_stream(ctx, stream)); });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_dev_topk_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n, int64_t k, uint64_t *d_out,
                                    void *stream)
 {
     return guarded(ctx, [&] { run_topk<uint64_t, false>(ctx, d_keys, n, k, d_out, nullptr, true, pick_stream(ctx, stream)); });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_dev_topk_f32(hpcs_ctx *ctx, const float *d_keys, int64_t n, int64_t k, float *d_out, void *stream)
 {
     return guarded(ctx, [&] {
@@ -2043,7 +2150,9 @@ HPCS_API int32_t hpcs_dev_topk_f32(hpcs_ctx *ctx, const float *d_keys, int64_t n
                                  reinterpret_cast<uint32_t *>(d_out), nullptr, true, pick_stream(ctx, stream));
     });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_dev_topk_f64(hpcs_ctx *ctx, const double *d_keys, int64_t n, int64_t k, double *d_out,
                                    void *stream)
 {
@@ -2052,6 +2161,7 @@ HPCS_API int32_t hpcs_dev_topk_f64(hpcs_ctx *ctx, const double *d_keys, int64_t 
                                  reinterpret_cast<uint64_t *>(d_out), nullptr, true, pick_stream(ctx, stream));
     });
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Signed keys: int[] / long[] (RadixSortLsd.cs:267-357, 444-532;
@@ -2060,30 +2170,39 @@ HPCS_API int32_t hpcs_dev_topk_f64(hpcs_ctx *ctx, const double *d_keys, int64_t 
 // unsigned sort, as in the reference (its top digit ^ 128).
 // ---------------------------------------------------------------------------
 
+#if HPCS_IN_PART(6)
 HPCS_API int32_t hpcs_sort_radix_i32(hpcs_ctx *ctx, int32_t *keys, int64_t n)
 {
     return guarded(ctx, [&] { host_sort<uint32_t, false, KIND_INT>(ctx, reinterpret_cast<uint32_t *>(keys), nullptr, n); });
 }
+#endif
 
+#if HPCS_IN_PART(2)
 HPCS_API int32_t hpcs_sort_radix_i64(hpcs_ctx *ctx, int64_t *keys, int64_t n)
 {
     return guarded(ctx, [&] { host_sort<uint64_t, false, KIND_INT>(ctx, reinterpret_cast<uint64_t *>(keys), nullptr, n); });
 }
+#endif
 
+#if HPCS_IN_PART(6)
 HPCS_API int32_t hpcs_dev_sort_radix_i32(hpcs_ctx *ctx, int32_t *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] {
         dev_sort<uint32_t, false, KIND_INT>(ctx, reinterpret_cast<uint32_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
     });
 }
+#endif
 
+#if HPCS_IN_PART(2)
 HPCS_API int32_t hpcs_dev_sort_radix_i64(hpcs_ctx *ctx, int64_t *d_keys, int64_t n, void *stream)
 {
     return guarded(ctx, [&] {
         dev_sort<uint64_t, false, KIND_INT>(ctx, reinterpret_cast<uint64_t *>(d_keys), nullptr, n, pick_stream(ctx, stream));
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_histogram_byte_components_i32(hpcs_ctx *ctx, const int32_t *a, int64_t l, int64_t r,
                                                     int32_t *counts)
 {
@@ -2091,7 +2210,9 @@ HPCS_API int32_t hpcs_histogram_byte_components_i32(hpcs_ctx *ctx, const int32_t
         host_histogram<uint32_t, KIND_INT>(ctx, reinterpret_cast<const uint32_t *>(a), l, r, counts);
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_histogram_byte_components_i64(hpcs_ctx *ctx, const int64_t *a, int64_t l, int64_t r,
                                                     int32_t *counts)
 {
@@ -2099,7 +2220,9 @@ HPCS_API int32_t hpcs_histogram_byte_components_i64(hpcs_ctx *ctx, const int64_t
         host_histogram<uint64_t, KIND_INT>(ctx, reinterpret_cast<const uint64_t *>(a), l, r, counts);
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_dev_histogram_byte_components_i32(hpcs_ctx *ctx, const int32_t *d_keys, int64_t n,
                                                         uint32_t *d_counts, void *stream)
 {
@@ -2108,7 +2231,9 @@ HPCS_API int32_t hpcs_dev_histogram_byte_components_i32(hpcs_ctx *ctx, const int
                                           pick_stream(ctx, stream));
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_dev_histogram_byte_components_i64(hpcs_ctx *ctx, const int64_t *d_keys, int64_t n,
                                                         uint32_t *d_counts, void *stream)
 {
@@ -2117,21 +2242,27 @@ HPCS_API int32_t hpcs_dev_histogram_byte_components_i64(hpcs_ctx *ctx, const int
                                           pick_stream(ctx, stream));
     });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_topk_i32(hpcs_ctx *ctx, const int32_t *keys, int64_t n, int64_t k, int32_t *out)
 {
     return guarded(ctx, [&] {
         host_topk<uint32_t, KIND_INT>(ctx, reinterpret_cast<const uint32_t *>(keys), n, k, reinterpret_cast<uint32_t *>(out));
     });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_topk_i64(hpcs_ctx *ctx, const int64_t *keys, int64_t n, int64_t k, int64_t *out)
 {
     return guarded(ctx, [&] {
         host_topk<uint64_t, KIND_INT>(ctx, reinterpret_cast<const uint64_t *>(keys), n, k, reinterpret_cast<uint64_t *>(out));
     });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_dev_topk_i32(hpcs_ctx *ctx, const int32_t *d_keys, int64_t n, int64_t k, int32_t *d_out,
                                    void *stream)
 {
@@ -2140,7 +2271,9 @@ HPCS_API int32_t hpcs_dev_topk_i32(hpcs_ctx *ctx, const int32_t *d_keys, int64_t
                                      reinterpret_cast<uint32_t *>(d_out), nullptr, true, pick_stream(ctx, stream));
     });
 }
+#endif
 
+#if HPCS_IN_PART(4)
 HPCS_API int32_t hpcs_dev_topk_i64(hpcs_ctx *ctx, const int64_t *d_keys, int64_t n, int64_t k, int64_t *d_out,
                                    void *stream)
 {
@@ -2149,6 +2282,7 @@ HPCS_API int32_t hpcs_dev_topk_i64(hpcs_ctx *ctx, const int64_t *d_keys, int64_t
                                      reinterpret_cast<uint64_t *>(d_out), nullptr, true, pick_stream(ctx, stream));
     });
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Key-extractor histograms: HistogramByteComponents<T>(T[], l, r,
@@ -2156,6 +2290,7 @@ HPCS_API int32_t hpcs_dev_topk_i64(hpcs_ctx *ctx, const int64_t *d_keys, int64_t
 // offset inside each record.
 // ---------------------------------------------------------------------------
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_histogram_byte_components_records(hpcs_ctx *ctx, const void *recs, int64_t l, int64_t r,
                                                         int32_t rec_size, int32_t key_offset, int32_t key_bits,
                                                         int32_t *counts)
@@ -2183,7 +2318,9 @@ HPCS_API int32_t hpcs_histogram_byte_components_records(hpcs_ctx *ctx, const voi
         for (int i = 0; i < P * RADIX; ++i) counts[i] = int32_t(tmp[i]);
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_dev_histogram_byte_components_records(hpcs_ctx *ctx, const void *d_recs, int64_t n,
                                                             int32_t rec_size, int32_t key_offset, int32_t key_bits,
                                                             uint32_t *d_counts, void *stream)
@@ -2192,11 +2329,13 @@ HPCS_API int32_t hpcs_dev_histogram_byte_components_records(hpcs_ctx *ctx, const
         dev_histogram_records(ctx, d_recs, n, rec_size, key_offset, key_bits, d_counts, pick_stream(ctx, stream));
     });
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Tuning tables
 // ---------------------------------------------------------------------------
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_ctx_load_tuning(hpcs_ctx *ctx, const char *path)
 {
     return guarded(ctx, [&] {
@@ -2212,7 +2351,9 @@ HPCS_API int32_t hpcs_ctx_load_tuning(hpcs_ctx *ctx, const char *path)
         ctx->tune.swap(rows);
     });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_tuning_check(const char *path, int32_t *rows)
 {
     if (!path) {
@@ -2232,6 +2373,7 @@ HPCS_API int32_t hpcs_tuning_check(const char *path, int32_t *rows)
     if (rows) *rows = int32_t(out.size());
     return HPCS_OK;
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // N-bit histograms: Histogram11bitComponents(uint[], l, r) (Histogram.cs:
@@ -2239,35 +2381,44 @@ HPCS_API int32_t hpcs_tuning_check(const char *path, int32_t *rows)
 // bits) (:381-427)
 // ---------------------------------------------------------------------------
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_histogram_nbit_components_u32(hpcs_ctx *ctx, const uint32_t *a, int64_t l, int64_t r,
                                                     int32_t bits, int32_t *counts)
 {
     return guarded(ctx, [&] { host_histogram_nbit<uint32_t>(ctx, a, l, r, bits, counts); });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_histogram_nbit_components_u64(hpcs_ctx *ctx, const uint64_t *a, int64_t l, int64_t r,
                                                     int32_t bits, int32_t *counts)
 {
     return guarded(ctx, [&] { host_histogram_nbit<uint64_t>(ctx, a, l, r, bits, counts); });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_dev_histogram_nbit_components_u32(hpcs_ctx *ctx, const uint32_t *d_keys, int64_t n,
                                                         int32_t bits, uint32_t *d_counts, void *stream)
 {
     return guarded(ctx, [&] { dev_histogram_nbit<uint32_t>(ctx, d_keys, n, bits, d_counts, pick_stream(ctx, stream)); });
 }
+#endif
 
+#if HPCS_IN_PART(0)
 HPCS_API int32_t hpcs_dev_histogram_nbit_components_u64(hpcs_ctx *ctx, const uint64_t *d_keys, int64_t n,
                                                         int32_t bits, uint32_t *d_counts, void *stream)
 {
     return guarded(ctx, [&] { dev_histogram_nbit<uint64_t>(ctx, d_keys, n, bits, d_counts, pick_stream(ctx, stream)); });
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Items by a separate key array: SortRadix<T>(this T[] inputArray, UInt32[] /
 // UInt64[] inKeys) (RadixSortLsd.cs:1183-1299) for items of any fixed size
 // ---------------------------------------------------------------------------
 
+#if HPCS_IN_PART(3)
 HPCS_API int32_t hpcs_sort_radix_keys_items(hpcs_ctx *ctx, void *keys, int32_t key_bits, void *items, int32_t item_size,
                                             int64_t n)
 {
@@ -2287,9 +2438,12 @@ HPCS_API int32_t hpcs_sort_radix_keys_items(hpcs_ctx *ctx, void *keys, int32_t k
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     });
 }
+#endif
 
+#if HPCS_IN_PART(3)
 HPCS_API int32_t hpcs_dev_sort_radix_keys_items(hpcs_ctx *ctx, void *d_keys, int32_t key_bits, void *d_items,
                                                 int32_t item_size, int64_t n, void *stream)
 {
     return guarded(ctx, [&] { dev_keys_items(ctx, d_keys, key_bits, d_items, item_size, n, pick_stream(ctx, stream)); });
 }
+#endif

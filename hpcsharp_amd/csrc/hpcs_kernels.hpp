@@ -170,7 +170,7 @@ __device__ inline uint32_t count_below(uint64_t mask)
 // k_fill: zero (or fill) a word range.  Used instead of hipMemsetAsync so the
 // sequence stays capturable in a HIP graph.
 // ---------------------------------------------------------------------------
-__global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t words, uint32_t value)
+static __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t words, uint32_t value)
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = value;
@@ -366,6 +366,10 @@ __global__ __launch_bounds__(RADIX) void k_plan(const uint32_t *__restrict__ his
 //                for a plain copy of the same bytes).
 // ---------------------------------------------------------------------------
 // Tiles per range of the reduce-then-scan offset scan (k_range_totals).
+// Measured on MI355X (2^28 u32): ranges of 16 tiles with 16-bit in-range
+// offsets took the range and column kernels from 32 + 9 to 68 + 24 us per
+// pass (four times the same-address digit-total atomics, and column loads
+// over four times the ranges).
 constexpr int SCAN_RANGE = 64;
 constexpr int RANK_BALLOT = 0;
 constexpr int RANK_LDSOR = 1;
@@ -799,18 +803,20 @@ __device__ inline void count_digit(uint32_t *sh, uint32_t d, bool valid, int lan
     }
 }
 
-// Per-tile digit counts of a pass.  When the previous pass that moved keys
+// Per-tile digit counts of a pass (16-bit: a tile holds fewer than 65536
+// keys).  When the previous pass that moved keys
 // wrote this pass's digits as bytes (`dig`, every pass after the first that
 // moves keys), the tile's TILE bytes are read (thread t: bytes t*ITEMS ..,
 // in 4-byte words); otherwise the keys are.
 template <typename K, int FLOATK, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ keysA, const K *__restrict__ keysB,
                                                        uint32_t n, int pass, const DevPlan *__restrict__ plan,
-                                                       uint32_t *__restrict__ counts, int vec,
+                                                       uint16_t *__restrict__ counts, int vec,
                                                        const uint8_t *__restrict__ dig)
 {
     static_assert(ITEMS % 4 == 0, "digit bytes are read in 4-byte words");
     constexpr int TILE = BLOCK * ITEMS;
+    static_assert(TILE < 65536, "tile counts are 16-bit");
     __shared__ uint32_t sh[RADIX];
     const uint32_t flags = plan->flags[pass];
     if (flags & PF_TRIVIAL) return;
@@ -839,7 +845,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
             }
         }
         __syncthreads();
-        for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
+        for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = uint16_t(sh[i]);
         return;
     }
     for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
@@ -867,7 +873,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
         for (int i = 0; i < ITEMS; ++i)
             count_digit(sh, (w[i / 4] >> (8 * (i % 4))) & 0xFFu, first + uint32_t(i) < cnt, lane);
         __syncthreads();
-        for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
+        for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = uint16_t(sh[i]);
     }
 }
 
@@ -879,24 +885,27 @@ constexpr int LINE_WORDS = 32;
 constexpr int ORSLOTS = 64;
 template <typename K> constexpr int OR_STRIDE = 128 / int(sizeof(K)); // one 128-byte line per slot
 
-// totals[r][d] = sum of counts[t][d] over the tiles of range r, each count
-// replaced by the sum of the range's earlier counts (the tile's offset within
-// the range), and the pass's digit totals dtot[d * LINE_WORDS] += totals[r][d].
-__global__ __launch_bounds__(RADIX) void k_range_totals(uint32_t *__restrict__ counts, uint32_t tiles,
-                                                        uint32_t *__restrict__ totals, uint32_t *__restrict__ dtot,
-                                                        int pass, const DevPlan *__restrict__ plan)
+// totals[r][d] = sum of counts[t][d] over the SCAN_RANGE tiles of range r,
+// offsets[t][d] = the sum of the range's earlier counts (the tile's offset
+// within the range), and the pass's digit totals dtot[d * LINE_WORDS] +=
+// totals[r][d].
+static __global__ __launch_bounds__(RADIX) void k_range_totals(const uint16_t *__restrict__ counts,
+                                                               uint32_t *__restrict__ offsets, uint32_t tiles,
+                                                               uint32_t *__restrict__ totals,
+                                                               uint32_t *__restrict__ dtot, int pass,
+                                                               const DevPlan *__restrict__ plan)
 {
     if (plan->flags[pass] & PF_TRIVIAL) return;
     const uint32_t r = blockIdx.x, d = threadIdx.x;
     const uint32_t t0 = r * SCAN_RANGE, t1 = min(tiles, t0 + SCAN_RANGE);
     uint32_t c[SCAN_RANGE];
 #pragma unroll
-    for (int k = 0; k < SCAN_RANGE; ++k) c[k] = t0 + k < t1 ? counts[size_t(t0 + k) * RADIX + d] : 0u;
-    // each tile's offset within the range, written over its count
+    for (int k = 0; k < SCAN_RANGE; ++k) c[k] = t0 + k < t1 ? uint32_t(counts[size_t(t0 + k) * RADIX + d]) : 0u;
+    // each tile's offset within the range
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_RANGE; ++k) {
-        if (t0 + k < t1) counts[size_t(t0 + k) * RADIX + d] = sum;
+        if (t0 + k < t1) offsets[size_t(t0 + k) * RADIX + d] = sum;
         sum += c[k];
     }
     totals[size_t(r) * RADIX + d] = sum;
@@ -906,7 +915,7 @@ __global__ __launch_bounds__(RADIX) void k_range_totals(uint32_t *__restrict__ c
 // One workgroup per digit d: the digit's global start (ComputeStartOfBins,
 // RadixSortLsd.cs:112-125: the sum of the smaller digits' totals), then the
 // exclusive scan of column d of the range totals, seeded with that start.
-__global__ __launch_bounds__(1024) void k_scan_columns(uint32_t *__restrict__ totals, uint32_t nranges,
+static __global__ __launch_bounds__(1024) void k_scan_columns(uint32_t *__restrict__ totals, uint32_t nranges,
                                                        const uint32_t *__restrict__ dtot, int pass,
                                                        DevPlan *__restrict__ plan)
 {
@@ -949,12 +958,13 @@ __global__ __launch_bounds__(1024) void k_scan_columns(uint32_t *__restrict__ to
 // The three scan kernels in one, for sorts of few tiles (the plan's digit
 // starts come from a full histogram, k_histogram + k_plan): workgroup d
 // scans digit d's column of the tile counts, the tiles split over its
-// threads, and writes the tiles' global offsets over the counts.
+// threads, and writes the tiles' 32-bit global offsets to `offsets`.
 // Measured on MI355X: faster than the five-launch pass up to 512 tiles (u32
 // 2^20: 0.109 -> 0.092 ms, u64 2^20: 0.240 -> 0.199 ms), slower from 1024
 // (the all-digit histogram and the column loads outgrow the launches saved).
 constexpr uint32_t SCAN_DIGIT_MAX_TILES = 512;
-__global__ __launch_bounds__(1024) void k_scan_digit(uint32_t *__restrict__ counts, uint32_t tiles, int pass,
+static __global__ __launch_bounds__(1024) void k_scan_digit(const uint16_t *__restrict__ counts,
+                                                     uint32_t *__restrict__ offsets, uint32_t tiles, int pass,
                                                      const DevPlan *__restrict__ plan)
 {
     __shared__ uint32_t s_w[1024 / WAVE];
@@ -977,7 +987,7 @@ __global__ __launch_bounds__(1024) void k_scan_digit(uint32_t *__restrict__ coun
     for (uint32_t t = t0; t < t1; ++t) {
         const size_t at = size_t(t) * RADIX + d;
         const uint32_t c = counts[at];
-        counts[at] = run;
+        offsets[at] = run;
         run += c;
     }
 }
@@ -993,9 +1003,10 @@ __global__ __launch_bounds__(1024) void k_scan_digit(uint32_t *__restrict__ coun
 
 template <typename K, int FLOATK, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, uint32_t n,
-                                                   uint32_t *__restrict__ counts, K *__restrict__ orslots, int vec)
+                                                   uint16_t *__restrict__ counts, K *__restrict__ orslots, int vec)
 {
     constexpr int TILE = BLOCK * ITEMS;
+    static_assert(TILE < 65536, "tile counts are 16-bit");
     constexpr int WAVES = BLOCK / WAVE;
     __shared__ uint32_t sh[RADIX];
     __shared__ K s_or[WAVES], s_nor[WAVES];
@@ -1035,7 +1046,7 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
         s_nor[wave] = vnor;
     }
     __syncthreads();
-    for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = sh[i];
+    for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = uint16_t(sh[i]);
     if (tid == 0) {
         K a = 0, b = 0;
         for (int w = 0; w < WAVES; ++w) {
@@ -1172,7 +1183,7 @@ __global__ void k_extract_keys(const unsigned char *__restrict__ recs, uint64_t 
 
 // One thread per 4-byte word of output when rec_size and the buffer allow it
 // (words4), one per byte otherwise.
-__global__ void k_gather_records(const unsigned char *__restrict__ in, unsigned char *__restrict__ out,
+static __global__ void k_gather_records(const unsigned char *__restrict__ in, unsigned char *__restrict__ out,
                                  const uint32_t *__restrict__ idx, uint64_t n, int rec_size, int words4)
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
@@ -1205,7 +1216,7 @@ __global__ void k_gather_records(const unsigned char *__restrict__ in, unsigned 
     }
 }
 
-__global__ void k_iota_u32(uint32_t *__restrict__ out, uint64_t n)
+static __global__ void k_iota_u32(uint32_t *__restrict__ out, uint64_t n)
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = uint32_t(i);

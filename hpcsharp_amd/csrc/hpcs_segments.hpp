@@ -189,7 +189,7 @@ __device__ inline void tile_rank_stage_ldsor(const K (&keys)[ITEMS], int wave_va
 // keys total at most n and their tiles at most n / TILE + nseg, the bound the
 // host sized the tile records and counts by (max_tiles; a plan beyond it
 // sorts no large segment).
-__global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ begin, int nseg, int64_t n,
+static __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ begin, int nseg, int64_t n,
                                                    SegPlan *__restrict__ sp, uint32_t *__restrict__ seg_tile0,
                                                    uint32_t *__restrict__ small_list,
                                                    uint32_t *__restrict__ medium_list,
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restrict__ b
 // One record per tile of the large segments: {first key, segment's first key,
 // keys in the tile, segment}, so the pass kernels find a tile with one load.
 // Grid-stride over the large segments, the segment's tiles across the threads.
-__global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restrict__ begin,
+static __global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restrict__ begin,
                                                    const SegPlan *__restrict__ sp,
                                                    const uint32_t *__restrict__ seg_tile0,
                                                    const uint32_t *__restrict__ large_list,
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(uint32_t *__restrict__ keys
 
 // One workgroup per tile of the large segments (the grid may exceed the tile
 // count; spare workgroups return): digit counts of pass `pass`.
-__global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
+static __global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
                                                           const uint32_t *__restrict__ keysB, int pass,
                                                           const SegPlan *__restrict__ sp,
                                                           const uint4 *__restrict__ tinfo,
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t *__rest
 // offsets), so the serial walk over a part is an eighth of the segment.
 constexpr int SEG_SCAN_GROUPS = 8;
 constexpr int SEG_SCAN_PAIRS = RADIX / 2;
-__global__ __launch_bounds__(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS) void k_seg_scan(const int64_t *__restrict__ begin,
+static __global__ __launch_bounds__(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS) void k_seg_scan(const int64_t *__restrict__ begin,
                                                                               const SegPlan *__restrict__ sp,
                                                                               const uint32_t *__restrict__ seg_tile0,
                                                                               const uint32_t *__restrict__ large_list,

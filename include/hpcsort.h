@@ -63,7 +63,8 @@ extern "C" {
                                  for it (the table's or the default shape is used) */
 #define HPCS_OPT_FALLBACK_ROUNDS 3 /* look-back rounds on an unpublished tile before counting it locally
                                       (default 24; 0 makes every wait use the local count, for tests) */
-#define HPCS_OPT_LOOKBACK_WINDOW 4 /* u32 keys: earlier tiles read per look-back round: 16, 32 (default), 64 */
+#define HPCS_OPT_LOOKBACK_WINDOW 4 /* 4-byte keys without values, 256x16 tiles: earlier tiles read per look-back
+                                      round: 16, 32 (default), 64; other sorts read 32 */
 #define HPCS_OPT_PASS_ALGO 5       /* 0: single-pass onesweep with look-back, 1: reduce-then-scan,
                                       2: by size (default: one workgroup up to 16384 4-byte / 8192 8-byte
                                       keys, onesweep for small arrays, reduce-then-scan beyond),
