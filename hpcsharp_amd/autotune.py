@@ -9,9 +9,10 @@ serves the key type, hpcs_tile_shape_query: 256, 512 and 1024 threads, 8 to
 fourth item by ballot; ballot only on the 1024-thread shapes).
 Size classes are powers of two; a class's row applies to every n up to it.
 The batched segmented sort (kind "segs") is tuned per class of total keys:
-the largest segment one workgroup sorts in LDS (4096 or 16384) and the
-ranking of the large segments' scatter, on a batch of segments with
-log-uniform lengths in [16, min(2^20, total / 16)].
+the largest segment one workgroup sorts in LDS (4096 or 16384), the tile
+shape of the large segments' passes (the 256- and 512-thread rows, 8 to 32
+keys per thread) and the ranking of their scatter, on a batch of segments
+with log-uniform lengths in [16, min(2^20, total / 16)].
 Each measured cell is appended to a JSON log as it finishes, and a rerun
 skips the cells the log already holds (the sweep resumes).
 
@@ -39,6 +40,7 @@ KINDS = {
 }
 RANKS = (1, 2, 3, 0)  # LDS-OR, every 2nd / 4th item by ballot, ballot
 SEG_SMALL = (4096, 16384)  # largest segment sorted in LDS (HPCS_OPT_SEG_SMALL)
+SEG_SHAPES = (0, 1, 2, 3, 4, 7, 8, 9)  # tile shapes of the large segments (HPCS_OPT_SEG_SHAPE)
 
 
 # largest arrays the one-workgroup sort (pass algorithm 3) takes
@@ -127,11 +129,12 @@ def seg_batch(lg: int):
     return datagen.segment_lengths(nseg, 0x5E6 + lg, lo, hi)
 
 
-def time_segments(torch, ctx, lg: int, small: int, rank: int, reps: int) -> float:
+def time_segments(torch, ctx, lg: int, small: int, rank: int, shape: int, reps: int) -> float:
     """Median seconds per batched call (device-resident keys)."""
     import numpy as np
     N.set_option(N.HPCS_OPT_SEG_SMALL, small, ctx)
     N.set_option(N.HPCS_OPT_SEG_RANK, rank, ctx)
+    N.set_option(N.HPCS_OPT_SEG_SHAPE, shape, ctx)
     lens = seg_batch(lg)
     begin = torch.from_numpy(np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)).cuda()
     n = int(lens.sum())
@@ -168,11 +171,12 @@ def table_text(results: dict) -> str:
             for name in alias[kind]:
                 lines.append(f"{name} {1 << lg} {best['algo']} {best['shape']} {best['rank']} {best.get('bits', 8)}"
                              f"   # {best['s'] * 1e3:.4f} ms")
-    lines.append("# segs max_n small_max rank   # median ms per batched call of the best variant")
+    lines.append("# segs max_n small_max rank shape   # median ms per batched call of the best variant")
     for lg in sorted({int(k.split(":")[1]) for k in results["cells"] if k.startswith("segs:")}):
         cells = [c for k, c in results["cells"].items() if k.startswith(f"segs:{lg}:")]
         best = min(cells, key=lambda c: c["s"])
-        lines.append(f"segs {1 << lg} {best['small']} {best['rank']}   # {best['s'] * 1e3:.4f} ms")
+        lines.append(f"segs {1 << lg} {best['small']} {best['rank']} {best.get('shape', 0)}"
+                     f"   # {best['s'] * 1e3:.4f} ms")
     return "\n".join(lines) + "\n"
 
 
@@ -199,12 +203,12 @@ def main(argv=None):
         for kind in args.kinds.split(","):
             if kind == "segs":
                 for lg in (int(x) for x in args.seg_classes.split(",")):
-                    for small, rank in itertools.product(SEG_SMALL, RANKS):
-                        key = f"segs:{lg}:{small}:{rank}"
+                    for small, rank, shape in itertools.product(SEG_SMALL, RANKS, SEG_SHAPES):
+                        key = f"segs:{lg}:{small}:{rank}:{shape}"
                         if key in results["cells"]:
                             continue
-                        s = time_segments(torch, ctx, lg, small, rank, args.reps)
-                        results["cells"][key] = {"small": small, "rank": rank, "s": s}
+                        s = time_segments(torch, ctx, lg, small, rank, shape, args.reps)
+                        results["cells"][key] = {"small": small, "rank": rank, "shape": shape, "s": s}
                         print(f"[autotune] {key}: {s * 1e3:.4f} ms", flush=True)
                         with open(args.log, "w") as f:
                             json.dump(results, f, indent=1, sort_keys=True)

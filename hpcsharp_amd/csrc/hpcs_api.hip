@@ -172,9 +172,10 @@ struct hpcs_ctx {
     // segment sorted in LDS (HPCS_OPT_SEG_RANK, HPCS_OPT_SEG_SMALL)
     int seg_rank = RANK_MIX2;
     int seg_small = SEG_SMALL;
+    int seg_shape = 0;
     std::vector<TuneEntry> tune;
     bool set_algo = false, set_shape = false, set_rank = false, set_bits = false;
-    bool set_seg_rank = false, set_seg_small = false;
+    bool set_seg_rank = false, set_seg_small = false, set_seg_shape = false;
     // kernel timing
     struct Pending {
         const char *name;
@@ -875,10 +876,10 @@ struct SegLayout {
     uint64_t max_tiles;
 };
 
-SegLayout seg_layout(int64_t n, int nseg)
+SegLayout seg_layout(int64_t n, int nseg, int tile)
 {
     SegLayout L{};
-    L.max_tiles = uint64_t(n) / SEG_TILE + uint64_t(nseg) + 1; // tiles of the large segments, at most
+    L.max_tiles = uint64_t(n) / uint64_t(tile) + uint64_t(nseg) + 1; // tiles of the large segments, at most
     L.off_plan = 0;
     L.off_tile0 = align_up(sizeof(SegPlan));
     L.off_small = align_up(L.off_tile0 + size_t(nseg) * 4);
@@ -896,12 +897,12 @@ SegLayout seg_layout(int64_t n, int nseg)
 // then the tuning table's `segs` row for the smallest size class that holds
 // n (the largest class beyond it), then the defaults.
 struct SegParams {
-    int small_max, rank;
+    int small_max, rank, shape;
 };
 
 SegParams resolve_seg_params(const hpcs_ctx *c, int64_t n)
 {
-    SegParams p{c->seg_small, c->seg_rank};
+    SegParams p{c->seg_small, c->seg_rank, c->seg_shape};
     const TuneEntry *best = nullptr, *largest = nullptr;
     for (const TuneEntry &e : c->tune) {
         if (e.key_bytes != 0) continue;
@@ -912,23 +913,60 @@ SegParams resolve_seg_params(const hpcs_ctx *c, int64_t n)
     if (best) {
         if (!c->set_seg_small) p.small_max = best->seg_small;
         if (!c->set_seg_rank) p.rank = best->rank;
+        if (!c->set_seg_shape && seg_shape_ok(best->shape)) p.shape = best->shape;
     }
     return p;
 }
 
-void launch_seg_scatter(int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass, const SegPlan *sp,
-                        const uint4 *tinfo, const uint32_t *offsets)
+// Runs f with the large segments' tile shape as a compile-time row of
+// TILE_SHAPES (the rows seg_shape_ok admits); any other index fails.
+template <int S = 0, typename F>
+void with_seg_shape(int shape, F &&f)
 {
-    if (rank == RANK_BALLOT)
-        hipLaunchKernelGGL(k_seg_scatter<RANK_BALLOT>, grid, dim3(SEG_BLOCK), 0, st, a, b, pass, sp, tinfo, offsets);
-    else if (rank == RANK_MIX2)
-        hipLaunchKernelGGL(k_seg_scatter<RANK_MIX2>, grid, dim3(SEG_BLOCK), 0, st, a, b, pass, sp, tinfo, offsets);
-    else if (rank == RANK_MIX4)
-        hipLaunchKernelGGL(k_seg_scatter<RANK_MIX4>, grid, dim3(SEG_BLOCK), 0, st, a, b, pass, sp, tinfo, offsets);
-    else if (rank == RANK_LDSOR)
-        hipLaunchKernelGGL(k_seg_scatter<RANK_LDSOR>, grid, dim3(SEG_BLOCK), 0, st, a, b, pass, sp, tinfo, offsets);
-    else
-        fail(HPCS_E_UNSUPPORTED, "unknown ranking mode " + std::to_string(rank));
+    if constexpr (S < NUM_TILE_SHAPES) {
+        if (shape == S) {
+            if constexpr (seg_shape_ok(S)) {
+                f(std::integral_constant<int, S>{});
+                return;
+            } else {
+                fail(HPCS_E_UNSUPPORTED, "tile shape " + std::to_string(shape) + " does not serve the segments");
+            }
+        }
+        with_seg_shape<S + 1>(shape, std::forward<F>(f));
+    } else {
+        fail(HPCS_E_UNSUPPORTED, "unknown tile shape " + std::to_string(shape));
+    }
+}
+
+void launch_seg_counts(int shape, dim3 grid, hipStream_t st, const uint32_t *a, const uint32_t *b, int pass,
+                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts)
+{
+    with_seg_shape(shape, [&](auto sc) {
+        constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
+        hipLaunchKernelGGL((k_seg_counts<T.block, T.items>), grid, dim3(T.block), 0, st, a, b, pass, sp, tinfo, counts);
+    });
+}
+
+void launch_seg_scatter(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
+                        const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets)
+{
+    with_seg_shape(shape, [&](auto sc) {
+        constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
+        if (rank == RANK_BALLOT)
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_BALLOT>), grid, dim3(T.block), 0, st, a, b, pass,
+                               sp, tinfo, offsets);
+        else if (rank == RANK_MIX2)
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX2>), grid, dim3(T.block), 0, st, a, b, pass,
+                               sp, tinfo, offsets);
+        else if (rank == RANK_MIX4)
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX4>), grid, dim3(T.block), 0, st, a, b, pass,
+                               sp, tinfo, offsets);
+        else if (rank == RANK_LDSOR)
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_LDSOR>), grid, dim3(T.block), 0, st, a, b, pass,
+                               sp, tinfo, offsets);
+        else
+            fail(HPCS_E_UNSUPPORTED, "unknown ranking mode " + std::to_string(rank));
+    });
 }
 
 void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_begin, int nseg, hipStream_t st)
@@ -937,25 +975,26 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     if (nseg == 0 || n <= 1) return;
     if (!d_keys || !d_begin) fail(HPCS_E_INVALID_ARG, "keys or segment offsets are null");
     if (n >= (int64_t(1) << 32)) fail(HPCS_E_UNSUPPORTED, "segments must lie below 2^32 keys");
-    const SegLayout L = seg_layout(n, nseg);
+    const SegParams prm = resolve_seg_params(c, n);
+    const int tile = shape_tile_keys(prm.shape);
+    const SegLayout L = seg_layout(n, nseg, tile);
     char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
     SegPlan *sp = reinterpret_cast<SegPlan *>(base + L.off_plan);
     uint32_t *tile0 = reinterpret_cast<uint32_t *>(base + L.off_tile0);
     uint32_t *small = reinterpret_cast<uint32_t *>(base + L.off_small);
     uint32_t *medium = reinterpret_cast<uint32_t *>(base + L.off_medium);
     uint32_t *large = reinterpret_cast<uint32_t *>(base + L.off_large);
-    const SegParams prm = resolve_seg_params(c, n);
     uint4 *tinfo = reinterpret_cast<uint4 *>(base + L.off_tinfo);
     uint16_t *counts = reinterpret_cast<uint16_t *>(base + L.off_counts);
     uint32_t *offsets = reinterpret_cast<uint32_t *>(base + L.off_offsets);
     uint32_t *keysB = reinterpret_cast<uint32_t *>(base + L.off_keysB);
     timed(c, st, "seg_plan", [&] {
         hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, d_begin, nseg, n, sp, tile0, small, medium,
-                           large, L.max_tiles, prm.small_max);
+                           large, L.max_tiles, prm.small_max, tile);
     });
     const int list_grid = std::min(nseg, c->num_cus * 8);
     timed(c, st, "seg_tiles", [&] {
-        hipLaunchKernelGGL(k_seg_tiles, dim3(list_grid), dim3(256), 0, st, d_begin, sp, tile0, large, tinfo);
+        hipLaunchKernelGGL(k_seg_tiles, dim3(list_grid), dim3(256), 0, st, d_begin, sp, tile0, large, tinfo, tile);
     });
     timed(c, st, "seg_small", [&] {
         hipLaunchKernelGGL((k_seg_small<SEG_BLOCK>), dim3(list_grid), dim3(SEG_BLOCK), 0, st, d_keys, d_begin, sp,
@@ -972,14 +1011,14 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     const dim3 tile_grid(unsigned(L.max_tiles));
     for (int p = 0; p < 4; ++p) {
         timed(c, st, "seg_counts", [&] {
-            hipLaunchKernelGGL(k_seg_counts, tile_grid, dim3(SEG_BLOCK), 0, st, d_keys, keysB, p, sp, tinfo, counts);
+            launch_seg_counts(prm.shape, tile_grid, st, d_keys, keysB, p, sp, tinfo, counts);
         });
         timed(c, st, "seg_scan", [&] {
             hipLaunchKernelGGL(k_seg_scan, dim3(list_grid), dim3(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS), 0, st, d_begin, sp,
-                               tile0, large, counts, offsets);
+                               tile0, large, counts, offsets, tile);
         });
         timed(c, st, "seg_scatter", [&] {
-            launch_seg_scatter(prm.rank, tile_grid, st, d_keys, keysB, p, sp, tinfo, offsets);
+            launch_seg_scatter(prm.shape, prm.rank, tile_grid, st, d_keys, keysB, p, sp, tinfo, offsets);
         });
     }
 }
@@ -1682,6 +1721,12 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             ctx->seg_rank = int(value);
             ctx->set_seg_rank = true;
             break;
+        case HPCS_OPT_SEG_SHAPE:
+            if (value < 0 || value >= NUM_TILE_SHAPES || !seg_shape_ok(int(value)))
+                fail(HPCS_E_INVALID_ARG, "the tile shape does not serve the batched segments");
+            ctx->seg_shape = int(value);
+            ctx->set_seg_shape = true;
+            break;
         case HPCS_OPT_SEG_SMALL:
             if (!seg_small_valid(value)) fail(HPCS_E_INVALID_ARG, "the LDS segment limit must be 4096 or 16384");
             ctx->seg_small = int(value);
@@ -1871,7 +1916,9 @@ HPCS_API int32_t hpcs_ctx_reserve_segments(hpcs_ctx *ctx, int64_t n, int32_t nse
     return guarded(ctx, [&] {
         if (n < 0 || nseg < 0) fail(HPCS_E_INVALID_ARG, "n and nseg must be non-negative");
         if (n >= (int64_t(1) << 32)) fail(HPCS_E_UNSUPPORTED, "segments must lie below 2^32 keys");
-        if (nseg > 0 && n > 1) ensure(ctx->scratch, ctx->scratch_cap, seg_layout(n, nseg).total);
+        // the smallest tile bounds the scratch of every call of up to n keys,
+        // whatever tile shape its size class takes
+        if (nseg > 0 && n > 1) ensure(ctx->scratch, ctx->scratch_cap, seg_layout(n, nseg, SEG_MIN_TILE).total);
     });
 }
 #endif

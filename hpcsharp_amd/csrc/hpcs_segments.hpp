@@ -10,14 +10,15 @@
 //                  small list, those of at most small_max (SEG_SMALL or
 //                  SEG_MEDIUM, a launch argument) to the medium list, both
 //                  sorted by k_seg_small; larger ones get a range of tiles of
-//                  TILE keys (a tile never crosses a segment)
+//                  `tile` keys (a launch argument: the tile shape's BLOCK x
+//                  ITEMS; a tile never crosses a segment)
 //   k_seg_tiles    one 16-byte record per tile (its keys, its segment)
 //   k_seg_small    one workgroup per small (256 threads) or medium (1024
 //                  threads) segment: the LSD passes entirely in LDS (one HBM
 //                  read and one write per key)
 //   per pass p of the large segments:
-//     k_seg_counts   per tile, the digit counts (16-bit: a tile holds at most
-//                    SEG_TILE keys)
+//     k_seg_counts   per tile, the digit counts (16-bit: a tile holds fewer
+//                    than 65536 keys)
 //     k_seg_scan     per segment, digit starts (segment base + exclusive scan
 //                    of the segment's digit totals) and per-tile offsets
 //                    (32-bit, positions within the segment)
@@ -29,10 +30,12 @@
 
 namespace hpcs {
 
+// The small list's workgroup (and the default tile of the large segments)
 constexpr int SEG_BLOCK = 256;
 constexpr int SEG_ITEMS = 16;
 constexpr int SEG_TILE = SEG_BLOCK * SEG_ITEMS; // 4096 keys
 constexpr int SEG_SMALL = SEG_TILE;             // largest segment of the small list (256 threads)
+constexpr int SEG_MIN_TILE = 2048;              // smallest tile of a shape that serves the large segments
 constexpr int SEG_MEDIUM_BLOCK = 1024;
 constexpr int SEG_MEDIUM = SEG_MEDIUM_BLOCK * SEG_ITEMS; // largest segment one workgroup sorts in LDS
 
@@ -194,7 +197,7 @@ static __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restr
                                                    uint32_t *__restrict__ small_list,
                                                    uint32_t *__restrict__ medium_list,
                                                    uint32_t *__restrict__ large_list, uint64_t max_tiles,
-                                                   int small_max)
+                                                   int small_max, int tile)
 {
     __shared__ uint64_t s_wsum[16];
     __shared__ int64_t s_wmax[16];
@@ -233,7 +236,7 @@ static __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restr
                 atomicAdd(&s_bad, 1u);
             } else {
                 const int64_t len = e - b;
-                if (len > small_max) tiles = uint32_t((len + SEG_TILE - 1) / SEG_TILE);
+                if (len > small_max) tiles = uint32_t((len + tile - 1) / tile);
                 else if (len > SEG_SMALL) medium = 1;
                 else if (len > 1) small = 1;
             }
@@ -241,7 +244,7 @@ static __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restr
         const uint32_t large = tiles ? 1u : 0u;
         // one block-wide exclusive scan of (tiles, small, large, medium)
         // packed in 64 bits: within a chunk of 1024 segments the tiles sum to
-        // at most n / SEG_TILE + 1024 < 2^21 (the sorted segments do not
+        // at most n / SEG_MIN_TILE + 1024 < 2^22 (the sorted segments do not
         // overlap and n < 2^32) and each list count to at most 1024 < 2^11
         constexpr int SH[4] = {0, 22, 33, 44};
         constexpr uint64_t MASK[4] = {(1ull << 22) - 1, (1ull << 11) - 1, (1ull << 11) - 1, (1ull << 11) - 1};
@@ -291,17 +294,17 @@ static __global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restr
                                                    const SegPlan *__restrict__ sp,
                                                    const uint32_t *__restrict__ seg_tile0,
                                                    const uint32_t *__restrict__ large_list,
-                                                   uint4 *__restrict__ tinfo)
+                                                   uint4 *__restrict__ tinfo, int tile)
 {
     const uint32_t nlarge = sp->num_large;
     for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
         const uint32_t s = large_list[k];
         const int64_t b = begin[s], len = begin[s + 1] - b;
         const uint32_t t0 = seg_tile0[s];
-        const uint32_t nt = uint32_t((len + SEG_TILE - 1) / SEG_TILE);
+        const uint32_t nt = uint32_t((len + tile - 1) / tile);
         for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
-            const int64_t tb = int64_t(t) * SEG_TILE;
-            tinfo[t0 + t] = make_uint4(uint32_t(b + tb), uint32_t(b), uint32_t(min(int64_t(SEG_TILE), len - tb)), s);
+            const int64_t tb = int64_t(t) * tile;
+            tinfo[t0 + t] = make_uint4(uint32_t(b + tb), uint32_t(b), uint32_t(min(int64_t(tile), len - tb)), s);
         }
     }
 }
@@ -383,15 +386,16 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(uint32_t *__restrict__ keys
     }
 }
 
-// One workgroup per tile of the large segments (the grid may exceed the tile
-// count; spare workgroups return): digit counts of pass `pass`.
-static __global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
-                                                          const uint32_t *__restrict__ keysB, int pass,
-                                                          const SegPlan *__restrict__ sp,
-                                                          const uint4 *__restrict__ tinfo,
-                                                          uint16_t *__restrict__ counts)
+// One workgroup per tile (BLOCK x ITEMS keys) of the large segments (the
+// grid may exceed the tile count; spare workgroups return): digit counts of
+// pass `pass`.
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
+                                                      const uint32_t *__restrict__ keysB, int pass,
+                                                      const SegPlan *__restrict__ sp, const uint4 *__restrict__ tinfo,
+                                                      uint16_t *__restrict__ counts)
 {
-    static_assert(SEG_TILE < 65536, "a tile's digit counts fit 16 bits");
+    static_assert(BLOCK * ITEMS < 65536, "a tile's digit counts fit 16 bits");
     __shared__ uint32_t sh[RADIX];
     const uint32_t t = blockIdx.x;
     if (t >= sp->num_tiles) return;
@@ -401,17 +405,17 @@ static __global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t 
     const uint4 ti = tinfo[t];
     const uint32_t tb = ti.x;
     const int cnt = int(ti.z);
-    sh[tid] = 0u;
+    for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
     __syncthreads();
-    uint32_t k[SEG_ITEMS];
+    uint32_t k[ITEMS];
 #pragma unroll
-    for (int j = 0; j < SEG_ITEMS; ++j) {
-        const int o = j * SEG_BLOCK + tid;
+    for (int j = 0; j < ITEMS; ++j) {
+        const int o = j * BLOCK + tid;
         k[j] = o < cnt ? kin[size_t(tb) + o] : 0u;
     }
 #pragma unroll
-    for (int j = 0; j < SEG_ITEMS; ++j) {
-        const bool valid = j * SEG_BLOCK + tid < cnt;
+    for (int j = 0; j < ITEMS; ++j) {
+        const bool valid = j * BLOCK + tid < cnt;
         const uint32_t d = (k[j] >> shift) & 0xFFu;
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
         if (__all(valid && d == d0)) {
@@ -421,7 +425,7 @@ static __global__ __launch_bounds__(SEG_BLOCK) void k_seg_counts(const uint32_t 
         }
     }
     __syncthreads();
-    counts[size_t(t) * RADIX + tid] = uint16_t(sh[tid]);
+    for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(t) * RADIX + i] = uint16_t(sh[i]);
 }
 
 // Grid-stride over the large segments, one workgroup each: digit starts of
@@ -437,7 +441,7 @@ static __global__ __launch_bounds__(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS) void k_seg
                                                                               const uint32_t *__restrict__ seg_tile0,
                                                                               const uint32_t *__restrict__ large_list,
                                                                               const uint16_t *__restrict__ counts,
-                                                                              uint32_t *__restrict__ offsets)
+                                                                              uint32_t *__restrict__ offsets, int tile)
 {
     static_assert(SEG_SCAN_PAIRS % WAVE == 0 && SEG_SCAN_PAIRS / WAVE == 2, "two waves of digit pairs");
     __shared__ uint32_t s_part[SEG_SCAN_GROUPS][RADIX];
@@ -452,7 +456,7 @@ static __global__ __launch_bounds__(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS) void k_seg
     for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
         const uint32_t s = large_list[k];
         const uint32_t t0 = seg_tile0[s];
-        const uint32_t nt = uint32_t((begin[s + 1] - begin[s] + SEG_TILE - 1) / SEG_TILE);
+        const uint32_t nt = uint32_t((begin[s + 1] - begin[s] + tile - 1) / tile);
         const uint32_t per = (nt + SEG_SCAN_GROUPS - 1) / SEG_SCAN_GROUPS;
         const uint32_t ta = min(nt, uint32_t(g) * per), tz = min(nt, ta + per); // this part's tiles
         uint32_t p0 = 0, p1 = 0;
@@ -512,13 +516,14 @@ static __global__ __launch_bounds__(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS) void k_seg
 // LDS atomics first and stages while it ranks, RANK_MIX2 / RANK_MIX4 match
 // every second / fourth item by ballots, RANK_BALLOT ranks by ballots with
 // wave-local counters and stages after the scan).
-template <int RANK>
-__global__ __launch_bounds__(SEG_BLOCK) void k_seg_scatter(uint32_t *__restrict__ keysA, uint32_t *__restrict__ keysB,
-                                                           int pass, const SegPlan *__restrict__ sp,
-                                                           const uint4 *__restrict__ tinfo,
-                                                           const uint32_t *__restrict__ offsets)
+template <int BLOCK, int ITEMS, int RANK>
+__global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ keysA, uint32_t *__restrict__ keysB,
+                                                       int pass, const SegPlan *__restrict__ sp,
+                                                       const uint4 *__restrict__ tinfo,
+                                                       const uint32_t *__restrict__ offsets)
 {
-    constexpr int BLOCK = SEG_BLOCK, ITEMS = SEG_ITEMS;
+    static_assert(BLOCK >= RADIX && BLOCK % WAVE == 0 && BLOCK <= 512, "one thread per digit, at most 8 waves");
+    constexpr int TILE = BLOCK * ITEMS;
     constexpr int WAVES = BLOCK / WAVE;
     constexpr int WAVE_TILE = WAVE * ITEMS;
     constexpr int SCAN_WAVES = RADIX / WAVE;
@@ -526,7 +531,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_scatter(uint32_t *__restrict_
     __shared__ uint64_t s_match[RANK != RANK_BALLOT ? WAVES : 1][RANK != RANK_BALLOT ? RADIX : 1];
     __shared__ uint32_t s_scan[SCAN_WAVES];
     __shared__ uint32_t s_base[RADIX];
-    __shared__ uint32_t s_keys[SEG_TILE];
+    __shared__ uint32_t s_keys[TILE];
     const uint32_t ntiles = sp->num_tiles;
     if (blockIdx.x >= ntiles) return;
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
@@ -561,7 +566,7 @@ __global__ __launch_bounds__(SEG_BLOCK) void k_seg_scatter(uint32_t *__restrict_
 
     // ---- per-wave digit counts (ballot ranking: the ranks themselves) ----
     uint32_t *cnt = s_cnt[wave];
-    const bool full = tile_valid == SEG_TILE;
+    const bool full = tile_valid == TILE;
     uint32_t rank2[RANK == RANK_BALLOT ? (ITEMS + 1) / 2 : 1];
     if constexpr (RANK == RANK_BALLOT) {
         auto rank_loop = [&](auto full_c) {

@@ -56,6 +56,10 @@ constexpr bool shape_serves(int s, int key_bytes, bool vals)
 
 constexpr int shape_tile_keys(int s) { return TILE_SHAPES[s].block * TILE_SHAPES[s].items; }
 
+// Shapes of the batched segments' large-segment tiles: the 256- and
+// 512-thread rows for 4-byte keys without values (LDS-OR or ballot ranking).
+constexpr bool seg_shape_ok(int s) { return shape_serves(s, 4, false) && TILE_SHAPES[s].block <= 512; }
+
 static_assert(shape_serves(0, 4, false) && shape_serves(0, 4, true) && shape_serves(0, 8, false) &&
                   shape_serves(0, 8, true),
               "shape 0 is the default of every sort");

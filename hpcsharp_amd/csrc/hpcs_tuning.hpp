@@ -1,7 +1,7 @@
 // hpcs_tuning.hpp -- tuning tables of libhpcsort.so: one row per line,
 //   <dtype> <max_n> <algo> <shape> <rank> [<digit_bits>]
 // or, for the batched segmented sort,
-//   segs <max_n> <small_max> <rank>
+//   segs <max_n> <small_max> <rank> [<shape>]
 // Plain C++ (no device code), so the parser is tested on the host with
 // sanitizers.
 #pragma once
@@ -17,8 +17,9 @@ namespace hpcs {
 // One row of a tuning table: for sorts of up to max_n keys of key_bytes
 // (with or without u32 values), the pass algorithm, tile shape, ranking and
 // digit width.  A `segs` row (key_bytes 0): for batched segmented sorts of
-// up to max_n keys in all, the largest segment sorted in LDS (seg_small) and
-// the ranking of the large segments' scatter (rank).
+// up to max_n keys in all, the largest segment sorted in LDS (seg_small), the
+// ranking of the large segments' scatter (rank) and their tile shape (shape,
+// a row of TILE_SHAPES with seg_shape_ok; 0 if the row omits it).
 struct TuneEntry {
     int key_bytes;
     int vals;
@@ -64,12 +65,16 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
         if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
         if (std::sscanf(line.c_str(), " %15s", dt) == 1 && std::string(dt) == "segs") {
             long long small_max = 0;
-            const int got = std::sscanf(line.c_str(), " %15s %lld %lld %d %n", dt, &max_n, &small_max, &rank, &consumed);
+            int consumed5 = 0;
+            const int got = std::sscanf(line.c_str(), " %15s %lld %lld %d %n%d %n", dt, &max_n, &small_max, &rank,
+                                        &consumed, &shape, &consumed5);
+            if (got == 5) consumed = consumed5;
             if (got < 4 || line.find_first_not_of(" \t\r", size_t(consumed)) != std::string::npos) {
-                err = "line " + std::to_string(line_no) + ": expected segs <max_n> <small_max> <rank>";
+                err = "line " + std::to_string(line_no) + ": expected segs <max_n> <small_max> <rank> [<shape>]";
                 return false;
             }
-            if (max_n < 1 || !seg_small_valid(small_max) || rank < 0 || rank > 3) {
+            if (got < 5) shape = 0;
+            if (max_n < 1 || !seg_small_valid(small_max) || rank < 0 || rank > 3 || !seg_shape_ok(shape)) {
                 err = "line " + std::to_string(line_no) + ": value out of range";
                 return false;
             }
@@ -77,6 +82,7 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
             e.key_bytes = 0;
             e.max_n = max_n;
             e.rank = rank;
+            e.shape = shape;
             e.bits = 8;
             e.seg_small = int(small_max);
             out.push_back(e);

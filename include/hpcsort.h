@@ -100,6 +100,10 @@ extern "C" {
 #define HPCS_OPT_SEG_RANK 13       /* hpcs_[dev_]sort_segments_u32: ranking of the large segments' scatter
                                       (as HPCS_OPT_RANK: 0 ballots, 1 LDS-OR, 2 / 3 LDS-OR with every second /
                                       fourth item by ballots); default from the tuning table, else 2 */
+#define HPCS_OPT_SEG_SHAPE 15      /* hpcs_[dev_]sort_segments_u32: tile shape of the large segments' passes, a
+                                      row of the shape table with 256 or 512 threads that serves 4-byte keys
+                                      without values (0-4, 7-9; see HPCS_OPT_TILE_SHAPE); default from the
+                                      tuning table, else 0 (256x16) */
 #define HPCS_OPT_SEG_SMALL 14      /* hpcs_[dev_]sort_segments_u32: segments of at most this many keys are
                                       sorted by one workgroup in LDS (one read and one write per key): 4096 or
                                       16384; default from the tuning table, else 4096 */

@@ -102,9 +102,12 @@ static int tuning_test()
     CHECK(hpcs::parse_tuning("segs 1000 4096 1 # lds\nu32 9 1 0 3\n segs 268435456 16384 3\r", rows, err));
     CHECK(rows.size() == 3 && rows[0].key_bytes == 0 && rows[0].seg_small == 4096 && rows[0].rank == 1);
     CHECK(rows[1].key_bytes == 4 && rows[2].key_bytes == 0 && rows[2].max_n == 268435456 &&
-          rows[2].seg_small == 16384 && rows[2].rank == 3);
+          rows[2].seg_small == 16384 && rows[2].rank == 3 && rows[0].shape == 0 && rows[2].shape == 0);
+    CHECK(hpcs::parse_tuning("segs 10 4096 2 9\nsegs 20 16384 0 3  # 512x32", rows, err) && rows.size() == 2);
+    CHECK(rows[0].shape == 9 && rows[1].shape == 3 && rows[1].rank == 0);
     const char *bad_segs[] = {"segs 1 8192 1", "segs 0 4096 1", "segs 1 4096 4", "segs 1 4096 -1", "segs 1 4096",
-                              "segs 1 4096 1 0", "segs 1 16384 x", "segs -3 4096 0"};
+                              "segs 1 4096 1 5", "segs 1 4096 1 6", "segs 1 4096 1 10", "segs 1 4096 1 0 0",
+                              "segs 1 16384 x", "segs -3 4096 0", "segs 1 4096 1 x"};
     for (const char *b : bad_segs) {
         CHECK(!hpcs::parse_tuning(b, rows, err));
         CHECK(err.rfind("line 1", 0) == 0);
