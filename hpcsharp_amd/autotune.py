@@ -43,13 +43,16 @@ SEG_SMALL = (4096, 16384)  # largest segment sorted in LDS (HPCS_OPT_SEG_SMALL)
 SEG_SHAPES = (0, 1, 2, 3, 4, 7, 8, 9)  # tile shapes of the large segments (HPCS_OPT_SEG_SHAPE)
 
 
-# largest arrays the one-workgroup sort (pass algorithm 3) takes
+# largest arrays the one-workgroup sort (pass algorithm 3) and the
+# block-sort + merge form (pass algorithm 5) take
 BLOCK_SORT_MAX = {4: 16384, 8: 8192}
+MERGE_SORT_MAX = 1 << 20
 
 
 def candidates(key_bytes: int, vals: bool, n: int | None = None):
     """8-bit digits: every pass algorithm x tile shape x ranking, and the
-    one-workgroup sort where n fits it (not the MSD split, pass algorithm 4:
+    one-workgroup sort where n fits it, the block-sort + merge form from
+    there up to 2^20 keys (not the MSD split, pass algorithm 4:
     its speed depends on how the keys spread, which uniform keys do not
     show); 11-bit digits (keys without values):
     their one form, onesweep passes with 256x16 tiles and ballot ranking."""
@@ -60,6 +63,8 @@ def candidates(key_bytes: int, vals: bool, n: int | None = None):
         out.append({"algo": 0, "shape": 0, "rank": 0, "bits": 11})
     if n is not None and n <= BLOCK_SORT_MAX[key_bytes]:
         out.append({"algo": 3, "shape": 0, "rank": 1, "bits": 8})
+    if n is not None and BLOCK_SORT_MAX[key_bytes] < n <= MERGE_SORT_MAX:
+        out.append({"algo": 5, "shape": 0, "rank": 1, "bits": 8})
     return out
 
 

@@ -73,6 +73,10 @@ constexpr int PASS_RTS = 1;
 constexpr int PASS_AUTO = 2;
 constexpr int PASS_BLOCK = 3; // one workgroup sorts the whole array in LDS
 constexpr int PASS_MSD = 4;   // one MSD split, then one workgroup per bucket (keys without values)
+constexpr int PASS_MERGE = 5; // runs sorted by one workgroup each in LDS, then stable merge passes
+// largest arrays the block-sort + merge form serves (beyond, the passes over
+// the whole array outnumber the radix passes)
+constexpr int64_t MERGE_SORT_MAX = int64_t(1) << 20;
 // largest arrays the MSD split serves (beyond, most buckets outgrow a tile)
 constexpr int64_t MSD_MAX4 = int64_t(1) << 22, MSD_MAX8 = int64_t(1) << 21;
 // largest arrays the one-workgroup sort takes (LDS: keys, values, counters)
@@ -482,6 +486,7 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
     const int64_t block_max = key_bytes == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8;
     if (p.algo == PASS_BLOCK && n > block_max) p.algo = PASS_AUTO;
     if (p.algo == PASS_MSD && (vals || n > (key_bytes == 4 ? MSD_MAX4 : MSD_MAX8))) p.algo = PASS_AUTO;
+    if (p.algo == PASS_MERGE && n > MERGE_SORT_MAX) p.algo = PASS_AUTO;
     // by size, measured on MI355X: one workgroup up to 16384 4-byte / 8192
     // 8-byte keys (with LDS-OR ranking it beats onesweep passes there: u32
     // 16384 keys 42.2 against 48.4 us)
@@ -508,11 +513,54 @@ void run_sort_block(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStre
     timed(c, st, "block_sort", [&] {
         if (n <= 256 * ITEMS)
             hipLaunchKernelGGL((k_sort_block<K, FLOATK, HAS_VALS, 256, ITEMS>), dim3(1), dim3(256), 0, st, d_keys,
-                               d_vals, uint32_t(n));
+                               d_vals, d_keys, d_vals, uint32_t(n));
         else
             hipLaunchKernelGGL((k_sort_block<K, FLOATK, HAS_VALS, 1024, ITEMS>), dim3(1), dim3(1024), 0, st, d_keys,
-                               d_vals, uint32_t(n));
+                               d_vals, d_keys, d_vals, uint32_t(n));
     });
+}
+
+// The block-sort + merge form (n <= MERGE_SORT_MAX): k_sort_block with one
+// workgroup per run of MERGE_RUN keys (a 256-thread tile) sorts every run in
+// LDS, then k_merge_runs merges pairs of runs until one is left, ping-pong
+// between the caller's buffer and n keys (and values) of scratch; the block
+// sort writes where the passes must start so that the last pass lands in the
+// caller's buffer.  1 + ceil(log2(runs)) launches, no counting pass, no plan.
+template <typename K> constexpr int merge_run_items() { return sizeof(K) == 4 ? 16 : 8; }
+template <typename K> constexpr int merge_run_keys() { return 256 * merge_run_items<K>(); }
+static_assert(256 * 8 % MERGE_TILE == 0, "a run is a multiple of the merge tile");
+
+size_t merge_sort_scratch_bytes(int64_t n, size_t key_size, bool vals)
+{
+    return align_up(size_t(n) * key_size) + (vals ? align_up(size_t(n) * sizeof(uint32_t)) : 0);
+}
+
+template <typename K, bool HAS_VALS, int FLOATK>
+void run_sort_merge(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t st)
+{
+    constexpr int RUN = merge_run_keys<K>();
+    const int64_t runs = (n + RUN - 1) / RUN;
+    int merges = 0;
+    while ((int64_t(RUN) << merges) < n) ++merges;
+    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, merge_sort_scratch_bytes(n, sizeof(K), HAS_VALS)));
+    K *kb = reinterpret_cast<K *>(base);
+    uint32_t *vb = HAS_VALS ? reinterpret_cast<uint32_t *>(base + align_up(size_t(n) * sizeof(K))) : nullptr;
+    // after an odd number of merge passes the runs must start in scratch
+    K *ks = (merges & 1) ? kb : d_keys, *kd = (merges & 1) ? d_keys : kb;
+    uint32_t *vs = (merges & 1) ? vb : d_vals, *vd = (merges & 1) ? d_vals : vb;
+    timed(c, st, "block_sort", [&] {
+        hipLaunchKernelGGL((k_sort_block<K, FLOATK, HAS_VALS, 256, merge_run_items<K>()>), dim3(unsigned(runs)),
+                           dim3(256), 0, st, d_keys, d_vals, ks, vs, uint32_t(n));
+    });
+    const unsigned grid = unsigned((n + MERGE_TILE - 1) / MERGE_TILE);
+    for (int64_t run = RUN; run < n; run *= 2) {
+        timed(c, st, "merge", [&] {
+            hipLaunchKernelGGL((k_merge_runs<K, HAS_VALS, FLOATK>), dim3(grid), dim3(MERGE_BLOCK), 0, st, ks, vs, kd,
+                               vd, uint64_t(n), uint64_t(run));
+        });
+        std::swap(ks, kd);
+        std::swap(vs, vd);
+    }
 }
 
 // Scratch of an 11-bit sort:
@@ -653,11 +701,12 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         // caller-provided scratch: the 8-bit passes (whose layout is
         // make_layout's), never the 11-bit or MSD-split layouts
         prm.bits = 8;
-        if (prm.algo == PASS_MSD)
+        if (prm.algo == PASS_MSD || prm.algo == PASS_MERGE)
             prm.algo = n <= (sizeof(K) == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8) ? PASS_BLOCK
                        : n < (sizeof(K) == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21)) ? PASS_ONESWEEP
                                                                                          : PASS_RTS;
     }
+    if (prm.algo == PASS_MERGE) return run_sort_merge<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
     if (prm.algo == PASS_BLOCK) {
         static_assert(BLOCK_SORT_MAX4 == 1024 * 16 && BLOCK_SORT_MAX8 == 1024 * 8, "k_sort_block sizes");
         return run_sort_block<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
@@ -784,10 +833,11 @@ size_t sort_scratch_bytes(const hpcs_ctx *c, int64_t n, bool ext)
     SortParams prm = resolve_params(c, int(sizeof(K)), HAS_VALS, n);
     if (ext) {
         prm.bits = 8;
-        if (prm.algo == PASS_MSD)
+        if (prm.algo == PASS_MSD || prm.algo == PASS_MERGE)
             prm.algo = n <= (sizeof(K) == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8) ? PASS_BLOCK : PASS_RTS;
     }
     if (prm.algo == PASS_BLOCK) return 0;
+    if (prm.algo == PASS_MERGE) return merge_sort_scratch_bytes(n, sizeof(K), HAS_VALS);
     if (!HAS_VALS && prm.bits == 11) return make_layout11(n, sizeof(K)).total;
     return make_layout(n, sizeof(K), HAS_VALS, shape_tile_keys(prm.shape),
                        prm.algo != PASS_MSD && digit_bytes_on(c, sizeof(K), HAS_VALS))
@@ -1698,7 +1748,7 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             break;
         case HPCS_OPT_PASS_ALGO:
             if (value != PASS_ONESWEEP && value != PASS_RTS && value != PASS_AUTO && value != PASS_BLOCK &&
-                value != PASS_MSD)
+                value != PASS_MSD && value != PASS_MERGE)
                 fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0, 1, 2, 3 or 4");
             ctx->pass_algo = int(value);
             ctx->set_algo = value != PASS_AUTO;
@@ -1842,6 +1892,20 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
         return HPCS_OK;
     }
     const int passes = int(ks * 8 / RADIX_BITS);
+    if (prm.algo == PASS_MERGE) {
+        // one read and one write per key for the runs, then per merge pass
+        const int64_t run = ks == 4 ? 4096 : 2048;
+        int merges = 0;
+        while ((run << merges) < n) ++merges;
+        info->digit_bits = RADIX_BITS;
+        info->passes = passes;
+        info->block_threads = 256;
+        info->tile_keys = run;
+        info->tiles = (n + run - 1) / run;
+        info->ideal_bytes = n * int64_t(1 + merges) * 2 * (int64_t(ks) + (with_values ? 4 : 0));
+        info->scratch_bytes = int64_t(merge_sort_scratch_bytes(n, ks, with_values != 0));
+        return HPCS_OK;
+    }
     if (prm.algo == PASS_BLOCK) {
         const int items = ks == 4 ? 16 : 8;
         info->digit_bits = RADIX_BITS;

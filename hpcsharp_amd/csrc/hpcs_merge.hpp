@@ -100,6 +100,57 @@ __global__ __launch_bounds__(MERGE_BLOCK) void k_merge(const K *__restrict__ s, 
     }
 }
 
+// One pass of the block-sort + merge sort (n up to a few times 10^5 keys,
+// where launches, not bytes, set the time): src holds sorted runs of `run`
+// keys (the last one may be shorter), and the runs of each pair (2p, 2p + 1)
+// are merged into dst, stably (the left run's key first among equal keys),
+// values (HAS_VALS) moving with their keys.  Workgroup w produces outputs
+// [w * MERGE_TILE, ...), which lie in one pair since 2 * run is a multiple of
+// MERGE_TILE; a last run without a partner is copied.  Keys compare by their
+// radix key (KIND), so the order is the radix sort's.
+template <typename K, bool HAS_VALS, int KIND>
+__global__ __launch_bounds__(MERGE_BLOCK) void k_merge_runs(const K *__restrict__ src,
+                                                            const uint32_t *__restrict__ vsrc, K *__restrict__ dst,
+                                                            uint32_t *__restrict__ vdst, uint64_t n, uint64_t run)
+{
+    __shared__ K t[MERGE_TILE];
+    __shared__ uint32_t tv[HAS_VALS ? MERGE_TILE : 1];
+    __shared__ uint64_t s_cut[2];
+    const uint64_t ds = uint64_t(blockIdx.x) * MERGE_TILE;
+    if (ds >= n) return; // workgroup-uniform
+    const uint64_t ps = ds / (2 * run) * (2 * run); // the pair's first key
+    const uint64_t na = min(run, n - ps);
+    const uint64_t nb = ps + na < n ? min(run, n - ps - na) : 0;
+    const K *a = src + ps, *b = src + ps + na;
+    const uint64_t d0 = ds - ps, d1 = min(d0 + uint64_t(MERGE_TILE), na + nb);
+    if (threadIdx.x < 2) s_cut[threadIdx.x] = merge_path_split<K, KIND>(a, na, b, nb, threadIdx.x ? d1 : d0);
+    __syncthreads();
+    const uint64_t ia = s_cut[0], ib = s_cut[1];
+    const int la = int(ib - ia), lb = int((d1 - ib) - (d0 - ia)); // la + lb = d1 - d0 <= MERGE_TILE
+    const uint64_t ja = d0 - ia;
+    for (int q = threadIdx.x; q < la + lb; q += MERGE_BLOCK) {
+        const uint64_t at = q < la ? ps + ia + uint64_t(q) : ps + na + ja + uint64_t(q - la);
+        t[q] = src[at];
+        if constexpr (HAS_VALS) tv[q] = vsrc[at];
+    }
+    __syncthreads();
+    const int len = la + lb;
+    const int dt = min(int(threadIdx.x) * MERGE_ITEMS, len);
+    int lo = dt > lb ? dt - lb : 0, hi = dt < la ? dt : la;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (key_enc<K, KIND>(t[mid]) <= key_enc<K, KIND>(t[la + dt - mid - 1])) lo = mid + 1;
+        else hi = mid;
+    }
+    int i = lo, j = dt - lo;
+    for (int k = 0; k < MERGE_ITEMS && dt + k < len; ++k) {
+        const bool take_a = j >= lb || (i < la && key_enc<K, KIND>(t[i]) <= key_enc<K, KIND>(t[la + j]));
+        const int q = take_a ? i++ : la + j++;
+        dst[ds + uint64_t(dt + k)] = t[q];
+        if constexpr (HAS_VALS) vdst[ds + uint64_t(dt + k)] = tv[q];
+    }
+}
+
 // dst[0, n) = src[0, n) (different buffers).
 template <typename K>
 __global__ void k_copy_keys(K *__restrict__ dst, const K *__restrict__ src, uint64_t n)

@@ -692,19 +692,28 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
 }
 
 // ---------------------------------------------------------------------------
-// k_sort_block: one workgroup sorts a whole array of n <= BLOCK * ITEMS keys
-// (and u32 values) in LDS: one read, the LSD passes over the varying digit
+// k_sort_block: workgroup b sorts its chunk [b * TILE, min(n, (b + 1) * TILE))
+// of keys_in (and u32 values) in LDS into the same place of keys_out (which
+// may be keys_in): one read, the LSD passes over the chunk's varying digit
 // positions (tile_rank_stage per pass, the float / signed key transform on
-// load and its inverse on store), one write.  One launch replaces the
-// count / plan / pass / copy-back sequence of the multi-tile sorts.
+// load and its inverse on store), one write.  With one workgroup (n <=
+// BLOCK * ITEMS) one launch replaces the count / plan / pass / copy-back
+// sequence of the multi-tile sorts; with many, it makes the sorted runs of
+// the block-sort + merge sort (hpcs_merge.hpp).
 // ---------------------------------------------------------------------------
 template <typename K, int FLOATK, bool HAS_VALS, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_sort_block(K *__restrict__ keys, uint32_t *__restrict__ vals, uint32_t n)
+__global__ __launch_bounds__(BLOCK) void k_sort_block(const K *keys_in, const uint32_t *vals_in, K *keys_out,
+                                                      uint32_t *vals_out, uint32_t n_all)
 {
     constexpr int WAVES = BLOCK / WAVE;
     constexpr int WAVE_TILE = WAVE * ITEMS;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int P = int(sizeof(K));
+    // this workgroup's chunk (the grid covers n_all; no workgroup is spare)
+    const uint32_t c0 = blockIdx.x * uint32_t(TILE);
+    const uint32_t n = min(uint32_t(TILE), n_all - c0);
+    const K *keys = keys_in + c0;
+    const uint32_t *vals = HAS_VALS ? vals_in + c0 : nullptr;
     // LDS-OR match ranking where its masks fit in LDS beside the keys and
     // values (measured on MI355X against ballot ranking: u32 4096 keys 25.4
     // -> 22.4 us, 16384 48.8 -> 42.4 us, u64 8192 53.5 -> 48.7 us)
@@ -772,10 +781,22 @@ __global__ __launch_bounds__(BLOCK) void k_sort_block(K *__restrict__ keys, uint
                                                        s_vals);
         staged = true;
     }
+    K *kout = keys_out + c0;
+    uint32_t *vout = HAS_VALS ? vals_out + c0 : nullptr;
     if (staged) {
         for (int j = tid; j < len; j += BLOCK) {
-            keys[j] = FLOATK ? key_dec<K, FLOATK>(s_keys[j]) : s_keys[j];
-            if constexpr (HAS_VALS) vals[j] = s_vals[j];
+            kout[j] = FLOATK ? key_dec<K, FLOATK>(s_keys[j]) : s_keys[j];
+            if constexpr (HAS_VALS) vout[j] = s_vals[j];
+        }
+    } else if (keys_out != keys_in) {
+        // every key of the chunk is equal (no pass ran): the chunk as it was
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int o = j * WAVE + lane;
+            if (o < wave_valid) {
+                kout[wave * WAVE_TILE + o] = FLOATK ? key_dec<K, FLOATK>(v[j]) : v[j];
+                if constexpr (HAS_VALS) vout[wave * WAVE_TILE + o] = pv[j];
+            }
         }
     }
 }

@@ -40,7 +40,9 @@ inline bool seg_small_valid(long long v) { return v == 4096 || v == 16384; }
 // 3 one workgroup in LDS (arrays of at most 16384 4-byte or 8192 8-byte keys;
 // larger arrays fall back to the choice by size) / 4 one MSD split, then one
 // workgroup per bucket (keys without values, at most 2^22 4-byte or 2^21
-// 8-byte keys; larger arrays fall back to the choice by size);
+// 8-byte keys; larger arrays fall back to the choice by size) / 5 runs of
+// 4096 4-byte or 2048 8-byte keys sorted in LDS, then stable merge passes
+// (arrays of at most 2^20 keys; larger ones fall back to the choice by size);
 // shape: a row of TILE_SHAPES (hpcs_shapes.hpp) that serves the dtype; rank
 // 0 ballot / 1 LDS-OR / 2 and 3 LDS-OR with every second / fourth item by
 // ballot (0 only for ballot-only shapes); digit_bits 8 (default) or 11.  11-bit digits exist in one form
@@ -105,7 +107,7 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
             err = "line " + std::to_string(line_no) + ": unknown dtype " + d;
             return false;
         }
-        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3 && algo != 4) || rank < 0 || rank > 3 ||
+        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3 && algo != 4 && algo != 5) || rank < 0 || rank > 3 ||
             !shape_serves(shape, e.key_bytes, e.vals != 0) || (rank != 0 && TILE_SHAPES[shape].ballot_only) ||
             (bits != 8 && bits != 11) ||
             (bits == 11 && (algo != 0 || shape != 0 || rank != 0 || e.vals)) || (algo == 4 && e.vals)) {

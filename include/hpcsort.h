@@ -74,7 +74,10 @@ extern "C" {
                                       other sorts: by size).  Opt-in only: for keys spread over their range
                                       it takes about half the time of 2: for 8-byte keys from 2^14 to 2^20,
                                       but keys clustered in one bucket (a bucket larger than a workgroup's
-                                      tile is sorted by that one workgroup) cost 4-60x */
+                                      tile is sorted by that one workgroup) cost 4-60x;
+                                      5: runs of 4096 4-byte / 2048 8-byte keys sorted in LDS by one
+                                      workgroup each, then stable merge passes (up to 2^20 keys; larger
+                                      arrays: by size) */
 #define HPCS_OPT_DIGIT_BITS 8      /* 8 (default) or 11: 11-bit digits (3 passes for 4-byte keys, 6 for 8-byte
                                       keys) in the onesweep form, for sorts of keys without values below 2^30
                                       keys; other sorts keep 8-bit digits */
