@@ -160,6 +160,10 @@ struct hpcs_ctx {
     size_t scratch_cap = 0;
     void *io = nullptr;
     size_t io_cap = 0;
+    // record and item sorts: [keys][indices][copy of the records], grown on
+    // demand like the scratch
+    void *work = nullptr;
+    size_t work_cap = 0;
     int rank_mode = RANK_LDSOR;  // tuning knob, env HPCS_RANK=ballot|ldsor|mix2|mix4
     int tile_shape = 0;          // tuning knob, env HPCS_TILE_SHAPE: a row of TILE_SHAPES
     int fallback_rounds = FALLBACK_ROUNDS;
@@ -1225,45 +1229,49 @@ template <typename K>
 void run_records(hpcs_ctx *c, void *d_recs, int64_t n, int32_t rec_size, int32_t key_offset, int32_t key_kind,
                  hipStream_t st)
 {
-    // Work buffer [keys n][idx n][gathered records n*rec_size], stream-ordered
-    // allocation; the sort's own scratch is the context's.
+    // The context's work buffer holds [keys n][idx n][copy of the records];
+    // the sort's own scratch is the context's scratch.  The records are
+    // gathered from the copy straight into place.
     const size_t off_idx = align_up(size_t(n) * sizeof(K));
-    const size_t off_out = align_up(off_idx + size_t(n) * sizeof(uint32_t));
-    const size_t need = align_up(off_out + size_t(n) * size_t(rec_size));
-    void *work = nullptr;
-    hip_check(hipMallocAsync(&work, need, st), "hipMallocAsync");
-    char *w = static_cast<char *>(work);
+    const size_t off_copy = align_up(off_idx + size_t(n) * sizeof(uint32_t));
+    char *w = static_cast<char *>(ensure(c->work, c->work_cap, off_copy + size_t(n) * size_t(rec_size)));
     K *keys = reinterpret_cast<K *>(w);
     uint32_t *idx = reinterpret_cast<uint32_t *>(w + off_idx);
-    unsigned char *out = reinterpret_cast<unsigned char *>(w + off_out);
+    unsigned char *copy = reinterpret_cast<unsigned char *>(w + off_copy);
+    const int words = gather_mode(d_recs, rec_size);
     const int grid = grid_for(uint64_t(n), 256, c->num_cus * 8);
-    hipLaunchKernelGGL((k_extract_keys<K>), dim3(grid), dim3(256), 0, st,
-                       static_cast<const unsigned char *>(d_recs), uint64_t(n), rec_size, key_offset,
-                       key_kind, keys, idx);
+    // 16-byte vector records are copied by the extraction, which reads their
+    // lines anyway; other records by a device copy
+    if (words != 2)
+        hip_check(hipMemcpyAsync(copy, d_recs, size_t(n) * size_t(rec_size), hipMemcpyDeviceToDevice, st),
+                  "hipMemcpyAsync records");
+    timed(c, st, "extract", [&] {
+        hipLaunchKernelGGL((k_extract_keys<K>), dim3(grid), dim3(256), 0, st,
+                           static_cast<const unsigned char *>(d_recs), uint64_t(n), rec_size, key_offset, key_kind,
+                           keys, idx, words == 2 ? reinterpret_cast<u32x4 *>(copy) : static_cast<u32x4 *>(nullptr));
+    });
     hip_check(hipGetLastError(), "k_extract_keys launch");
     run_sort<K, true, false>(c, keys, idx, n, st);
-    const int words = gather_mode(d_recs, rec_size);
-    hipLaunchKernelGGL(k_gather_records, dim3(grid_for(uint64_t(n) * uint64_t(rec_size) / 4 + 1, 256, c->num_cus * 8)),
-                       dim3(256), 0, st, static_cast<const unsigned char *>(d_recs), out, idx, uint64_t(n), rec_size,
-                       words);
+    timed(c, st, "gather", [&] {
+        hipLaunchKernelGGL(k_gather_records,
+                           dim3(grid_for(uint64_t(n) * uint64_t(rec_size) / 4 + 1, 256, c->num_cus * 8)), dim3(256),
+                           0, st, copy, static_cast<unsigned char *>(d_recs), idx, uint64_t(n), rec_size, words);
+    });
     hip_check(hipGetLastError(), "k_gather_records launch");
-    hip_check(hipMemcpyAsync(d_recs, out, size_t(n) * size_t(rec_size), hipMemcpyDeviceToDevice, st),
-              "hipMemcpyAsync records");
-    hip_check(hipFreeAsync(work, st), "hipFreeAsync");
 }
 
 // Items sorted by a separate key array (the keys are sorted alongside):
 // (key, index) pairs through the stable sort, then the items gathered by
-// index into a work buffer and copied back.
+// index from a copy (the context's work buffer) back into place.
 template <typename K>
 void run_keys_items(hpcs_ctx *c, K *d_keys, void *d_items, int64_t n, int32_t item_size, hipStream_t st)
 {
-    const size_t off_out = align_up(size_t(n) * sizeof(uint32_t));
-    const size_t need = align_up(off_out + size_t(n) * size_t(item_size));
-    void *work = nullptr;
-    hip_check(hipMallocAsync(&work, need, st), "hipMallocAsync");
-    uint32_t *idx = static_cast<uint32_t *>(work);
-    unsigned char *out = static_cast<unsigned char *>(work) + off_out;
+    const size_t off_copy = align_up(size_t(n) * sizeof(uint32_t));
+    char *w = static_cast<char *>(ensure(c->work, c->work_cap, off_copy + size_t(n) * size_t(item_size)));
+    uint32_t *idx = reinterpret_cast<uint32_t *>(w);
+    unsigned char *copy = reinterpret_cast<unsigned char *>(w + off_copy);
+    hip_check(hipMemcpyAsync(copy, d_items, size_t(n) * size_t(item_size), hipMemcpyDeviceToDevice, st),
+              "hipMemcpyAsync items");
     hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(uint64_t(n), 256, c->num_cus * 8)), dim3(256), 0, st, idx,
                        uint64_t(n));
     hip_check(hipGetLastError(), "k_iota_u32 launch");
@@ -1271,11 +1279,8 @@ void run_keys_items(hpcs_ctx *c, K *d_keys, void *d_items, int64_t n, int32_t it
     const int words = gather_mode(d_items, item_size);
     hipLaunchKernelGGL(k_gather_records,
                        dim3(grid_for(uint64_t(n) * uint64_t(item_size) / 4 + 1, 256, c->num_cus * 8)), dim3(256), 0,
-                       st, static_cast<const unsigned char *>(d_items), out, idx, uint64_t(n), item_size, words);
+                       st, copy, static_cast<unsigned char *>(d_items), idx, uint64_t(n), item_size, words);
     hip_check(hipGetLastError(), "k_gather_records launch");
-    hip_check(hipMemcpyAsync(d_items, out, size_t(n) * size_t(item_size), hipMemcpyDeviceToDevice, st),
-              "hipMemcpyAsync items");
-    hip_check(hipFreeAsync(work, st), "hipFreeAsync");
 }
 
 void check_keys_items_args(int64_t n, int32_t key_bits, int32_t item_size)
@@ -1500,7 +1505,7 @@ void dev_histogram_records(hpcs_ctx *c, const void *d_recs, int64_t n, int32_t r
             timed(c, st, "extract", [&] {
                 hipLaunchKernelGGL((k_extract_keys<uint32_t>), dim3(grid), dim3(256), 0, st, recs, uint64_t(n), rec_size,
                                    key_offset, int(HPCS_KEY_UINT), reinterpret_cast<uint32_t *>(base),
-                                   static_cast<uint32_t *>(nullptr));
+                                   static_cast<uint32_t *>(nullptr), static_cast<u32x4 *>(nullptr));
             });
         dev_histogram<uint32_t, KIND_UINT>(c, reinterpret_cast<const uint32_t *>(base), n, d_counts, st);
     } else {
@@ -1508,7 +1513,7 @@ void dev_histogram_records(hpcs_ctx *c, const void *d_recs, int64_t n, int32_t r
             timed(c, st, "extract", [&] {
                 hipLaunchKernelGGL((k_extract_keys<uint64_t>), dim3(grid), dim3(256), 0, st, recs, uint64_t(n), rec_size,
                                    key_offset, int(HPCS_KEY_UINT), reinterpret_cast<uint64_t *>(base),
-                                   static_cast<uint32_t *>(nullptr));
+                                   static_cast<uint32_t *>(nullptr), static_cast<u32x4 *>(nullptr));
             });
         dev_histogram<uint64_t, KIND_UINT>(c, reinterpret_cast<const uint64_t *>(base), n, d_counts, st);
     }
@@ -1696,8 +1701,9 @@ HPCS_API int32_t hpcs_ctx_trim(hpcs_ctx *ctx)
         graph_cache_clear(ctx);
         if (ctx->scratch) hip_check(hipFree(ctx->scratch), "hipFree");
         if (ctx->io) hip_check(hipFree(ctx->io), "hipFree");
-        ctx->scratch = ctx->io = nullptr;
-        ctx->scratch_cap = ctx->io_cap = 0;
+        if (ctx->work) hip_check(hipFree(ctx->work), "hipFree");
+        ctx->scratch = ctx->io = ctx->work = nullptr;
+        ctx->scratch_cap = ctx->io_cap = ctx->work_cap = 0;
     });
 }
 #endif

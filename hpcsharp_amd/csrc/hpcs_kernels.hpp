@@ -1162,13 +1162,21 @@ __global__ __launch_bounds__(256) void k_histogram_nbit(const K *__restrict__ ke
 // ---------------------------------------------------------------------------
 // Records: extract (radix key, index) pairs, and gather records by index.
 // ---------------------------------------------------------------------------
+// With copy16 (records 16-byte aligned, size a multiple of 16) each thread
+// also copies its record there in 16-byte vectors: the extraction reads the
+// records' lines anyway.
 template <typename K>
 __global__ void k_extract_keys(const unsigned char *__restrict__ recs, uint64_t n, int rec_size, int key_offset,
-                               int kind, K *__restrict__ keys, uint32_t *__restrict__ idx)
+                               int kind, K *__restrict__ keys, uint32_t *__restrict__ idx, u32x4 *__restrict__ copy16)
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         const unsigned char *p = recs + i * uint64_t(rec_size) + key_offset;
+        if (copy16) {
+            const uint32_t v = uint32_t(rec_size) >> 4;
+            const u32x4 *src = reinterpret_cast<const u32x4 *>(recs) + i * v;
+            for (uint32_t j = 0; j < v; ++j) copy16[i * v + j] = src[j];
+        }
         K k;
         if ((reinterpret_cast<uintptr_t>(p) & (sizeof(K) - 1)) == 0) {
             k = *reinterpret_cast<const K *>(p);

@@ -186,6 +186,48 @@ def test_records_float_key_unaligned(oracle):
     _assert_records_equal(rec, ref)
 
 
+def test_records_32_byte_and_work_buffer_reuse(oracle):
+    """32-byte records (copied in two 16-byte vectors per record by the key
+    extraction) and 12-byte ones (word copy), through one context whose work
+    buffer grows and is then reused by a smaller call: the oracle's order,
+    padding bytes included, on host arrays and on a device sub-array."""
+    import torch
+    P = ctypes.c_void_p
+    dt32 = np.dtype({"names": ["payload", "key", "tail"], "formats": [np.uint32, np.uint64, "u8,u8"],
+                     "offsets": [0, 8, 16], "itemsize": 32})
+    dt12 = np.dtype({"names": ["key", "payload"], "formats": [np.uint64, np.uint32], "offsets": [0, 8],
+                     "itemsize": 12})
+    ctx = N.new_context(0)
+    try:
+        for n, dt, koff in ((1000, dt32, 8), (300001, dt32, 8), (250007, dt12, 0), (5000, dt32, 8)):
+            print(f"records {dt.itemsize} B, n={n}", flush=True)
+            rec = np.zeros(n, dtype=dt)
+            rec["key"] = datagen.keys_u64(n, 40 + n) & np.uint64(0xFFFF0000000000FF)   # ties: stability
+            rec["payload"] = np.arange(n, dtype=np.uint32)
+            if "tail" in dt.names:
+                rec["tail"]["f0"] = datagen.keys_u64(n, 41 + n)
+                rec["tail"]["f1"] = np.arange(n, dtype=np.uint64) * np.uint64(3)
+            ref = _byte_copy(rec)
+            oracle.sort_radix_records(ref, koff, 64)
+            N.check(N.lib().hpcs_sort_radix_records(ctx, rec.ctypes.data_as(P), n, dt.itemsize, koff, 64, 0))
+            _assert_records_equal(rec, ref)
+        # device call on records that start 32 bytes into a buffer (16-byte aligned)
+        n = 70001
+        rec = np.zeros(n + 1, dtype=dt32)
+        rec["key"] = datagen.keys_u64(n + 1, 77)
+        rec["payload"] = np.arange(n + 1, dtype=np.uint32)
+        ref = _byte_copy(rec)
+        oracle.sort_radix_records(ref[1:], 8, 64)
+        t = torch.from_numpy(np.frombuffer(rec.tobytes(), np.uint8).copy()).cuda()
+        st = P(torch.cuda.current_stream().cuda_stream)
+        N.check(N.lib().hpcs_dev_sort_radix_records(ctx, P(t.data_ptr() + 32), n, 32, 8, 64, 0, st))
+        torch.cuda.synchronize()
+        got = np.frombuffer(t.cpu().numpy().tobytes(), dtype=dt32)
+        _assert_records_equal(got, ref)
+    finally:
+        N.destroy_context(ctx)
+
+
 @pytest.mark.parametrize("case", FIXTURES["hist"], ids=lambda c: str(c["n"]))
 def test_histograms(case):
     n = case["n"]
