@@ -295,6 +295,39 @@ def histogram_figures(torch, N, datagen, keys_host: np.ndarray, reps: int):
     return out
 
 
+def histogram_cases(torch, N, datagen, reps: int):
+    """The byte histogram at 2^28 on ascending and constant u32 keys and on
+    uniform u64 keys (bench.py --only histogram), counts checked with
+    torch.bincount."""
+    lib, ctx = N.lib(), N.context(0)
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    P = ctypes.c_void_p
+    n = 1 << 28
+    out = {}
+    for tag, gen, fn, kb in (("u32_sorted", lambda: datagen.keys_u32(n, SEED, "sorted"),
+                              "hpcs_dev_histogram_byte_components_u32", 4),
+                             ("u32_constant", lambda: datagen.keys_u32(n, SEED, "constant"),
+                              "hpcs_dev_histogram_byte_components_u32", 4),
+                             ("u64_uniform", lambda: datagen.keys_u64(n, SEED),
+                              "hpcs_dev_histogram_byte_components_u64", 8)):
+        h = gen()
+        x = torch.from_numpy(h.view(np.int32 if kb == 4 else np.int64)).cuda()
+        del h
+        counts = torch.empty(kb * 256, dtype=torch.int32, device="cuda")
+        f = getattr(lib, fn)
+        ms = event_ms(torch, lambda: None,
+                      lambda: N.check(f(ctx, P(x.data_ptr()), n, P(counts.data_ptr()), sp)), reps)
+        xl = x.to(torch.int64)
+        want = torch.stack([torch.bincount((xl >> (8 * d)) & 255, minlength=256) for d in range(kb)]).reshape(-1)
+        if not torch.equal(counts.to(torch.int64), want):
+            print(f"bench: {tag} histogram differs from torch.bincount", file=sys.stderr)
+            sys.exit(3)
+        out[tag] = {"n": n, "histogram_us": round(ms * 1e3, 2), "histogram_GBs": round(kb * n / ms / 1e6, 1),
+                    "histogram_roofline_frac": round(kb * n / ms / 1e6 / HBM_PEAK_GBS, 4)}
+        del x, counts, xl, want
+    return out
+
+
 def cfg3_figures(torch, N, datagen, reps: int):
     """Floating and record keys at 2^27: double[] with +-0.0, subnormals and
     +-1e300 (plain sort), and {u64 key, u32 payload} records through the
@@ -504,7 +537,9 @@ def segments_only(torch, N, datagen, nseg: int, seed: int, reps: int):
 def only_section(torch, N, datagen, args):
     """One workload alone (bench.py --only X), for profiling runs."""
     if args.only == "histogram":
-        return histogram_figures(torch, N, datagen, datagen.keys_u32(args.n, SEED, "uniform"), 10)
+        r = histogram_figures(torch, N, datagen, datagen.keys_u32(args.n, SEED, "uniform"), 10)
+        r.update(histogram_cases(torch, N, datagen, 10))
+        return r
     if args.only == "topk":
         return topk_figures(torch, N, datagen, 5)
     if args.only == "segments":

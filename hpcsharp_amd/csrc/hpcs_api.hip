@@ -1130,12 +1130,6 @@ void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_r
     K *cand[2] = {reinterpret_cast<K *>(base + off_c0), reinterpret_cast<K *>(base + off_c1)};
     const int vec = (reinterpret_cast<uintptr_t>(d_in) % 16 == 0) ? 1 : 0;
     constexpr int TILE = TopkTile<K>::KEYS;
-    launch_fill(c, ts, sizeof(TopkState), st);
-    timed(c, st, "topk_hist", [&] {
-        hipLaunchKernelGGL((k_topk_hist<K, FLOATK>),
-                           dim3(grid_for(uint64_t(n), TOPK_HIST_BLOCK * TopkTile<K>::ITEMS, c->num_cus * 2)),
-                           dim3(TOPK_HIST_BLOCK), 0, st, d_in, un, vec, ts);
-    });
     // one resident wave of workgroups (a second, partial round would leave
     // most CUs idle at the end of the grid-stride loop)
     static const int occ = [] { // resident compaction workgroups per CU, initialised once
@@ -1146,10 +1140,29 @@ void run_topk(hpcs_ctx *c, const K *d_in, int64_t n, int64_t m, K *d_out, K *d_r
         return o;
     }();
     const int cgrid = grid_for(uint64_t(n), TILE, c->num_cus * occ);
+    // large inputs without `rest`: sample, then keep the keys of the top bins
+    // in one read (level 0 then descends over those alone)
+    const int sampled = (!d_rest && un >= TOPK_FILTER_MIN && uint64_t(m) * 64u <= uint64_t(n)) ? 1 : 0;
+    launch_fill(c, ts, sizeof(TopkState), st);
+    if (sampled) {
+        timed(c, st, "topk_sample", [&] {
+            hipLaunchKernelGGL((k_topk_sample<K, FLOATK>), dim3(TOPK_SAMPLE_BLOCKS), dim3(TOPK_BLOCK), 0, st, d_in,
+                               un, ts);
+        });
+        timed(c, st, "topk_filter", [&] {
+            hipLaunchKernelGGL((k_topk_filter<K, FLOATK>), dim3(cgrid), dim3(TOPK_BLOCK), 0, st, d_in, un, vec, ts,
+                               cand[0], um);
+        });
+    }
+    timed(c, st, "topk_hist", [&] {
+        hipLaunchKernelGGL((k_topk_hist<K, FLOATK>),
+                           dim3(grid_for(uint64_t(n), TOPK_HIST_BLOCK * TopkTile<K>::ITEMS, c->num_cus * 2)),
+                           dim3(TOPK_HIST_BLOCK), 0, st, d_in, un, vec, ts, sampled, um);
+    });
     for (int l = 0; l < LV::LEVELS; ++l) {
         timed(c, st, "topk_compact", [&] {
             hipLaunchKernelGGL((k_topk_compact<K, FLOATK>), dim3(cgrid), dim3(TOPK_BLOCK), 0, st, d_in, un, vec,
-                               cand[l & 1], cand[(l + 1) & 1], l, ts, d_out, d_rest, um, idesc);
+                               cand[l & 1], cand[(l + 1) & 1], l, ts, d_out, d_rest, um, idesc, sampled);
         });
     }
     const uint64_t span = std::max<uint64_t>(uint64_t(m), uint64_t(n - m));

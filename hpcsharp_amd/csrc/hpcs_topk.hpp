@@ -17,6 +17,16 @@
 //   k_topk_finish   after the last level the m-th largest key T is known:
 //                   k_rem copies of T close the output, the surplus copies of
 //                   T go to `rest`
+// Large inputs without a `rest` output first take a sample and filter:
+//   k_topk_sample   level-0 digit counts of an evenly spaced sample (2^16
+//                   keys)
+//   k_topk_filter   one read: from the sample, a level-0 bin b_est below the
+//                   sample's (2 m S / n + 16)-th largest key; keys of bins >=
+//                   b_est are kept as the candidate list and counted (the
+//                   level-0 counts of those bins)
+// and level 0 then descends over the kept keys alone.  If fewer than m were
+// kept (a sample that misrepresents the input), k_topk_hist counts the keys
+// below b_est and level 0 reads the whole input, as without the filter.
 // Output positions come from atomics, so the selected keys are then sorted
 // (one workgroup: by counting ranks up to TOPK_RANK_SORT keys, by an LSD
 // radix sort in LDS up to TOPK_BLOCK_SORT keys; beyond, the full radix sort)
@@ -88,7 +98,16 @@ struct TopkState {
     uint32_t done;       // 1 + the level whose bin was taken whole; 0 while descending
     uint32_t pad[9];
     TopkSel sel[TOPK_MAX_LEVELS + 1]; // sel[l + 1] written by level l
+    // sampled filter (inputs of at least TOPK_FILTER_MIN keys, no rest)
+    uint32_t shist[TOPK_BINS]; // level-0 counts of the sample
+    uint32_t n_filt;           // keys kept by the filter (level-0 bin >= b_est)
+    uint32_t b_est;
+    uint32_t pad3[2];
 };
+constexpr uint32_t TOPK_FILTER_MIN = 1u << 20; // smallest input that is sampled
+constexpr int TOPK_SAMPLE_BLOCKS = 64;
+constexpr int TOPK_SAMPLE_ITEMS = 4;
+constexpr uint32_t TOPK_SAMPLE = uint32_t(TOPK_SAMPLE_BLOCKS) * TOPK_BLOCK * TOPK_SAMPLE_ITEMS; // 2^16 keys
 
 // Order key of a value: the key-kind transform (float: sign flip / all-bits
 // flip; signed: sign flip; unsigned: the value itself).
@@ -123,13 +142,18 @@ __device__ inline void wave_count(uint32_t *s_hist, uint32_t d, bool valid)
 }
 
 // Level-0 counts over the whole input (grid-stride over tiles).
+// With `sampled` (the filter ran before): nothing to do when the filter kept
+// at least m keys; otherwise only the keys below the filter's bin are
+// counted, the filter having counted the others.
 template <typename K, int FLOATK>
 __global__ __launch_bounds__(TOPK_HIST_BLOCK) void k_topk_hist(const K *__restrict__ in, uint32_t n, int vec,
-                                                               TopkState *__restrict__ st)
+                                                               TopkState *__restrict__ st, int sampled, uint32_t m)
 {
     using LV = TopkLevels<K>;
     constexpr int ITEMS = TopkTile<K>::ITEMS, TILE = TOPK_HIST_BLOCK * ITEMS;
     __shared__ uint32_t s_hist[TOPK_BINS];
+    if (sampled && st->n_filt >= m) return;
+    const uint32_t below = sampled ? st->b_est : uint32_t(TOPK_BINS);
     for (int i = threadIdx.x; i < TOPK_BINS; i += TOPK_HIST_BLOCK) s_hist[i] = 0u;
     __syncthreads();
     for (uint32_t base = blockIdx.x * uint32_t(TILE); base < n; base += gridDim.x * uint32_t(TILE)) {
@@ -138,9 +162,10 @@ __global__ __launch_bounds__(TOPK_HIST_BLOCK) void k_topk_hist(const K *__restri
         K u[ITEMS];
         load_tile_keys<K, TOPK_HIST_BLOCK, ITEMS>(in + base, cnt, vec, u);
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j)
-            wave_count(s_hist, LV::digit(order_key<K, FLOATK>(u[j]), 0),
-                       full || j * TOPK_HIST_BLOCK + int(threadIdx.x) < int(cnt));
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint32_t d = LV::digit(order_key<K, FLOATK>(u[j]), 0);
+            wave_count(s_hist, d, (full || j * TOPK_HIST_BLOCK + int(threadIdx.x) < int(cnt)) && d < below);
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < TOPK_BINS; i += TOPK_HIST_BLOCK)
@@ -230,6 +255,97 @@ template <typename K> struct WaveStage {
     }
 };
 
+// Level-0 counts of TOPK_SAMPLE keys spaced n / TOPK_SAMPLE apart (grid:
+// TOPK_SAMPLE_BLOCKS workgroups of TOPK_BLOCK threads; n >= TOPK_FILTER_MIN).
+template <typename K, int FLOATK>
+__global__ __launch_bounds__(TOPK_BLOCK) void k_topk_sample(const K *__restrict__ in, uint32_t n,
+                                                            TopkState *__restrict__ st)
+{
+    using LV = TopkLevels<K>;
+    __shared__ uint32_t s_hist[TOPK_BINS];
+    for (int i = threadIdx.x; i < TOPK_BINS; i += TOPK_BLOCK) s_hist[i] = 0u;
+    const uint32_t stride = n / TOPK_SAMPLE;
+    K u[TOPK_SAMPLE_ITEMS];
+#pragma unroll
+    for (int j = 0; j < TOPK_SAMPLE_ITEMS; ++j) {
+        const uint32_t q = (blockIdx.x * uint32_t(TOPK_SAMPLE_ITEMS) + uint32_t(j)) * TOPK_BLOCK + threadIdx.x;
+        u[j] = in[size_t(q) * stride + stride / 2];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < TOPK_SAMPLE_ITEMS; ++j) atomicAdd(&s_hist[LV::digit(order_key<K, FLOATK>(u[j]), 0)], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < TOPK_BINS; i += TOPK_BLOCK)
+        if (s_hist[i]) atomicAdd(&st->shist[i], s_hist[i]);
+}
+
+// One read of the input: the level-0 bin b_est one below the bin of the
+// sample's (2 m S / n + 16)-th largest key (S sample keys; every workgroup
+// derives the same bin), and every key of a bin >= b_est appended to `kept`
+// (raw values, capacity n) and counted into the level-0 counts.
+template <typename K, int FLOATK>
+__global__ __launch_bounds__(TOPK_BLOCK) void k_topk_filter(const K *__restrict__ in, uint32_t n, int vec,
+                                                            TopkState *__restrict__ st, K *__restrict__ kept,
+                                                            uint32_t m)
+{
+    using LV = TopkLevels<K>;
+    constexpr int ITEMS = TopkTile<K>::ITEMS, TILE = TopkTile<K>::KEYS;
+    constexpr int WAVES = TOPK_BLOCK / WAVE;
+    __shared__ uint32_t s_hist[TOPK_BINS];
+    __shared__ K s_stage[WAVES][TOPK_STAGE];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    const uint64_t target = min(2ull * uint64_t(m) * TOPK_SAMPLE / uint64_t(n) + 16ull, uint64_t(TOPK_SAMPLE));
+    uint32_t sel[3];
+    topk_find_bin(st->shist, uint32_t(target), sel);
+    const uint32_t b = sel[0] > 0u ? sel[0] - 1u : 0u;
+    if (blockIdx.x == 0 && tid == 0) st->b_est = b;
+    for (int i = tid; i < TOPK_BINS; i += TOPK_BLOCK) s_hist[i] = 0u;
+    __syncthreads();
+    WaveStage<K> sk{s_stage[wave], 0u, kept, &st->n_filt, n};
+    const uint32_t step = gridDim.x * uint32_t(TILE);
+    K nx[ITEMS];
+    uint32_t base = blockIdx.x * uint32_t(TILE);
+    if (base < n) load_tile_keys<K, TOPK_BLOCK, ITEMS>(in + base, min(uint32_t(TILE), n - base), vec, nx);
+    for (; base < n; base += step) {
+        const uint32_t cnt = min(uint32_t(TILE), n - base);
+        const bool full = vec && cnt == uint32_t(TILE);
+        K u[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) u[j] = nx[j];
+        if (base + step < n)
+            load_tile_keys<K, TOPK_BLOCK, ITEMS>(in + base + step, min(uint32_t(TILE), n - base - step), vec, nx);
+        uint32_t d[ITEMS];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            d[j] = LV::digit(order_key<K, FLOATK>(u[j]), 0);
+            any |= (full || j * TOPK_BLOCK + tid < int(cnt)) && d[j] >= b;
+        }
+        if (__ballot(any) == 0ull) continue;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const bool keep = (full || j * TOPK_BLOCK + tid < int(cnt)) && d[j] >= b;
+            sk.append(keep, u[j]);
+            if (keep) atomicAdd(&s_hist[d[j]], 1u);
+        }
+    }
+    // the waves' remaining stages: one counter atomic per workgroup
+    __shared__ uint32_t s_cnt[WAVES], s_base;
+    if (lane == 0) s_cnt[wave] = sk.cnt;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < WAVES; ++w) tot += s_cnt[w];
+        s_base = tot ? atomicAdd(sk.ctr, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t b0 = s_base;
+    for (int w = 0; w < wave; ++w) b0 += s_cnt[w];
+    sk.write_at(b0);
+    for (int i = tid; i < TOPK_BINS; i += TOPK_BLOCK)
+        if (s_hist[i]) atomicAdd(&st->hist[0][i], s_hist[i]);
+}
+
 // Level `level`: choose the threshold bin b from the level's counts (every
 // workgroup computes the same choice; workgroup 0 records it), then classify
 // the level's candidates.  Keys above b (and all of b when every key of b is
@@ -242,7 +358,7 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_topk_compact(const K *__restrict
                                                              const K *__restrict__ cand_src, K *__restrict__ cand_dst,
                                                              int level, TopkState *__restrict__ st,
                                                              K *__restrict__ out, K *__restrict__ rest, uint32_t m,
-                                                             int desc)
+                                                             int desc, int sampled)
 {
     using LV = TopkLevels<K>;
     constexpr int ITEMS = TopkTile<K>::ITEMS, TILE = TopkTile<K>::KEYS;
@@ -273,9 +389,12 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_topk_compact(const K *__restrict
     const bool last = level + 1 == LV::LEVELS;
     const bool keep_cand = !take_bin && !last;
     const int nlevel = last ? level : level + 1;
-    const uint32_t n = level == 0 ? n_in : st->n_cand[level % 3];
-    const K *src = level == 0 ? in : cand_src;
-    const bool vec = level == 0 ? in_vec != 0 : true;
+    // level 0 reads the filter's kept keys (in cand_src) when they hold the
+    // m largest, else the whole input
+    const bool filtered = level == 0 && sampled && st->n_filt >= m;
+    const uint32_t n = level == 0 ? (filtered ? st->n_filt : n_in) : st->n_cand[level % 3];
+    const K *src = level == 0 && !filtered ? in : cand_src;
+    const bool vec = level == 0 && !filtered ? in_vec != 0 : true;
     // capacities keep the writes inside the buffers even if the counts were
     // inconsistent (they are not, by construction)
     WaveStage<K> so{s_stage[wave][0], 0u, out, &st->n_out, m};

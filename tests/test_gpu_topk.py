@@ -87,6 +87,34 @@ def test_topk_types_and_distributions(dtype, dist, n, k):
     assert np.array_equal(_bits(got), _bits(_expected_desc(orig, k)))
 
 
+@pytest.mark.parametrize("dtype", [np.uint32, np.uint64, np.float32, np.float64])
+@pytest.mark.parametrize("dist", ["uniform", "low8", "sorted", "constant"])
+def test_topk_sampled_filter(dtype, dist):
+    """Inputs of at least 2^20 keys with k <= n / 64 are sampled and their top
+    level-0 bins kept in one read before the descent: every key kind and
+    distribution (constant keys: every key kept), k from 1 to the limit."""
+    if dtype in (np.float32, np.float64) and dist in ("low8", "sorted"):
+        pytest.skip("distribution defined for unsigned keys")
+    n = (1 << 20) + 77
+    x = _keys(dtype, n, 515, dist)
+    for k in (1, 1000, n // 64):
+        got = Algorithm.TopK(x, k)
+        assert np.array_equal(_bits(got), _bits(_expected_desc(x, k))), k
+
+
+def test_topk_sampled_filter_falls_back():
+    """A sample that overstates the keys (the sampled positions hold the only
+    large keys, and k = n / 64 exceeds the 2^16 sampled keys) keeps fewer
+    than k keys: level 0 then reads the whole input.  Same result."""
+    n = 1 << 23
+    stride = n >> 16
+    a = datagen.keys_u32(n, 77) & np.uint32(0x000FFFFF)
+    pos = np.arange(1 << 16, dtype=np.int64) * stride + stride // 2
+    a[pos] |= np.uint32(0xFFF00000)
+    for k in (n // 64, n // 100):
+        assert np.array_equal(Algorithm.TopK(a, k), _expected_desc(a, k)), k
+
+
 @pytest.mark.parametrize("mask", [0xF, 0x3FF, 0xFFFF0000, 0x80000001])
 def test_topk_ties(mask):
     """Narrow keys: the threshold key has many copies, some needed, some not."""
