@@ -13,6 +13,7 @@
 #include "hpcs_merge.hpp"
 #include "hpcs_msd.hpp"
 #include "hpcs_segments.hpp"
+#include "hpcs_hybrid.hpp"
 #include "hpcs_topk.hpp"
 #include "hpcs_tuning.hpp"
 #include "../../include/hpcsort.h"
@@ -74,6 +75,7 @@ constexpr int PASS_AUTO = 2;
 constexpr int PASS_BLOCK = 3; // one workgroup sorts the whole array in LDS
 constexpr int PASS_MSD = 4;   // one MSD split, then one workgroup per bucket (keys without values)
 constexpr int PASS_MERGE = 5; // runs sorted by one workgroup each in LDS, then stable merge passes
+constexpr int PASS_HYBRID = 6; // 4-byte unsigned keys without values: top two bytes MSD-first, then LDS buckets
 // largest arrays the block-sort + merge form serves (beyond, the passes over
 // the whole array outnumber the radix passes)
 constexpr int64_t MERGE_SORT_MAX = int64_t(1) << 20;
@@ -491,6 +493,9 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
     if (p.algo == PASS_BLOCK && n > block_max) p.algo = PASS_AUTO;
     if (p.algo == PASS_MSD && (vals || n > (key_bytes == 4 ? MSD_MAX4 : MSD_MAX8))) p.algo = PASS_AUTO;
     if (p.algo == PASS_MERGE && n > MERGE_SORT_MAX) p.algo = PASS_AUTO;
+    // the MSD-first form sorts 4-byte keys without values (run_sort sends
+    // float and signed keys to the reduce-then-scan passes)
+    if (p.algo == PASS_HYBRID && (key_bytes != 4 || vals || n <= BLOCK_SORT_MAX4)) p.algo = PASS_AUTO;
     // by size, measured on MI355X: one workgroup up to 16384 4-byte / 8192
     // 8-byte keys (with LDS-OR ranking it beats onesweep passes there: u32
     // 16384 keys 42.2 against 48.4 us)
@@ -683,6 +688,195 @@ void run_sort_msd(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st, const SortP
     timed(c, st, "msd_buckets", [&] { launch_sort_buckets<K, FLOATK>(n, st, d_keys, keysB, plan); });
 }
 
+void launch_seg_counts(int shape, dim3 grid, hipStream_t st, const uint32_t *a, const uint32_t *b, int pass,
+                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift);
+void launch_seg_scatter(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
+                        const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
+                        const uint32_t *sbase, const uint32_t *dshift);
+
+// Scratch of the MSD-first form (hpcs_hybrid.hpp): the reduce-then-scan
+// layout, then the top-byte tile counts, the bucket bounds, level 2's and the
+// overflow's segment plans and lists, and the tile records, counts and
+// offsets both share (level 2 is done when the overflow starts).
+struct HybLayout {
+    Layout L;
+    size_t off_cnt3, off_begin1, off_sp, off_sp2, off_tile0, off_small, off_medium, off_large, off_tinfo, off_scnt,
+        off_soff, off_rbase, off_total, off_sbase, off_dstart, off_pairs, off_medium2, off_large2, off_tile02, total;
+    uint64_t max_tiles1, max_tiles2; // tile bounds of level 2 and of the overflow passes
+    uint64_t max_ranges, max_large2; // ranges of SEGX_RANGE tile records; overflow segments of more than SEG_MEDIUM keys
+    int seg_shape, seg_tile;
+};
+
+HybLayout make_hyb_layout(int64_t n, int sort_shape)
+{
+    HybLayout H{};
+    H.L = make_layout(n, 4, false, shape_tile_keys(sort_shape), false);
+    H.seg_shape = seg_shape_ok(sort_shape) ? sort_shape : 0;
+    H.seg_tile = shape_tile_keys(H.seg_shape);
+    const uint64_t tiles = uint64_t(n) / uint64_t(H.seg_tile);
+    H.max_tiles1 = tiles + RADIX + 1;                           // the 256 buckets' tiles
+    H.max_tiles2 = tiles + uint64_t(n) / uint64_t(SEG_MEDIUM) + 2; // overflow segments exceed SEG_MEDIUM keys
+    const uint64_t mt = std::max(H.max_tiles1, H.max_tiles2);
+    H.off_cnt3 = H.L.total;
+    H.off_begin1 = align_up(H.off_cnt3 + size_t(H.L.tiles) * RADIX * sizeof(uint16_t));
+    H.off_sp = align_up(H.off_begin1 + (RADIX + 1) * sizeof(int64_t));
+    H.off_sp2 = align_up(H.off_sp + sizeof(SegPlan));
+    H.off_tile0 = align_up(H.off_sp2 + sizeof(SegPlan));
+    H.off_small = align_up(H.off_tile0 + RADIX * 4);
+    H.off_medium = align_up(H.off_small + RADIX * 4);
+    H.off_large = align_up(H.off_medium + RADIX * 4);
+    H.off_tinfo = align_up(H.off_large + RADIX * 4);
+    H.off_scnt = align_up(H.off_tinfo + mt * sizeof(uint4));
+    H.off_soff = align_up(H.off_scnt + mt * RADIX * sizeof(uint16_t));
+    H.max_ranges = (mt + SEGX_RANGE - 1) / SEGX_RANGE;
+    H.max_large2 = uint64_t(n) / uint64_t(SEG_MEDIUM) + 1;
+    H.off_rbase = align_up(H.off_soff + mt * RADIX * sizeof(uint32_t));
+    H.off_total = align_up(H.off_rbase + H.max_ranges * RADIX * sizeof(uint32_t));
+    H.off_sbase = align_up(H.off_total + RADIX * sizeof(uint32_t));
+    H.off_dstart = align_up(H.off_sbase + std::max<uint64_t>(RADIX, H.max_large2) * RADIX * sizeof(uint32_t));
+    H.off_pairs = align_up(H.off_dstart + size_t(RADIX) * RADIX * sizeof(uint32_t));
+    H.off_medium2 = align_up(H.off_pairs + size_t(HYB_SUB) * 2 * sizeof(int64_t));
+    H.off_large2 = align_up(H.off_medium2 + size_t(HYB_SUB) * 4);
+    H.off_tile02 = align_up(H.off_large2 + size_t(HYB_SUB) * 4);
+    H.total = align_up(H.off_tile02 + size_t(HYB_SUB) * 2 * 4);
+    return H;
+}
+
+// The MSD-first form (pass algorithm 6) for 4-byte unsigned keys without
+// values; inputs whose top two bytes do not both vary take the plain
+// reduce-then-scan passes inside the same launch sequence (device plan).
+void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, const SortParams &prm)
+{
+    using K = uint32_t;
+    const int shape = prm.shape, rank_mode = prm.rank;
+    const HybLayout H = make_hyb_layout(n, shape);
+    const Layout &L = H.L;
+    char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, H.total));
+    DevPlan *plan = reinterpret_cast<DevPlan *>(base + L.off_plan);
+    uint32_t *offs = reinterpret_cast<uint32_t *>(base + L.off_lb0), *totals = reinterpret_cast<uint32_t *>(base + L.off_lb1);
+    uint32_t *dtot = reinterpret_cast<uint32_t *>(base + L.off_dtot);
+    uint16_t *counts = reinterpret_cast<uint16_t *>(base + L.off_cnt);
+    uint16_t *counts3 = reinterpret_cast<uint16_t *>(base + H.off_cnt3);
+    K *keysB = reinterpret_cast<K *>(base + L.off_keysB);
+    K *orslots = reinterpret_cast<K *>(base + L.off_hist);
+    int64_t *begin1 = reinterpret_cast<int64_t *>(base + H.off_begin1);
+    SegPlan *sp = reinterpret_cast<SegPlan *>(base + H.off_sp);
+    SegPlan *sp2 = reinterpret_cast<SegPlan *>(base + H.off_sp2);
+    uint32_t *tile0 = reinterpret_cast<uint32_t *>(base + H.off_tile0);
+    uint32_t *small = reinterpret_cast<uint32_t *>(base + H.off_small);
+    uint32_t *medium = reinterpret_cast<uint32_t *>(base + H.off_medium);
+    uint32_t *large = reinterpret_cast<uint32_t *>(base + H.off_large);
+    uint4 *tinfo = reinterpret_cast<uint4 *>(base + H.off_tinfo);
+    uint16_t *scnt = reinterpret_cast<uint16_t *>(base + H.off_scnt);
+    uint32_t *soff = reinterpret_cast<uint32_t *>(base + H.off_soff);
+    uint32_t *rbase = reinterpret_cast<uint32_t *>(base + H.off_rbase);
+    uint32_t *stotal = reinterpret_cast<uint32_t *>(base + H.off_total);
+    uint32_t *sbase = reinterpret_cast<uint32_t *>(base + H.off_sbase);
+    uint32_t *dstart = reinterpret_cast<uint32_t *>(base + H.off_dstart);
+    int64_t *pairs = reinterpret_cast<int64_t *>(base + H.off_pairs);
+    uint32_t *medium2 = reinterpret_cast<uint32_t *>(base + H.off_medium2);
+    uint32_t *large2 = reinterpret_cast<uint32_t *>(base + H.off_large2);
+    uint32_t *tile02 = reinterpret_cast<uint32_t *>(base + H.off_tile02);
+    const uint32_t tiles = uint32_t(L.tiles);
+    const uint32_t nranges = (tiles + SCAN_RANGE - 1) / SCAN_RANGE;
+    const int stile = H.seg_tile;
+
+    launch_fill(c, base, L.off_lb0, st); // OR slots, plan, per-pass digit totals
+    timed(c, st, "upsweep", [&] {
+        with_shape<4, false>(shape, [&](auto sc) {
+            constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
+            hipLaunchKernelGGL((k_upsweep<K, 0, T.block, T.items, true>), dim3(tiles), dim3(T.block), 0, st, d_keys,
+                               uint32_t(n), counts, orslots, int(aligned16(d_keys)), counts3);
+        });
+    });
+    timed(c, st, "plan", [&] {
+        hipLaunchKernelGGL((k_plan_rts<K>), dim3(1), dim3(64), 0, st, orslots, plan, uint64_t(n), 0, 1);
+    });
+    // the LSD loop: in the MSD-first form only pass 3 (the top byte) runs
+    for (int p = 0; p < 4; ++p) {
+        if (p > 0)
+            timed(c, st, "tile_counts", [&] {
+                launch_tile_counts<K, false, 0>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan, counts,
+                                                nullptr);
+            });
+        timed(c, st, "scan", [&] {
+            uint32_t *dtot_p = dtot + size_t(p) * RADIX * LINE_WORDS;
+            hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, p == 3 ? counts3 : counts,
+                               offs, tiles, totals, dtot_p, p, plan);
+            hipLaunchKernelGGL(k_scan_columns, dim3(RADIX), dim3(1024), 0, st, totals, nranges, dtot_p, p, plan);
+        });
+        timed(c, st, "scatter", [&] {
+            launch_onesweep<K, false, 0>(rank_mode, shape, 0, c->fallback_rounds, dim3(tiles), st, d_keys, keysB,
+                                         nullptr, nullptr, uint32_t(n), p, plan, offs, totals, nullptr);
+        });
+    }
+    // level 2: the top-byte buckets in B, split by byte 2 into A (buckets of
+    // at most SEG_SMALL keys sorted whole into A)
+    timed(c, st, "hyb_begin", [&] {
+        hipLaunchKernelGGL(k_hyb_begin1, dim3(1), dim3(RADIX), 0, st, plan, uint64_t(n), begin1, sp2);
+    });
+    timed(c, st, "seg_plan", [&] {
+        hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, begin1, RADIX, n, sp, tile0, small, medium, large,
+                           H.max_tiles1, SEG_SMALL, stile, 1);
+    });
+    timed(c, st, "seg_tiles", [&] {
+        hipLaunchKernelGGL(k_seg_tiles, dim3(RADIX), dim3(256), 0, st, begin1, sp, tile0, large, tinfo, stile);
+    });
+    timed(c, st, "seg_small", [&] {
+        hipLaunchKernelGGL((k_seg_small<SEG_BLOCK>), dim3(RADIX), dim3(SEG_BLOCK), 0, st, keysB, d_keys, begin1, sp,
+                           small);
+    });
+    timed(c, st, "seg_counts", [&] {
+        launch_seg_counts(H.seg_shape, dim3(unsigned(H.max_tiles1)), st, keysB, d_keys, 2, sp, tinfo, scnt,
+                          &plan->hyb_shift2);
+    });
+    // the large segments' scan: range sums, column scan, segment bases
+    auto segx_scan = [&](const int64_t *bg, const SegPlan *p, const uint32_t *t0, const uint32_t *lst, uint64_t mt,
+                         uint64_t nlarge_max, uint32_t *ds) {
+        timed(c, st, "seg_scan", [&] {
+            hipLaunchKernelGGL(k_segx_ranges, dim3(unsigned((mt + SEGX_RANGE - 1) / SEGX_RANGE)), dim3(RADIX), 0, st,
+                               p, scnt, soff, rbase);
+            hipLaunchKernelGGL(k_segx_columns, dim3(RADIX), dim3(1024), 0, st, p, rbase, stotal);
+            hipLaunchKernelGGL(k_segx_bases, dim3(unsigned(std::min<uint64_t>(nlarge_max, uint64_t(c->num_cus) * 8))),
+                               dim3(RADIX), 0, st, bg, p, t0, lst, soff, rbase, stotal, sbase, ds, stile);
+        });
+    };
+    segx_scan(begin1, sp, tile0, large, H.max_tiles1, RADIX, dstart);
+    timed(c, st, "seg_scatter", [&] {
+        launch_seg_scatter(H.seg_shape, rank_mode, dim3(unsigned(H.max_tiles1)), st, keysB, d_keys, 2, sp, tinfo,
+                           soff, rbase, sbase, &plan->hyb_shift2);
+    });
+    // level 3: every sub-bucket in LDS, in place
+    timed(c, st, "hyb_local", [&] {
+        hipLaunchKernelGGL((k_hyb_local<HYB_BLOCK, HYB_ITEMS>), dim3(HYB_SUB), dim3(HYB_BLOCK), 0, st, d_keys, begin1,
+                           plan, sp, large, dstart, sp2, pairs, medium2, large2, tile02, stile);
+    });
+    // overflow: medium sub-buckets in LDS, large ones by bytes 0 and 1
+    timed(c, st, "hyb_medium", [&] {
+        hipLaunchKernelGGL((k_seg_small<SEG_MEDIUM_BLOCK>), dim3(c->num_cus), dim3(SEG_MEDIUM_BLOCK), 0, st, d_keys,
+                           d_keys, pairs, sp2, medium2);
+    });
+    timed(c, st, "seg_tiles", [&] {
+        hipLaunchKernelGGL(k_seg_tiles, dim3(c->num_cus * 8), dim3(256), 0, st, pairs, sp2, tile02, large2, tinfo,
+                           stile);
+    });
+    for (int p = 0; p < 2; ++p) {
+        timed(c, st, "seg_counts", [&] {
+            launch_seg_counts(H.seg_shape, dim3(unsigned(H.max_tiles2)), st, d_keys, keysB, p, sp2, tinfo, scnt,
+                              nullptr);
+        });
+        segx_scan(pairs, sp2, tile02, large2, H.max_tiles2, H.max_large2, nullptr);
+        timed(c, st, "seg_scatter", [&] {
+            launch_seg_scatter(H.seg_shape, rank_mode, dim3(unsigned(H.max_tiles2)), st, d_keys, keysB, p, sp2, tinfo,
+                               soff, rbase, sbase, nullptr);
+        });
+    }
+    timed(c, st, "copyback", [&] {
+        hipLaunchKernelGGL((k_copyback<K>), dim3(grid_for(uint64_t(n), 256, c->num_cus * 8)), dim3(256), 0, st,
+                           d_keys, keysB, nullptr, nullptr, uint64_t(n), plan);
+    });
+}
+
 // Scratch memory a caller hands to run_sort instead of the context's pool
 // (the no-allocation entry point): its device address and size.
 struct ExtScratch {
@@ -705,10 +899,15 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
         // caller-provided scratch: the 8-bit passes (whose layout is
         // make_layout's), never the 11-bit or MSD-split layouts
         prm.bits = 8;
-        if (prm.algo == PASS_MSD || prm.algo == PASS_MERGE)
+        if (prm.algo == PASS_MSD || prm.algo == PASS_MERGE || prm.algo == PASS_HYBRID)
             prm.algo = n <= (sizeof(K) == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8) ? PASS_BLOCK
                        : n < (sizeof(K) == 4 ? (int64_t(1) << 19) : (int64_t(1) << 21)) ? PASS_ONESWEEP
                                                                                          : PASS_RTS;
+    }
+    if (prm.algo == PASS_HYBRID) {
+        if constexpr (std::is_same<K, uint32_t>::value && FLOATK == 0 && !HAS_VALS)
+            return run_sort_hybrid(c, d_keys, n, st, prm);
+        prm.algo = PASS_RTS; // float and signed keys: the same passes without the bucket stages
     }
     if (prm.algo == PASS_MERGE) return run_sort_merge<K, HAS_VALS, FLOATK>(c, d_keys, d_vals, n, st);
     if (prm.algo == PASS_BLOCK) {
@@ -803,8 +1002,8 @@ void run_sort(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipStream_t s
                 });
             timed(c, st, "scan", [&] {
                 uint32_t *dtot_p = dtot + size_t(p) * RADIX * LINE_WORDS;
-                hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, offs, tiles, totals,
-                                   dtot_p, p, plan);
+                hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, counts, offs, tiles,
+                                   totals, dtot_p, p, plan);
                 hipLaunchKernelGGL(k_scan_columns, dim3(RADIX), dim3(1024), 0, st, totals, nranges, dtot_p, p, plan);
             });
             timed(c, st, "scatter", [&] {
@@ -837,10 +1036,11 @@ size_t sort_scratch_bytes(const hpcs_ctx *c, int64_t n, bool ext)
     SortParams prm = resolve_params(c, int(sizeof(K)), HAS_VALS, n);
     if (ext) {
         prm.bits = 8;
-        if (prm.algo == PASS_MSD || prm.algo == PASS_MERGE)
+        if (prm.algo == PASS_MSD || prm.algo == PASS_MERGE || prm.algo == PASS_HYBRID)
             prm.algo = n <= (sizeof(K) == 4 ? BLOCK_SORT_MAX4 : BLOCK_SORT_MAX8) ? PASS_BLOCK : PASS_RTS;
     }
     if (prm.algo == PASS_BLOCK) return 0;
+    if (prm.algo == PASS_HYBRID && sizeof(K) == 4 && !HAS_VALS) return make_hyb_layout(n, prm.shape).total;
     if (prm.algo == PASS_MERGE) return merge_sort_scratch_bytes(n, sizeof(K), HAS_VALS);
     if (!HAS_VALS && prm.bits == 11) return make_layout11(n, sizeof(K)).total;
     return make_layout(n, sizeof(K), HAS_VALS, shape_tile_keys(prm.shape),
@@ -855,15 +1055,26 @@ void run_sort_tight(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st)
 {
     const int64_t h = (n + 1) / 2, nb = n - h;
     const size_t merge_bytes = align_up(size_t(h) * sizeof(K)) + align_up(sizeof(MergeSplit));
-    const size_t need = std::max({merge_bytes, sort_scratch_bytes<K, false>(c, h, true),
-                                  sort_scratch_bytes<K, false>(c, nb, true)});
+    // halves whose tuned sort is the block-sort + merge form take it (its
+    // scratch is one half's keys); others take the 8-bit passes' layout
+    const bool merge_halves = resolve_params(c, int(sizeof(K)), false, h).algo == PASS_MERGE &&
+                              resolve_params(c, int(sizeof(K)), false, nb).algo == PASS_MERGE;
+    const size_t halves = merge_halves ? merge_sort_scratch_bytes(h, sizeof(K), false)
+                                       : std::max(sort_scratch_bytes<K, false>(c, h, true),
+                                                  sort_scratch_bytes<K, false>(c, nb, true));
+    const size_t need = std::max(merge_bytes, halves);
     if (c->scratch_limit && need > c->scratch_limit)
         fail(HPCS_E_BUDGET, "the sort needs " + std::to_string(need) + " bytes of scratch even with halves merged (" +
                                 "limit " + std::to_string(c->scratch_limit) + ")");
     char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, need));
-    const ExtScratch ext{base, std::max(need, c->scratch_cap)};
-    run_sort<K, false, FLOATK>(c, d_keys, nullptr, h, st, &ext);
-    run_sort<K, false, FLOATK>(c, d_keys + h, nullptr, nb, st, &ext);
+    if (merge_halves) {
+        run_sort_merge<K, false, FLOATK>(c, d_keys, nullptr, h, st);
+        run_sort_merge<K, false, FLOATK>(c, d_keys + h, nullptr, nb, st);
+    } else {
+        const ExtScratch ext{base, std::max(need, c->scratch_cap)};
+        run_sort<K, false, FLOATK>(c, d_keys, nullptr, h, st, &ext);
+        run_sort<K, false, FLOATK>(c, d_keys + h, nullptr, nb, st, &ext);
+    }
     K *S = reinterpret_cast<K *>(base);
     MergeSplit *sp = reinterpret_cast<MergeSplit *>(base + align_up(size_t(h) * sizeof(K)));
     K *R = d_keys + h;
@@ -993,31 +1204,33 @@ void with_seg_shape(int shape, F &&f)
 }
 
 void launch_seg_counts(int shape, dim3 grid, hipStream_t st, const uint32_t *a, const uint32_t *b, int pass,
-                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts)
+                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift)
 {
     with_seg_shape(shape, [&](auto sc) {
         constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
-        hipLaunchKernelGGL((k_seg_counts<T.block, T.items>), grid, dim3(T.block), 0, st, a, b, pass, sp, tinfo, counts);
+        hipLaunchKernelGGL((k_seg_counts<T.block, T.items>), grid, dim3(T.block), 0, st, a, b, pass, sp, tinfo, counts,
+                           dshift);
     });
 }
 
 void launch_seg_scatter(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
-                        const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets)
+                        const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
+                        const uint32_t *sbase, const uint32_t *dshift)
 {
     with_seg_shape(shape, [&](auto sc) {
         constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
         if (rank == RANK_BALLOT)
             hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_BALLOT>), grid, dim3(T.block), 0, st, a, b, pass,
-                               sp, tinfo, offsets);
+                               sp, tinfo, offsets, rbase, sbase, dshift);
         else if (rank == RANK_MIX2)
             hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX2>), grid, dim3(T.block), 0, st, a, b, pass,
-                               sp, tinfo, offsets);
+                               sp, tinfo, offsets, rbase, sbase, dshift);
         else if (rank == RANK_MIX4)
             hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX4>), grid, dim3(T.block), 0, st, a, b, pass,
-                               sp, tinfo, offsets);
+                               sp, tinfo, offsets, rbase, sbase, dshift);
         else if (rank == RANK_LDSOR)
             hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_LDSOR>), grid, dim3(T.block), 0, st, a, b, pass,
-                               sp, tinfo, offsets);
+                               sp, tinfo, offsets, rbase, sbase, dshift);
         else
             fail(HPCS_E_UNSUPPORTED, "unknown ranking mode " + std::to_string(rank));
     });
@@ -1044,35 +1257,36 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     uint32_t *keysB = reinterpret_cast<uint32_t *>(base + L.off_keysB);
     timed(c, st, "seg_plan", [&] {
         hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(1024), 0, st, d_begin, nseg, n, sp, tile0, small, medium,
-                           large, L.max_tiles, prm.small_max, tile);
+                           large, L.max_tiles, prm.small_max, tile, 2);
     });
     const int list_grid = std::min(nseg, c->num_cus * 8);
     timed(c, st, "seg_tiles", [&] {
         hipLaunchKernelGGL(k_seg_tiles, dim3(list_grid), dim3(256), 0, st, d_begin, sp, tile0, large, tinfo, tile);
     });
     timed(c, st, "seg_small", [&] {
-        hipLaunchKernelGGL((k_seg_small<SEG_BLOCK>), dim3(list_grid), dim3(SEG_BLOCK), 0, st, d_keys, d_begin, sp,
-                           small);
+        hipLaunchKernelGGL((k_seg_small<SEG_BLOCK>), dim3(list_grid), dim3(SEG_BLOCK), 0, st, d_keys, d_keys,
+                           d_begin, sp, small);
     });
     // the medium list is empty when small_max is SEG_SMALL (its workgroups
     // return at once; the launch sequence stays the same)
     timed(c, st, "seg_medium", [&] {
         hipLaunchKernelGGL((k_seg_small<SEG_MEDIUM_BLOCK>), dim3(std::min(nseg, c->num_cus)), dim3(SEG_MEDIUM_BLOCK),
-                           0, st, d_keys, d_begin, sp, medium);
+                           0, st, d_keys, d_keys, d_begin, sp, medium);
     });
     // one workgroup per tile of the large segments (the device knows how
     // many; the grid is the bound and spare workgroups return)
     const dim3 tile_grid(unsigned(L.max_tiles));
     for (int p = 0; p < 4; ++p) {
         timed(c, st, "seg_counts", [&] {
-            launch_seg_counts(prm.shape, tile_grid, st, d_keys, keysB, p, sp, tinfo, counts);
+            launch_seg_counts(prm.shape, tile_grid, st, d_keys, keysB, p, sp, tinfo, counts, nullptr);
         });
         timed(c, st, "seg_scan", [&] {
             hipLaunchKernelGGL(k_seg_scan, dim3(list_grid), dim3(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS), 0, st, d_begin, sp,
                                tile0, large, counts, offsets, tile);
         });
         timed(c, st, "seg_scatter", [&] {
-            launch_seg_scatter(prm.shape, prm.rank, tile_grid, st, d_keys, keysB, p, sp, tinfo, offsets);
+            launch_seg_scatter(prm.shape, prm.rank, tile_grid, st, d_keys, keysB, p, sp, tinfo, offsets, nullptr,
+                               nullptr, nullptr);
         });
     }
 }
@@ -1767,8 +1981,8 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             break;
         case HPCS_OPT_PASS_ALGO:
             if (value != PASS_ONESWEEP && value != PASS_RTS && value != PASS_AUTO && value != PASS_BLOCK &&
-                value != PASS_MSD && value != PASS_MERGE)
-                fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0, 1, 2, 3 or 4");
+                value != PASS_MSD && value != PASS_MERGE && value != PASS_HYBRID)
+                fail(HPCS_E_INVALID_ARG, "pass algorithm must be 0 to 6");
             ctx->pass_algo = int(value);
             ctx->set_algo = value != PASS_AUTO;
             break;

@@ -72,9 +72,28 @@ struct DevPlan {
     uint32_t msd_flags;                 // PF_* of that scatter (PF_TRIVIAL: every key equal)
     uint32_t msd_shift;
     uint32_t msd_start[RADIX];          // digit counts (k_msd_hist), then exclusive starts
-    uint32_t pad[45];
+    // MSD-first form of the 4-byte keys-only sort (k_plan_rts with `hybrid`):
+    // 1 when it runs (some bit from 23 up varies), so that pass 3 alone moves
+    // keys in the LSD loop (by the 8-bit digit at hyb_shift1) and the
+    // two-level bucket stages (the digit at hyb_shift2 = hyb_shift1 - 8, then
+    // the low 16 bits) finish the sort
+    uint32_t hybrid;
+    uint32_t hyb_shift1, hyb_shift2;
+    uint32_t pad[42];
 };
 constexpr uint32_t PF_TRIVIAL = 1u, PF_SRC_B = 2u, PF_XIN = 4u, PF_XOUT = 8u;
+// The pass's tile counts were written by the upsweep (the MSD-first form's
+// top-byte pass): its counting kernel does not run, its range kernel reads
+// the upsweep's counts.
+constexpr uint32_t PF_PRECOUNTED = 32u;
+// The pass's digit sits at bit (flags >> PF_SHIFT_POS) & 63 of the key, not
+// at 8 * pass (the MSD-first form's first split).
+constexpr uint32_t PF_SHIFT = 64u;
+constexpr int PF_SHIFT_POS = 16;
+__device__ inline int pass_shift(uint32_t flags, int pass)
+{
+    return (flags & PF_SHIFT) ? int((flags >> PF_SHIFT_POS) & 63u) : pass * RADIX_BITS;
+}
 // Reduce-then-scan plans only: PF_FIRST marks the first pass that moves keys,
 // and bits PF_NEXT_SHIFT.. of a pass's flags hold the next pass that moves
 // keys (PF_NEXT_NONE after the last one).
@@ -164,6 +183,84 @@ __device__ inline uint32_t xcd_contiguous(uint32_t b, uint32_t g)
 __device__ inline uint32_t count_below(uint64_t mask)
 {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// The valid lanes that hold the first lane's digit, when at least PEEL_MIN
+// do (else 0; a wave-uniform value).  Many lanes of a wave with one digit
+// (sorted or skewed keys) would queue their LDS atomics on one address; these
+// lanes are counted and ranked by this one ballot instead.
+constexpr int PEEL_MIN = 16;
+__device__ inline uint64_t peel_mask(uint32_t d, bool valid)
+{
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const uint64_t m0 = __ballot(valid && d == d0);
+    return __popcll(m0) >= PEEL_MIN ? m0 : 0ull;
+}
+
+// cnt[d] += 1 for every valid lane, one LDS atomic per lane outside the
+// peeled group and one for the group.
+template <typename C>
+__device__ inline void count_peeled(C *cnt, uint32_t d, bool valid, int lane)
+{
+    const uint64_t m0 = peel_mask(d, valid);
+    if (m0) {
+        if (lane == int(__builtin_ctzll(m0))) atomicAdd(&cnt[d], uint32_t(__popcll(m0)));
+        else if (valid && !((m0 >> lane) & 1ull)) atomicAdd(&cnt[d], 1u);
+    } else if (valid) {
+        atomicAdd(&cnt[d], 1u);
+    }
+}
+
+// Whether a wave's keys look skewed: many lanes share the first lane's digit
+// in the first or the middle item (sorted or dominated inputs).  Such waves
+// count and rank with peeling (count_peeled, ldsor_rank_item<true>); the
+// others skip its per-item test.
+__device__ inline bool wave_skewed(uint32_t da, bool va, uint32_t db, bool vb)
+{
+    return (peel_mask(da, va) | peel_mask(db, vb)) != 0ull;
+}
+
+// cnt[d] += 1 for every valid lane: count_peeled in skewed waves, else one
+// LDS atomic per lane.
+template <typename C>
+__device__ inline void count_item(C *cnt, uint32_t d, bool valid, int lane, bool skewed)
+{
+    if (skewed) count_peeled(cnt, d, valid, lane);
+    else if (valid) atomicAdd(&cnt[d], 1u);
+}
+
+// One item of LDS-OR ranking in wave-local tables: each valid lane ORs its
+// bit into its digit's 64-bit mask (match, as 32-bit halves in match32),
+// reads the mask back (its peers), and the lowest peer advances the wave's
+// counter cnt[d] by the peer count and clears the mask.  With PEEL a peeled
+// group takes its mask from the ballot.  Returns the lane's rank: cnt[d]
+// before the item plus the peers below the lane (lane order, so stable).
+template <bool PEEL, typename C>
+__device__ inline uint32_t ldsor_rank_item(uint32_t d, bool valid, int lane, C *cnt, uint32_t *match32,
+                                           uint64_t *match)
+{
+    const uint32_t lanebit = 1u << (lane & 31);
+    const int half = lane >> 5;
+    const uint64_t m0 = PEEL ? peel_mask(d, valid) : 0ull;
+    uint64_t peers;
+    bool mine0 = false;
+    if (PEEL && m0) {
+        mine0 = (m0 >> lane) & 1ull;
+        if (valid && !mine0) atomicOr(&match32[2 * d + half], lanebit);
+        __builtin_amdgcn_wave_barrier();
+        peers = mine0 ? m0 : match[d];
+    } else {
+        if (valid) atomicOr(&match32[2 * d + half], lanebit);
+        __builtin_amdgcn_wave_barrier();
+        peers = match[d];
+    }
+    const uint32_t below = count_below(peers);
+    const uint32_t old = cnt[d];
+    if (valid && below == 0) {
+        cnt[d] = C(old + uint32_t(__popcll(peers)));
+        if (!mine0) match[d] = 0ull;
+    }
+    return old + below;
 }
 
 // ---------------------------------------------------------------------------
@@ -438,7 +535,7 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
     uint32_t *__restrict__ vout = src ? valsA : valsB;
     const bool xin = FLOATK && (flags & PF_XIN);
     const bool xout = FLOATK && (flags & PF_XOUT);
-    const int shift = pass < 0 ? int(plan->msd_shift) : pass * RADIX_BITS;
+    const int shift = pass < 0 ? int(plan->msd_shift) : pass_shift(flags, pass);
     const uint32_t *__restrict__ gstart = pass < 0 ? plan->msd_start : plan->gstart[pass];
 
     // With precomputed offsets the tile order is free: give each XCD (work-
@@ -513,6 +610,9 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
     const bool full = n - tile_base >= uint32_t(TILE);
     // in-wave ranks (below 64 * ITEMS), two 16-bit ranks per register
     static_assert(WAVE * ITEMS < 65536, "ranks fit 16 bits");
+    const bool skewed = RANK != RANK_BALLOT &&
+                        wave_skewed(KeyOps<K>::digit(keys[0], shift), lane < wave_valid,
+                                    KeyOps<K>::digit(keys[ITEMS / 2], shift), (ITEMS / 2) * WAVE + lane < wave_valid);
     uint32_t rank2[RANK == RANK_BALLOT ? (ITEMS + 1) / 2 : 1];
     if constexpr (RANK == RANK_BALLOT) {
         auto rank_loop = [&](auto full_c) {
@@ -542,11 +642,11 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
     } else {
         if (full) {
 #pragma unroll
-            for (int i = 0; i < ITEMS; ++i) atomicAdd(&cnt[KeyOps<K>::digit(keys[i], shift)], 1u);
+            for (int i = 0; i < ITEMS; ++i) count_item(cnt, KeyOps<K>::digit(keys[i], shift), true, lane, skewed);
         } else {
 #pragma unroll
             for (int i = 0; i < ITEMS; ++i)
-                if (i * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[i], shift)], 1u);
+                count_item(cnt, KeyOps<K>::digit(keys[i], shift), i * WAVE + lane < wave_valid, lane, skewed);
         }
     }
     __syncthreads();
@@ -582,8 +682,9 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
 
     // ---- 3. rank and stage the tile in LDS in sorted order ----
     uint32_t pos[HAS_VALS ? ITEMS : 1];
-    auto stage_loop = [&](auto full_c) {
+    auto stage_loop = [&](auto full_c, auto peel_c) {
         constexpr bool FULL = decltype(full_c)::value;
+        constexpr bool PEEL = decltype(peel_c)::value;
         if constexpr (RANK == RANK_BALLOT) {
             // cnt[d] now holds the wave's base for digit d
 #pragma unroll
@@ -597,8 +698,6 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
             }
         } else {
             uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
-            const uint32_t lanebit = 1u << (lane & 31);
-            const int half = lane >> 5;
             constexpr int MIX = rank_mix_period<RANK>();
 #pragma unroll
             for (int i = 0; i < ITEMS; ++i) {
@@ -614,17 +713,9 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
                     if constexpr (HAS_VALS) pos[i] = old + below;
                     __builtin_amdgcn_wave_barrier();
                 } else {
-                    if (valid) atomicOr(&match32[2 * d + half], lanebit);
-                    __builtin_amdgcn_wave_barrier();
-                    const uint64_t peers = s_match[wave][d];
-                    const uint32_t below = count_below(peers);
-                    const uint32_t old = cnt[d];
-                    if (valid && below == 0) {
-                        cnt[d] = Cnt(old + uint32_t(__popcll(peers)));
-                        s_match[wave][d] = 0ull;
-                    }
-                    if (valid) s_keys[old + below] = keys[i];
-                    if constexpr (HAS_VALS) pos[i] = old + below;
+                    const uint32_t at = ldsor_rank_item<PEEL>(d, valid, lane, cnt, match32, s_match[wave]);
+                    if (valid) s_keys[at] = keys[i];
+                    if constexpr (HAS_VALS) pos[i] = at;
                     __builtin_amdgcn_wave_barrier();
                 }
             }
@@ -635,8 +726,16 @@ __global__ __launch_bounds__(BLOCK, onesweep_waves_per_eu<BLOCK>()) void k_onesw
                 if (FULL || i * WAVE + lane < wave_valid) s_vals[pos[i]] = vals[i];
         }
     };
-    if (full) stage_loop(std::true_type{});
-    else stage_loop(std::false_type{});
+    if constexpr (RANK == RANK_BALLOT) {
+        if (full) stage_loop(std::true_type{}, std::false_type{});
+        else stage_loop(std::false_type{}, std::false_type{});
+    } else if (full) {
+        if (skewed) stage_loop(std::true_type{}, std::true_type{});
+        else stage_loop(std::true_type{}, std::false_type{});
+    } else {
+        if (skewed) stage_loop(std::false_type{}, std::true_type{});
+        else stage_loop(std::false_type{}, std::false_type{});
+    }
 
     // ---- 4. decoupled look-back over a window of earlier tiles ----
     // A round reads LOOKBACK_WINDOW words at once; after meeting an
@@ -791,16 +890,11 @@ __device__ inline void load_tile_keys(const K *__restrict__ p, uint32_t cnt, boo
     }
 }
 
-// Count one digit per LDS atomic; a wave whose lanes all hold the same digit
-// adds 64 once.
+// Count one digit per LDS atomic; the lanes that share the first lane's
+// digit, when many do, add once (count_peeled).
 __device__ inline void count_digit(uint32_t *sh, uint32_t d, bool valid, int lane)
 {
-    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-    if (__all(valid && d == d0)) {
-        if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
-    } else if (valid) {
-        atomicAdd(&sh[d], 1u);
-    }
+    count_peeled(sh, d, valid, lane);
 }
 
 // Per-tile digit counts of a pass (16-bit: a tile holds fewer than 65536
@@ -819,10 +913,10 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
     static_assert(TILE < 65536, "tile counts are 16-bit");
     __shared__ uint32_t sh[RADIX];
     const uint32_t flags = plan->flags[pass];
-    if (flags & PF_TRIVIAL) return;
+    if (flags & (PF_TRIVIAL | PF_PRECOUNTED)) return;
     const K *__restrict__ kin = (flags & PF_SRC_B) ? keysB : keysA;
     const bool xin = FLOATK && (flags & PF_XIN);
-    const int shift = pass * RADIX_BITS;
+    const int shift = pass_shift(flags, pass);
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
     const uint32_t base = blockIdx.x * uint32_t(TILE);
     if (base >= n) return; // a grid beyond the tiles (every launch sizes it to them): nothing to count
@@ -888,14 +982,17 @@ template <typename K> constexpr int OR_STRIDE = 128 / int(sizeof(K)); // one 128
 // totals[r][d] = sum of counts[t][d] over the SCAN_RANGE tiles of range r,
 // offsets[t][d] = the sum of the range's earlier counts (the tile's offset
 // within the range), and the pass's digit totals dtot[d * LINE_WORDS] +=
-// totals[r][d].
-static __global__ __launch_bounds__(RADIX) void k_range_totals(const uint16_t *__restrict__ counts,
+// totals[r][d].  A PF_PRECOUNTED pass reads counts_pre (the upsweep's).
+static __global__ __launch_bounds__(RADIX) void k_range_totals(const uint16_t *__restrict__ counts_own,
+                                                               const uint16_t *__restrict__ counts_pre,
                                                                uint32_t *__restrict__ offsets, uint32_t tiles,
                                                                uint32_t *__restrict__ totals,
                                                                uint32_t *__restrict__ dtot, int pass,
                                                                const DevPlan *__restrict__ plan)
 {
-    if (plan->flags[pass] & PF_TRIVIAL) return;
+    const uint32_t flags = plan->flags[pass];
+    if (flags & PF_TRIVIAL) return;
+    const uint16_t *__restrict__ counts = (flags & PF_PRECOUNTED) ? counts_pre : counts_own;
     const uint32_t r = blockIdx.x, d = threadIdx.x;
     const uint32_t t0 = r * SCAN_RANGE, t1 = min(tiles, t0 + SCAN_RANGE);
     uint32_t c[SCAN_RANGE];
@@ -1001,14 +1098,17 @@ static __global__ __launch_bounds__(1024) void k_scan_digit(const uint16_t *__re
 // ORSLOTS word pairs (by block) to keep atomics on one address few.
 // ---------------------------------------------------------------------------
 
-template <typename K, int FLOATK, int BLOCK, int ITEMS>
+template <typename K, int FLOATK, int BLOCK, int ITEMS, bool TOP = false>
 __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, uint32_t n,
-                                                   uint16_t *__restrict__ counts, K *__restrict__ orslots, int vec)
+                                                   uint16_t *__restrict__ counts, K *__restrict__ orslots, int vec,
+                                                   uint16_t *__restrict__ counts_top = nullptr)
 {
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(TILE < 65536, "tile counts are 16-bit");
+    static_assert(!TOP || (sizeof(K) == 4 && FLOATK == 0), "top-byte counts serve the MSD-first u32 form");
     constexpr int WAVES = BLOCK / WAVE;
     __shared__ uint32_t sh[RADIX];
+    __shared__ uint32_t sh_top[TOP ? RADIX : 1];
     __shared__ K s_or[WAVES], s_nor[WAVES];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
     const uint32_t base = blockIdx.x * uint32_t(TILE);
@@ -1017,7 +1117,10 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
     // the loads go out first, so their latency covers the LDS set-up
     K k[ITEMS];
     load_tile_keys<K, BLOCK, ITEMS>(keys + base, cnt, vec, k);
-    for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
+    for (int i = tid; i < RADIX; i += BLOCK) {
+        sh[i] = 0u;
+        if constexpr (TOP) sh_top[i] = 0u;
+    }
     __syncthreads();
     K vor = 0, vnor = 0;
     auto count_loop = [&](auto full_c) {
@@ -1032,6 +1135,7 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
                 vnor |= ~key;
             }
             count_digit(sh, KeyOps<K>::digit(key, 0), valid, lane);
+            if constexpr (TOP) count_digit(sh_top, KeyOps<K>::digit(key, 24), valid, lane);
         }
     };
     if (cnt == uint32_t(TILE)) count_loop(std::true_type{});
@@ -1046,7 +1150,10 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
         s_nor[wave] = vnor;
     }
     __syncthreads();
-    for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(blockIdx.x) * RADIX + i] = uint16_t(sh[i]);
+    for (int i = tid; i < RADIX; i += BLOCK) {
+        counts[size_t(blockIdx.x) * RADIX + i] = uint16_t(sh[i]);
+        if constexpr (TOP) counts_top[size_t(blockIdx.x) * RADIX + i] = uint16_t(sh_top[i]);
+    }
     if (tid == 0) {
         K a = 0, b = 0;
         for (int w = 0; w < WAVES; ++w) {
@@ -1061,9 +1168,16 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
 
 // k_plan_rts: trivial passes from the varying bits; buffer selection and the
 // float transform's first/last pass.  Digit starts come later, per pass, from
-// that pass's counts (k_scan_ranges).
+// that pass's counts (k_scan_ranges).  With `hybrid` (4-byte unsigned keys
+// without values whose upsweep also counted the top byte): if a bit from 23
+// up varies, the MSD-first form runs instead -- only pass 3 moves keys in the
+// LSD loop (A to B), by the 8 bits that end at the top varying bit (at most
+// the top byte, at least bits 16..23; counted by the upsweep when that is the
+// top byte, else by the loop's counting kernel), the bucket stages
+// (hpcs_hybrid.hpp) bring them back to A sorted, and plan->hybrid says so.
 template <typename K>
-__global__ void k_plan_rts(const K *__restrict__ orslots, DevPlan *__restrict__ plan, uint64_t n, int floatk)
+__global__ void k_plan_rts(const K *__restrict__ orslots, DevPlan *__restrict__ plan, uint64_t n, int floatk,
+                           int hybrid = 0)
 {
     if (threadIdx.x != 0) return;
     constexpr int P = int(sizeof(K));
@@ -1073,6 +1187,25 @@ __global__ void k_plan_rts(const K *__restrict__ orslots, DevPlan *__restrict__ 
         b |= orslots[s * OR_STRIDE<K> + 1];
     }
     const K varying = a & b;
+    if (P == 4 && hybrid && !floatk && n > 1 && (uint64_t(varying) >> 23) != 0) {
+        const int top = 63 - __builtin_clzll(uint64_t(varying)); // 23 .. 31
+        const int s1 = min(24, top - 7);
+        for (int p = 0; p < P; ++p) {
+            plan->trivial[p] = p == 3 ? 0u : 1u;
+            plan->src[p] = 0u;
+            plan->xin[p] = 0u;
+            plan->xout[p] = 0u;
+        }
+        plan->copyback = 0u;
+        plan->hybrid = 1u;
+        plan->hyb_shift1 = uint32_t(s1);
+        plan->hyb_shift2 = uint32_t(s1 - 8);
+        pack_plan_flags(plan, P);
+        for (int p = 0; p < P; ++p) plan->flags[p] |= PF_NEXT_NONE << PF_NEXT_SHIFT;
+        plan->flags[3] |= PF_FIRST | PF_SHIFT | (uint32_t(s1) << PF_SHIFT_POS) | (s1 == 24 ? PF_PRECOUNTED : 0u);
+        return;
+    }
+    plan->hybrid = 0u;
     uint32_t cur = 0;
     int first = -1, last = -1;
     for (int p = 0; p < P; ++p) {

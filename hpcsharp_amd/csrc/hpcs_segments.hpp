@@ -130,9 +130,11 @@ __device__ inline void tile_rank_stage_ldsor(const K (&keys)[ITEMS], int wave_va
     }
     __syncthreads();
     uint32_t *cnt = s_cnt[wave];
+    const bool skewed = wave_skewed(KeyOps<K>::digit(keys[0], shift), lane < wave_valid,
+                                    KeyOps<K>::digit(keys[ITEMS / 2], shift), (ITEMS / 2) * WAVE + lane < wave_valid);
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
-        if (j * WAVE + lane < wave_valid) atomicAdd(&cnt[KeyOps<K>::digit(keys[j], shift)], 1u);
+        count_item(cnt, KeyOps<K>::digit(keys[j], shift), j * WAVE + lane < wave_valid, lane, skewed);
     __syncthreads();
     uint32_t tile_cnt = 0, tile_start = 0;
     if (tid < RADIX) {
@@ -160,27 +162,22 @@ __device__ inline void tile_rank_stage_ldsor(const K (&keys)[ITEMS], int wave_va
     }
     __syncthreads();
     uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
-    const uint32_t lanebit = 1u << (lane & 31);
-    const int half = lane >> 5;
+    auto rank_loop = [&](auto peel_c) {
+        constexpr bool PEEL = decltype(peel_c)::value;
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t d = KeyOps<K>::digit(keys[j], shift);
-        const bool valid = j * WAVE + lane < wave_valid;
-        if (valid) atomicOr(&match32[2 * d + half], lanebit);
-        __builtin_amdgcn_wave_barrier();
-        const uint64_t peers = s_match[wave][d];
-        const uint32_t below = count_below(peers);
-        const uint32_t old = cnt[d];
-        if (valid && below == 0) {
-            cnt[d] = old + uint32_t(__popcll(peers));
-            s_match[wave][d] = 0ull;
+        for (int j = 0; j < ITEMS; ++j) {
+            const bool valid = j * WAVE + lane < wave_valid;
+            const uint32_t at =
+                ldsor_rank_item<PEEL>(KeyOps<K>::digit(keys[j], shift), valid, lane, cnt, match32, s_match[wave]);
+            if (valid) {
+                s_keys[at] = keys[j];
+                if constexpr (HAS_VALS) s_vals[at] = vals[j];
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        if (valid) {
-            s_keys[old + below] = keys[j];
-            if constexpr (HAS_VALS) s_vals[old + below] = vals[j];
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
+    };
+    if (skewed) rank_loop(std::true_type{});
+    else rank_loop(std::false_type{});
     __syncthreads();
 }
 
@@ -197,7 +194,7 @@ static __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restr
                                                    uint32_t *__restrict__ small_list,
                                                    uint32_t *__restrict__ medium_list,
                                                    uint32_t *__restrict__ large_list, uint64_t max_tiles,
-                                                   int small_max, int tile)
+                                                   int small_max, int tile, int min_small)
 {
     __shared__ uint64_t s_wsum[16];
     __shared__ int64_t s_wmax[16];
@@ -238,7 +235,7 @@ static __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restr
                 const int64_t len = e - b;
                 if (len > small_max) tiles = uint32_t((len + tile - 1) / tile);
                 else if (len > SEG_SMALL) medium = 1;
-                else if (len > 1) small = 1;
+                else if (len >= min_small) small = 1;
             }
         }
         const uint32_t large = tiles ? 1u : 0u;
@@ -288,7 +285,8 @@ static __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restr
 }
 
 // One record per tile of the large segments: {first key, segment's first key,
-// keys in the tile, segment}, so the pass kernels find a tile with one load.
+// keys in the tile, the segment's place in the large list}, so the pass
+// kernels find a tile with one load.
 // Grid-stride over the large segments, the segment's tiles across the threads.
 static __global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restrict__ begin,
                                                    const SegPlan *__restrict__ sp,
@@ -304,7 +302,7 @@ static __global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restr
         const uint32_t nt = uint32_t((len + tile - 1) / tile);
         for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
             const int64_t tb = int64_t(t) * tile;
-            tinfo[t0 + t] = make_uint4(uint32_t(b + tb), uint32_t(b), uint32_t(min(int64_t(tile), len - tb)), s);
+            tinfo[t0 + t] = make_uint4(uint32_t(b + tb), uint32_t(b), uint32_t(min(int64_t(tile), len - tb)), k);
         }
     }
 }
@@ -313,9 +311,12 @@ static __global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restr
 // BLOCK = SEG_BLOCK: the small list (at most SEG_SMALL keys); BLOCK =
 // SEG_MEDIUM_BLOCK: the medium list (at most SEG_MEDIUM keys, one workgroup
 // per CU by its LDS).  Digit positions where every key of the segment agrees
-// are skipped.
+// are skipped.  The sorted segment goes to the same place of keys_out, which
+// is keys or another buffer (then every listed segment is written, sorted or
+// not).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_seg_small(uint32_t *__restrict__ keys, const int64_t *__restrict__ begin,
+__global__ __launch_bounds__(BLOCK) void k_seg_small(const uint32_t *keys, uint32_t *keys_out,
+                                                     const int64_t *__restrict__ begin,
                                                      const SegPlan *__restrict__ sp,
                                                      const uint32_t *__restrict__ list)
 {
@@ -335,7 +336,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(uint32_t *__restrict__ keys
         const uint32_t s = list[k];
         const int64_t b0 = begin[s];
         const int len = int(begin[s + 1] - b0); // at most TILE (k_seg_plan)
-        uint32_t *seg = keys + b0;
+        const uint32_t *seg = keys + b0;
+        uint32_t *seg_out = keys_out + b0;
         const int wave_valid = len > wave * WAVE_TILE ? min(WAVE_TILE, len - wave * WAVE_TILE) : 0;
         uint32_t v[SEG_ITEMS];
         uint32_t vor = 0, vnor = 0;
@@ -380,7 +382,13 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(uint32_t *__restrict__ keys
             staged = true;
         }
         if (staged) {
-            for (int j = tid; j < len; j += BLOCK) seg[j] = s_keys[j];
+            for (int j = tid; j < len; j += BLOCK) seg_out[j] = s_keys[j];
+        } else if (keys_out != keys) {
+#pragma unroll
+            for (int j = 0; j < SEG_ITEMS; ++j) {
+                const int o = j * WAVE + lane;
+                if (o < wave_valid) seg_out[wave * WAVE_TILE + o] = v[j];
+            }
         }
         __syncthreads();
     }
@@ -393,7 +401,8 @@ template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
                                                       const uint32_t *__restrict__ keysB, int pass,
                                                       const SegPlan *__restrict__ sp, const uint4 *__restrict__ tinfo,
-                                                      uint16_t *__restrict__ counts)
+                                                      uint16_t *__restrict__ counts,
+                                                      const uint32_t *__restrict__ dshift)
 {
     static_assert(BLOCK * ITEMS < 65536, "a tile's digit counts fit 16 bits");
     __shared__ uint32_t sh[RADIX];
@@ -401,7 +410,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict
     if (t >= sp->num_tiles) return;
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
-    const int shift = 8 * pass;
+    const int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
     const uint4 ti = tinfo[t];
     const uint32_t tb = ti.x;
     const int cnt = int(ti.z);
@@ -414,16 +423,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict
         k[j] = o < cnt ? kin[size_t(tb) + o] : 0u;
     }
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const bool valid = j * BLOCK + tid < cnt;
-        const uint32_t d = (k[j] >> shift) & 0xFFu;
-        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        if (__all(valid && d == d0)) {
-            if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
-        } else if (valid) {
-            atomicAdd(&sh[d], 1u);
-        }
-    }
+    for (int j = 0; j < ITEMS; ++j) count_digit(sh, (k[j] >> shift) & 0xFFu, j * BLOCK + tid < cnt, lane);
     __syncthreads();
     for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(t) * RADIX + i] = uint16_t(sh[i]);
 }
@@ -505,6 +505,115 @@ static __global__ __launch_bounds__(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS) void k_seg
     }
 }
 
+// The large segments' scan for segments of any length (k_seg_scan walks a
+// segment's tiles in one workgroup, which is slow for segments of many
+// thousands of tiles).  With P(t, d) = the count of digit d over the tile
+// records before t (records of one segment are consecutive):
+//   k_segx_ranges  per range of SEGX_RANGE records: in-range prefixes into
+//                  offsets, range sums into rbase
+//   k_segx_columns per digit: exclusive scan of the range sums (rbase now
+//                  holds P at each range's first record) and the total
+//   k_segx_bases   per large segment k (records t0 .. te): its digit totals
+//                  P(te, d) - P(t0, d), their exclusive scan dstart[k][d]
+//                  (digit starts within the segment) and sbase[k][d] =
+//                  dstart[k][d] - P(t0, d)
+// so that the record t of segment k writes digit d from position
+// sbase[k][d] + rbase[t / SEGX_RANGE][d] + offsets[t][d] of its segment
+// (unsigned arithmetic; the sum lies within the segment).
+constexpr int SEGX_RANGE = 64;
+
+static __global__ __launch_bounds__(RADIX) void k_segx_ranges(const SegPlan *__restrict__ sp,
+                                                              const uint16_t *__restrict__ counts,
+                                                              uint32_t *__restrict__ offsets,
+                                                              uint32_t *__restrict__ rbase)
+{
+    const uint32_t nt = sp->num_tiles;
+    const uint32_t r = blockIdx.x, d = threadIdx.x;
+    const uint32_t t0 = r * SEGX_RANGE;
+    if (t0 >= nt) return;
+    const uint32_t t1 = min(nt, t0 + SEGX_RANGE);
+    uint32_t c[SEGX_RANGE];
+#pragma unroll
+    for (int k = 0; k < SEGX_RANGE; ++k) c[k] = t0 + k < t1 ? uint32_t(counts[size_t(t0 + k) * RADIX + d]) : 0u;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SEGX_RANGE; ++k) {
+        if (t0 + k < t1) offsets[size_t(t0 + k) * RADIX + d] = sum;
+        sum += c[k];
+    }
+    rbase[size_t(r) * RADIX + d] = sum;
+}
+
+// One workgroup of 1024 threads per digit d.
+static __global__ __launch_bounds__(1024) void k_segx_columns(const SegPlan *__restrict__ sp,
+                                                              uint32_t *__restrict__ rbase,
+                                                              uint32_t *__restrict__ total)
+{
+    __shared__ uint32_t s_w[1024 / WAVE];
+    const uint32_t d = blockIdx.x, tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    const uint32_t nr = (sp->num_tiles + SEGX_RANGE - 1) / SEGX_RANGE;
+    const uint32_t per = (nr + 1023) / 1024;
+    const uint32_t r0 = min(nr, tid * per), r1 = min(nr, r0 + per);
+    uint32_t local = 0;
+    for (uint32_t r = r0; r < r1; ++r) local += rbase[size_t(r) * RADIX + d];
+    uint32_t x = local;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= uint32_t(off)) x += y;
+    }
+    if (lane == WAVE - 1) s_w[wave] = x;
+    __syncthreads();
+    uint32_t run = x - local;
+    for (uint32_t w = 0; w < wave; ++w) run += s_w[w];
+    for (uint32_t r = r0; r < r1; ++r) {
+        const size_t at = size_t(r) * RADIX + d;
+        const uint32_t c = rbase[at];
+        rbase[at] = run;
+        run += c;
+    }
+    if (tid == 1023) total[d] = run; // every range summed (the last thread's run ends at nr)
+}
+
+// Grid-stride over the large segments, RADIX threads (one per digit).
+static __global__ __launch_bounds__(RADIX) void k_segx_bases(const int64_t *__restrict__ begin,
+                                                             const SegPlan *__restrict__ sp,
+                                                             const uint32_t *__restrict__ seg_tile0,
+                                                             const uint32_t *__restrict__ large_list,
+                                                             const uint32_t *__restrict__ offsets,
+                                                             const uint32_t *__restrict__ rbase,
+                                                             const uint32_t *__restrict__ total,
+                                                             uint32_t *__restrict__ sbase, uint32_t *__restrict__ dstart,
+                                                             int tile)
+{
+    __shared__ uint32_t s_w[RADIX / WAVE];
+    const uint32_t d = threadIdx.x, lane = d & (WAVE - 1), wave = d / WAVE;
+    const uint32_t nlarge = sp->num_large, nt_all = sp->num_tiles;
+    auto prefix = [&](uint32_t t) -> uint32_t {
+        return t < nt_all ? rbase[size_t(t / SEGX_RANGE) * RADIX + d] + offsets[size_t(t) * RADIX + d] : total[d];
+    };
+    for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
+        const uint32_t s = large_list[k];
+        const uint32_t t0 = seg_tile0[s];
+        const uint32_t nt = uint32_t((begin[s + 1] - begin[s] + tile - 1) / tile);
+        const uint32_t p0 = prefix(t0);
+        const uint32_t tot = prefix(t0 + nt) - p0;
+        uint32_t x = tot;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= uint32_t(off)) x += y;
+        }
+        if (lane == WAVE - 1) s_w[wave] = x;
+        __syncthreads();
+        uint32_t start = x - tot;
+        for (uint32_t w = 0; w < wave; ++w) start += s_w[w];
+        sbase[size_t(k) * RADIX + d] = start - p0;
+        if (dstart) dstart[size_t(k) * RADIX + d] = start;
+        __syncthreads();
+    }
+}
+
 // Scatter of pass `pass` for the large segments' tiles, one workgroup per
 // tile (spare workgroups of the grid return): the plain sort's prescanned
 // scatter (k_onesweep with precomputed offsets) on a tile record.  Workgroup
@@ -520,7 +629,10 @@ template <int BLOCK, int ITEMS, int RANK>
 __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ keysA, uint32_t *__restrict__ keysB,
                                                        int pass, const SegPlan *__restrict__ sp,
                                                        const uint4 *__restrict__ tinfo,
-                                                       const uint32_t *__restrict__ offsets)
+                                                       const uint32_t *__restrict__ offsets,
+                                                       const uint32_t *__restrict__ rbase,
+                                                       const uint32_t *__restrict__ sbase,
+                                                       const uint32_t *__restrict__ dshift)
 {
     static_assert(BLOCK >= RADIX && BLOCK % WAVE == 0 && BLOCK <= 512, "one thread per digit, at most 8 waves");
     constexpr int TILE = BLOCK * ITEMS;
@@ -537,7 +649,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
     uint32_t *__restrict__ kout = (pass & 1) ? keysA : keysB;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
-    const int shift = 8 * pass;
+    const int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
     const uint32_t t = xcd_contiguous(blockIdx.x, ntiles);
     const uint4 ti = tinfo[t];
     const uint32_t tb = ti.x, sb = ti.y;
@@ -556,8 +668,11 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
             keys[i] = off < wave_valid ? kin[tb + uint32_t(wave * WAVE_TILE + off)] : ~0u;
         }
     }
-    // the tile's digit offsets within its segment
-    const uint32_t pre_off = tid < RADIX ? offsets[size_t(t) * RADIX + tid] : 0u;
+    // the tile's digit offsets within its segment (k_seg_scan's, or the sum
+    // of k_segx_* parts: in-range prefix, range base, segment base)
+    uint32_t pre_off = tid < RADIX ? offsets[size_t(t) * RADIX + tid] : 0u;
+    if (rbase != nullptr && tid < RADIX)
+        pre_off += rbase[size_t(t / SEGX_RANGE) * RADIX + tid] + sbase[size_t(ti.w) * RADIX + tid];
 
     for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
     if constexpr (RANK != RANK_BALLOT)
@@ -567,6 +682,9 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     // ---- per-wave digit counts (ballot ranking: the ranks themselves) ----
     uint32_t *cnt = s_cnt[wave];
     const bool full = tile_valid == TILE;
+    const bool skewed = RANK != RANK_BALLOT && wave_skewed((keys[0] >> shift) & (RADIX - 1), lane < wave_valid,
+                                                           (keys[ITEMS / 2] >> shift) & (RADIX - 1),
+                                                           (ITEMS / 2) * WAVE + lane < wave_valid);
     uint32_t rank2[RANK == RANK_BALLOT ? (ITEMS + 1) / 2 : 1];
     if constexpr (RANK == RANK_BALLOT) {
         auto rank_loop = [&](auto full_c) {
@@ -589,11 +707,11 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     } else {
         if (full) {
 #pragma unroll
-            for (int i = 0; i < ITEMS; ++i) atomicAdd(&cnt[(keys[i] >> shift) & (RADIX - 1)], 1u);
+            for (int i = 0; i < ITEMS; ++i) count_item(cnt, (keys[i] >> shift) & (RADIX - 1), true, lane, skewed);
         } else {
 #pragma unroll
             for (int i = 0; i < ITEMS; ++i)
-                if (i * WAVE + lane < wave_valid) atomicAdd(&cnt[(keys[i] >> shift) & (RADIX - 1)], 1u);
+                count_item(cnt, (keys[i] >> shift) & (RADIX - 1), i * WAVE + lane < wave_valid, lane, skewed);
         }
     }
     __syncthreads();
@@ -626,8 +744,9 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     __syncthreads();
 
     // ---- rank and stage the tile in LDS in sorted order ----
-    auto stage_loop = [&](auto full_c) {
+    auto stage_loop = [&](auto full_c, auto peel_c) {
         constexpr bool FULL = decltype(full_c)::value;
+        constexpr bool PEEL = decltype(peel_c)::value;
         if constexpr (RANK == RANK_BALLOT) {
 #pragma unroll
             for (int i = 0; i < ITEMS; ++i)
@@ -636,8 +755,6 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
                         keys[i];
         } else {
             uint32_t *match32 = reinterpret_cast<uint32_t *>(s_match[wave]);
-            const uint32_t lanebit = 1u << (lane & 31);
-            const int half = lane >> 5;
             constexpr int MIX = rank_mix_period<RANK>();
 #pragma unroll
             for (int i = 0; i < ITEMS; ++i) {
@@ -651,23 +768,23 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
                     if (valid) s_keys[old + below] = keys[i];
                     __builtin_amdgcn_wave_barrier();
                 } else {
-                    if (valid) atomicOr(&match32[2 * d + half], lanebit);
-                    __builtin_amdgcn_wave_barrier();
-                    const uint64_t peers = s_match[wave][d];
-                    const uint32_t below = count_below(peers);
-                    const uint32_t old = cnt[d];
-                    if (valid && below == 0) {
-                        cnt[d] = old + uint32_t(__popcll(peers));
-                        s_match[wave][d] = 0ull;
-                    }
-                    if (valid) s_keys[old + below] = keys[i];
+                    const uint32_t at = ldsor_rank_item<PEEL>(d, valid, lane, cnt, match32, s_match[wave]);
+                    if (valid) s_keys[at] = keys[i];
                     __builtin_amdgcn_wave_barrier();
                 }
             }
         }
     };
-    if (full) stage_loop(std::true_type{});
-    else stage_loop(std::false_type{});
+    if constexpr (RANK == RANK_BALLOT) {
+        if (full) stage_loop(std::true_type{}, std::false_type{});
+        else stage_loop(std::false_type{}, std::false_type{});
+    } else if (full) {
+        if (skewed) stage_loop(std::true_type{}, std::true_type{});
+        else stage_loop(std::true_type{}, std::false_type{});
+    } else {
+        if (skewed) stage_loop(std::false_type{}, std::true_type{});
+        else stage_loop(std::false_type{}, std::false_type{});
+    }
     __syncthreads();
 
     // ---- write the tile out, each digit's run contiguous ----

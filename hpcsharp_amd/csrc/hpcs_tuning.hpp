@@ -42,7 +42,10 @@ inline bool seg_small_valid(long long v) { return v == 4096 || v == 16384; }
 // workgroup per bucket (keys without values, at most 2^22 4-byte or 2^21
 // 8-byte keys; larger arrays fall back to the choice by size) / 5 runs of
 // 4096 4-byte or 2048 8-byte keys sorted in LDS, then stable merge passes
-// (arrays of at most 2^20 keys; larger ones fall back to the choice by size);
+// (arrays of at most 2^20 keys; larger ones fall back to the choice by size)
+// / 6 the MSD-first form of 4-byte keys without values (top byte, then byte
+// 2, then each (byte 3, byte 2) bucket in LDS; unsigned keys only, float and
+// signed keys of the row take algo 1 with its shape and ranking);
 // shape: a row of TILE_SHAPES (hpcs_shapes.hpp) that serves the dtype; rank
 // 0 ballot / 1 LDS-OR / 2 and 3 LDS-OR with every second / fourth item by
 // ballot (0 only for ballot-only shapes); digit_bits 8 (default) or 11.  11-bit digits exist in one form
@@ -107,10 +110,11 @@ inline bool parse_tuning(const std::string &text, std::vector<TuneEntry> &out, s
             err = "line " + std::to_string(line_no) + ": unknown dtype " + d;
             return false;
         }
-        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3 && algo != 4 && algo != 5) || rank < 0 || rank > 3 ||
-            !shape_serves(shape, e.key_bytes, e.vals != 0) || (rank != 0 && TILE_SHAPES[shape].ballot_only) ||
-            (bits != 8 && bits != 11) ||
-            (bits == 11 && (algo != 0 || shape != 0 || rank != 0 || e.vals)) || (algo == 4 && e.vals)) {
+        if (max_n < 1 || (algo != 0 && algo != 1 && algo != 3 && algo != 4 && algo != 5 && algo != 6) || rank < 0 ||
+            rank > 3 || !shape_serves(shape, e.key_bytes, e.vals != 0) ||
+            (rank != 0 && TILE_SHAPES[shape].ballot_only) || (bits != 8 && bits != 11) ||
+            (bits == 11 && (algo != 0 || shape != 0 || rank != 0 || e.vals)) || (algo == 4 && e.vals) ||
+            (algo == 6 && (e.vals || e.key_bytes != 4 || bits != 8))) {
             err = "line " + std::to_string(line_no) + ": value out of range";
             return false;
         }

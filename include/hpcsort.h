@@ -77,7 +77,11 @@ extern "C" {
                                       tile is sorted by that one workgroup) cost 4-60x;
                                       5: runs of 4096 4-byte / 2048 8-byte keys sorted in LDS by one
                                       workgroup each, then stable merge passes (up to 2^20 keys; larger
-                                      arrays: by size) */
+                                      arrays: by size),
+                                      6: 4-byte unsigned keys without values, more than 16384: MSD-first by
+                                      the top byte, then byte 2, then each of the (up to 65536) buckets in
+                                      LDS; keys whose top two bytes do not both vary take the passes of 1
+                                      (float and signed keys always do; other sorts: by size) */
 #define HPCS_OPT_DIGIT_BITS 8      /* 8 (default) or 11: 11-bit digits (3 passes for 4-byte keys, 6 for 8-byte
                                       keys) in the onesweep form, for sorts of keys without values below 2^30
                                       keys; other sorts keep 8-bit digits */

@@ -84,12 +84,13 @@ def test_tuning_table_parser(tmp_path):
     lib = N.lib()
     rows = ctypes.c_int32()
     good = ("# comment\nu32 4096 0 0 1   # best\n\nf64 268435456 1 0 1\npairs64 1024 0 0 0\nu64 65536 0 0 0 11\n"
-            "i32 1048576 4 4 1\npairs64 65536 5 0 1\nsegs 268435456 4096 2 9\n")
+            "i32 1048576 4 4 1\npairs64 65536 5 0 1\nu32 268435456 6 0 3\nsegs 268435456 4096 2 9\n")
     assert lib.hpcs_tuning_check(_table(tmp_path, good), ctypes.byref(rows)) == 0
-    assert rows.value == 7
+    assert rows.value == 8
     for bad in ("u16 4096 0 0 1\n", "u64 4096 0 2 1\n", "u32 4096 0 0 1 7\n", "u32 0 0 0 1\n",
                 "u32 4096 2 0 1\n", "u32 4096 3 0 0 11\n", "pairs32 100 0 1 0\n", "u32 4096 0 0\n", "u32 4096 1 0 0 11\n",
-                "pairs64 4096 0 0 0 11\n", "pairs32 4096 4 0 1\n", "u32 4096 6 0 1\n", "u64 4096 4 0 0 11\n",
+                "pairs64 4096 0 0 0 11\n", "pairs32 4096 4 0 1\n", "u32 4096 7 0 1\n", "u64 4096 6 0 0\n",
+                "pairs32 4096 6 0 1\n", "u32 4096 6 0 0 11\n", "u64 4096 4 0 0 11\n",
                 "u32 4096 5 0 0 11\n", "segs 4096 4096 2 5\n"):
         assert lib.hpcs_tuning_check(_table(tmp_path, bad), ctypes.byref(rows)) == N.HPCS_E_INVALID_ARG, bad
         assert b"line 1" in lib.hpcs_last_error()
