@@ -601,16 +601,23 @@ def main():
     gkeys = n / t_step / 1e9
     log(f"[bench] uniform: {t_step * 1e3:.3f} ms/sort, {gkeys:.2f} Gkeys/s; kernels {stats}")
 
-    # dominant kernel: the pass scatter (reduce-then-scan) or onesweep kernel;
+    # dominant kernel: the one with the most time per sort among those that
+    # move every key once (read + write: the pass scatter, onesweep, and in
+    # the MSD-first form its bucket sort and its byte-2 scatter); its
     # algorithmic bytes per launch = one read and one write of every key
-    dom = "scatter" if "scatter" in stats else "onesweep"
+    movers = {"scatter": "k_onesweep as the pass scatter", "onesweep": "k_onesweep",
+              "hyb_local": "k_hyb_local (the LDS bucket sort of the MSD-first form)",
+              "seg_scatter": "k_seg_scatter (the byte-2 split of the MSD-first form)"}
+    msd_first = "hyb_local" in stats
+    dom = max((k for k in movers if k in stats), key=lambda k: stats[k][1])
     d_launch, d_ms = stats.get(dom, (0, 0.0))
-    d_avg_ms = d_ms / max(d_launch, 1)
+    # launches that move keys: in the MSD-first form one of the LSD loop's
+    # four scatter launches (three find nothing to do) and one byte-2 scatter
+    real = {"scatter": args.steps, "seg_scatter": args.steps} if msd_first else {}
+    d_avg_ms = d_ms / max(real.get(dom, d_launch), 1)
     d_bytes = n * 4 * 2
     d_gbs = d_bytes / (d_avg_ms * 1e-3) / 1e9 if d_avg_ms > 0 else 0.0
     sort_gbs = info.ideal_bytes / t_step / 1e9
-    per_key = {"scatter": 8, "onesweep": 8, "tile_counts": 4, "upsweep": 4, "histogram": 4}
-    kernel_gbs = {k: round(per_key[k] * n / (v[1] / v[0] * 1e-3) / 1e9, 1) for k, v in stats.items() if k in per_key}
 
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes of this command (calibrated against a
@@ -619,15 +626,18 @@ def main():
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "sort_u32_2p28_uniform_traffic.json")
     if n == N_DEFAULT and os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = int(json.load(f)["traffic_bytes_per_launch"])
+            tj = json.load(f)
+        if tj.get("timed_name") == dom:  # a profile of this kernel
+            traffic = int(tj["traffic_bytes_per_launch"])
 
     extra = {"sort_ms": round(t_step * 1e3, 4),
              "sort_ms_with_kernel_events": round(t_events * 1e3, 4),
              "sort_ideal_bytes": int(info.ideal_bytes),
              "sort_roofline_frac": round(sort_gbs / HBM_PEAK_GBS, 4),
-             "kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in stats.items()},
+             "kernel_ms_per_sort": {k: round(v[1] / args.steps, 4) for k, v in stats.items()},
              "kernel_launches_per_sort": {k: v[0] // args.steps for k, v in stats.items()},
-             "kernel_algorithmic_GBs": kernel_gbs}
+             "sort_form": "MSD-first: top-byte and byte-2 splits, then 65536 LDS buckets (pass algorithm 6)"
+                          if msd_first else "LSD passes"}
     if not args.no_dists:
         dists = {}
         for dist in ("low8", "sorted", "constant"):
@@ -697,9 +707,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(d_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(d_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": d_bytes,
-                     "kernel": f"k_onesweep as {dom} (one 8-bit pass: read + write 4 B per key); launch "
-                               f"durations from HIP events on the launch stream over a second timed region of "
-                               f"{args.steps} sorts"},
+                     "kernel": f"{movers[dom]} (read + write 4 B per key); launch durations from HIP events "
+                               f"on the launch stream over a second timed region of {args.steps} sorts"},
         "cpu_baseline": cpu,
         "extra": extra,
     }

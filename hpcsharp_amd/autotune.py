@@ -3,7 +3,8 @@ GPU and write the tuning table libhpcsort.so reads (hpcs_ctx_load_tuning).
 
 Variants: digit width (8 bits; 11 bits for keys without values), pass
 algorithm (onesweep with look-back, reduce-then-scan, one workgroup in LDS
-for small arrays), tile shape (every row of the library's shape table that
+for small arrays, block-sort + merge up to 2^20 keys, the MSD-first form for
+4-byte keys without values), tile shape (every row of the library's shape table that
 serves the key type, hpcs_tile_shape_query: 256, 512 and 1024 threads, 8 to
 32 keys per thread) and ranking (ballot, LDS-OR, LDS-OR with every second or
 fourth item by ballot; ballot only on the 1024-thread shapes).
@@ -65,6 +66,12 @@ def candidates(key_bytes: int, vals: bool, n: int | None = None):
         out.append({"algo": 3, "shape": 0, "rank": 1, "bits": 8})
     if n is not None and BLOCK_SORT_MAX[key_bytes] < n <= MERGE_SORT_MAX:
         out.append({"algo": 5, "shape": 0, "rank": 1, "bits": 8})
+    if key_bytes == 4 and not vals and n is not None and n > BLOCK_SORT_MAX[4]:
+        # the MSD-first form: its top-byte pass and its segmented byte-2 pass
+        # take the shape (rows of at most 512 threads) and the ranking
+        out.extend({"algo": 6, "shape": s, "rank": r, "bits": 8}
+                   for (s, ballot_only), r in itertools.product(shapes_for(4, False), RANKS)
+                   if not ballot_only)
     return out
 
 

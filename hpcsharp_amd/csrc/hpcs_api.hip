@@ -76,6 +76,10 @@ constexpr int PASS_BLOCK = 3; // one workgroup sorts the whole array in LDS
 constexpr int PASS_MSD = 4;   // one MSD split, then one workgroup per bucket (keys without values)
 constexpr int PASS_MERGE = 5; // runs sorted by one workgroup each in LDS, then stable merge passes
 constexpr int PASS_HYBRID = 6; // 4-byte unsigned keys without values: top two bytes MSD-first, then LDS buckets
+// Largest sort of the MSD-first form: its 65536 buckets then hold 4608 keys on
+// average for uniform keys, within the LDS sort's HYB_CAP (5120) with margin;
+// larger arrays take the reduce-then-scan passes.
+constexpr int64_t HYB_MAX_KEYS = (int64_t(1) << 28) + (int64_t(1) << 25);
 // largest arrays the block-sort + merge form serves (beyond, the passes over
 // the whole array outnumber the radix passes)
 constexpr int64_t MERGE_SORT_MAX = int64_t(1) << 20;
@@ -495,7 +499,8 @@ SortParams resolve_params(const hpcs_ctx *c, int key_bytes, bool vals, int64_t n
     if (p.algo == PASS_MERGE && n > MERGE_SORT_MAX) p.algo = PASS_AUTO;
     // the MSD-first form sorts 4-byte keys without values (run_sort sends
     // float and signed keys to the reduce-then-scan passes)
-    if (p.algo == PASS_HYBRID && (key_bytes != 4 || vals || n <= BLOCK_SORT_MAX4)) p.algo = PASS_AUTO;
+    if (p.algo == PASS_HYBRID && (key_bytes != 4 || vals || n <= BLOCK_SORT_MAX4 || n > HYB_MAX_KEYS))
+        p.algo = PASS_AUTO;
     // by size, measured on MI355X: one workgroup up to 16384 4-byte / 8192
     // 8-byte keys (with LDS-OR ranking it beats onesweep passes there: u32
     // 16384 keys 42.2 against 48.4 us)
@@ -2173,8 +2178,11 @@ HPCS_API int32_t hpcs_plan_query(hpcs_ctx *ctx, int32_t dtype, int64_t n, int32_
     info->tile_keys = tile_keys;
     info->block_threads = TILE_SHAPES[shape].block;
     info->tiles = L.tiles;
+    // the LSD form's ideal (one counting read, a read and a write per pass),
+    // also for the MSD-first form, which moves fewer bytes
     info->ideal_bytes = n * per_key;
     info->scratch_bytes = int64_t(L.total);
+    if (prm.algo == PASS_HYBRID && dtype == HPCS_DTYPE_U32) info->scratch_bytes = int64_t(make_hyb_layout(n, shape).total);
     return HPCS_OK;
 }
 #endif
@@ -2198,6 +2206,8 @@ HPCS_API int32_t hpcs_ctx_reserve(hpcs_ctx *ctx, int32_t dtype, int64_t n)
             if (!shape_serves(shape, int(ks), false)) continue;
             const int tk = shape_tile_keys(shape);
             need = std::max(need, make_layout(n, ks, false, tk).total);
+            if (ks == 4 && shape_serves(shape, 4, false) && n > BLOCK_SORT_MAX4 && n <= HYB_MAX_KEYS)
+                need = std::max(need, make_hyb_layout(n, shape).total); // the MSD-first form
         }
         if (n < LOOKBACK_MAX_KEYS) need = std::max(need, make_layout11(n, ks).total);
         const size_t topk = align_up(align_up(align_up(sizeof(TopkState)) + size_t(n) * ks) + size_t(n) * ks);

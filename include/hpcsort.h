@@ -78,7 +78,7 @@ extern "C" {
                                       5: runs of 4096 4-byte / 2048 8-byte keys sorted in LDS by one
                                       workgroup each, then stable merge passes (up to 2^20 keys; larger
                                       arrays: by size),
-                                      6: 4-byte unsigned keys without values, more than 16384: MSD-first by
+                                      6: 4-byte unsigned keys without values, 16385 to 2^28 + 2^25: MSD-first by
                                       the top byte, then byte 2, then each of the (up to 65536) buckets in
                                       LDS; keys whose top two bytes do not both vary take the passes of 1
                                       (float and signed keys always do; other sorts: by size) */
