@@ -56,13 +56,9 @@ static __global__ __launch_bounds__(RADIX) void k_hyb_begin1(const DevPlan *__re
     if (d < int(sizeof(SegPlan) / 4)) reinterpret_cast<uint32_t *>(sp2)[d] = 0u;
 }
 
-// Bin counters are 16-bit, two per LDS word (a sub-bucket holds fewer than
-// 65536 keys): bin b is half b & 1 of word b >> 1.  LDS slot of word w: one
-// padding word per 8, so the scan's thread t (words 8t .. 8t + 7, bins 16t ..
-// 16t + 15) meets no bank conflict at any step.
-constexpr int HYB_WORDS = HYB_NBIN / 2;
-__device__ __forceinline__ int hyb_slot(int w) { return w + (w >> 3); }
-__device__ __forceinline__ uint32_t hyb_half(uint32_t word, int b) { return (word >> ((b & 1) << 4)) & 0xFFFFu; }
+// LDS slot of bin b: one padding word per 16 bins, so the scan's thread t
+// (bins 16t .. 16t + 15) meets no bank conflict at any step.
+__device__ __forceinline__ int hyb_slot(int b) { return b + (b >> 4); }
 
 
 // Workgroup b = k * RADIX + d sorts sub-bucket d (byte 2 = d) of the k-th
@@ -84,10 +80,10 @@ __global__ __launch_bounds__(BLOCK) void k_hyb_local(uint32_t *__restrict__ keys
 {
     constexpr int CAP = BLOCK * ITEMS;
     constexpr int WAVES = BLOCK / WAVE;
-    constexpr int PER = HYB_WORDS / BLOCK; // counter words per thread in the scan
-    static_assert(HYB_WORDS == 8 * BLOCK, "the scan: eight words (16 bins) per thread");
-    static_assert(CAP < 65536, "in-bucket positions and bin counts fit 16 bits");
-    __shared__ uint32_t s_cnt[HYB_WORDS + HYB_WORDS / 8];
+    constexpr int PER = HYB_NBIN / BLOCK; // bins per thread in the scan
+    static_assert(HYB_NBIN == 16 * BLOCK, "the scan: 16 bins per thread");
+    static_assert(CAP < 65536, "in-bucket slots fit 16 bits");
+    __shared__ uint32_t s_cnt[HYB_NBIN + HYB_NBIN / 16];
     __shared__ uint16_t s_low[CAP];
     __shared__ uint32_t s_wsum[WAVES], s_wmax[WAVES], s_top;
     const uint32_t k = blockIdx.x / RADIX, d = blockIdx.x % RADIX;
@@ -114,24 +110,21 @@ __global__ __launch_bounds__(BLOCK) void k_hyb_local(uint32_t *__restrict__ keys
             const int o = j * BLOCK + tid;
             v[j] = o < len ? seg[o] : 0u;
         }
-        for (int i = tid; i < HYB_WORDS + HYB_WORDS / 8; i += BLOCK) s_cnt[i] = 0u;
+        for (int i = tid; i < HYB_NBIN + HYB_NBIN / 16; i += BLOCK) s_cnt[i] = 0u;
         if (tid == 0) s_top = v[0] & 0xFFFF0000u; // key 0: every key of the sub-bucket shares bits 16..31
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
-            if (j * BLOCK + tid < len) {
-                const int b = int((v[j] >> bs) & (HYB_NBIN - 1));
-                atomicAdd(&s_cnt[hyb_slot(b >> 1)], 1u << ((b & 1) << 4));
-            }
+            if (j * BLOCK + tid < len) atomicAdd(&s_cnt[hyb_slot(int((v[j] >> bs) & (HYB_NBIN - 1)))], 1u);
         __syncthreads();
-        // exclusive scan of the bins (thread t owns words PER t ..) and the largest bin
+        // exclusive scan of the bins (thread t owns bins PER t ..) and the largest bin
         uint32_t c[PER];
         uint32_t sum = 0, mx = 0;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             c[i] = s_cnt[hyb_slot(PER * tid + i)];
-            sum += (c[i] & 0xFFFFu) + (c[i] >> 16);
-            mx = max(mx, max(c[i] & 0xFFFFu, c[i] >> 16));
+            sum += c[i];
+            mx = max(mx, c[i]);
         }
         uint32_t x = sum;
 #pragma unroll
@@ -154,9 +147,8 @@ __global__ __launch_bounds__(BLOCK) void k_hyb_local(uint32_t *__restrict__ keys
         if (!over) {
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
-                const uint32_t lo16 = c[i] & 0xFFFFu;
-                s_cnt[hyb_slot(PER * tid + i)] = run | ((run + lo16) << 16);
-                run += lo16 + (c[i] >> 16);
+                s_cnt[hyb_slot(PER * tid + i)] = run;
+                run += c[i];
             }
         }
     }
@@ -180,13 +172,13 @@ __global__ __launch_bounds__(BLOCK) void k_hyb_local(uint32_t *__restrict__ keys
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
         if (j * BLOCK + tid < len) {
-            const int b = int((v[j] >> bs) & (HYB_NBIN - 1));
-            const uint32_t old = atomicAdd(&s_cnt[hyb_slot(b >> 1)], 1u << ((b & 1) << 4));
-            s_low[hyb_half(old, b)] = uint16_t(v[j]);
+            const uint32_t pos = atomicAdd(&s_cnt[hyb_slot(int((v[j] >> bs) & (HYB_NBIN - 1)))], 1u);
+            s_low[pos] = uint16_t(v[j]);
         }
     __syncthreads();
     // s_cnt now holds each bin's end; the key at slot p of bin [lb, hb) goes
-    // to lb + (keys of the bin below it) + (equal keys at earlier slots).
+    // to lb + (keys of the bin below it) + (equal keys at earlier slots):
+    // the keys (low 16 bits, slot) before it in one 32-bit order.
     const uint32_t top = s_top;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
@@ -194,13 +186,11 @@ __global__ __launch_bounds__(BLOCK) void k_hyb_local(uint32_t *__restrict__ keys
         if (p < len) {
             const uint32_t xk = s_low[p];
             const int bin = int((xk >> bs) & (HYB_NBIN - 1));
-            const uint32_t hb = hyb_half(s_cnt[hyb_slot(bin >> 1)], bin);
-            const uint32_t lb = bin ? hyb_half(s_cnt[hyb_slot((bin - 1) >> 1)], bin - 1) : 0u;
+            const uint32_t hb = s_cnt[hyb_slot(bin)];
+            const uint32_t lb = bin ? s_cnt[hyb_slot(bin - 1)] : 0u;
+            const uint32_t me = (xk << 16) | uint32_t(p);
             uint32_t r = 0;
-            for (uint32_t q = lb; q < hb; ++q) {
-                const uint32_t y = s_low[q];
-                r += (y < xk || (y == xk && q < uint32_t(p))) ? 1u : 0u;
-            }
+            for (uint32_t q = lb; q < hb; ++q) r += (((uint32_t(s_low[q]) << 16) | q) < me) ? 1u : 0u;
             seg[lb + r] = top | xk;
         }
     }
