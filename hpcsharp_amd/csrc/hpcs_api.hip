@@ -694,10 +694,10 @@ void run_sort_msd(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st, const SortP
 }
 
 void launch_seg_counts(int shape, dim3 grid, hipStream_t st, const uint32_t *a, const uint32_t *b, int pass,
-                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift);
+                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift, bool peel);
 void launch_seg_scatter(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
                         const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
-                        const uint32_t *sbase, const uint32_t *dshift);
+                        const uint32_t *sbase, const uint32_t *dshift, bool peel);
 
 // Scratch of the MSD-first form (hpcs_hybrid.hpp): the reduce-then-scan
 // layout, then the top-byte tile counts, the bucket bounds, level 2's and the
@@ -797,17 +797,18 @@ void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, c
     timed(c, st, "plan", [&] {
         hipLaunchKernelGGL((k_plan_rts<K>), dim3(1), dim3(64), 0, st, orslots, plan, uint64_t(n), 0, 1);
     });
-    // the LSD loop: in the MSD-first form only pass 3 (the top byte) runs
-    for (int p = 0; p < 4; ++p) {
-        if (p > 0)
-            timed(c, st, "tile_counts", [&] {
-                launch_tile_counts<K, false, 0>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan, counts,
-                                                nullptr);
-            });
+    // the global pass slot (plan pass 3): the first split, or the LSD pass
+    // over the lowest varying byte
+    {
+        constexpr int p = 3;
+        timed(c, st, "tile_counts", [&] {
+            launch_tile_counts<K, false, 0>(shape, dim3(tiles), st, d_keys, keysB, uint32_t(n), p, plan, counts,
+                                            nullptr);
+        });
         timed(c, st, "scan", [&] {
             uint32_t *dtot_p = dtot + size_t(p) * RADIX * LINE_WORDS;
-            hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, p == 3 ? counts3 : counts,
-                               offs, tiles, totals, dtot_p, p, plan);
+            hipLaunchKernelGGL(k_range_totals, dim3(nranges), dim3(RADIX), 0, st, counts, counts3, offs, tiles,
+                               totals, dtot_p, p, plan);
             hipLaunchKernelGGL(k_scan_columns, dim3(RADIX), dim3(1024), 0, st, totals, nranges, dtot_p, p, plan);
         });
         timed(c, st, "scatter", [&] {
@@ -833,7 +834,7 @@ void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, c
     });
     timed(c, st, "seg_counts", [&] {
         launch_seg_counts(H.seg_shape, dim3(unsigned(H.max_tiles1)), st, keysB, d_keys, 2, sp, tinfo, scnt,
-                          &plan->hyb_shift2);
+                          &plan->hyb_shift2, true);
     });
     // the large segments' scan: range sums, column scan, segment bases
     auto segx_scan = [&](const int64_t *bg, const SegPlan *p, const uint32_t *t0, const uint32_t *lst, uint64_t mt,
@@ -849,7 +850,7 @@ void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, c
     segx_scan(begin1, sp, tile0, large, H.max_tiles1, RADIX, dstart);
     timed(c, st, "seg_scatter", [&] {
         launch_seg_scatter(H.seg_shape, rank_mode, dim3(unsigned(H.max_tiles1)), st, keysB, d_keys, 2, sp, tinfo,
-                           soff, rbase, sbase, &plan->hyb_shift2);
+                           soff, rbase, sbase, &plan->hyb_shift2, true);
     });
     // level 3: every sub-bucket in LDS, in place
     timed(c, st, "hyb_local", [&] {
@@ -868,12 +869,12 @@ void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, c
     for (int p = 0; p < 2; ++p) {
         timed(c, st, "seg_counts", [&] {
             launch_seg_counts(H.seg_shape, dim3(unsigned(H.max_tiles2)), st, d_keys, keysB, p, sp2, tinfo, scnt,
-                              nullptr);
+                              &plan->hyb_ov_shift[p], true);
         });
         segx_scan(pairs, sp2, tile02, large2, H.max_tiles2, H.max_large2, nullptr);
         timed(c, st, "seg_scatter", [&] {
             launch_seg_scatter(H.seg_shape, rank_mode, dim3(unsigned(H.max_tiles2)), st, d_keys, keysB, p, sp2, tinfo,
-                               soff, rbase, sbase, nullptr);
+                               soff, rbase, sbase, &plan->hyb_ov_shift[p], true);
         });
     }
     timed(c, st, "copyback", [&] {
@@ -1208,33 +1209,58 @@ void with_seg_shape(int shape, F &&f)
     }
 }
 
+// Segmented passes: `peel` (the MSD-first form's stages, whose inputs keep
+// the caller's runs of equal digits) counts and ranks skewed waves by
+// peeling; the batched segments' passes do without its per-wave test.
 void launch_seg_counts(int shape, dim3 grid, hipStream_t st, const uint32_t *a, const uint32_t *b, int pass,
-                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift)
+                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift, bool peel)
 {
     with_seg_shape(shape, [&](auto sc) {
         constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
-        hipLaunchKernelGGL((k_seg_counts<T.block, T.items>), grid, dim3(T.block), 0, st, a, b, pass, sp, tinfo, counts,
-                           dshift);
+        if (peel)
+            hipLaunchKernelGGL((k_seg_counts<T.block, T.items, true>), grid, dim3(T.block), 0, st, a, b, pass, sp,
+                               tinfo, counts, dshift);
+        else
+            hipLaunchKernelGGL((k_seg_counts<T.block, T.items, false>), grid, dim3(T.block), 0, st, a, b, pass, sp,
+                               tinfo, counts, dshift);
     });
 }
 
 void launch_seg_scatter(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
                         const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
-                        const uint32_t *sbase, const uint32_t *dshift)
+                        const uint32_t *sbase, const uint32_t *dshift, bool peel);
+
+template <bool PEEL>
+void launch_seg_scatter_t(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
+                          const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
+                          const uint32_t *sbase, const uint32_t *dshift);
+
+void launch_seg_scatter(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
+                        const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
+                        const uint32_t *sbase, const uint32_t *dshift, bool peel)
+{
+    if (peel) return launch_seg_scatter_t<true>(shape, rank, grid, st, a, b, pass, sp, tinfo, offsets, rbase, sbase, dshift);
+    launch_seg_scatter_t<false>(shape, rank, grid, st, a, b, pass, sp, tinfo, offsets, rbase, sbase, dshift);
+}
+
+template <bool PEEL>
+void launch_seg_scatter_t(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
+                          const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
+                          const uint32_t *sbase, const uint32_t *dshift)
 {
     with_seg_shape(shape, [&](auto sc) {
         constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
         if (rank == RANK_BALLOT)
-            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_BALLOT>), grid, dim3(T.block), 0, st, a, b, pass,
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_BALLOT, PEEL>), grid, dim3(T.block), 0, st, a, b, pass,
                                sp, tinfo, offsets, rbase, sbase, dshift);
         else if (rank == RANK_MIX2)
-            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX2>), grid, dim3(T.block), 0, st, a, b, pass,
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX2, PEEL>), grid, dim3(T.block), 0, st, a, b, pass,
                                sp, tinfo, offsets, rbase, sbase, dshift);
         else if (rank == RANK_MIX4)
-            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX4>), grid, dim3(T.block), 0, st, a, b, pass,
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX4, PEEL>), grid, dim3(T.block), 0, st, a, b, pass,
                                sp, tinfo, offsets, rbase, sbase, dshift);
         else if (rank == RANK_LDSOR)
-            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_LDSOR>), grid, dim3(T.block), 0, st, a, b, pass,
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_LDSOR, PEEL>), grid, dim3(T.block), 0, st, a, b, pass,
                                sp, tinfo, offsets, rbase, sbase, dshift);
         else
             fail(HPCS_E_UNSUPPORTED, "unknown ranking mode " + std::to_string(rank));
@@ -1283,7 +1309,7 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     const dim3 tile_grid(unsigned(L.max_tiles));
     for (int p = 0; p < 4; ++p) {
         timed(c, st, "seg_counts", [&] {
-            launch_seg_counts(prm.shape, tile_grid, st, d_keys, keysB, p, sp, tinfo, counts, nullptr);
+            launch_seg_counts(prm.shape, tile_grid, st, d_keys, keysB, p, sp, tinfo, counts, nullptr, false);
         });
         timed(c, st, "seg_scan", [&] {
             hipLaunchKernelGGL(k_seg_scan, dim3(list_grid), dim3(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS), 0, st, d_begin, sp,
@@ -1291,7 +1317,7 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
         });
         timed(c, st, "seg_scatter", [&] {
             launch_seg_scatter(prm.shape, prm.rank, tile_grid, st, d_keys, keysB, p, sp, tinfo, offsets, nullptr,
-                               nullptr, nullptr);
+                               nullptr, nullptr, false);
         });
     }
 }

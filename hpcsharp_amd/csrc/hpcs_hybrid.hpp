@@ -7,25 +7,32 @@
 // Sequence (every launch planned on the device, graph-capturable):
 //   k_upsweep<TOP>   one read: OR / AND of the keys, per-tile counts of byte 0
 //                    and of byte 3
-//   k_plan_rts       if bytes 3 and 2 both vary: the MSD-first form (plan->
-//                    hybrid); otherwise the plain LSD passes run as always and
-//                    every stage below finds nothing to do
-//   LSD loop         only pass 3 moves keys: range sums, column scan, scatter
-//                    A -> B by the top byte (256 buckets, contiguous)
-//   k_hyb_begin1     the buckets' bounds from the pass's digit starts
+//   k_plan_rts       if a bit from 23 up varies: the MSD-first form, split
+//                    digits s1 = min(24, top varying bit - 7) and s2 = s1 - 8
+//                    (HYB_MSD); otherwise the LSD passes over the varying
+//                    bytes, one per stage slot below (HYB_LSD)
+//   global pass      pass slot 3 of the plan: tile counts (the upsweep's when
+//                    the digit is the top byte or byte 0), range sums, column
+//                    scan, scatter A -> B by the 8 bits at s1 (256 buckets)
+//   k_hyb_begin1     the segments of the segmented pass: the buckets
 //   level 2          the batched segments' machinery (hpcs_segments.hpp) on
 //                    the 256 buckets in B: buckets of at most SEG_SMALL keys
-//                    sorted whole in LDS into A, larger ones split by byte 2
-//                    (counts, scan, scatter B -> A) into 256 sub-buckets each
+//                    sorted whole in LDS into A, larger ones split by the 8
+//                    bits at s2 (counts, scan, scatter B -> A) into 256
+//                    sub-buckets each
 //   k_hyb_local      one workgroup per sub-bucket, in place in A: its keys
-//                    share bytes 3 and 2, so a counting pass over 12-bit bins
-//                    (bits 4..15) in LDS and a rank within each bin (bits
-//                    0..3, at most HYB_MAX_BIN keys per bin) finish it: one
-//                    read and one write per key
+//                    share bits 16..31, so a counting pass over the 12-bit
+//                    bins below s2 in LDS and a rank within each bin (at most
+//                    HYB_MAX_BIN keys) finish it: one read and one write per
+//                    key
 //   overflow         sub-buckets beyond HYB_CAP keys or with a larger bin:
 //                    up to SEG_MEDIUM keys one 1024-thread workgroup in LDS,
-//                    larger ones two segmented passes (bytes 0, 1: A -> B ->
-//                    A)
+//                    larger ones two segmented passes (bits 0-7, 8-15: A -> B
+//                    -> A)
+// With HYB_LSD the lowest varying byte takes the global pass, the second the
+// segmented pass over one segment of all keys, the third the first overflow
+// pass, and a copy back follows an odd number of them; each slot without a
+// byte does nothing.
 // Bytes moved per key: upsweep 4, top-byte pass 8, byte-2 pass 12 (count,
 // read, write), local sort 8 = 32 (the LSD passes move 48 in this form: one
 // counting read per later pass).
@@ -41,18 +48,20 @@ constexpr int HYB_NBIN = 4096;                 // 12-bit bins: key bits 4..15
 constexpr int HYB_MAX_BIN = 256;               // keys per bin the rank loop takes
 constexpr int HYB_SUB = RADIX * RADIX;         // sub-buckets: one per (byte 3, byte 2)
 
-// The bounds of the 256 top-byte buckets (begin1[0..256]) from pass 3's digit
-// starts, and a clear overflow plan (sp2).  Without the MSD-first form every
-// bucket is empty, so the later stages do nothing.  One workgroup of RADIX
-// threads.
+// The segments of the segmented pass (begin1[0..256]) and a clear overflow
+// plan (sp2).  MSD-first form: the 256 buckets of the first split, from pass
+// 3's digit starts.  LSD passes with a second varying byte: segment 0 holds
+// every key.  Otherwise every segment is empty, so the later stages do
+// nothing.  One workgroup of RADIX threads.
 static __global__ __launch_bounds__(RADIX) void k_hyb_begin1(const DevPlan *__restrict__ plan, uint64_t n,
                                                           int64_t *__restrict__ begin1, SegPlan *__restrict__ sp2)
 {
     static_assert(sizeof(SegPlan) / 4 <= RADIX, "one thread per word of the overflow plan");
     const int d = threadIdx.x;
-    const bool on = plan->hybrid != 0u;
-    begin1[d] = on ? int64_t(plan->gstart[3][d]) : int64_t(0);
-    if (d == 0) begin1[RADIX] = on ? int64_t(n) : int64_t(0);
+    const uint32_t mode = plan->hybrid;
+    const bool whole = mode == HYB_LSD && plan->hyb_shift2 < HYB_SKIP;
+    begin1[d] = mode == HYB_MSD ? int64_t(plan->gstart[3][d]) : (whole && d > 0) ? int64_t(n) : int64_t(0);
+    if (d == 0) begin1[RADIX] = (mode == HYB_MSD || whole) ? int64_t(n) : int64_t(0);
     if (d < int(sizeof(SegPlan) / 4)) reinterpret_cast<uint32_t *>(sp2)[d] = 0u;
 }
 
@@ -87,8 +96,22 @@ __global__ __launch_bounds__(BLOCK) void k_hyb_local(uint32_t *__restrict__ keys
     __shared__ uint16_t s_low[CAP];
     __shared__ uint32_t s_wsum[WAVES], s_wmax[WAVES], s_top;
     const uint32_t k = blockIdx.x / RADIX, d = blockIdx.x % RADIX;
-    if (k >= sp->num_large) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    if (plan->hybrid != HYB_MSD) {
+        // the LSD passes: a third varying byte takes the first overflow
+        // pass over one segment of all keys
+        if (plan->hybrid == HYB_LSD && plan->hyb_ov_shift[0] < HYB_SKIP && blockIdx.x == 0 && tid == 0) {
+            const int64_t n = begin1[RADIX];
+            pairs[0] = 0;
+            pairs[1] = n;
+            tile0_2[0] = 0u;
+            large2[0] = 0u;
+            sp2->num_large = 1u;
+            sp2->num_tiles = uint32_t((n + tile - 1) / tile);
+        }
+        return;
+    }
+    if (k >= sp->num_large) return;
     const uint32_t s = large_list[k];
     const int64_t base = begin1[s];
     const uint32_t *__restrict__ first = dstart + size_t(k) * RADIX;
@@ -178,8 +201,18 @@ __global__ __launch_bounds__(BLOCK) void k_hyb_local(uint32_t *__restrict__ keys
     __syncthreads();
     // s_cnt now holds each bin's end; the key at slot p of bin [lb, hb) goes
     // to lb + (keys of the bin below it) + (equal keys at earlier slots):
-    // the keys (low 16 bits, slot) before it in one 32-bit order.
+    // the keys (low 16 bits, slot) before it in one 32-bit order.  When the
+    // bins cover every bit below s2 (s2 <= 12), a bin holds equal keys only
+    // and slot p is the key's place.
     const uint32_t top = s_top;
+    if (plan->hyb_shift2 <= 12u) {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int p = j * BLOCK + tid;
+            if (p < len) seg[p] = top | s_low[p];
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const int p = j * BLOCK + tid;

@@ -73,13 +73,13 @@ struct DevPlan {
     uint32_t msd_shift;
     uint32_t msd_start[RADIX];          // digit counts (k_msd_hist), then exclusive starts
     // MSD-first form of the 4-byte keys-only sort (k_plan_rts with `hybrid`):
-    // 1 when it runs (some bit from 23 up varies), so that pass 3 alone moves
-    // keys in the LSD loop (by the 8-bit digit at hyb_shift1) and the
-    // two-level bucket stages (the digit at hyb_shift2 = hyb_shift1 - 8, then
-    // the low 16 bits) finish the sort
+    // the stages' mode (HYB_*), the digit positions of its global pass slot
+    // (pass 3 of the plan) and of its segmented pass, and the digit
+    // positions of the two overflow passes (HYB_SKIP: the pass does nothing)
     uint32_t hybrid;
     uint32_t hyb_shift1, hyb_shift2;
-    uint32_t pad[42];
+    uint32_t hyb_ov_shift[2];
+    uint32_t pad[40];
 };
 constexpr uint32_t PF_TRIVIAL = 1u, PF_SRC_B = 2u, PF_XIN = 4u, PF_XOUT = 8u;
 // The pass's tile counts were written by the upsweep (the MSD-first form's
@@ -90,6 +90,14 @@ constexpr uint32_t PF_PRECOUNTED = 32u;
 // at 8 * pass (the MSD-first form's first split).
 constexpr uint32_t PF_SHIFT = 64u;
 constexpr int PF_SHIFT_POS = 16;
+// The pass's tile counts are the upsweep's byte-0 counts, already in the
+// counts array: its counting kernel does not run.
+constexpr uint32_t PF_PRE0 = 128u;
+// Modes of the MSD-first form's stages (DevPlan::hybrid): nothing to do;
+// the MSD-first form (top varying bit 23 or higher); the LSD passes over the
+// varying bytes of keys below 2^23, one per stage slot.
+constexpr uint32_t HYB_NONE = 0u, HYB_MSD = 1u, HYB_LSD = 2u;
+constexpr uint32_t HYB_SKIP = 32u; // a digit position meaning "no such pass"
 __device__ inline int pass_shift(uint32_t flags, int pass)
 {
     return (flags & PF_SHIFT) ? int((flags >> PF_SHIFT_POS) & 63u) : pass * RADIX_BITS;
@@ -913,7 +921,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_counts(const K *__restrict__ key
     static_assert(TILE < 65536, "tile counts are 16-bit");
     __shared__ uint32_t sh[RADIX];
     const uint32_t flags = plan->flags[pass];
-    if (flags & (PF_TRIVIAL | PF_PRECOUNTED)) return;
+    if (flags & (PF_TRIVIAL | PF_PRECOUNTED | PF_PRE0)) return;
     const K *__restrict__ kin = (flags & PF_SRC_B) ? keysB : keysA;
     const bool xin = FLOATK && (flags & PF_XIN);
     const int shift = pass_shift(flags, pass);
@@ -1169,12 +1177,16 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K *__restrict__ keys, u
 // k_plan_rts: trivial passes from the varying bits; buffer selection and the
 // float transform's first/last pass.  Digit starts come later, per pass, from
 // that pass's counts (k_scan_ranges).  With `hybrid` (4-byte unsigned keys
-// without values whose upsweep also counted the top byte): if a bit from 23
-// up varies, the MSD-first form runs instead -- only pass 3 moves keys in the
-// LSD loop (A to B), by the 8 bits that end at the top varying bit (at most
-// the top byte, at least bits 16..23; counted by the upsweep when that is the
-// top byte, else by the loop's counting kernel), the bucket stages
-// (hpcs_hybrid.hpp) bring them back to A sorted, and plan->hybrid says so.
+// without values whose upsweep also counted the top byte; the launch
+// sequence of hpcs_hybrid.hpp, whose LSD loop has pass slot 3 only): if a bit
+// from 23 up varies, the MSD-first form -- slot 3 splits A to B by the 8 bits
+// that end at the top varying bit (at most the top byte, at least bits
+// 16..23; counted by the upsweep when that is the top byte, else by the
+// loop's counting kernel), the bucket stages bring the keys back to A sorted;
+// otherwise the LSD passes over the (at most three) varying bytes, lowest
+// first: slot 3 (A to B; the upsweep's byte-0 counts when that is byte 0),
+// the segmented pass over one segment of all keys (B to A), the first
+// overflow pass (A to B), and a copy back after an odd number of them.
 template <typename K>
 __global__ void k_plan_rts(const K *__restrict__ orslots, DevPlan *__restrict__ plan, uint64_t n, int floatk,
                            int hybrid = 0)
@@ -1187,22 +1199,44 @@ __global__ void k_plan_rts(const K *__restrict__ orslots, DevPlan *__restrict__ 
         b |= orslots[s * OR_STRIDE<K> + 1];
     }
     const K varying = a & b;
-    if (P == 4 && hybrid && !floatk && n > 1 && (uint64_t(varying) >> 23) != 0) {
-        const int top = 63 - __builtin_clzll(uint64_t(varying)); // 23 .. 31
-        const int s1 = min(24, top - 7);
+    if (P == 4 && hybrid && !floatk) {
+        uint32_t mode = HYB_NONE, s1 = 0, s2 = HYB_SKIP, ov0 = HYB_SKIP, ov1 = HYB_SKIP, copy = 0, pre = 0;
+        if (n > 1 && varying != 0) {
+            const int top = 63 - __builtin_clzll(uint64_t(varying));
+            if (top >= 23) {
+                mode = HYB_MSD;
+                s1 = uint32_t(min(24, top - 7));
+                s2 = s1 - 8;
+                ov0 = 0;
+                ov1 = 8;
+                pre = s1 == 24 ? PF_PRECOUNTED : 0u;
+            } else {
+                mode = HYB_LSD;
+                uint32_t v[3], k = 0;
+                for (uint32_t b = 0; b < 3; ++b)
+                    if ((varying >> (8 * b)) & K(0xFF)) v[k++] = 8 * b;
+                s1 = v[0];
+                pre = s1 == 0 ? PF_PRE0 : 0u;
+                if (k >= 2) s2 = v[1];
+                if (k >= 3) ov0 = v[2];
+                copy = k & 1u;
+            }
+        }
         for (int p = 0; p < P; ++p) {
-            plan->trivial[p] = p == 3 ? 0u : 1u;
+            plan->trivial[p] = (p == 3 && mode != HYB_NONE) ? 0u : 1u;
             plan->src[p] = 0u;
             plan->xin[p] = 0u;
             plan->xout[p] = 0u;
         }
-        plan->copyback = 0u;
-        plan->hybrid = 1u;
-        plan->hyb_shift1 = uint32_t(s1);
-        plan->hyb_shift2 = uint32_t(s1 - 8);
+        plan->copyback = copy;
+        plan->hybrid = mode;
+        plan->hyb_shift1 = s1;
+        plan->hyb_shift2 = s2;
+        plan->hyb_ov_shift[0] = ov0;
+        plan->hyb_ov_shift[1] = ov1;
         pack_plan_flags(plan, P);
         for (int p = 0; p < P; ++p) plan->flags[p] |= PF_NEXT_NONE << PF_NEXT_SHIFT;
-        plan->flags[3] |= PF_FIRST | PF_SHIFT | (uint32_t(s1) << PF_SHIFT_POS) | (s1 == 24 ? PF_PRECOUNTED : 0u);
+        if (mode != HYB_NONE) plan->flags[3] |= PF_FIRST | PF_SHIFT | (s1 << PF_SHIFT_POS) | pre;
         return;
     }
     plan->hybrid = 0u;

@@ -397,7 +397,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(const uint32_t *keys, uint3
 // One workgroup per tile (BLOCK x ITEMS keys) of the large segments (the
 // grid may exceed the tile count; spare workgroups return): digit counts of
 // pass `pass`.
-template <int BLOCK, int ITEMS>
+template <int BLOCK, int ITEMS, bool PEEL = true>
 __global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
                                                       const uint32_t *__restrict__ keysB, int pass,
                                                       const SegPlan *__restrict__ sp, const uint4 *__restrict__ tinfo,
@@ -411,6 +411,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
     const int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
+    if (shift >= 32) return;                            // a pass the plan skips
     const uint4 ti = tinfo[t];
     const uint32_t tb = ti.x;
     const int cnt = int(ti.z);
@@ -423,7 +424,20 @@ __global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict
         k[j] = o < cnt ? kin[size_t(tb) + o] : 0u;
     }
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) count_digit(sh, (k[j] >> shift) & 0xFFu, j * BLOCK + tid < cnt, lane);
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t d = (k[j] >> shift) & 0xFFu;
+        if constexpr (PEEL) {
+            count_digit(sh, d, j * BLOCK + tid < cnt, lane);
+        } else {
+            // one digit for the whole wave: one add
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            if (__all(j * BLOCK + tid < cnt && d == d0)) {
+                if (lane == 0) atomicAdd(&sh[d0], uint32_t(WAVE));
+            } else if (j * BLOCK + tid < cnt) {
+                atomicAdd(&sh[d], 1u);
+            }
+        }
+    }
     __syncthreads();
     for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(t) * RADIX + i] = uint16_t(sh[i]);
 }
@@ -625,7 +639,7 @@ static __global__ __launch_bounds__(RADIX) void k_segx_bases(const int64_t *__re
 // LDS atomics first and stages while it ranks, RANK_MIX2 / RANK_MIX4 match
 // every second / fourth item by ballots, RANK_BALLOT ranks by ballots with
 // wave-local counters and stages after the scan).
-template <int BLOCK, int ITEMS, int RANK>
+template <int BLOCK, int ITEMS, int RANK, bool PEEL = true>
 __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ keysA, uint32_t *__restrict__ keysB,
                                                        int pass, const SegPlan *__restrict__ sp,
                                                        const uint4 *__restrict__ tinfo,
@@ -650,6 +664,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     uint32_t *__restrict__ kout = (pass & 1) ? keysA : keysB;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
     const int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
+    if (shift >= 32) return;                            // a pass the plan skips
     const uint32_t t = xcd_contiguous(blockIdx.x, ntiles);
     const uint4 ti = tinfo[t];
     const uint32_t tb = ti.x, sb = ti.y;
@@ -682,7 +697,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     // ---- per-wave digit counts (ballot ranking: the ranks themselves) ----
     uint32_t *cnt = s_cnt[wave];
     const bool full = tile_valid == TILE;
-    const bool skewed = RANK != RANK_BALLOT && wave_skewed((keys[0] >> shift) & (RADIX - 1), lane < wave_valid,
+    const bool skewed = PEEL && RANK != RANK_BALLOT && wave_skewed((keys[0] >> shift) & (RADIX - 1), lane < wave_valid,
                                                            (keys[ITEMS / 2] >> shift) & (RADIX - 1),
                                                            (ITEMS / 2) * WAVE + lane < wave_valid);
     uint32_t rank2[RANK == RANK_BALLOT ? (ITEMS + 1) / 2 : 1];
@@ -746,7 +761,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     // ---- rank and stage the tile in LDS in sorted order ----
     auto stage_loop = [&](auto full_c, auto peel_c) {
         constexpr bool FULL = decltype(full_c)::value;
-        constexpr bool PEEL = decltype(peel_c)::value;
+        constexpr bool PEELED = decltype(peel_c)::value;
         if constexpr (RANK == RANK_BALLOT) {
 #pragma unroll
             for (int i = 0; i < ITEMS; ++i)
@@ -768,14 +783,14 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
                     if (valid) s_keys[old + below] = keys[i];
                     __builtin_amdgcn_wave_barrier();
                 } else {
-                    const uint32_t at = ldsor_rank_item<PEEL>(d, valid, lane, cnt, match32, s_match[wave]);
+                    const uint32_t at = ldsor_rank_item<PEELED>(d, valid, lane, cnt, match32, s_match[wave]);
                     if (valid) s_keys[at] = keys[i];
                     __builtin_amdgcn_wave_barrier();
                 }
             }
         }
     };
-    if constexpr (RANK == RANK_BALLOT) {
+    if constexpr (RANK == RANK_BALLOT || !PEEL) {
         if (full) stage_loop(std::true_type{}, std::false_type{});
         else stage_loop(std::false_type{}, std::false_type{});
     } else if (full) {

@@ -105,18 +105,20 @@ def test_hybrid_2pow24(hctx, dist):
 
 
 def test_hybrid_profile_shows_stages(hctx):
-    """Uniform keys run the bucket stages; keys below 2^23 run the LSD passes
-    and the stages find nothing (the same launch sequence either way)."""
+    """Uniform keys and keys below 2^23 (the LSD passes through the same
+    stage slots) run one launch sequence: one global pass slot, the
+    segmented pass, the bucket sort and the overflow passes."""
     n = 1 << 22
-    for dist, hybrid in (("uniform", True), ("below2^23", False)):
+    for dist in ("uniform", "below2^23", "low8"):
         a = _keys(n, 0x7A6 + len(dist), dist)
         N.profile_enable(True, hctx)
         got = _sort(hctx, a)
         prof = N.profile_read(hctx)
         N.profile_enable(False, hctx)
-        assert np.array_equal(got, np.sort(a))
+        assert np.array_equal(got, np.sort(a)), dist
         assert "hyb_local" in prof and "seg_scatter" in prof, prof
-        assert prof["scatter"][0] == 4  # the LSD loop's four scatter launches (three do nothing when hybrid)
+        assert prof["scatter"][0] == 1 and prof["tile_counts"][0] == 1, prof  # the one global pass slot
+        assert prof["seg_scatter"][0] == 3, prof  # the segmented pass and two overflow passes
 
 
 def test_hybrid_float_and_signed_take_the_passes(oracle, hctx):
