@@ -33,6 +33,7 @@ HPCS_OPT_SCRATCH_LIMIT = 12
 HPCS_OPT_SEG_RANK = 13
 HPCS_OPT_SEG_SMALL = 14
 HPCS_OPT_SEG_SHAPE = 15
+HPCS_OPT_SEG_MSD = 16
 HPCS_OP_SORT, HPCS_OP_TOPK = 0, 1
 HPCS_SERVE_REALTIME, HPCS_SERVE_ALONE = 0, 1
 

@@ -187,6 +187,7 @@ struct hpcs_ctx {
     int seg_rank = RANK_MIX2;
     int seg_small = SEG_SMALL;
     int seg_shape = 0;
+    int64_t seg_msd = 0; // segments longer than this take the MSD-first form (0: none)
     std::vector<TuneEntry> tune;
     bool set_algo = false, set_shape = false, set_rank = false, set_bits = false;
     bool set_seg_rank = false, set_seg_small = false, set_seg_shape = false;
@@ -826,7 +827,8 @@ void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, c
                            H.max_tiles1, SEG_SMALL, stile, 1);
     });
     timed(c, st, "seg_tiles", [&] {
-        hipLaunchKernelGGL(k_seg_tiles, dim3(RADIX), dim3(256), 0, st, begin1, sp, tile0, large, tinfo, stile);
+        hipLaunchKernelGGL(k_seg_tiles, dim3(RADIX), dim3(256), 0, st, begin1, sp, tile0, large, tinfo, stile,
+                           int64_t(0));
     });
     timed(c, st, "seg_small", [&] {
         hipLaunchKernelGGL((k_seg_small<SEG_BLOCK>), dim3(RADIX), dim3(SEG_BLOCK), 0, st, keysB, d_keys, begin1, sp,
@@ -864,7 +866,7 @@ void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, c
     });
     timed(c, st, "seg_tiles", [&] {
         hipLaunchKernelGGL(k_seg_tiles, dim3(c->num_cus * 8), dim3(256), 0, st, pairs, sp2, tile02, large2, tinfo,
-                           stile);
+                           stile, int64_t(0));
     });
     for (int p = 0; p < 2; ++p) {
         timed(c, st, "seg_counts", [&] {
@@ -1141,16 +1143,28 @@ void run_sort_limited(hpcs_ctx *c, K *d_keys, uint32_t *d_vals, int64_t n, hipSt
 // Batched segments: scratch [SegPlan | seg_tile0 | small | medium | large |
 // tile records | 16-bit counts | offsets | keys B].  The plan's word `bad`
 // counts rejected segments.
+//
+// With the MSD-first form (segments longer than msd_min > 0 keys) its
+// bucket bounds and overflow follow: [overflow plan | digit positions |
+// bucket bounds | pairs | medium list | large list | first tiles]; at most
+// n / (msd_min + 1) segments take the form (256 buckets each), and an
+// overflow bucket holds more than HYB_MAX_BIN keys, so at most
+// n / (HYB_MAX_BIN + 1) of them, each large one more than SEG_MEDIUM.
 struct SegLayout {
     size_t off_plan, off_tile0, off_small, off_medium, off_large, off_tinfo, off_counts, off_offsets, off_keysB,
-        total;
-    uint64_t max_tiles;
+        off_sp2, off_shifts, off_bounds, off_pairs, off_medium2, off_large2, off_tile02, total;
+    uint64_t max_tiles;          // tile records: the large segments' or the overflow's, the larger
+    uint64_t max_pairs, max_large2, max_msd;
 };
 
-SegLayout seg_layout(int64_t n, int nseg, int tile)
+SegLayout seg_layout(int64_t n, int nseg, int tile, int64_t msd_min)
 {
+    const bool msd = msd_min > 0;
     SegLayout L{};
     L.max_tiles = uint64_t(n) / uint64_t(tile) + uint64_t(nseg) + 1; // tiles of the large segments, at most
+    L.max_pairs = uint64_t(n) / uint64_t(HYB_MAX_BIN + 1) + 1;
+    L.max_large2 = uint64_t(n) / uint64_t(SEG_MEDIUM + 1) + 1;
+    if (msd) L.max_tiles = std::max(L.max_tiles, uint64_t(n) / uint64_t(tile) + L.max_large2 + 1);
     L.off_plan = 0;
     L.off_tile0 = align_up(sizeof(SegPlan));
     L.off_small = align_up(L.off_tile0 + size_t(nseg) * 4);
@@ -1161,6 +1175,17 @@ SegLayout seg_layout(int64_t n, int nseg, int tile)
     L.off_offsets = align_up(L.off_counts + L.max_tiles * RADIX * sizeof(uint16_t));
     L.off_keysB = align_up(L.off_offsets + L.max_tiles * RADIX * sizeof(uint32_t));
     L.total = align_up(L.off_keysB + size_t(n) * 4);
+    if (msd) {
+        L.off_sp2 = L.total;
+        L.off_shifts = align_up(L.off_sp2 + sizeof(SegPlan));
+        L.max_msd = uint64_t(n) / uint64_t(msd_min + 1) + 1;
+        L.off_bounds = align_up(L.off_shifts + 4 * sizeof(uint32_t));
+        L.off_pairs = align_up(L.off_bounds + L.max_msd * RADIX * sizeof(uint2));
+        L.off_medium2 = align_up(L.off_pairs + L.max_pairs * 2 * sizeof(int64_t));
+        L.off_large2 = align_up(L.off_medium2 + L.max_pairs * 4);
+        L.off_tile02 = align_up(L.off_large2 + L.max_large2 * 4);
+        L.total = align_up(L.off_tile02 + L.max_pairs * 2 * 4);
+    }
     return L;
 }
 
@@ -1275,7 +1300,9 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     if (n >= (int64_t(1) << 32)) fail(HPCS_E_UNSUPPORTED, "segments must lie below 2^32 keys");
     const SegParams prm = resolve_seg_params(c, n);
     const int tile = shape_tile_keys(prm.shape);
-    const SegLayout L = seg_layout(n, nseg, tile);
+    const int64_t msd_min = c->seg_msd; // segments longer than this: the MSD-first form
+    const bool msd = msd_min > 0 && n > msd_min;
+    const SegLayout L = seg_layout(n, nseg, tile, msd ? msd_min : int64_t(0));
     char *base = static_cast<char *>(ensure(c->scratch, c->scratch_cap, L.total));
     SegPlan *sp = reinterpret_cast<SegPlan *>(base + L.off_plan);
     uint32_t *tile0 = reinterpret_cast<uint32_t *>(base + L.off_tile0);
@@ -1292,7 +1319,8 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     });
     const int list_grid = std::min(nseg, c->num_cus * 8);
     timed(c, st, "seg_tiles", [&] {
-        hipLaunchKernelGGL(k_seg_tiles, dim3(list_grid), dim3(256), 0, st, d_begin, sp, tile0, large, tinfo, tile);
+        hipLaunchKernelGGL(k_seg_tiles, dim3(list_grid), dim3(256), 0, st, d_begin, sp, tile0, large, tinfo, tile,
+                           msd ? msd_min : int64_t(0));
     });
     timed(c, st, "seg_small", [&] {
         hipLaunchKernelGGL((k_seg_small<SEG_BLOCK>), dim3(list_grid), dim3(SEG_BLOCK), 0, st, d_keys, d_keys,
@@ -1311,13 +1339,66 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
         timed(c, st, "seg_counts", [&] {
             launch_seg_counts(prm.shape, tile_grid, st, d_keys, keysB, p, sp, tinfo, counts, nullptr, false);
         });
+        // the MSD-first segments keep pass 0's offsets (their bucket bounds)
+        const int64_t skip = (msd && p > 0) ? msd_min : INT64_MAX;
         timed(c, st, "seg_scan", [&] {
             hipLaunchKernelGGL(k_seg_scan, dim3(list_grid), dim3(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS), 0, st, d_begin, sp,
-                               tile0, large, counts, offsets, tile);
+                               tile0, large, counts, offsets, tile, skip);
         });
         timed(c, st, "seg_scatter", [&] {
             launch_seg_scatter(prm.shape, prm.rank, tile_grid, st, d_keys, keysB, p, sp, tinfo, offsets, nullptr,
                                nullptr, nullptr, false);
+        });
+    }
+    if (!msd) return;
+    // the MSD-first segments: 256 top-byte buckets each in B, sorted into A
+    // (hpcs_hybrid.hpp); overflow buckets by one 1024-thread workgroup in LDS
+    // or by the LSD passes over bytes 0, 1, 2 (B -> A -> B -> A)
+    SegPlan *sp2 = reinterpret_cast<SegPlan *>(base + L.off_sp2);
+    uint32_t *shifts = reinterpret_cast<uint32_t *>(base + L.off_shifts);
+    int64_t *pairs = reinterpret_cast<int64_t *>(base + L.off_pairs);
+    uint32_t *medium2 = reinterpret_cast<uint32_t *>(base + L.off_medium2);
+    uint32_t *large2 = reinterpret_cast<uint32_t *>(base + L.off_large2);
+    uint32_t *tile02 = reinterpret_cast<uint32_t *>(base + L.off_tile02);
+    uint2 *bounds = reinterpret_cast<uint2 *>(base + L.off_bounds);
+    timed(c, st, "seg_msd_init", [&] {
+        hipLaunchKernelGGL(k_segmsd_init, dim3(1), dim3(WAVE), 0, st, sp2, shifts);
+        hipLaunchKernelGGL(k_segmsd_bounds, dim3(list_grid), dim3(RADIX), 0, st, d_begin, sp, tile0, large, offsets,
+                           msd_min, sp2, bounds);
+    });
+    // one resident round of workgroups (grid-stride over the buckets)
+    static const int occ = [] { // resident bucket workgroups per CU, initialised once
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_segmsd_local<HYB_BLOCK, HYB_ITEMS>, HYB_BLOCK, 0) !=
+                hipSuccess ||
+            o < 1)
+            o = 1;
+        return o;
+    }();
+    timed(c, st, "seg_msd_local", [&] {
+        hipLaunchKernelGGL((k_segmsd_local<HYB_BLOCK, HYB_ITEMS>), dim3(c->num_cus * occ), dim3(HYB_BLOCK), 0, st, keysB,
+                           d_keys, bounds, sp2, pairs, medium2, large2, tile02, tile);
+    });
+    timed(c, st, "seg_medium", [&] {
+        hipLaunchKernelGGL((k_seg_small<SEG_MEDIUM_BLOCK>), dim3(c->num_cus), dim3(SEG_MEDIUM_BLOCK), 0, st, keysB,
+                           d_keys, pairs, sp2, medium2);
+    });
+    timed(c, st, "seg_tiles", [&] {
+        hipLaunchKernelGGL(k_seg_tiles, dim3(c->num_cus * 8), dim3(256), 0, st, pairs, sp2, tile02, large2, tinfo,
+                           tile, int64_t(0));
+    });
+    const int list_grid2 = int(std::min<uint64_t>(L.max_large2, uint64_t(c->num_cus) * 8));
+    for (int p = 1; p < 4; ++p) {
+        timed(c, st, "seg_counts", [&] {
+            launch_seg_counts(prm.shape, tile_grid, st, d_keys, keysB, p, sp2, tinfo, counts, shifts + (p - 1), false);
+        });
+        timed(c, st, "seg_scan", [&] {
+            hipLaunchKernelGGL(k_seg_scan, dim3(list_grid2), dim3(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS), 0, st, pairs, sp2,
+                               tile02, large2, counts, offsets, tile, INT64_MAX);
+        });
+        timed(c, st, "seg_scatter", [&] {
+            launch_seg_scatter(prm.shape, prm.rank, tile_grid, st, d_keys, keysB, p, sp2, tinfo, offsets, nullptr,
+                               nullptr, shifts + (p - 1), false);
         });
     }
 }
@@ -2046,6 +2127,11 @@ HPCS_API int32_t hpcs_ctx_set_option(hpcs_ctx *ctx, int32_t option, int64_t valu
             ctx->seg_small = int(value);
             ctx->set_seg_small = true;
             break;
+        case HPCS_OPT_SEG_MSD:
+            if (value != 0 && (value <= SEG_MEDIUM || value >= (int64_t(1) << 32)))
+                fail(HPCS_E_INVALID_ARG, "the MSD-first threshold must be 0 or in [16384, 2^32)");
+            ctx->seg_msd = value;
+            break;
         case HPCS_OPT_FALLBACK_ROUNDS:
             if (value < 0 || value > (1 << 20)) fail(HPCS_E_INVALID_ARG, "fallback rounds out of range");
             ctx->fallback_rounds = int(value);
@@ -2250,8 +2336,10 @@ HPCS_API int32_t hpcs_ctx_reserve_segments(hpcs_ctx *ctx, int64_t n, int32_t nse
         if (n < 0 || nseg < 0) fail(HPCS_E_INVALID_ARG, "n and nseg must be non-negative");
         if (n >= (int64_t(1) << 32)) fail(HPCS_E_UNSUPPORTED, "segments must lie below 2^32 keys");
         // the smallest tile bounds the scratch of every call of up to n keys,
-        // whatever tile shape its size class takes
-        if (nseg > 0 && n > 1) ensure(ctx->scratch, ctx->scratch_cap, seg_layout(n, nseg, SEG_MIN_TILE).total);
+        // whatever tile shape its size class takes (with the MSD-first
+        // form's overflow plan when the context has a threshold)
+        if (nseg > 0 && n > 1)
+            ensure(ctx->scratch, ctx->scratch_cap, seg_layout(n, nseg, SEG_MIN_TILE, ctx->seg_msd).total);
     });
 }
 #endif

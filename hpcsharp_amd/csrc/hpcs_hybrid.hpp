@@ -229,4 +229,206 @@ __global__ __launch_bounds__(BLOCK) void k_hyb_local(uint32_t *__restrict__ keys
     }
 }
 
+// ---- Batched segments, MSD-first (hpcs_[dev_]sort_segments_u32 with an
+// MSD threshold): segments longer than msd_min keys take pass 0 of the
+// segmented passes by their top byte, A -> B (their tile records carry
+// SEG_MSD_TILE; passes 1..3 skip them), which leaves 256 buckets per segment
+// whose keys share bits 24..31; then each bucket is sorted in LDS from B into
+// the same place of A.  Keys alone carry no identity, so the output equals
+// the LSD passes'.  Bytes moved per key of such a segment: count 4, top-byte
+// pass 8, bucket sort 8 = 20 (the LSD passes: 48).
+
+// The overflow plan cleared and the overflow passes' digit positions
+// (bits 0, 8, 16: the buckets share the top byte).  One wave.
+static __global__ __launch_bounds__(WAVE) void k_segmsd_init(SegPlan *__restrict__ sp2, uint32_t *__restrict__ shifts)
+{
+    static_assert(sizeof(SegPlan) / 4 <= WAVE, "one lane per word of the overflow plan");
+    const int i = threadIdx.x;
+    if (i < int(sizeof(SegPlan) / 4)) reinterpret_cast<uint32_t *>(sp2)[i] = 0u;
+    if (i < 3) shifts[i] = 8u * uint32_t(i);
+}
+
+// The buckets of the MSD-first segments as (first, end) key positions:
+// grid-stride over the large list, one thread per digit; a segment longer
+// than msd_min keys takes the next 256 entries of `bounds` (counted in
+// sp2->num_small, which the overflow plan does not use).  Bucket d of
+// segment s is [b + row[d], b + row[d + 1]) with b = begin[s] and row =
+// k_seg_scan's pass-0 offsets of the segment's first tile (the later passes
+// leave the row alone); the last one ends at the segment's end.
+static __global__ __launch_bounds__(RADIX) void k_segmsd_bounds(const int64_t *__restrict__ begin,
+                                                                const SegPlan *__restrict__ sp,
+                                                                const uint32_t *__restrict__ seg_tile0,
+                                                                const uint32_t *__restrict__ large_list,
+                                                                const uint32_t *__restrict__ offsets, int64_t msd_min,
+                                                                SegPlan *__restrict__ sp2, uint2 *__restrict__ bounds)
+{
+    __shared__ uint32_t s_slot;
+    const uint32_t d = threadIdx.x, nlarge = sp->num_large;
+    for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
+        const uint32_t s = large_list[k];
+        const int64_t b = begin[s], e = begin[s + 1];
+        if (e - b <= msd_min) continue; // the LSD passes sorted it
+        if (d == 0) s_slot = atomicAdd(&sp2->num_small, 1u);
+        __syncthreads();
+        const uint32_t *__restrict__ row = offsets + size_t(seg_tile0[s]) * RADIX;
+        const int64_t hi = d + 1 < uint32_t(RADIX) ? b + row[d + 1] : e;
+        bounds[size_t(s_slot) * RADIX + d] = make_uint2(uint32_t(b + row[d]), uint32_t(hi));
+        __syncthreads();
+    }
+}
+
+// One bucket of len (2 .. CAP) keys in v (striped over the workgroup; every
+// key shares bits 24..31) sorted in LDS into dst: counts of the 2^NB bins of
+// bits 24 - NB .. 23, their scan, each key placed at its bin's next free
+// slot, then each key ranked within its bin by (the bits below the bin,
+// slot).  Returns false, having written nothing, when a bin holds more than
+// HYB_MAX_BIN keys.  Every thread calls it; it ends after a barrier.
+template <int BLOCK, int ITEMS, int NB>
+__device__ inline bool segmsd_sort_bucket(const uint32_t (&v)[ITEMS], uint32_t len, uint32_t *__restrict__ dst,
+                                          uint32_t *s_cnt, uint32_t *s_key, uint32_t *s_wsum, uint32_t *s_wmax)
+{
+    constexpr int WAVES = BLOCK / WAVE;
+    constexpr int NBIN = 1 << NB;
+    constexpr int PER = NBIN / BLOCK; // bins per thread in the scan
+    constexpr int BS = 24 - NB;       // the bins' lowest bit
+    static_assert(PER >= 1 && PER <= 16, "1 to 16 bins per thread");
+    static_assert(BLOCK * ITEMS <= (1 << (8 + NB)), "a slot fits below the compared bits");
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
+    for (int i = tid; i < NBIN + NBIN / 16; i += BLOCK) s_cnt[i] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+        if (uint32_t(j * BLOCK + tid) < len) atomicAdd(&s_cnt[hyb_slot(int((v[j] >> BS) & (NBIN - 1)))], 1u);
+    __syncthreads();
+    uint32_t c[PER];
+    uint32_t sum = 0, mx = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        c[i] = s_cnt[hyb_slot(PER * tid + i)];
+        sum += c[i];
+        mx = max(mx, c[i]);
+    }
+    uint32_t x = sum;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) mx = max(mx, __shfl_xor(mx, off));
+    if (lane == WAVE - 1) s_wsum[wave] = x;
+    if (lane == 0) s_wmax[wave] = mx;
+    __syncthreads();
+    uint32_t run = x - sum, gmax = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+        if (w < wave) run += s_wsum[w];
+        gmax = max(gmax, s_wmax[w]);
+    }
+    if (gmax > uint32_t(HYB_MAX_BIN)) return false; // the same for every thread; s_cnt was read before the barrier
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        s_cnt[hyb_slot(PER * tid + i)] = run;
+        run += c[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+        if (uint32_t(j * BLOCK + tid) < len) {
+            const uint32_t pos = atomicAdd(&s_cnt[hyb_slot(int((v[j] >> BS) & (NBIN - 1)))], 1u);
+            s_key[pos] = v[j];
+        }
+    __syncthreads();
+    // s_cnt holds each bin's end; the key at slot p of bin [lb, hb) goes to
+    // lb + the keys of the bin before it in (bits below the bin, slot) order
+    // (the bin's keys agree on every higher bit)
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t p = uint32_t(j * BLOCK + tid);
+        if (p < len) {
+            const uint32_t xk = s_key[p];
+            const int bin = int((xk >> BS) & (NBIN - 1));
+            const uint32_t hb = s_cnt[hyb_slot(bin)];
+            const uint32_t lb = bin ? s_cnt[hyb_slot(bin - 1)] : 0u;
+            const uint32_t me = (xk << (8 + NB)) | p;
+            uint32_t r = 0;
+            for (uint32_t q = lb; q < hb; ++q) r += ((s_key[q] << (8 + NB)) | q) < me ? 1u : 0u;
+            dst[lb + r] = xk;
+        }
+    }
+    __syncthreads(); // the next bucket reuses s_cnt and s_key
+    return true;
+}
+
+// Grid-stride over the buckets g < 256 * sp2->num_small (bounds[g]; lane l
+// of every wave fetches the bounds of the workgroup's next 64 buckets at
+// once).  A bucket of keysB of at most CAP keys is sorted in LDS into the
+// same place of keysA (segmsd_sort_bucket) with 2^10 bins up to 2048 keys,
+// 2^11 up to 4096 and 2^12 beyond, so the bins' set-up and scan stay in
+// proportion to the keys.  Buckets beyond CAP keys or with a bin beyond
+// HYB_MAX_BIN keys go to the overflow plan sp2 as k_hyb_local lists them
+// (pairs, medium2, large2 with tiles in tile0_2).
+template <int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_segmsd_local(const uint32_t *__restrict__ keysB,
+                                                        uint32_t *__restrict__ keysA,
+                                                        const uint2 *__restrict__ bounds, SegPlan *__restrict__ sp2,
+                                                        int64_t *__restrict__ pairs, uint32_t *__restrict__ medium2,
+                                                        uint32_t *__restrict__ large2, uint32_t *__restrict__ tile0_2,
+                                                        int tile)
+{
+    constexpr int CAP = BLOCK * ITEMS;
+    constexpr int WAVES = BLOCK / WAVE;
+    static_assert(HYB_NBIN == 16 * BLOCK, "at most 16 bins per thread");
+    static_assert(CAP < 65536, "in-bucket slots fit 16 bits");
+    __shared__ uint32_t s_cnt[HYB_NBIN + HYB_NBIN / 16];
+    __shared__ uint32_t s_key[CAP];
+    __shared__ uint32_t s_wsum[WAVES], s_wmax[WAVES];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const uint32_t total = sp2->num_small * uint32_t(RADIX), G = gridDim.x;
+    // an overflow bucket: listed by thread 0
+    auto overflow = [&](uint32_t lo, uint32_t hi) {
+        if (tid != 0) return;
+        const uint32_t len = hi - lo;
+        const uint32_t i = atomicAdd(&sp2->bad, 1u); // pairs listed so far (no segment is bad here)
+        pairs[2 * size_t(i)] = int64_t(lo);
+        pairs[2 * size_t(i) + 1] = int64_t(hi);
+        if (len <= uint32_t(SEG_MEDIUM)) {
+            medium2[atomicAdd(&sp2->num_medium, 1u)] = 2 * i;
+        } else {
+            tile0_2[2 * size_t(i)] = atomicAdd(&sp2->num_tiles, (len + uint32_t(tile) - 1) / uint32_t(tile));
+            large2[atomicAdd(&sp2->num_large, 1u)] = 2 * i;
+        }
+    };
+    uint2 nb = make_uint2(0u, 0u); // lane l: the bounds of iteration (it & ~63) + l
+    uint32_t it = 0;
+    for (uint32_t g = blockIdx.x; g < total; g += G, ++it) {
+        if ((it & (WAVE - 1)) == 0) {
+            const uint32_t gl = g + uint32_t(lane) * G;
+            nb = gl < total ? bounds[gl] : make_uint2(0u, 0u);
+        }
+        const uint32_t lo = __builtin_amdgcn_readlane(nb.x, int(it & (WAVE - 1)));
+        const uint32_t hi = __builtin_amdgcn_readlane(nb.y, int(it & (WAVE - 1)));
+        const uint32_t len = hi - lo;
+        if (len <= 1u) {
+            if (len == 1u && tid == 0) keysA[lo] = keysB[lo];
+            continue;
+        }
+        if (len > uint32_t(CAP)) {
+            overflow(lo, hi);
+            continue;
+        }
+        const uint32_t *__restrict__ src = keysB + lo;
+        uint32_t v[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint32_t o = uint32_t(j * BLOCK + tid);
+            v[j] = o < len ? src[o] : 0u;
+        }
+        const bool done = len <= 2048u ? segmsd_sort_bucket<BLOCK, ITEMS, 10>(v, len, keysA + lo, s_cnt, s_key, s_wsum, s_wmax)
+                          : len <= 4096u ? segmsd_sort_bucket<BLOCK, ITEMS, 11>(v, len, keysA + lo, s_cnt, s_key, s_wsum, s_wmax)
+                                         : segmsd_sort_bucket<BLOCK, ITEMS, 12>(v, len, keysA + lo, s_cnt, s_key, s_wsum, s_wmax);
+        if (!done) overflow(lo, hi);
+    }
+}
+
 } // namespace hpcs

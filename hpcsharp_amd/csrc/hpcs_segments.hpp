@@ -38,6 +38,10 @@ constexpr int SEG_SMALL = SEG_TILE;             // largest segment of the small 
 constexpr int SEG_MIN_TILE = 2048;              // smallest tile of a shape that serves the large segments
 constexpr int SEG_MEDIUM_BLOCK = 1024;
 constexpr int SEG_MEDIUM = SEG_MEDIUM_BLOCK * SEG_ITEMS; // largest segment one workgroup sorts in LDS
+// Tile record flag (in .w, beside the large-list index): the tile's segment
+// takes the MSD-first form, whose only segmented pass is pass 0 by the top
+// byte (bits 24..31); passes 1..3 skip its tiles.
+constexpr uint32_t SEG_MSD_TILE = 0x80000000u;
 
 struct SegPlan {
     uint32_t num_tiles;  // tiles of the large segments
@@ -286,13 +290,14 @@ static __global__ __launch_bounds__(1024) void k_seg_plan(const int64_t *__restr
 
 // One record per tile of the large segments: {first key, segment's first key,
 // keys in the tile, the segment's place in the large list}, so the pass
-// kernels find a tile with one load.
+// kernels find a tile with one load.  With msd_min > 0, tiles of segments
+// longer than msd_min keys carry SEG_MSD_TILE.
 // Grid-stride over the large segments, the segment's tiles across the threads.
 static __global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restrict__ begin,
                                                    const SegPlan *__restrict__ sp,
                                                    const uint32_t *__restrict__ seg_tile0,
                                                    const uint32_t *__restrict__ large_list,
-                                                   uint4 *__restrict__ tinfo, int tile)
+                                                   uint4 *__restrict__ tinfo, int tile, int64_t msd_min)
 {
     const uint32_t nlarge = sp->num_large;
     for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
@@ -300,9 +305,10 @@ static __global__ __launch_bounds__(256) void k_seg_tiles(const int64_t *__restr
         const int64_t b = begin[s], len = begin[s + 1] - b;
         const uint32_t t0 = seg_tile0[s];
         const uint32_t nt = uint32_t((len + tile - 1) / tile);
+        const uint32_t w = k | (msd_min > 0 && len > msd_min ? SEG_MSD_TILE : 0u);
         for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
             const int64_t tb = int64_t(t) * tile;
-            tinfo[t0 + t] = make_uint4(uint32_t(b + tb), uint32_t(b), uint32_t(min(int64_t(tile), len - tb)), k);
+            tinfo[t0 + t] = make_uint4(uint32_t(b + tb), uint32_t(b), uint32_t(min(int64_t(tile), len - tb)), w);
         }
     }
 }
@@ -410,9 +416,13 @@ __global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict
     if (t >= sp->num_tiles) return;
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
-    const int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
-    if (shift >= 32) return;                            // a pass the plan skips
+    int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
+    if (shift >= 32) return;                      // a pass the plan skips
     const uint4 ti = tinfo[t];
+    if (ti.w & SEG_MSD_TILE) { // MSD-first segment: pass 0 by the top byte only
+        if (pass != 0) return;
+        shift = 24;
+    }
     const uint32_t tb = ti.x;
     const int cnt = int(ti.z);
     for (int i = tid; i < RADIX; i += BLOCK) sh[i] = 0u;
@@ -455,7 +465,8 @@ static __global__ __launch_bounds__(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS) void k_seg
                                                                               const uint32_t *__restrict__ seg_tile0,
                                                                               const uint32_t *__restrict__ large_list,
                                                                               const uint16_t *__restrict__ counts,
-                                                                              uint32_t *__restrict__ offsets, int tile)
+                                                                              uint32_t *__restrict__ offsets, int tile,
+                                                                              int64_t skip_above)
 {
     static_assert(SEG_SCAN_PAIRS % WAVE == 0 && SEG_SCAN_PAIRS / WAVE == 2, "two waves of digit pairs");
     __shared__ uint32_t s_part[SEG_SCAN_GROUPS][RADIX];
@@ -469,6 +480,7 @@ static __global__ __launch_bounds__(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS) void k_seg
     const uint32_t nlarge = sp->num_large;
     for (uint32_t k = blockIdx.x; k < nlarge; k += gridDim.x) {
         const uint32_t s = large_list[k];
+        if (begin[s + 1] - begin[s] > skip_above) continue; // an MSD-first segment after its one pass
         const uint32_t t0 = seg_tile0[s];
         const uint32_t nt = uint32_t((begin[s + 1] - begin[s] + tile - 1) / tile);
         const uint32_t per = (nt + SEG_SCAN_GROUPS - 1) / SEG_SCAN_GROUPS;
@@ -663,10 +675,14 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
     uint32_t *__restrict__ kout = (pass & 1) ? keysA : keysB;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
-    const int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
-    if (shift >= 32) return;                            // a pass the plan skips
+    int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
+    if (shift >= 32) return;                      // a pass the plan skips
     const uint32_t t = xcd_contiguous(blockIdx.x, ntiles);
     const uint4 ti = tinfo[t];
+    if (ti.w & SEG_MSD_TILE) { // MSD-first segment: pass 0 by the top byte only
+        if (pass != 0) return;
+        shift = 24;
+    }
     const uint32_t tb = ti.x, sb = ti.y;
     const int tile_valid = int(ti.z);
     const int wave_valid = tile_valid > wave * WAVE_TILE ? min(WAVE_TILE, tile_valid - wave * WAVE_TILE) : 0;
@@ -687,7 +703,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     // of k_segx_* parts: in-range prefix, range base, segment base)
     uint32_t pre_off = tid < RADIX ? offsets[size_t(t) * RADIX + tid] : 0u;
     if (rbase != nullptr && tid < RADIX)
-        pre_off += rbase[size_t(t / SEGX_RANGE) * RADIX + tid] + sbase[size_t(ti.w) * RADIX + tid];
+        pre_off += rbase[size_t(t / SEGX_RANGE) * RADIX + tid] + sbase[size_t(ti.w & ~SEG_MSD_TILE) * RADIX + tid];
 
     for (int i = tid; i < WAVES * RADIX; i += BLOCK) (&s_cnt[0][0])[i] = 0u;
     if constexpr (RANK != RANK_BALLOT)

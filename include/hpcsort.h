@@ -114,6 +114,10 @@ extern "C" {
 #define HPCS_OPT_SEG_SMALL 14      /* hpcs_[dev_]sort_segments_u32: segments of at most this many keys are
                                       sorted by one workgroup in LDS (one read and one write per key): 4096 or
                                       16384; default from the tuning table, else 4096 */
+#define HPCS_OPT_SEG_MSD 16        /* hpcs_[dev_]sort_segments_u32: segments of more than this many keys take
+                                      the MSD-first form (one segmented pass by the top byte, then each of
+                                      the 256 buckets sorted in LDS: 20 bytes of HBM traffic per key against
+                                      48 for the four passes); same output.  0 off; at least 16385 */
 
 #define HPCS_DTYPE_U32 0
 #define HPCS_DTYPE_U64 1
