@@ -138,7 +138,7 @@ def batched_vs_plain(torch, N, datagen, nseg: int, seed: int, reps: int):
             "keys": total, "batched_ms": round(tb * 1e3, 3), "plain_loop_ms": round(tp * 1e3, 3),
             "batched_gkeys": round(total / tb / 1e9, 3), "plain_loop_gkeys": round(total / tp / 1e9, 3),
             "speedup": round(tp / tb, 2), "identical_outputs": identical,
-            "batched": "one hipGraph replay of the batched call (fixed 16-launch sequence), HIP events around "
+            "batched": "one hipGraph replay of the batched call (its fixed launch sequence), HIP events around "
                        "the replay; the plain loop timed on the host"}
 
 

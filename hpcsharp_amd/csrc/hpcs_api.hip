@@ -187,7 +187,7 @@ struct hpcs_ctx {
     int seg_rank = RANK_MIX2;
     int seg_small = SEG_SMALL;
     int seg_shape = 0;
-    int64_t seg_msd = 0; // segments longer than this take the MSD-first form (0: none)
+    int64_t seg_msd = SEG_MSD_DEFAULT; // segments longer than this take the MSD-first form (0: none)
     std::vector<TuneEntry> tune;
     bool set_algo = false, set_shape = false, set_rank = false, set_bits = false;
     bool set_seg_rank = false, set_seg_small = false, set_seg_shape = false;

@@ -47,6 +47,9 @@ constexpr int HYB_CAP = HYB_BLOCK * HYB_ITEMS; // largest sub-bucket of the loca
 constexpr int HYB_NBIN = 4096;                 // 12-bit bins: key bits 4..15
 constexpr int HYB_MAX_BIN = 256;               // keys per bin the rank loop takes
 constexpr int HYB_SUB = RADIX * RADIX;         // sub-buckets: one per (byte 3, byte 2)
+// Batched segments: segments longer than this many keys take the MSD-first
+// form by default (its 256 buckets then hold at least 512 keys on average)
+constexpr int64_t SEG_MSD_DEFAULT = 131072;
 
 // The segments of the segmented pass (begin1[0..256]) and a clear overflow
 // plan (sp2).  MSD-first form: the 256 buckets of the first split, from pass
@@ -363,13 +366,13 @@ __device__ inline bool segmsd_sort_bucket(const uint32_t (&v)[ITEMS], uint32_t l
 // Grid-stride over the buckets g < 256 * sp2->num_small (bounds[g]; lane l
 // of every wave fetches the bounds of the workgroup's next 64 buckets at
 // once).  A bucket of keysB of at most CAP keys is sorted in LDS into the
-// same place of keysA (segmsd_sort_bucket) with 2^10 bins up to 2048 keys,
-// 2^11 up to 4096 and 2^12 beyond, so the bins' set-up and scan stay in
-// proportion to the keys.  Buckets beyond CAP keys or with a bin beyond
+// same place of keysA (segmsd_sort_bucket) with 2^10 bins up to 2048 keys
+// and 2^11 beyond, so the bins' set-up and scan stay in proportion to the
+// keys (and the LDS, 29 KB, leaves room for five workgroups per CU).  Buckets beyond CAP keys or with a bin beyond
 // HYB_MAX_BIN keys go to the overflow plan sp2 as k_hyb_local lists them
 // (pairs, medium2, large2 with tiles in tile0_2).
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_segmsd_local(const uint32_t *__restrict__ keysB,
+__global__ __launch_bounds__(BLOCK, 5) void k_segmsd_local(const uint32_t *__restrict__ keysB,
                                                         uint32_t *__restrict__ keysA,
                                                         const uint2 *__restrict__ bounds, SegPlan *__restrict__ sp2,
                                                         int64_t *__restrict__ pairs, uint32_t *__restrict__ medium2,
@@ -378,9 +381,10 @@ __global__ __launch_bounds__(BLOCK) void k_segmsd_local(const uint32_t *__restri
 {
     constexpr int CAP = BLOCK * ITEMS;
     constexpr int WAVES = BLOCK / WAVE;
-    static_assert(HYB_NBIN == 16 * BLOCK, "at most 16 bins per thread");
+    constexpr int NBIN = 2048;
+    static_assert(NBIN <= 16 * BLOCK, "at most 16 bins per thread");
     static_assert(CAP < 65536, "in-bucket slots fit 16 bits");
-    __shared__ uint32_t s_cnt[HYB_NBIN + HYB_NBIN / 16];
+    __shared__ uint32_t s_cnt[NBIN + NBIN / 16];
     __shared__ uint32_t s_key[CAP];
     __shared__ uint32_t s_wsum[WAVES], s_wmax[WAVES];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
@@ -424,9 +428,9 @@ __global__ __launch_bounds__(BLOCK) void k_segmsd_local(const uint32_t *__restri
             const uint32_t o = uint32_t(j * BLOCK + tid);
             v[j] = o < len ? src[o] : 0u;
         }
-        const bool done = len <= 2048u ? segmsd_sort_bucket<BLOCK, ITEMS, 10>(v, len, keysA + lo, s_cnt, s_key, s_wsum, s_wmax)
-                          : len <= 4096u ? segmsd_sort_bucket<BLOCK, ITEMS, 11>(v, len, keysA + lo, s_cnt, s_key, s_wsum, s_wmax)
-                                         : segmsd_sort_bucket<BLOCK, ITEMS, 12>(v, len, keysA + lo, s_cnt, s_key, s_wsum, s_wmax);
+        const bool done = len <= 2048u
+                              ? segmsd_sort_bucket<BLOCK, ITEMS, 10>(v, len, keysA + lo, s_cnt, s_key, s_wsum, s_wmax)
+                              : segmsd_sort_bucket<BLOCK, ITEMS, 11>(v, len, keysA + lo, s_cnt, s_key, s_wsum, s_wmax);
         if (!done) overflow(lo, hi);
     }
 }

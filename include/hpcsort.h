@@ -117,7 +117,8 @@ extern "C" {
 #define HPCS_OPT_SEG_MSD 16        /* hpcs_[dev_]sort_segments_u32: segments of more than this many keys take
                                       the MSD-first form (one segmented pass by the top byte, then each of
                                       the 256 buckets sorted in LDS: 20 bytes of HBM traffic per key against
-                                      48 for the four passes); same output.  0 off; at least 16385 */
+                                      48 for the four passes); same output.  0 off; otherwise at least
+                                      16385; default 131072 */
 
 #define HPCS_DTYPE_U32 0
 #define HPCS_DTYPE_U64 1
