@@ -1369,14 +1369,14 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     // one resident round of workgroups (grid-stride over the buckets)
     static const int occ = [] { // resident bucket workgroups per CU, initialised once
         int o = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_segmsd_local<HYB_BLOCK, HYB_ITEMS>, HYB_BLOCK, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_segmsd_local<SEGMSD_BLOCK, SEGMSD_ITEMS>, SEGMSD_BLOCK, 0) !=
                 hipSuccess ||
             o < 1)
             o = 1;
         return o;
     }();
     timed(c, st, "seg_msd_local", [&] {
-        hipLaunchKernelGGL((k_segmsd_local<HYB_BLOCK, HYB_ITEMS>), dim3(c->num_cus * occ), dim3(HYB_BLOCK), 0, st, keysB,
+        hipLaunchKernelGGL((k_segmsd_local<SEGMSD_BLOCK, SEGMSD_ITEMS>), dim3(c->num_cus * occ), dim3(SEGMSD_BLOCK), 0, st, keysB,
                            d_keys, bounds, sp2, pairs, medium2, large2, tile02, tile);
     });
     timed(c, st, "seg_medium", [&] {

@@ -50,6 +50,8 @@ constexpr int HYB_SUB = RADIX * RADIX;         // sub-buckets: one per (byte 3, 
 // Batched segments: segments longer than this many keys take the MSD-first
 // form by default (its 256 buckets then hold at least 512 keys on average)
 constexpr int64_t SEG_MSD_DEFAULT = 131072;
+constexpr int SEGMSD_BLOCK = 256; // the bucket kernel's workgroup: 256 x 20 keys
+constexpr int SEGMSD_ITEMS = 20;
 
 // The segments of the segmented pass (begin1[0..256]) and a clear overflow
 // plan (sp2).  MSD-first form: the 256 buckets of the first split, from pass
@@ -372,7 +374,7 @@ __device__ inline bool segmsd_sort_bucket(const uint32_t (&v)[ITEMS], uint32_t l
 // HYB_MAX_BIN keys go to the overflow plan sp2 as k_hyb_local lists them
 // (pairs, medium2, large2 with tiles in tile0_2).
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK, 5) void k_segmsd_local(const uint32_t *__restrict__ keysB,
+__global__ __launch_bounds__(BLOCK, BLOCK == 512 ? 8 : 5) void k_segmsd_local(const uint32_t *__restrict__ keysB,
                                                         uint32_t *__restrict__ keysA,
                                                         const uint2 *__restrict__ bounds, SegPlan *__restrict__ sp2,
                                                         int64_t *__restrict__ pairs, uint32_t *__restrict__ medium2,
