@@ -48,8 +48,8 @@ constexpr int HYB_NBIN = 4096;                 // 12-bit bins: key bits 4..15
 constexpr int HYB_MAX_BIN = 256;               // keys per bin the rank loop takes
 constexpr int HYB_SUB = RADIX * RADIX;         // sub-buckets: one per (byte 3, byte 2)
 // Batched segments: segments longer than this many keys take the MSD-first
-// form by default (its 256 buckets then hold at least 512 keys on average)
-constexpr int64_t SEG_MSD_DEFAULT = 131072;
+// form by default (its 256 buckets then hold at least 256 keys on average)
+constexpr int64_t SEG_MSD_DEFAULT = 65536;
 constexpr int SEGMSD_BLOCK = 256; // the bucket kernel's workgroup: 256 x 20 keys
 constexpr int SEGMSD_ITEMS = 20;
 

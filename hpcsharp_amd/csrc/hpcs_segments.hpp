@@ -402,18 +402,18 @@ __global__ __launch_bounds__(BLOCK) void k_seg_small(const uint32_t *keys, uint3
 
 // One workgroup per tile (BLOCK x ITEMS keys) of the large segments (the
 // grid may exceed the tile count; spare workgroups return): digit counts of
-// pass `pass`.
-template <int BLOCK, int ITEMS, bool PEEL = true>
-__global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
-                                                      const uint32_t *__restrict__ keysB, int pass,
-                                                      const SegPlan *__restrict__ sp, const uint4 *__restrict__ tinfo,
-                                                      uint16_t *__restrict__ counts,
-                                                      const uint32_t *__restrict__ dshift)
+// pass `pass`.  STRIDE: a grid smaller than the tile count walks the tiles
+// grid-stride (the overflow stages' launches, which are mostly empty; a
+// separate instantiation, since the loop costs the one-tile form registers
+// and time).
+template <int BLOCK, int ITEMS, bool PEEL>
+__device__ __forceinline__ void seg_counts_tile(uint32_t t, const uint32_t *__restrict__ keysA,
+                                                const uint32_t *__restrict__ keysB, int pass,
+                                                const uint4 *__restrict__ tinfo, uint16_t *__restrict__ counts,
+                                                const uint32_t *__restrict__ dshift)
 {
     static_assert(BLOCK * ITEMS < 65536, "a tile's digit counts fit 16 bits");
     __shared__ uint32_t sh[RADIX];
-    const uint32_t t = blockIdx.x;
-    if (t >= sp->num_tiles) return;
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
     int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
@@ -450,6 +450,25 @@ __global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict
     }
     __syncthreads();
     for (int i = tid; i < RADIX; i += BLOCK) counts[size_t(t) * RADIX + i] = uint16_t(sh[i]);
+}
+
+template <int BLOCK, int ITEMS, bool PEEL = true, bool STRIDE = false>
+__global__ __launch_bounds__(BLOCK) void k_seg_counts(const uint32_t *__restrict__ keysA,
+                                                      const uint32_t *__restrict__ keysB, int pass,
+                                                      const SegPlan *__restrict__ sp, const uint4 *__restrict__ tinfo,
+                                                      uint16_t *__restrict__ counts,
+                                                      const uint32_t *__restrict__ dshift)
+{
+    const uint32_t nt = sp->num_tiles;
+    if constexpr (!STRIDE) {
+        if (blockIdx.x < nt) seg_counts_tile<BLOCK, ITEMS, PEEL>(blockIdx.x, keysA, keysB, pass, tinfo, counts, dshift);
+        return;
+    }
+#pragma nounroll
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        seg_counts_tile<BLOCK, ITEMS, PEEL>(t, keysA, keysB, pass, tinfo, counts, dshift);
+        if (t + gridDim.x < nt) __syncthreads(); // the next tile clears the counters
+    }
 }
 
 // Grid-stride over the large segments, one workgroup each: digit starts of
@@ -641,7 +660,8 @@ static __global__ __launch_bounds__(RADIX) void k_segx_bases(const int64_t *__re
 }
 
 // Scatter of pass `pass` for the large segments' tiles, one workgroup per
-// tile (spare workgroups of the grid return): the plain sort's prescanned
+// tile (spare workgroups of the grid return; STRIDE, as for k_seg_counts:
+// a smaller grid walks the tiles grid-stride): the plain sort's prescanned
 // scatter (k_onesweep with precomputed offsets) on a tile record.  Workgroup
 // b < num_tiles takes tile xcd_contiguous(b, num_tiles): XCD b % 8 walks a
 // contiguous share of the tiles, so neighbouring tiles (adjacent digit runs)
@@ -651,14 +671,14 @@ static __global__ __launch_bounds__(RADIX) void k_segx_bases(const int64_t *__re
 // LDS atomics first and stages while it ranks, RANK_MIX2 / RANK_MIX4 match
 // every second / fourth item by ballots, RANK_BALLOT ranks by ballots with
 // wave-local counters and stages after the scan).
-template <int BLOCK, int ITEMS, int RANK, bool PEEL = true>
-__global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ keysA, uint32_t *__restrict__ keysB,
-                                                       int pass, const SegPlan *__restrict__ sp,
-                                                       const uint4 *__restrict__ tinfo,
-                                                       const uint32_t *__restrict__ offsets,
-                                                       const uint32_t *__restrict__ rbase,
-                                                       const uint32_t *__restrict__ sbase,
-                                                       const uint32_t *__restrict__ dshift)
+template <int BLOCK, int ITEMS, int RANK, bool PEEL>
+__device__ __forceinline__ void seg_scatter_tile(uint32_t vb, uint32_t ntiles, uint32_t *__restrict__ keysA,
+                                                 uint32_t *__restrict__ keysB, int pass,
+                                                 const uint4 *__restrict__ tinfo,
+                                                 const uint32_t *__restrict__ offsets,
+                                                 const uint32_t *__restrict__ rbase,
+                                                 const uint32_t *__restrict__ sbase,
+                                                 const uint32_t *__restrict__ dshift)
 {
     static_assert(BLOCK >= RADIX && BLOCK % WAVE == 0 && BLOCK <= 512, "one thread per digit, at most 8 waves");
     constexpr int TILE = BLOCK * ITEMS;
@@ -670,14 +690,12 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
     __shared__ uint32_t s_scan[SCAN_WAVES];
     __shared__ uint32_t s_base[RADIX];
     __shared__ uint32_t s_keys[TILE];
-    const uint32_t ntiles = sp->num_tiles;
-    if (blockIdx.x >= ntiles) return;
     const uint32_t *__restrict__ kin = (pass & 1) ? keysB : keysA;
     uint32_t *__restrict__ kout = (pass & 1) ? keysA : keysB;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid >> 6;
     int shift = dshift ? int(*dshift) : 8 * pass; // the digit's bit; pass picks the buffers
     if (shift >= 32) return;                      // a pass the plan skips
-    const uint32_t t = xcd_contiguous(blockIdx.x, ntiles);
+    const uint32_t t = xcd_contiguous(vb, ntiles);
     const uint4 ti = tinfo[t];
     if (ti.w & SEG_MSD_TILE) { // MSD-first segment: pass 0 by the top byte only
         if (pass != 0) return;
@@ -836,6 +854,30 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ ke
                 out[s_base[(k >> shift) & (RADIX - 1)] + uint32_t(j)] = k;
             }
         }
+    }
+}
+
+template <int BLOCK, int ITEMS, int RANK, bool PEEL = true, bool STRIDE = false>
+__global__ __launch_bounds__(BLOCK) void k_seg_scatter(uint32_t *__restrict__ keysA, uint32_t *__restrict__ keysB,
+                                                       int pass, const SegPlan *__restrict__ sp,
+                                                       const uint4 *__restrict__ tinfo,
+                                                       const uint32_t *__restrict__ offsets,
+                                                       const uint32_t *__restrict__ rbase,
+                                                       const uint32_t *__restrict__ sbase,
+                                                       const uint32_t *__restrict__ dshift)
+{
+    const uint32_t ntiles = sp->num_tiles;
+    if constexpr (!STRIDE) {
+        if (blockIdx.x < ntiles)
+            seg_scatter_tile<BLOCK, ITEMS, RANK, PEEL>(blockIdx.x, ntiles, keysA, keysB, pass, tinfo, offsets, rbase,
+                                                       sbase, dshift);
+        return;
+    }
+#pragma nounroll
+    for (uint32_t vb = blockIdx.x; vb < ntiles; vb += gridDim.x) {
+        seg_scatter_tile<BLOCK, ITEMS, RANK, PEEL>(vb, ntiles, keysA, keysB, pass, tinfo, offsets, rbase, sbase,
+                                                   dshift);
+        if (vb + gridDim.x < ntiles) __syncthreads(); // the next tile reuses the LDS
     }
 }
 

@@ -118,7 +118,7 @@ extern "C" {
                                       the MSD-first form (one segmented pass by the top byte, then each of
                                       the 256 buckets sorted in LDS: 20 bytes of HBM traffic per key against
                                       48 for the four passes); same output.  0 off; otherwise at least
-                                      16385; default 131072 */
+                                      16385; default 65536 */
 
 #define HPCS_DTYPE_U32 0
 #define HPCS_DTYPE_U64 1
