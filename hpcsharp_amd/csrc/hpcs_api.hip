@@ -695,11 +695,18 @@ void run_sort_msd(hpcs_ctx *c, K *d_keys, int64_t n, hipStream_t st, const SortP
 }
 
 void launch_seg_counts(int shape, dim3 grid, hipStream_t st, const uint32_t *a, const uint32_t *b, int pass,
-                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift, bool peel,
+                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift,
                        bool stride = false);
 void launch_seg_scatter(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
                         const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
-                        const uint32_t *sbase, const uint32_t *dshift, bool peel, bool stride = false);
+                        const uint32_t *sbase, const uint32_t *dshift, bool stride = false);
+
+// Workgroups per CU of the overflow stages' segmented passes (grid-stride
+// over their tiles): few enough that the usual empty launch costs ~2 us
+// instead of ~14 (one workgroup per tile bound), many enough that a full one
+// (keys crowded into few buckets) runs as fast as one workgroup per tile
+// (2^28 keys, 90% in one top byte: 3.26 ms against 3.25; 8 per CU: 3.80).
+constexpr int OV_GRID_PER_CU = 128;
 
 // Scratch of the MSD-first form (hpcs_hybrid.hpp): the reduce-then-scan
 // layout, then the top-byte tile counts, the bucket bounds, level 2's and the
@@ -837,7 +844,7 @@ void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, c
     });
     timed(c, st, "seg_counts", [&] {
         launch_seg_counts(H.seg_shape, dim3(unsigned(H.max_tiles1)), st, keysB, d_keys, 2, sp, tinfo, scnt,
-                          &plan->hyb_shift2, true);
+                          &plan->hyb_shift2);
     });
     // the large segments' scan: range sums, column scan, segment bases
     auto segx_scan = [&](const int64_t *bg, const SegPlan *p, const uint32_t *t0, const uint32_t *lst, uint64_t mt,
@@ -853,7 +860,7 @@ void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, c
     segx_scan(begin1, sp, tile0, large, H.max_tiles1, RADIX, dstart);
     timed(c, st, "seg_scatter", [&] {
         launch_seg_scatter(H.seg_shape, rank_mode, dim3(unsigned(H.max_tiles1)), st, keysB, d_keys, 2, sp, tinfo,
-                           soff, rbase, sbase, &plan->hyb_shift2, true);
+                           soff, rbase, sbase, &plan->hyb_shift2);
     });
     // level 3: every sub-bucket in LDS, in place
     timed(c, st, "hyb_local", [&] {
@@ -869,18 +876,18 @@ void run_sort_hybrid(hpcs_ctx *c, uint32_t *d_keys, int64_t n, hipStream_t st, c
         hipLaunchKernelGGL(k_seg_tiles, dim3(c->num_cus * 8), dim3(256), 0, st, pairs, sp2, tile02, large2, tinfo,
                            stile, int64_t(0));
     });
-    // the overflow passes walk their tiles grid-stride from a grid of a few
-    // workgroups per CU (usually they have none)
-    const dim3 ov_grid(unsigned(std::min<uint64_t>(H.max_tiles2, uint64_t(c->num_cus) * 8)));
+    // the overflow passes walk their tiles grid-stride from OV_GRID_PER_CU
+    // workgroups per CU (usually they have no tiles at all)
+    const dim3 ov_grid(unsigned(std::min<uint64_t>(H.max_tiles2, uint64_t(c->num_cus) * OV_GRID_PER_CU)));
     for (int p = 0; p < 2; ++p) {
         timed(c, st, "seg_counts", [&] {
             launch_seg_counts(H.seg_shape, ov_grid, st, d_keys, keysB, p, sp2, tinfo, scnt, &plan->hyb_ov_shift[p],
-                              true, true);
+                              true);
         });
         segx_scan(pairs, sp2, tile02, large2, H.max_tiles2, H.max_large2, nullptr);
         timed(c, st, "seg_scatter", [&] {
             launch_seg_scatter(H.seg_shape, rank_mode, ov_grid, st, d_keys, keysB, p, sp2, tinfo, soff, rbase, sbase,
-                               &plan->hyb_ov_shift[p], true, true);
+                               &plan->hyb_ov_shift[p], true);
         });
     }
     timed(c, st, "copyback", [&] {
@@ -1238,35 +1245,25 @@ void with_seg_shape(int shape, F &&f)
     }
 }
 
-// Segmented passes: `peel` (the MSD-first form's stages, whose inputs keep
-// the caller's runs of equal digits) counts and ranks skewed waves by
-// peeling; the batched segments' passes do without its per-wave test.
-// `stride`: the grid-stride instantiation, for grids smaller than the tile
-// count (the overflow stages).
+// Segmented passes, counting and ranking skewed waves by peeling (runs of
+// equal digits: crowded buckets, keys that share their top byte; no cost on
+// spread keys).  `stride`: the grid-stride instantiation, for grids smaller
+// than the tile count (the overflow stages).
 void launch_seg_counts(int shape, dim3 grid, hipStream_t st, const uint32_t *a, const uint32_t *b, int pass,
-                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift, bool peel,
-                       bool stride)
+                       const SegPlan *sp, const uint4 *tinfo, uint16_t *counts, const uint32_t *dshift, bool stride)
 {
     with_seg_shape(shape, [&](auto sc) {
         constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
-        if (stride) {
-            if (peel)
-                hipLaunchKernelGGL((k_seg_counts<T.block, T.items, true, true>), grid, dim3(T.block), 0, st, a, b,
-                                   pass, sp, tinfo, counts, dshift);
-            else
-                hipLaunchKernelGGL((k_seg_counts<T.block, T.items, false, true>), grid, dim3(T.block), 0, st, a, b,
-                                   pass, sp, tinfo, counts, dshift);
-        } else if (peel) {
+        if (stride)
+            hipLaunchKernelGGL((k_seg_counts<T.block, T.items, true, true>), grid, dim3(T.block), 0, st, a, b, pass,
+                               sp, tinfo, counts, dshift);
+        else
             hipLaunchKernelGGL((k_seg_counts<T.block, T.items, true>), grid, dim3(T.block), 0, st, a, b, pass, sp,
                                tinfo, counts, dshift);
-        } else {
-            hipLaunchKernelGGL((k_seg_counts<T.block, T.items, false>), grid, dim3(T.block), 0, st, a, b, pass, sp,
-                               tinfo, counts, dshift);
-        }
     });
 }
 
-template <bool PEEL, bool STRIDE>
+template <bool STRIDE>
 void launch_seg_scatter_t(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
                           const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
                           const uint32_t *sbase, const uint32_t *dshift)
@@ -1274,16 +1271,16 @@ void launch_seg_scatter_t(int shape, int rank, dim3 grid, hipStream_t st, uint32
     with_seg_shape(shape, [&](auto sc) {
         constexpr TileShape T = TILE_SHAPES[decltype(sc)::value];
         if (rank == RANK_BALLOT)
-            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_BALLOT, PEEL, STRIDE>), grid, dim3(T.block), 0, st,
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_BALLOT, true, STRIDE>), grid, dim3(T.block), 0, st,
                                a, b, pass, sp, tinfo, offsets, rbase, sbase, dshift);
         else if (rank == RANK_MIX2)
-            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX2, PEEL, STRIDE>), grid, dim3(T.block), 0, st, a,
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX2, true, STRIDE>), grid, dim3(T.block), 0, st, a,
                                b, pass, sp, tinfo, offsets, rbase, sbase, dshift);
         else if (rank == RANK_MIX4)
-            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX4, PEEL, STRIDE>), grid, dim3(T.block), 0, st, a,
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_MIX4, true, STRIDE>), grid, dim3(T.block), 0, st, a,
                                b, pass, sp, tinfo, offsets, rbase, sbase, dshift);
         else if (rank == RANK_LDSOR)
-            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_LDSOR, PEEL, STRIDE>), grid, dim3(T.block), 0, st,
+            hipLaunchKernelGGL((k_seg_scatter<T.block, T.items, RANK_LDSOR, true, STRIDE>), grid, dim3(T.block), 0, st,
                                a, b, pass, sp, tinfo, offsets, rbase, sbase, dshift);
         else
             fail(HPCS_E_UNSUPPORTED, "unknown ranking mode " + std::to_string(rank));
@@ -1293,19 +1290,10 @@ void launch_seg_scatter_t(int shape, int rank, dim3 grid, hipStream_t st, uint32
 // `stride`: as for launch_seg_counts.
 void launch_seg_scatter(int shape, int rank, dim3 grid, hipStream_t st, uint32_t *a, uint32_t *b, int pass,
                         const SegPlan *sp, const uint4 *tinfo, const uint32_t *offsets, const uint32_t *rbase,
-                        const uint32_t *sbase, const uint32_t *dshift, bool peel, bool stride)
+                        const uint32_t *sbase, const uint32_t *dshift, bool stride)
 {
-    if (stride) {
-        if (peel)
-            return launch_seg_scatter_t<true, true>(shape, rank, grid, st, a, b, pass, sp, tinfo, offsets, rbase, sbase,
-                                                    dshift);
-        return launch_seg_scatter_t<false, true>(shape, rank, grid, st, a, b, pass, sp, tinfo, offsets, rbase, sbase,
-                                                 dshift);
-    }
-    if (peel)
-        return launch_seg_scatter_t<true, false>(shape, rank, grid, st, a, b, pass, sp, tinfo, offsets, rbase, sbase,
-                                                 dshift);
-    launch_seg_scatter_t<false, false>(shape, rank, grid, st, a, b, pass, sp, tinfo, offsets, rbase, sbase, dshift);
+    if (stride) return launch_seg_scatter_t<true>(shape, rank, grid, st, a, b, pass, sp, tinfo, offsets, rbase, sbase, dshift);
+    launch_seg_scatter_t<false>(shape, rank, grid, st, a, b, pass, sp, tinfo, offsets, rbase, sbase, dshift);
 }
 
 void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_begin, int nseg, hipStream_t st)
@@ -1353,7 +1341,7 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     const dim3 tile_grid(unsigned(L.max_tiles));
     for (int p = 0; p < 4; ++p) {
         timed(c, st, "seg_counts", [&] {
-            launch_seg_counts(prm.shape, tile_grid, st, d_keys, keysB, p, sp, tinfo, counts, nullptr, false);
+            launch_seg_counts(prm.shape, tile_grid, st, d_keys, keysB, p, sp, tinfo, counts, nullptr);
         });
         // the MSD-first segments keep pass 0's offsets (their bucket bounds)
         const int64_t skip = (msd && p > 0) ? msd_min : INT64_MAX;
@@ -1363,7 +1351,7 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
         });
         timed(c, st, "seg_scatter", [&] {
             launch_seg_scatter(prm.shape, prm.rank, tile_grid, st, d_keys, keysB, p, sp, tinfo, offsets, nullptr,
-                               nullptr, nullptr, false);
+                               nullptr, nullptr);
         });
     }
     if (!msd) return;
@@ -1405,11 +1393,10 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
     });
     const int list_grid2 = int(std::min<uint64_t>(L.max_large2, uint64_t(c->num_cus) * 8));
     // grid-stride over the overflow tiles (usually none)
-    const dim3 ov_grid(unsigned(std::min<uint64_t>(L.max_tiles, uint64_t(c->num_cus) * 8)));
+    const dim3 ov_grid(unsigned(std::min<uint64_t>(L.max_tiles, uint64_t(c->num_cus) * OV_GRID_PER_CU)));
     for (int p = 1; p < 4; ++p) {
         timed(c, st, "seg_counts", [&] {
-            launch_seg_counts(prm.shape, ov_grid, st, d_keys, keysB, p, sp2, tinfo, counts, shifts + (p - 1), false,
-                              true);
+            launch_seg_counts(prm.shape, ov_grid, st, d_keys, keysB, p, sp2, tinfo, counts, shifts + (p - 1), true);
         });
         timed(c, st, "seg_scan", [&] {
             hipLaunchKernelGGL(k_seg_scan, dim3(list_grid2), dim3(SEG_SCAN_GROUPS * SEG_SCAN_PAIRS), 0, st, pairs, sp2,
@@ -1417,7 +1404,7 @@ void run_segments(hpcs_ctx *c, uint32_t *d_keys, int64_t n, const int64_t *d_beg
         });
         timed(c, st, "seg_scatter", [&] {
             launch_seg_scatter(prm.shape, prm.rank, ov_grid, st, d_keys, keysB, p, sp2, tinfo, offsets, nullptr,
-                               nullptr, shifts + (p - 1), false, true);
+                               nullptr, shifts + (p - 1), true);
         });
     }
 }
